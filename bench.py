@@ -1,0 +1,218 @@
+"""Benchmark: FSP solves/s on one MI355X, with the mat-vec roofline and a CPU baseline.
+
+Workload (one "step" = one complete solve): the two-species toggle switch on the fixed
+state space 0..1023 x 0..1023 (1,048,576 states + 4 sinks, 5,240,832 reference
+nonzeros), production 50/(1+(S/20)^2.5)+1, degradation 1 per molecule, initial state
+(0,0), outputs at t = 0,1,...,10, device Krylov expv with m = 30. Storage FP32 (matrix
+values and Krylov basis) with FP64 diagonal, accumulation and master state; the l1
+distance of every snapshot to the FP64 GPU solve is measured in the same run (the FP64
+path itself is checked against the exact oracle by the test suite).
+
+    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64] [--no-cpu]
+
+Prints one JSON line. Inputs are resident on the GPU before the timed region; each
+timed solve restarts from the device-resident initial state and keeps its 11
+snapshots on the device.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "numcme.jl_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def toggle_model(nh):
+    S = np.array([[1, -1, 0, 0], [0, 0, 1, -1]])
+    props = [
+        nh.propensity(lambda x, p: p[3] + p[0] / (1.0 + (x[1] / p[1]) ** p[2])),
+        nh.propensity(lambda x, p: p[4] * x[0]),
+        nh.propensity(lambda x, p: p[3] + p[0] / (1.0 + (x[0] / p[1]) ** p[2])),
+        nh.propensity(lambda x, p: p[4] * x[1]),
+    ]
+    return nh.CmeModel(S, props, [50.0, 20.0, 2.5, 1.0, 1.0])
+
+
+def box_states(n):
+    a = np.arange(n)
+    s1, s2 = np.meshgrid(a, a, indexing="xy")  # species 0 fastest
+    return np.stack([s1.ravel(), s2.ravel()], axis=1)
+
+
+class Solver:
+    def __init__(self, L, A, precision, time_kernels):
+        o = L.nc_solve_opts()
+        L.call("nc_solve_opts_default", C.byref(o))
+        o.method = L.NC_METHOD_KRYLOV
+        o.krylov_m = 30
+        o.krylov_tol = 1e-12
+        o.time_kernels = int(time_kernels)
+        h = C.c_void_p()
+        L.call("nc_solver_create", A.handle, C.byref(o), C.byref(h))
+        self.L, self.h, self.A = L, h.value, A
+
+    def set_state(self, u0):
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        self.L.call("nc_solver_set_state", self.h, 0.0, self.L.ptr(u0, C.c_double))
+
+    def solve(self, ts):
+        self.L.call("nc_solver_restart", self.h)
+        self.L.call("nc_solver_run", self.h, len(ts), self.L.ptr(ts, C.c_double))
+
+    def stats(self):
+        st = self.L.nc_solve_stats()
+        self.L.call("nc_solver_stats", self.h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in self.L.nc_solve_stats._fields_}
+
+    def snapshot(self, k, n):
+        out = np.empty(n)
+        self.L.call("nc_solver_get_snapshot", self.h, k, self.L.ptr(out, C.c_double))
+        return out
+
+    def free(self):
+        self.L.call("nc_solver_destroy", self.h)
+
+
+def cpu_baseline(model_theta, gpu_fp64_steps, max_steps=3):
+    """Oracle-built CSC matrix, C single-thread CSC mat-vec + Arnoldi; sample of Krylov steps."""
+    from oracle.cpu_ref import CscExpv, build
+    from oracle.fspmatrix import FspMatrix, TimeInvariant
+    from oracle.statespace import StateSpace
+
+    build()
+    S = np.array([[1, -1, 0, 0], [0, 0, 1, -1]])
+    th = model_theta
+    hill = lambda y: th[3] + th[0] / (1.0 + (y / th[1]) ** th[2])
+    props = [TimeInvariant(lambda x, p: hill(x[1])), TimeInvariant(lambda x, p: th[4] * x[0]),
+             TimeInvariant(lambda x, p: hill(x[0])), TimeInvariant(lambda x, p: th[4] * x[1])]
+    sp = StateSpace(S, box_states(1024))
+    A = FspMatrix(sp, props, th).to_csc(0.0)
+    ex = CscExpv(A, m=30, tol=1e-12)
+    u0 = np.zeros(A.shape[0])
+    u0[0] = 1.0
+    _, steps, secs = ex.run(u0, 10.0, max_steps=max_steps)
+    per_step = secs / max(steps, 1)
+    return {
+        "value": 1.0 / (per_step * gpu_fp64_steps),
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} Krylov steps (m=30, {steps * 31} CSC column-scatter mat-vecs + CGS2) of the same solve "
+                  f"in FP64, {secs:.2f} s, scaled by the {gpu_fp64_steps} steps of the full FP64 solve",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs one FSP solve per GPU; multi-GPU runs are not part of this "
+                                   "build (see DESIGN.md for how it would shard)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numcme_hip as nh
+    from numcme_hip import _lib as L
+
+    t_build = time.perf_counter()
+    model = toggle_model(nh)
+    states = box_states(1024)
+    space = nh.StateSpaceSparse(model.stoich_matrix, states)
+    A = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision=args.precision, model=model)
+    A64 = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision="fp64", model=model)
+    info = A.info()
+    N = info["n_rows"]
+    build_s = time.perf_counter() - t_build
+    u0 = np.zeros(N)
+    u0[0] = 1.0
+    ts = np.arange(0.0, 11.0)
+
+    # FP64 reference solve on the GPU (accuracy check of the timed precision)
+    s64 = Solver(L, A64, "fp64", False)
+    s64.set_state(u0)
+    s64.solve(ts)
+    ref = [s64.snapshot(k, N) for k in range(len(ts))]
+    fp64_steps = s64.stats()["steps"]
+    t0 = time.perf_counter()
+    s64.solve(ts)
+    fp64_solve_s = time.perf_counter() - t0
+    s64.free()
+
+    sol = Solver(L, A, args.precision, True)
+    sol.set_state(u0)
+    for _ in range(args.warmup):
+        sol.solve(ts)
+    l1 = max(float(np.abs(sol.snapshot(k, N) - ref[k]).sum()) for k in range(len(ts)))
+    sol.L.call("nc_solver_restart", sol.h)  # clears stats (and the kernel timers below)
+    A_timed_before = sol.stats()
+    t0 = time.perf_counter()
+    steps_stats = []
+    for _ in range(args.steps):
+        sol.solve(ts)
+        steps_stats.append(sol.stats())
+    elapsed = time.perf_counter() - t0
+    mv_ms = steps_stats[-1]["matvec_ms"] - A_timed_before["matvec_ms"]
+    mv_n = steps_stats[-1]["matvecs_timed"] - A_timed_before["matvecs_timed"]
+    sol.free()
+
+    avg_mv_s = (mv_ms / mv_n) * 1e-3 if mv_n else float("nan")
+    achieved = info["csr_bytes_per_matvec"] / avg_mv_s / 1e9
+    line = {
+        "metric": "FSP solves/s at l1<=1e-6 (toggle 1024^2, Krylov m=30); SpMV HBM GB/s",
+        "value": args.steps / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-storage/f64-accumulate" if args.precision == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "toggle switch, fixed FSP space 0..1023^2 (1,048,576 states + 4 sinks, 5,240,832 nnz), "
+                        "t=0..10 at 11 outputs, device Krylov expv m=30",
+            "precision": args.precision,
+            "krylov_steps_per_solve": steps_stats[-1]["steps"],
+            "matvecs_per_solve": steps_stats[-1]["matvecs"],
+        },
+        "l1_vs_fp64_gpu": l1,
+        "fp64_solves_per_s": 1.0 / fp64_solve_s,
+        "setup_s": build_s,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "k_matvec (FSP SpMV)",
+            "bytes_per_launch_csr": info["csr_bytes_per_matvec"],
+            "bytes_per_launch_own_layout": info["bytes_per_matvec"],
+            "avg_launch_us": avg_mv_s * 1e6,
+            "launches_timed": mv_n,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(model.parameters, fp64_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,236 @@
+/*
+ * numcme_hip.h — C ABI of the MI355X-native finite-state-projection (FSP) engine.
+ *
+ * The library replaces, for one GPU, the FSP hot path of NumCME.jl: the sparse state
+ * space, the FSP generator and its mat-vec, the time integrator with the sink-mass
+ * budget, and the R-step space adapters. Each entry point below names the reference
+ * code it replaces (paths relative to the NumCME.jl repository).
+ *
+ * Conventions (all entry points):
+ *  - Every function returns an int status (NC_OK == 0). On failure a message is
+ *    available from nc_last_error() in the calling thread. No C++ exception crosses
+ *    the ABI.
+ *  - Handles are opaque and owned by the library; destroy them with *_destroy.
+ *  - Host pointers are borrowed for the duration of the call only. Outputs are copied
+ *    into caller-allocated arrays whose sizes are stated per function.
+ *  - Indices are 0-based. "None" is -1 (state connectivity) and reaction numbers are
+ *    0-based (sink connectivity). A 1-based host shim adds 1 to every index.
+ *  - Each handle owns one HIP stream (default priority). Handles are not thread-safe;
+ *    calls on one handle serialise, as in the single-threaded reference.
+ *  - The library needs a HIP device; with none it fails with NC_ERR_NO_DEVICE. There
+ *    is no CPU fallback.
+ */
+#ifndef NUMCME_HIP_H
+#define NUMCME_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------------------- */
+enum {
+  NC_OK = 0,
+  NC_ERR_ARG = 1,         /* invalid argument (the reference throws ArgumentError)        */
+  NC_ERR_HIP = 2,         /* HIP runtime error                                            */
+  NC_ERR_ALLOC = 3,       /* device or host allocation failed                             */
+  NC_ERR_OVERFLOW = 4,    /* a size or index exceeds the library's 32-bit limits          */
+  NC_ERR_UNSUPPORTED = 5, /* feature outside what this build supports                    */
+  NC_ERR_NUMERIC = 6,     /* integrator failure (step size underflow, too many steps)     */
+  NC_ERR_INTERNAL = 7,
+  NC_ERR_NO_DEVICE = 8
+};
+
+/* Storage precision of matrix values and probability vectors. */
+enum { NC_FP64 = 0, NC_FP32 = 1 };
+
+/* Copy the calling thread's last error message into buf (always NUL-terminated). */
+int nc_last_error(char* buf, size_t len);
+/* Library version and the HIP device it runs on. */
+int nc_version(int32_t* major, int32_t* minor);
+int nc_device_name(char* buf, size_t len);
+
+/* ======================================================================================
+ * State space — replaces StateSpaceSparse (src/statespace/sparse/sparsestatespace.jl).
+ * States are int32 vectors of ns species; a state space holds at most 2^31-2 states
+ * and at most 32 reactions.
+ * ==================================================================================== */
+typedef struct nc_space_s* nc_space;
+
+/* StateSpaceSparse(stoich_matrix, initstates) — sparsestatespace.jl:103-124 (and the
+ * single-state form :142-144). stoich: ns x nr, column-major (column r = s_r).
+ * init_states: n_init x ns, row-major. Duplicates and states with a negative entry are
+ * dropped, first occurrence wins, exactly as _addstates! (:208-267). */
+int nc_space_create(int32_t ns, int32_t nr, const int32_t* stoich, int64_t n_init,
+                    const int32_t* init_states, nc_space* out);
+int nc_space_destroy(nc_space space);
+/* expand!(space, levels; onlyreactions) — sparsestatespace.jl:153-194. only_reactions
+ * holds n_only 0-based reaction numbers; n_only == 0 means all reactions. The new states
+ * get the indices the reference assigns (LIFO exploration, first occurrence wins). */
+int nc_space_expand(nc_space space, int32_t levels, int32_t n_only, const int32_t* only_reactions);
+/* deleteat!(space, ids) — sparsestatespace.jl:276-331. ids: n_ids 0-based indices,
+ * strictly increasing (the reference requires sorted unique ids). */
+int nc_space_delete(nc_space space, int64_t n_ids, const int64_t* ids);
+/* get_state_count / get_sink_count — sparsestatespace.jl:55,62 */
+int nc_space_count(nc_space space, int64_t* n_states, int32_t* n_sinks);
+/* get_states — sparsestatespace.jl:69. out: n_states x ns int32, row-major. */
+int nc_space_get_states(nc_space space, int32_t* out);
+/* get_state_connectivity — sparsestatespace.jl:78. out: n_states x nr int32; entry (i,r)
+ * is the index of x_i - s_r, or -1. */
+int nc_space_get_state_conn(nc_space space, int32_t* out);
+/* get_sink_connectivity — sparsestatespace.jl:80. out: n_states x nr int32; entry (i,r)
+ * is r when x_i + s_r is non-negative and absent from the space, else -1. */
+int nc_space_get_sink_conn(nc_space space, int32_t* out);
+/* get_statedict lookups — sparsestatespace.jl:76. states: n x ns row-major; out: index
+ * or -1 per state. */
+int nc_space_lookup(nc_space space, int64_t n, const int32_t* states, int64_t* out);
+
+/* ======================================================================================
+ * Model — replaces CmeModel and the propensity kinds (src/cmemodel/cmemodel.jl:49-53,
+ * src/cmemodel/propensity.jl:50-153). Propensities are postfix programs evaluated in
+ * FP64 on a stack of depth NC_PROG_STACK.
+ * ==================================================================================== */
+#define NC_PROG_STACK 16
+enum nc_opcode {
+  NC_OP_CONST = 0,  /* push c                              */
+  NC_OP_PARAM = 1,  /* push theta[arg]                     */
+  NC_OP_SPECIES = 2,/* push x[arg]                         */
+  NC_OP_TIME = 3,   /* push t                              */
+  NC_OP_ADD = 4, NC_OP_SUB = 5, NC_OP_MUL = 6, NC_OP_DIV = 7, NC_OP_POW = 8,
+  NC_OP_NEG = 9, NC_OP_EXP = 10, NC_OP_LOG = 11, NC_OP_SIN = 12, NC_OP_COS = 13,
+  NC_OP_SQRT = 14, NC_OP_ABS = 15,
+  NC_OP_MAX = 16,   /* max(a,b) = (b < a) ? a : b, as Julia's Base.max  */
+  NC_OP_MIN = 17,   /* min(a,b) = (b < a) ? b : a                      */
+  NC_OP_LT = 18, NC_OP_LE = 19, NC_OP_GT = 20, NC_OP_GE = 21, /* push 1.0 or 0.0     */
+  NC_OP_SELECT = 22 /* pops c, a, b (b on top): push (c != 0) ? a : b */
+};
+typedef struct {
+  int32_t op;
+  int32_t arg;
+  double c;
+} nc_instr;
+
+/* Propensity kinds — propensity.jl:50-52 (StandardTimeInvariantPropensity), :91-94
+ * (SeparableTimeVaryingPropensity), :122-124 (JointTimeVaryingPropensity). */
+enum { NC_PROP_TIME_INVARIANT = 0, NC_PROP_SEPARABLE = 1, NC_PROP_JOINT = 2 };
+
+typedef struct nc_model_s* nc_model;
+/* CmeModel(stoich_matrix, propensities, parameters) — cmemodel.jl:49-53.
+ * kinds[r] is the propensity kind of reaction r. prog_offsets has 2*nr+1 entries;
+ * reaction r's "x program" is code[prog_offsets[2r] .. prog_offsets[2r+1]) and its
+ * "t program" is code[prog_offsets[2r+1] .. prog_offsets[2r+2]):
+ *   time-invariant: x program f(x, theta); t program empty
+ *   separable:      x program statefactor(x, theta); t program tfactor(t, theta)
+ *   joint:          x program f(t, x, theta); t program empty                        */
+int nc_model_create(int32_t ns, int32_t nr, const int32_t* stoich, int32_t n_params,
+                    const double* params, const int32_t* kinds, const int64_t* prog_offsets,
+                    const nc_instr* code, nc_model* out);
+int nc_model_destroy(nc_model model);
+/* Partial derivatives of the propensities — CmeModelWithSensitivity (cmemodel.jl:104-108,
+ * 157-170) with PropensityGradient semantics (propensitygrad.jl:56-67). pattern is
+ * nr x n_params row-major (1 where a_r depends on theta_p, the reaction x parameter
+ * sparsity pattern). For pair k = r*n_params + p, dprog_offsets[2k..2k+2) delimit two
+ * programs: d(x program)/d theta_p and d(t program)/d theta_p (the latter only for
+ * separable reactions; empty otherwise). dprog_offsets has 2*nr*n_params+1 entries. */
+int nc_model_set_gradients(nc_model model, const uint8_t* pattern, const int64_t* dprog_offsets,
+                           const nc_instr* dcode);
+/* Evaluate reaction r's x program (or, when which == 1, its t program) at n states
+ * (row-major n x ns) and time t — the propensity call methods of propensity.jl:145-153.
+ * Runs on the device; used to check the program evaluator. */
+int nc_model_eval(nc_model model, int32_t r, int32_t which, double t, int64_t n,
+                  const int32_t* states, double* out);
+
+/* ======================================================================================
+ * FSP matrix — replaces FspMatrixSparse (src/fspmatrix/sparse/fspsparsematrix.jl).
+ * Rows/columns: the states of the space in order, then one sink per reaction.
+ * ==================================================================================== */
+typedef struct nc_matrix_s* nc_matrix;
+typedef struct {
+  int64_t n_rows;        /* states + sinks                                         */
+  int64_t n_states;
+  int32_t n_sinks;
+  int32_t ell_width;     /* merged time-invariant off-diagonal slots per state row */
+  int32_t n_tv;          /* time-varying (separable + joint) reactions             */
+  int32_t precision;
+  int64_t nnz_reference; /* nonzeros of the reference's sparse matrices (explicit
+                            zeros included, duplicates merged per matrix)           */
+  int64_t n_sink_entries;
+  double bytes_per_matvec;   /* bytes the mat-vec kernel moves (its own layout)     */
+  double csr_bytes_per_matvec; /* nnz*(s_v+4) + (N+1)*4 + 2*N*s_v, CSR-equivalent   */
+  double norm1;          /* max column abs sum (2 * max |diagonal|)                 */
+} nc_matrix_info_t;
+
+/* FspMatrixSparse{RealT}(space, propensities; parameters) — fspsparsematrix.jl:47-108.
+ * precision NC_FP64 stores FP64 values; NC_FP32 stores FP32 off-diagonal and sink
+ * values with an FP64 diagonal equal to minus the exact sum of the FP32 column entries,
+ * so every column of the stored generator sums to zero exactly. */
+int nc_matrix_create(nc_space space, nc_model model, int32_t precision, nc_matrix* out);
+int nc_matrix_destroy(nc_matrix A);
+int nc_matrix_info(nc_matrix A, nc_matrix_info_t* info);
+/* matvec!(out, t, A, v) (accumulate == 0) and matvecadd!(out, t, A, v) (accumulate != 0)
+ * — fspsparsematrix.jl:196-247. v, out: host arrays of n_rows doubles. */
+int nc_matrix_matvec(nc_matrix A, double t, const double* v, double* out, int32_t accumulate);
+/* The generator A(t) as COO triplets (row, col, value) in the stored precision, for
+ * inspection. Call with rows == NULL to get *nnz only. */
+int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t* cols, double* vals);
+/* Repeat the mat-vec kernel `reps` times on device-resident vectors and report the mean
+ * kernel duration from HIP events on the matrix's stream. */
+int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms);
+
+/* ======================================================================================
+ * Time integration — replaces the integrator loop of solve() (fspsolve.jl:10-41) with
+ * a device-resident Krylov expv (time-invariant generators) or an adaptive
+ * Dormand-Prince 5(4) Runge-Kutta (any generator).
+ * ==================================================================================== */
+enum { NC_METHOD_AUTO = 0, NC_METHOD_KRYLOV = 1, NC_METHOD_RK45 = 2 };
+typedef struct {
+  int32_t method;      /* NC_METHOD_*                                              */
+  int32_t krylov_m;    /* Krylov subspace dimension (default 30)                   */
+  double krylov_tol;   /* expv local error tolerance per unit time (default 1e-12) */
+  double rtol, atol;   /* RK tolerances (defaults 1e-8, 1e-14)                     */
+  double h0;           /* first RK step, 0 = automatic                             */
+  int64_t max_steps;   /* 0 = unlimited                                            */
+  int32_t time_kernels;/* record HIP events around every mat-vec launch            */
+  int32_t reserved;
+} nc_solve_opts;
+
+typedef struct {
+  int64_t steps, rejected, matvecs, host_syncs;
+  int64_t matvecs_timed;
+  double matvec_ms;    /* summed event-timed mat-vec kernel durations              */
+  double error_estimate;
+} nc_solve_stats;
+
+typedef struct nc_solver_s* nc_solver;
+int nc_solve_opts_default(nc_solve_opts* opts);
+int nc_solver_create(nc_matrix A, const nc_solve_opts* opts, nc_solver* out);
+int nc_solver_destroy(nc_solver s);
+/* Upload the ODE state u = [p; sinks] (n_rows doubles) at time t0. */
+int nc_solver_set_state(nc_solver s, double t0, const double* u);
+/* Reset the state to the one uploaded by the last nc_solver_set_state (kept on the
+ * device), so repeated solves need no host transfer. */
+int nc_solver_restart(nc_solver s);
+/* Integrate from the current time through the ascending output times t_out[0..n_out),
+ * keeping one snapshot per output time on the device (DE.solve with saveat,
+ * fspsolve.jl:27). Returns when the device work is complete. */
+int nc_solver_run(nc_solver s, int32_t n_out, const double* t_out);
+/* Copy snapshot k (n_rows doubles) to the host. */
+int nc_solver_get_snapshot(nc_solver s, int32_t k, double* u);
+int nc_solver_get_state(nc_solver s, double* t, double* u);
+int nc_solver_stats(nc_solver s, nc_solve_stats* st);
+
+/* Fixed-space solve — fspsolve.jl:10-41. u0: n_rows doubles; u_out: n_out x n_rows. */
+int nc_solve_fixed(nc_matrix A, const nc_solve_opts* opts, const double* u0, int32_t n_out,
+                   const double* t_out, double* u_out, nc_solve_stats* st);
+
+/* ---- helpers exposed for testing --------------------------------------------------- */
+/* Dense matrix exponential of an n x n row-major matrix (Pade 13 scaling and squaring,
+ * Higham 2005) — the host-side Hessenberg exponential of the Krylov integrator. */
+int nc_expm_dense(int32_t n, const double* a, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NUMCME_HIP_H */

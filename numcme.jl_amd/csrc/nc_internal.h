@@ -1,0 +1,131 @@
+// Internal helpers shared by the numcme-hip library sources: error state, HIP
+// checks, device buffers and small launch utilities. Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "numcme_hip.h"
+
+namespace nc {
+
+// ---- error handling --------------------------------------------------------------------
+// Every C entry point catches nc::Error (and std::exception) and turns it into a
+// status code plus a thread-local message retrievable with nc_last_error().
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define NC_CHECK_HIP(expr)                                                                 \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      throw ::nc::Error(NC_ERR_HIP, std::string(#expr " failed: ") + hipGetErrorString(_e) + \
+                                        " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define NC_REQUIRE(cond, code, msg)              \
+  do {                                           \
+    if (!(cond)) throw ::nc::Error((code), (msg)); \
+  } while (0)
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return NC_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return NC_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return NC_ERR_INTERNAL;
+  }
+}
+
+// ---- device buffers -------------------------------------------------------------------
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(NC_ERR_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+    }
+    n = count;
+  }
+  // grow to at least `count` elements; contents are NOT preserved
+  void reserve(size_t count) { if (count > n) alloc(count); }
+  // grow to at least `count` elements, preserving the first `keep` elements
+  void grow_keep(size_t count, size_t keep, hipStream_t s) {
+    if (count <= n) return;
+    T* q = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&q), count * sizeof(T));
+    if (e != hipSuccess) throw Error(NC_ERR_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    if (keep && p) NC_CHECK_HIP(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    release();
+    p = q;
+    n = count;
+  }
+  void release() {
+    if (p) {
+      // work queued on any stream may still read the buffer
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
+    p = nullptr;
+    n = 0;
+  }
+  T* get() const { return p; }
+};
+
+inline unsigned grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
+// Exclusive prefix sum of `n` uint32 values (rocPRIM); the total is written to *total_host
+// when non-null (synchronises `s`). Temporary storage lives in `tmp`.
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, DevBuf<unsigned char>& tmp,
+                        hipStream_t s, uint32_t* total_host);
+
+// Dense n x n row-major matrix exponential on the host (Pade 13 scaling and squaring).
+void expm_dense(int n, const double* a, double* out);
+
+// Stream with library-owned lifetime; default priority only.
+struct Stream {
+  hipStream_t s = nullptr;
+  Stream() { NC_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~Stream() { if (s) (void)hipStreamDestroy(s); }
+  Stream(const Stream&) = delete;
+  Stream& operator=(const Stream&) = delete;
+};
+
+}  // namespace nc

@@ -1,0 +1,65 @@
+// FSP generator storage and mat-vec (internal).
+//
+// Layout in HBM for a space of n states and nr reactions (VT = float or double):
+//   diag    double [n]       merged time-invariant diagonal
+//   col     int32  [K][n]    merged time-invariant off-diagonal columns, slot-major
+//   val     VT     [K][n]    (rows with fewer than K predecessors are padded: col = row, val = 0)
+//   tvd     VT     [T][n]    per time-varying reaction: diagonal (statefactor or joint value)
+//   tvc     int32  [T][n]    per time-varying reaction: predecessor column (or row)
+//   tvv     VT     [T][n]    per time-varying reaction: predecessor value (or 0)
+//   sink_off int64 [nr+1]    sink row r holds entries sink_off[r] .. sink_off[r+1)
+//   sink_src int32 [E]       source state of each sink entry
+//   sink_val VT    [E]
+// Row i of A(t) = diag*v_i + sum_k val*v[col] + sum_q scale_q(t)*(tvd*v_i + tvv*v[tvc]);
+// sink row r = scale_r(t) * sum_e sink_val*v[sink_src]. Slot-major arrays make every
+// load of a wavefront one contiguous 256-byte (FP32) or 512-byte (FP64) segment.
+#pragma once
+
+#include "nc_model.h"
+#include "nc_space.h"
+
+struct nc_matrix_s {
+  int precision = NC_FP64;
+  size_t vsize = 8;  // bytes per stored value / vector entry
+  int ns = 0, nr = 0;
+  int64_t n = 0;       // states
+  int64_t n_rows = 0;  // states + sinks
+  int K = 0;
+  nc::DevBuf<double> diag;
+  nc::DevBuf<int32_t> col;
+  nc::DevBuf<unsigned char> val;
+  std::vector<int> tv_r;     // reaction index of each time-varying reaction
+  std::vector<int> tv_kind;  // NC_PROP_SEPARABLE or NC_PROP_JOINT
+  nc::DevBuf<unsigned char> tvd, tvv;
+  nc::DevBuf<int32_t> tvc;
+  std::vector<int64_t> sink_off_h;
+  nc::DevBuf<int64_t> sink_off;
+  nc::DevBuf<int32_t> sink_src;
+  nc::DevBuf<unsigned char> sink_val;
+  nc::DevBuf<int32_t> states;      // kept for joint refreshes
+  nc::DevBuf<int32_t> state_conn;  // kept for joint refreshes
+  nc_model_s* model = nullptr;     // borrowed; must outlive the matrix
+  double t_cache = -INFINITY;
+  nc::Stream stream;
+  // reporting
+  int64_t nnz_ref = 0, n_sink_entries = 0;
+  double norm1 = 0.0;
+  // kernel timing (HIP events around mat-vec launches)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  double timed_ms = 0.0;
+  int64_t timed_count = 0;
+  int64_t launches = 0;
+  ~nc_matrix_s();
+};
+
+namespace nc {
+// out = A(t) v (+ out when accumulate); v is a device vector of n_rows VT values, out holds
+// n_rows VT values, or FP64 values when out_fp64 is set.
+void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
+                   bool out_fp64 = false);
+// Fold recorded kernel timings into timed_ms (synchronises on the recorded events).
+void matvec_harvest_timing(nc_matrix_s* A);
+double matrix_bytes_per_matvec(const nc_matrix_s* A);
+}  // namespace nc

@@ -1,0 +1,691 @@
+// FSP generator build and mat-vec on the device.
+//
+// Replaces FspMatrixSparse (NumCME.jl src/fspmatrix/sparse/fspsparsematrix.jl):
+//  * the constructor (:47-108) and _generate_sparsematrix_entries (:120-152) become one
+//    count kernel and one fill kernel over the state rows plus a sink-list build;
+//    time-invariant reactions with identical stoichiometry share a predecessor and are
+//    merged into one slot, exactly the duplicates SparseArrays.sparse sums (:74);
+//  * _update_sparsematrix! (:154-166) becomes a refresh kernel for joint time-varying
+//    reactions, run when t differs from the cached time (:206-213);
+//  * matvec!/matvecadd! (:196-247) become one kernel: one thread per state row (slot-major
+//    ELL, FP64 accumulation) and one workgroup per sink row (segmented reduction).
+#include <cfloat>
+
+#include "nc_matrix.h"
+
+nc_matrix_s::~nc_matrix_s() {
+  for (auto e : ev) (void)hipEventDestroy(e);
+}
+
+namespace nc {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Groups {
+  int32_t ng;
+  int32_t leader[kMaxReactions];   // first reaction of each group
+  int32_t start[kMaxReactions + 1];
+  int32_t members[kMaxReactions];  // reactions, grouped, ascending within a group
+};
+
+struct TvList {
+  int32_t n;
+  int32_t r[kMaxReactions];
+  int32_t kind[kMaxReactions];
+};
+
+struct Progs {
+  const nc_instr* code;
+  const int64_t* offs;  // 2*nr+1
+  const double* theta;
+};
+
+__device__ inline double xprog(const Progs& p, int r, double t, const int32_t* x) {
+  const int64_t o = p.offs[2 * r];
+  return prog_eval(p.code + o, p.offs[2 * r + 1] - o, t, x, p.theta);
+}
+
+__global__ void k_build_count(int64_t n, int nr, const int32_t* state_conn, Groups g, TvList tv, int32_t* kmax,
+                              unsigned long long* offdiag_total, unsigned long long* tv_pred) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int cnt = 0;
+  for (int k = 0; k < g.ng; ++k) cnt += state_conn[i * nr + g.leader[k]] >= 0;
+  atomicMax(kmax, cnt);
+  if (cnt) atomicAdd(offdiag_total, (unsigned long long)cnt);
+  for (int q = 0; q < tv.n; ++q)
+    if (state_conn[i * nr + tv.r[q]] >= 0) atomicAdd(&tv_pred[q], 1ull);
+}
+
+template <typename VT>
+__global__ void k_build_fill(int64_t n, int ns, int nr, int K, const int32_t* states, const int32_t* state_conn,
+                             const uint32_t* sink_mask, const int32_t* stoich, Groups g, TvList tv, Progs pr,
+                             double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc, VT* tvv,
+                             unsigned long long* diag_max_bits) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t x[kMaxSpecies], y[kMaxSpecies];
+  for (int s = 0; s < ns; ++s) x[s] = states[i * ns + s];
+  const uint32_t smask = sink_mask[i];
+  int slot = 0;
+  double d = 0.0;
+  for (int k = 0; k < g.ng; ++k) {
+    const int lead = g.leader[k];
+    // incoming entry of this row from the group's predecessor x - s_g
+    const int32_t j = state_conn[i * nr + lead];
+    if (j >= 0) {
+      for (int s = 0; s < ns; ++s) y[s] = x[s] - stoich[s + lead * ns];
+      double a = 0.0;
+      for (int m = g.start[k]; m < g.start[k + 1]; ++m) a += xprog(pr, g.members[m], 0.0, y);
+      col[(int64_t)slot * n + i] = j;
+      val[(int64_t)slot * n + i] = (VT)a;
+      ++slot;
+    }
+    // this row's outflow through the group: the diagonal is minus the stored column entries
+    bool nonneg = true;
+    for (int s = 0; s < ns; ++s) nonneg = nonneg && (x[s] + stoich[s + lead * ns] >= 0);
+    if (!nonneg) {
+      // no successor and no sink: the mass leaks, as in the reference (:129-132)
+      for (int m = g.start[k]; m < g.start[k + 1]; ++m) d += xprog(pr, g.members[m], 0.0, x);
+    } else if (smask & (1u << lead)) {
+      for (int m = g.start[k]; m < g.start[k + 1]; ++m) d += (double)(VT)xprog(pr, g.members[m], 0.0, x);
+    } else {
+      double a = 0.0;
+      for (int m = g.start[k]; m < g.start[k + 1]; ++m) a += xprog(pr, g.members[m], 0.0, x);
+      d += (double)(VT)a;
+    }
+  }
+  for (; slot < K; ++slot) {
+    col[(int64_t)slot * n + i] = (int32_t)i;
+    val[(int64_t)slot * n + i] = (VT)0;
+  }
+  diag[i] = -d;
+  atomicMax(diag_max_bits, (unsigned long long)__double_as_longlong(d > 0 ? d : -d));
+  for (int q = 0; q < tv.n; ++q) {
+    const int r = tv.r[q];
+    const int32_t j = state_conn[i * nr + r];
+    if (tv.kind[q] == NC_PROP_SEPARABLE) {
+      tvd[(int64_t)q * n + i] = -(VT)xprog(pr, r, 0.0, x);
+      if (j >= 0) {
+        for (int s = 0; s < ns; ++s) y[s] = x[s] - stoich[s + r * ns];
+        tvv[(int64_t)q * n + i] = (VT)xprog(pr, r, 0.0, y);
+      } else {
+        tvv[(int64_t)q * n + i] = (VT)0;
+      }
+    } else {  // joint: values are written by the refresh kernel
+      tvd[(int64_t)q * n + i] = (VT)0;
+      tvv[(int64_t)q * n + i] = (VT)0;
+    }
+    tvc[(int64_t)q * n + i] = j >= 0 ? j : (int32_t)i;
+  }
+}
+
+__global__ void k_sink_flags(int64_t n, int nr, const uint32_t* sink_mask, uint32_t* flags) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * nr) return;
+  const int r = (int)(t / n);
+  const int64_t j = t % n;
+  flags[t] = (sink_mask[j] >> r) & 1u;
+}
+
+template <typename VT>
+__global__ void k_sink_fill(int64_t n, int ns, int nr, const int32_t* states, const uint32_t* flags,
+                            const uint32_t* pos, const int32_t* kinds, Progs pr, int32_t* sink_src, VT* sink_val) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * nr || !flags[t]) return;
+  const int r = (int)(t / n);
+  const int64_t j = t % n;
+  int32_t x[kMaxSpecies];
+  for (int s = 0; s < ns; ++s) x[s] = states[j * ns + s];
+  const uint32_t e = pos[t];
+  sink_src[e] = (int32_t)j;
+  sink_val[e] = kinds[r] == NC_PROP_JOINT ? (VT)0 : (VT)xprog(pr, r, 0.0, x);
+}
+
+// _update_sparsematrix! (fspsparsematrix.jl:154-166) for every joint reaction at time t.
+template <typename VT>
+__global__ void k_joint_refresh(int64_t n, int ns, int nr, const int32_t* states, const int32_t* state_conn,
+                                const int32_t* stoich, TvList tv, Progs pr, double t, VT* tvd, VT* tvv) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t x[kMaxSpecies], y[kMaxSpecies];
+  for (int s = 0; s < ns; ++s) x[s] = states[i * ns + s];
+  for (int q = 0; q < tv.n; ++q) {
+    if (tv.kind[q] != NC_PROP_JOINT) continue;
+    const int r = tv.r[q];
+    tvd[(int64_t)q * n + i] = -(VT)xprog(pr, r, t, x);
+    const int32_t j = state_conn[i * nr + r];
+    if (j >= 0) {
+      for (int s = 0; s < ns; ++s) y[s] = x[s] - stoich[s + r * ns];
+      tvv[(int64_t)q * n + i] = (VT)xprog(pr, r, t, y);
+    }
+  }
+}
+
+template <typename VT>
+__global__ void k_joint_sink_refresh(int64_t e0, int64_t e1, int r, int ns, const int32_t* states,
+                                     const int32_t* sink_src, Progs pr, double t, VT* sink_val) {
+  int64_t e = e0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= e1) return;
+  int32_t x[kMaxSpecies];
+  const int64_t j = sink_src[e];
+  for (int s = 0; s < ns; ++s) x[s] = states[j * ns + s];
+  sink_val[e] = (VT)xprog(pr, r, t, x);
+}
+
+// ---- mat-vec ---------------------------------------------------------------------------
+template <typename VT>
+struct MatArgs {
+  int64_t n;
+  int32_t K;
+  int32_t ntv;
+  int32_t nr;
+  const double* diag;
+  const int32_t* col;
+  const VT* val;
+  const VT* tvd;
+  const int32_t* tvc;
+  const VT* tvv;
+  const int64_t* sink_off;
+  const int32_t* sink_src;
+  const VT* sink_val;
+  double tv_scale[kMaxReactions];
+  double sink_scale[kMaxReactions];
+};
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <typename VT, typename OT, int KT>
+__global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __restrict__ v, OT* __restrict__ out,
+                                                   int accumulate) {
+  const int64_t row_blocks = (a.n + kBlock - 1) / kBlock;
+  if ((int64_t)blockIdx.x < row_blocks) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const double vi = (double)v[i];
+    double acc = a.diag[i] * vi;
+    if constexpr (KT > 0) {
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const int64_t o = (int64_t)k * a.n + i;
+        acc += (double)a.val[o] * (double)v[a.col[o]];
+      }
+    } else {
+      for (int k = 0; k < a.K; ++k) {
+        const int64_t o = (int64_t)k * a.n + i;
+        acc += (double)a.val[o] * (double)v[a.col[o]];
+      }
+    }
+    for (int q = 0; q < a.ntv; ++q) {
+      const int64_t o = (int64_t)q * a.n + i;
+      acc += a.tv_scale[q] * ((double)a.tvd[o] * vi + (double)a.tvv[o] * (double)v[a.tvc[o]]);
+    }
+    if (accumulate) acc += (double)out[i];
+    out[i] = (OT)acc;
+  } else {
+    const int r = (int)((int64_t)blockIdx.x - row_blocks);
+    const int64_t e0 = a.sink_off[r], e1 = a.sink_off[r + 1];
+    double acc = 0.0;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) acc += (double)a.sink_val[e] * (double)v[a.sink_src[e]];
+    __shared__ double part[kBlock / 64];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) s += part[w];
+      s *= a.sink_scale[r];
+      const int64_t o = a.n + r;
+      out[o] = (OT)(accumulate ? (double)out[o] + s : s);
+    }
+  }
+}
+
+template <typename VT, typename OT>
+void dispatch_k(int K, unsigned grid, hipStream_t s, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
+  switch (K) {
+    case 1: k_matvec<VT, OT, 1><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 2: k_matvec<VT, OT, 2><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 3: k_matvec<VT, OT, 3><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 4: k_matvec<VT, OT, 4><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 5: k_matvec<VT, OT, 5><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 6: k_matvec<VT, OT, 6><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 7: k_matvec<VT, OT, 7><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 8: k_matvec<VT, OT, 8><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    default: k_matvec<VT, OT, 0><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+  }
+}
+
+template <typename VT>
+void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
+                     bool out_fp64) {
+  MatArgs<VT> a{};
+  a.n = A->n;
+  a.K = A->K;
+  a.ntv = (int32_t)A->tv_r.size();
+  a.nr = A->nr;
+  a.diag = A->diag.p;
+  a.col = A->col.p;
+  a.val = reinterpret_cast<const VT*>(A->val.p);
+  a.tvd = reinterpret_cast<const VT*>(A->tvd.p);
+  a.tvc = A->tvc.p;
+  a.tvv = reinterpret_cast<const VT*>(A->tvv.p);
+  a.sink_off = A->sink_off.p;
+  a.sink_src = A->sink_src.p;
+  a.sink_val = reinterpret_cast<const VT*>(A->sink_val.p);
+  const nc_model_s* m = A->model;
+  for (int r = 0; r < A->nr; ++r) a.sink_scale[r] = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
+  for (size_t q = 0; q < A->tv_r.size(); ++q)
+    a.tv_scale[q] = A->tv_kind[q] == NC_PROP_SEPARABLE ? m->tfactor(A->tv_r[q], t) : 1.0;
+  const int64_t row_blocks = (A->n + kBlock - 1) / kBlock;
+  const unsigned grid = (unsigned)(row_blocks + A->nr);
+  const VT* vv = reinterpret_cast<const VT*>(v);
+  const int acc = accumulate ? 1 : 0;
+  if (out_fp64)
+    dispatch_k<VT, double>(A->K, grid, s, a, vv, reinterpret_cast<double*>(out), acc);
+  else
+    dispatch_k<VT, VT>(A->K, grid, s, a, vv, reinterpret_cast<VT*>(out), acc);
+}
+
+template <typename VT>
+void joint_refresh(nc_matrix_s* A, double t) {
+  bool any = false;
+  for (int k : A->tv_kind) any = any || k == NC_PROP_JOINT;
+  if (!any || t == A->t_cache) return;
+  A->t_cache = t;
+  const nc_model_s* m = A->model;
+  TvList tv{};
+  tv.n = (int32_t)A->tv_r.size();
+  for (int q = 0; q < tv.n; ++q) tv.r[q] = A->tv_r[q], tv.kind[q] = A->tv_kind[q];
+  Progs pr{m->d_code.p, m->d_offs.p, m->d_params.p};
+  hipStream_t s = A->stream.s;
+  k_joint_refresh<VT><<<grid_for(A->n, kBlock), kBlock, 0, s>>>(
+      A->n, A->ns, A->nr, A->states.p, A->state_conn.p, m->d_stoich.p, tv, pr, t,
+      reinterpret_cast<VT*>(A->tvd.p), reinterpret_cast<VT*>(A->tvv.p));
+  for (int r = 0; r < A->nr; ++r) {
+    if (m->kinds[r] != NC_PROP_JOINT) continue;
+    const int64_t e0 = A->sink_off_h[r], e1 = A->sink_off_h[r + 1];
+    if (e1 > e0)
+      k_joint_sink_refresh<VT><<<grid_for(e1 - e0, kBlock), kBlock, 0, s>>>(e0, e1, r, A->ns, A->states.p,
+                                                                             A->sink_src.p, pr, t,
+                                                                             reinterpret_cast<VT*>(A->sink_val.p));
+  }
+  NC_CHECK_HIP(hipGetLastError());
+}
+
+template <typename VT>
+void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
+  hipStream_t s = A->stream.s;
+  const int64_t n = sp->n;
+  const int nr = sp->nr, ns = sp->ns;
+  // groups of time-invariant reactions with identical stoichiometry (merged by sparse())
+  Groups g{};
+  std::vector<std::vector<int>> groups;
+  for (int r = 0; r < nr; ++r) {
+    if (m->kinds[r] != NC_PROP_TIME_INVARIANT) continue;
+    bool placed = false;
+    for (auto& gr : groups) {
+      const int l = gr[0];
+      bool same = true;
+      for (int sidx = 0; sidx < ns; ++sidx) same = same && m->stoich[sidx + l * ns] == m->stoich[sidx + r * ns];
+      if (same) {
+        gr.push_back(r);
+        placed = true;
+        break;
+      }
+    }
+    if (!placed) groups.push_back({r});
+  }
+  g.ng = (int32_t)groups.size();
+  int mcount = 0;
+  for (int k = 0; k < g.ng; ++k) {
+    g.leader[k] = groups[k][0];
+    g.start[k] = mcount;
+    for (int r : groups[k]) g.members[mcount++] = r;
+  }
+  g.start[g.ng] = mcount;
+  TvList tv{};
+  for (int r = 0; r < nr; ++r)
+    if (m->kinds[r] != NC_PROP_TIME_INVARIANT) {
+      tv.r[tv.n] = r;
+      tv.kind[tv.n] = m->kinds[r];
+      A->tv_r.push_back(r);
+      A->tv_kind.push_back(m->kinds[r]);
+      ++tv.n;
+    }
+  // pass 1: ELL width and counts
+  DevBuf<int32_t> d_kmax(1);
+  DevBuf<unsigned long long> d_counts(2 + kMaxReactions);
+  NC_CHECK_HIP(hipMemsetAsync(d_kmax.p, 0, sizeof(int32_t), s));
+  NC_CHECK_HIP(hipMemsetAsync(d_counts.p, 0, sizeof(unsigned long long) * (2 + kMaxReactions), s));
+  if (n)
+    k_build_count<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, nr, sp->state_conn.p, g, tv, d_kmax.p, d_counts.p,
+                                                         d_counts.p + 2);
+  NC_CHECK_HIP(hipGetLastError());
+  int32_t K = 0;
+  unsigned long long counts[2 + kMaxReactions];
+  NC_CHECK_HIP(hipMemcpyAsync(&K, d_kmax.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipMemcpyAsync(counts, d_counts.p, sizeof(counts), hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipStreamSynchronize(s));
+  A->K = K;
+  const size_t vs = sizeof(VT);
+  A->diag.alloc(std::max<int64_t>(n, 1));
+  A->col.alloc(std::max<int64_t>((int64_t)K * n, 1));
+  A->val.alloc(std::max<int64_t>((int64_t)K * n, 1) * vs);
+  A->tvd.alloc(std::max<int64_t>((int64_t)tv.n * n, 1) * vs);
+  A->tvv.alloc(std::max<int64_t>((int64_t)tv.n * n, 1) * vs);
+  A->tvc.alloc(std::max<int64_t>((int64_t)tv.n * n, 1));
+  Progs pr{m->d_code.p, m->d_offs.p, m->d_params.p};
+  DevBuf<unsigned long long> d_dmax(1);
+  NC_CHECK_HIP(hipMemsetAsync(d_dmax.p, 0, sizeof(unsigned long long), s));
+  if (n)
+    k_build_fill<VT><<<grid_for(n, kBlock), kBlock, 0, s>>>(
+        n, ns, nr, K, sp->states.p, sp->state_conn.p, sp->sink_mask.p, sp->d_stoich.p, g, tv, pr, A->diag.p,
+        A->col.p, reinterpret_cast<VT*>(A->val.p), reinterpret_cast<VT*>(A->tvd.p), A->tvc.p,
+        reinterpret_cast<VT*>(A->tvv.p), d_dmax.p);
+  NC_CHECK_HIP(hipGetLastError());
+  // sink entries, grouped by sink row (reaction), ascending source state within a row
+  DevBuf<uint32_t> flags(std::max<int64_t>(n * nr, 1)), pos(std::max<int64_t>(n * nr, 1));
+  DevBuf<unsigned char> scan_tmp;
+  uint32_t total = 0;
+  if (n) {
+    k_sink_flags<<<grid_for(n * nr, kBlock), kBlock, 0, s>>>(n, nr, sp->sink_mask.p, flags.p);
+    exclusive_scan_u32(flags.p, pos.p, n * nr, scan_tmp, s, &total);
+  }
+  A->sink_off_h.assign(nr + 1, 0);
+  if (n) {
+    std::vector<uint32_t> starts(nr);
+    for (int r = 0; r < nr; ++r)
+      NC_CHECK_HIP(hipMemcpyAsync(&starts[r], pos.p + (int64_t)r * n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    for (int r = 0; r < nr; ++r) A->sink_off_h[r] = starts[r];
+  }
+  A->sink_off_h[nr] = total;
+  A->n_sink_entries = total;
+  A->sink_off.alloc(nr + 1);
+  NC_CHECK_HIP(hipMemcpyAsync(A->sink_off.p, A->sink_off_h.data(), sizeof(int64_t) * (nr + 1), hipMemcpyHostToDevice, s));
+  A->sink_src.alloc(std::max<uint32_t>(total, 1));
+  A->sink_val.alloc(std::max<uint32_t>(total, 1) * vs);
+  DevBuf<int32_t> d_kinds(nr);
+  NC_CHECK_HIP(hipMemcpyAsync(d_kinds.p, m->kinds.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, s));
+  if (n && total)
+    k_sink_fill<VT><<<grid_for(n * nr, kBlock), kBlock, 0, s>>>(n, ns, nr, sp->states.p, flags.p, pos.p, d_kinds.p,
+                                                                pr, A->sink_src.p,
+                                                                reinterpret_cast<VT*>(A->sink_val.p));
+  NC_CHECK_HIP(hipGetLastError());
+  bool any_joint = false;
+  for (int k : A->tv_kind) any_joint = any_joint || k == NC_PROP_JOINT;
+  if (any_joint && n) {
+    A->states.alloc((size_t)n * ns);
+    A->state_conn.alloc((size_t)n * nr);
+    NC_CHECK_HIP(hipMemcpyAsync(A->states.p, sp->states.p, sizeof(int32_t) * n * ns, hipMemcpyDeviceToDevice, s));
+    NC_CHECK_HIP(hipMemcpyAsync(A->state_conn.p, sp->state_conn.p, sizeof(int32_t) * n * nr, hipMemcpyDeviceToDevice, s));
+  }
+  unsigned long long dmax_bits = 0;
+  NC_CHECK_HIP(hipMemcpyAsync(&dmax_bits, d_dmax.p, sizeof(dmax_bits), hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipStreamSynchronize(s));
+  double dmax;
+  std::memcpy(&dmax, &dmax_bits, sizeof(dmax));
+  A->norm1 = 2.0 * dmax;
+  // nonzeros of the reference's matrices: merged TI matrix + one matrix per TV reaction
+  std::vector<uint32_t> sinks_per_r(nr);
+  for (int r = 0; r < nr; ++r) sinks_per_r[r] = (uint32_t)(A->sink_off_h[r + 1] - A->sink_off_h[r]);
+  int64_t nnz = 0;
+  if (g.ng) {
+    nnz += n + (int64_t)counts[0];
+    for (int r = 0; r < nr; ++r)
+      if (m->kinds[r] == NC_PROP_TIME_INVARIANT) nnz += sinks_per_r[r];
+  }
+  for (int q = 0; q < tv.n; ++q) nnz += n + (int64_t)counts[2 + q] + sinks_per_r[tv.r[q]];
+  A->nnz_ref = nnz;
+}
+
+}  // namespace
+
+double matrix_bytes_per_matvec(const nc_matrix_s* A) {
+  const double vs = (double)A->vsize;
+  const double n = (double)A->n;
+  double b = n * 8.0                                  // diag
+             + n * A->K * (4.0 + vs)                  // ELL slots
+             + n * A->tv_r.size() * (4.0 + 2.0 * vs)  // time-varying slots
+             + (double)A->n_sink_entries * (4.0 + vs)  // sink lists
+             + (double)(A->nr + 1) * 8.0;
+  b += 2.0 * (double)A->n_rows * vs;  // read v, write out
+  return b;
+}
+
+void matvec_harvest_timing(nc_matrix_s* A) {
+  if (!A->ev_used) return;
+  NC_CHECK_HIP(hipEventSynchronize(A->ev[2 * A->ev_used - 1]));
+  for (size_t k = 0; k < A->ev_used; ++k) {
+    float ms = 0.f;
+    NC_CHECK_HIP(hipEventElapsedTime(&ms, A->ev[2 * k], A->ev[2 * k + 1]));
+    A->timed_ms += ms;
+    A->timed_count++;
+  }
+  A->ev_used = 0;
+}
+
+void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
+                   bool out_fp64) {
+  if (A->precision == NC_FP32)
+    joint_refresh<float>(A, t);
+  else
+    joint_refresh<double>(A, t);
+  if (s != A->stream.s) {
+    // joint refresh runs on the matrix stream; order it before the launch on `s`
+    hipEvent_t e;
+    NC_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    NC_CHECK_HIP(hipEventRecord(e, A->stream.s));
+    NC_CHECK_HIP(hipStreamWaitEvent(s, e, 0));
+    NC_CHECK_HIP(hipEventDestroy(e));
+  }
+  const bool timed = A->timing;
+  if (timed) {
+    if (A->ev_used * 2 >= A->ev.size()) {
+      if (A->ev.size() < 8192) {
+        size_t old = A->ev.size();
+        A->ev.resize(old ? old * 2 : 256);
+        for (size_t k = old; k < A->ev.size(); ++k) NC_CHECK_HIP(hipEventCreate(&A->ev[k]));
+      } else {
+        matvec_harvest_timing(A);
+      }
+    }
+    NC_CHECK_HIP(hipEventRecord(A->ev[2 * A->ev_used], s));
+  }
+  if (A->precision == NC_FP32)
+    launch_matvec_t<float>(A, t, v, out, accumulate, s, out_fp64);
+  else
+    launch_matvec_t<double>(A, t, v, out, accumulate, s, out_fp64);
+  NC_CHECK_HIP(hipGetLastError());
+  if (timed) {
+    NC_CHECK_HIP(hipEventRecord(A->ev[2 * A->ev_used + 1], s));
+    A->ev_used++;
+  }
+  A->launches++;
+}
+
+}  // namespace nc
+
+using namespace nc;
+
+namespace {
+
+template <typename VT>
+__global__ void k_to_vt(const double* in, VT* out, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (VT)in[i];
+}
+template <typename VT>
+__global__ void k_from_vt(const VT* in, double* out, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (double)in[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int nc_matrix_create(nc_space sp, nc_model m, int32_t precision, nc_matrix* out) {
+  return guard([&] {
+    NC_REQUIRE(sp && m && out, NC_ERR_ARG, "null argument");
+    NC_REQUIRE(precision == NC_FP64 || precision == NC_FP32, NC_ERR_ARG, "precision must be NC_FP64 or NC_FP32");
+    NC_REQUIRE(sp->ns == m->ns && sp->nr == m->nr, NC_ERR_ARG, "model and state space sizes differ");
+    NC_REQUIRE(sp->stoich == m->stoich, NC_ERR_ARG, "model and state space stoichiometry differ");
+    auto A = new nc_matrix_s();
+    try {
+      A->precision = precision;
+      A->vsize = precision == NC_FP32 ? 4 : 8;
+      A->ns = sp->ns;
+      A->nr = sp->nr;
+      A->n = sp->n;
+      A->n_rows = sp->n + sp->nr;
+      A->model = m;
+      NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
+      if (precision == NC_FP32)
+        build_t<float>(A, sp, m);
+      else
+        build_t<double>(A, sp, m);
+      NC_CHECK_HIP(hipStreamSynchronize(A->stream.s));
+    } catch (...) {
+      delete A;
+      throw;
+    }
+    *out = A;
+  });
+}
+
+int nc_matrix_destroy(nc_matrix A) {
+  return guard([&] { delete A; });
+}
+
+int nc_matrix_info(nc_matrix A, nc_matrix_info_t* info) {
+  return guard([&] {
+    NC_REQUIRE(A && info, NC_ERR_ARG, "null argument");
+    info->n_rows = A->n_rows;
+    info->n_states = A->n;
+    info->n_sinks = A->nr;
+    info->ell_width = A->K;
+    info->n_tv = (int32_t)A->tv_r.size();
+    info->precision = A->precision;
+    info->nnz_reference = A->nnz_ref;
+    info->n_sink_entries = A->n_sink_entries;
+    info->bytes_per_matvec = matrix_bytes_per_matvec(A);
+    const double sv = (double)A->vsize;
+    info->csr_bytes_per_matvec =
+        (double)A->nnz_ref * (sv + 4.0) + (double)(A->n_rows + 1) * 4.0 + 2.0 * (double)A->n_rows * sv;
+    info->norm1 = A->norm1;
+  });
+}
+
+int nc_matrix_matvec(nc_matrix A, double t, const double* v, double* out, int32_t accumulate) {
+  return guard([&] {
+    NC_REQUIRE(A && v && out, NC_ERR_ARG, "null argument");
+    hipStream_t s = A->stream.s;
+    const int64_t N = A->n_rows;
+    DevBuf<double> dv(N), dout(N);
+    DevBuf<unsigned char> vv(N * A->vsize), vo(N * A->vsize);
+    NC_CHECK_HIP(hipMemcpyAsync(dv.p, v, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    if (accumulate) NC_CHECK_HIP(hipMemcpyAsync(dout.p, out, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    if (A->precision == NC_FP32) {
+      k_to_vt<float><<<grid_for(N, 256), 256, 0, s>>>(dv.p, reinterpret_cast<float*>(vv.p), N);
+      if (accumulate) k_to_vt<float><<<grid_for(N, 256), 256, 0, s>>>(dout.p, reinterpret_cast<float*>(vo.p), N);
+      matvec_launch(A, t, vv.p, vo.p, accumulate != 0, s);
+      k_from_vt<float><<<grid_for(N, 256), 256, 0, s>>>(reinterpret_cast<float*>(vo.p), dout.p, N);
+    } else {
+      matvec_launch(A, t, dv.p, dout.p, accumulate != 0, s);
+    }
+    NC_CHECK_HIP(hipGetLastError());
+    NC_CHECK_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    A->ev_used = 0;
+  });
+}
+
+int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t* cols, double* vals) {
+  return guard([&] {
+    NC_REQUIRE(A && nnz, NC_ERR_ARG, "null argument");
+    const int64_t n = A->n;
+    const int ntv = (int)A->tv_r.size();
+    const int64_t total = n * (1 + A->K + 2 * ntv) + A->n_sink_entries;
+    *nnz = total;
+    if (!rows) return;
+    NC_REQUIRE(cols && vals, NC_ERR_ARG, "null output arrays");
+    if (A->precision == NC_FP32)
+      joint_refresh<float>(A, t);
+    else
+      joint_refresh<double>(A, t);
+    NC_CHECK_HIP(hipStreamSynchronize(A->stream.s));
+    auto fetch = [&](const unsigned char* src, int64_t count) {
+      std::vector<double> out(count);
+      if (!count) return out;
+      if (A->vsize == 8) {
+        NC_CHECK_HIP(hipMemcpy(out.data(), src, 8 * count, hipMemcpyDeviceToHost));
+      } else {
+        std::vector<float> f(count);
+        NC_CHECK_HIP(hipMemcpy(f.data(), src, 4 * count, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < count; ++k) out[k] = f[k];
+      }
+      return out;
+    };
+    std::vector<double> diag(n);
+    if (n) NC_CHECK_HIP(hipMemcpy(diag.data(), A->diag.p, 8 * n, hipMemcpyDeviceToHost));
+    std::vector<int32_t> col((size_t)A->K * n), tvc((size_t)ntv * n), src(A->n_sink_entries);
+    if (!col.empty()) NC_CHECK_HIP(hipMemcpy(col.data(), A->col.p, 4 * col.size(), hipMemcpyDeviceToHost));
+    if (!tvc.empty()) NC_CHECK_HIP(hipMemcpy(tvc.data(), A->tvc.p, 4 * tvc.size(), hipMemcpyDeviceToHost));
+    if (!src.empty()) NC_CHECK_HIP(hipMemcpy(src.data(), A->sink_src.p, 4 * src.size(), hipMemcpyDeviceToHost));
+    auto val = fetch(A->val.p, (int64_t)A->K * n);
+    auto tvd = fetch(A->tvd.p, (int64_t)ntv * n);
+    auto tvv = fetch(A->tvv.p, (int64_t)ntv * n);
+    auto sv = fetch(A->sink_val.p, A->n_sink_entries);
+    const nc_model_s* m = A->model;
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = i, vals[k] = diag[i], ++k;
+    for (int s = 0; s < A->K; ++s)
+      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = col[s * n + i], vals[k] = val[s * n + i], ++k;
+    for (int q = 0; q < ntv; ++q) {
+      const double sc = A->tv_kind[q] == NC_PROP_SEPARABLE ? m->tfactor(A->tv_r[q], t) : 1.0;
+      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = i, vals[k] = sc * tvd[q * n + i], ++k;
+      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = tvc[q * n + i], vals[k] = sc * tvv[q * n + i], ++k;
+    }
+    for (int r = 0; r < A->nr; ++r) {
+      const double sc = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
+      for (int64_t e = A->sink_off_h[r]; e < A->sink_off_h[r + 1]; ++e)
+        rows[k] = n + r, cols[k] = src[e], vals[k] = sc * sv[e], ++k;
+    }
+  });
+}
+
+int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms) {
+  return guard([&] {
+    NC_REQUIRE(A && mean_ms && reps > 0, NC_ERR_ARG, "bad arguments");
+    hipStream_t s = A->stream.s;
+    const int64_t N = A->n_rows;
+    DevBuf<unsigned char> v(N * A->vsize), o(N * A->vsize);
+    NC_CHECK_HIP(hipMemsetAsync(v.p, 0, N * A->vsize, s));
+    NC_CHECK_HIP(hipMemsetAsync(o.p, 0, N * A->vsize, s));
+    matvec_launch(A, t, v.p, o.p, false, s);  // warm-up (and joint refresh)
+    hipEvent_t e0, e1;
+    NC_CHECK_HIP(hipEventCreate(&e0));
+    NC_CHECK_HIP(hipEventCreate(&e1));
+    const bool was = A->timing;
+    A->timing = false;
+    NC_CHECK_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; ++k) matvec_launch(A, t, (k & 1) ? o.p : v.p, (k & 1) ? v.p : o.p, false, s);
+    NC_CHECK_HIP(hipEventRecord(e1, s));
+    NC_CHECK_HIP(hipEventSynchronize(e1));
+    A->timing = was;
+    float ms = 0.f;
+    NC_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *mean_ms = ms / reps;
+  });
+}
+
+}  // extern "C"

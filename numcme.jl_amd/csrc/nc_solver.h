@@ -1,0 +1,46 @@
+// Time integrators (internal): device-resident Krylov expv and Dormand-Prince 5(4).
+#pragma once
+
+#include "nc_matrix.h"
+
+struct nc_solver_s {
+  nc_matrix_s* A = nullptr;  // borrowed; must outlive the solver
+  nc_solve_opts opts{};
+  int method = NC_METHOD_KRYLOV;
+  int64_t N = 0;
+  size_t vs = 8;
+  double t = 0.0;
+  bool have_state = false;
+  hipStream_t s = nullptr;  // the matrix's stream
+  nc::DevBuf<double> u;     // FP64 master state [p; sinks]
+  nc::DevBuf<double> u0;    // state uploaded by the last set_state (for restart)
+  double t0 = 0.0, beta0 = 0.0;
+  nc::DevBuf<double> snaps; // n_out x N
+  int32_t n_snaps = 0;
+  // reduction scratch
+  nc::DevBuf<double> partials;
+  nc::DevBuf<unsigned int> counter;
+  unsigned red_blocks = 0;
+  // Krylov
+  int m = 30;
+  nc::DevBuf<unsigned char> V;  // (m+1) x N, VT, unnormalised
+  nc::DevBuf<double> q, p;      // FP64 work vectors
+  nc::DevBuf<double> kbuf;      // H (m+2)^2 | h (m+2) | h2 (m+2) | sc (m+2) | misc (8)
+  nc::DevBuf<double> coef;      // combination coefficients (m+2)
+  std::vector<double> hbuf;     // host copy of kbuf
+  double beta = 0.0, t_new = 0.0, anorm = 0.0;
+  bool started = false;
+  // RK
+  nc::DevBuf<unsigned char> rkv;   // VT copy of the stage input
+  nc::DevBuf<double> k;            // 7 x N stage derivatives (FP64)
+  nc::DevBuf<double> ytmp;         // N
+  double h = 0.0;
+  bool fsal_valid = false;
+  nc_solve_stats st{};
+};
+
+namespace nc {
+void krylov_advance(nc_solver_s* S, double t_target);
+void rk_advance(nc_solver_s* S, double t_target);
+void solver_reduce_setup(nc_solver_s* S, int nvals);
+}  // namespace nc

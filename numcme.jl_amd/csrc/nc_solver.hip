@@ -1,0 +1,809 @@
+// Device-resident time integration of the FSP ODE du/dt = A(t) u.
+//
+// Replaces the integrator loop of NumCME.jl's solve (src/transientcme/sparse/fspsolve.jl:
+// 10-41, a DifferentialEquations.jl/CVODE call around matvec!) with:
+//
+//  * Krylov expv for time-invariant generators: Arnoldi with classical Gram-Schmidt and
+//    one full reorthogonalisation (CGS2), FP64 dot products and coefficients, basis
+//    vectors stored in the matrix precision; the (m+2)x(m+2) Hessenberg exponential,
+//    the local error estimate and the step-size control run on the host, following
+//    R. B. Sidje, "Expokit: a software package for computing matrix exponentials",
+//    ACM TOMS 24(1), 1998 (algorithm of its EXPV routine). One host synchronisation
+//    per Krylov step.
+//  * Dormand-Prince 5(4) (J. R. Dormand & P. J. Prince, J. Comput. Appl. Math. 6, 1980)
+//    with FSAL, the Hairer-Norsett-Wanner RMS error norm computed on the device in FP64,
+//    and output times hit exactly by step clamping.
+//
+// Vectors: the master state u is FP64; mat-vec inputs are stored in the matrix precision.
+#include <algorithm>
+#include <cmath>
+
+#include "nc_solver.h"
+
+namespace nc {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxRedBlocks = 1024;
+
+enum Finish : int { FIN_NONE = 0, FIN_H1, FIN_H2, FIN_COL, FIN_AVN, FIN_BETA, FIN_SUMS };
+
+// kbuf layout (doubles)
+struct KLayout {
+  int m;
+  __host__ __device__ int H(int i, int j) const { return i * (m + 2) + j; }
+  __host__ __device__ int h(int i) const { return (m + 2) * (m + 2) + i; }
+  __host__ __device__ int h2(int i) const { return (m + 2) * (m + 2) + (m + 2) + i; }
+  __host__ __device__ int sc(int i) const { return (m + 2) * (m + 2) + 2 * (m + 2) + i; }
+  __host__ __device__ int misc(int i) const { return (m + 2) * (m + 2) + 3 * (m + 2) + i; }
+  __host__ __device__ int size() const { return (m + 2) * (m + 2) + 3 * (m + 2) + 8; }
+};
+enum { MISC_AVN = 0, MISC_BETA = 1 };
+
+template <typename BT>
+struct VecArgs {
+  int64_t N;
+  int nv;
+  const BT* V;            // nv x N basis (raw)
+  const double* sc;       // per-vector scale (device, may be null = 1)
+  const double* coef;     // per-vector coefficient (device, may be null)
+  double csign;           // multiplies coef
+  const double* xin;      // FP64 input (may be null = 0)
+  const double* xscale;   // device scalar multiplying xin (may be null = 1)
+  double* xout;           // FP64 outputs (nullable)
+  double* xout2;
+  void* vout;             // VT output (nullable)
+  int vout_fp32;          // 1: vout is float, 0: double
+  int dots, norm;
+  double* partials;
+  unsigned int* counter;
+  int finish;
+  double* kb;
+  KLayout L;
+  int j;
+  double btol;
+};
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// x = xscale*xin + csign * sum_i coef[i]*sc[i]*V_i ; optional stores; optional dots
+// sum_n (sc[i] V_i[n]) x[n] and norm sum_n x[n]^2, reduced across the grid by the last
+// block to finish (deterministic fixed-order reduction).
+template <typename BT, int MAXV>
+__global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
+  const int nv = a.nv;
+  double cs[MAXV > 0 ? MAXV : 1], sci[MAXV > 0 ? MAXV : 1];
+#pragma unroll
+  for (int i = 0; i < MAXV; ++i) {
+    if (i < nv) {
+      sci[i] = a.sc ? a.sc[i] : 1.0;
+      cs[i] = a.coef ? a.csign * a.coef[i] * sci[i] : 0.0;
+    } else {
+      sci[i] = 0.0, cs[i] = 0.0;
+    }
+  }
+  const double xs = a.xscale ? *a.xscale : 1.0;
+  double dacc[MAXV > 0 ? MAXV : 1];
+#pragma unroll
+  for (int i = 0; i < MAXV; ++i) dacc[i] = 0.0;
+  double nacc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < a.N; n += stride) {
+    double vloc[MAXV > 0 ? MAXV : 1];
+    double x = a.xin ? xs * a.xin[n] : 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) {
+      if (i < nv) {
+        vloc[i] = (double)a.V[(int64_t)i * a.N + n];
+        if (a.coef) x += cs[i] * vloc[i];
+      }
+    }
+    if (a.xout) a.xout[n] = x;
+    if (a.xout2) a.xout2[n] = x;
+    if (a.vout) {
+      if (a.vout_fp32)
+        reinterpret_cast<float*>(a.vout)[n] = (float)x;
+      else
+        reinterpret_cast<double*>(a.vout)[n] = x;
+    }
+    if (a.dots) {
+#pragma unroll
+      for (int i = 0; i < MAXV; ++i)
+        if (i < nv) dacc[i] += sci[i] * vloc[i] * x;
+    }
+    nacc += x * x;
+  }
+  if (a.finish == FIN_NONE) return;
+  // block reduction of nv dots + 1 norm
+  __shared__ double red[(MAXV > 0 ? MAXV : 1) + 1][kBlock / 64];
+  __shared__ bool last;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nvals = (a.dots ? nv : 0) + 1;
+  if (a.dots) {
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) {
+      if (i < nv) {
+        double v = wave_sum(dacc[i]);
+        if (lane == 0) red[i][wave] = v;
+      }
+    }
+  }
+  {
+    double v = wave_sum(nacc);
+    if (lane == 0) red[nvals - 1][wave] = v;
+  }
+  __syncthreads();
+  const int pstride = MAXV + 1;
+  if (threadIdx.x < nvals) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
+    a.partials[(int64_t)blockIdx.x * pstride + threadIdx.x] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  __shared__ double tot[(MAXV > 0 ? MAXV : 1) + 1];
+  if (threadIdx.x < nvals) {
+    double s = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b)
+      s += __hip_atomic_load(&a.partials[(int64_t)b * pstride + threadIdx.x], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *a.counter = 0u;
+    const KLayout& L = a.L;
+    double* kb = a.kb;
+    const double nrm2 = tot[nvals - 1];
+    switch (a.finish) {
+      case FIN_H1:
+        for (int i = 0; i < nv; ++i) kb[L.h(i)] = tot[i];
+        break;
+      case FIN_H2:
+        for (int i = 0; i < nv; ++i) kb[L.h2(i)] = tot[i];
+        break;
+      case FIN_COL: {
+        const double s = sqrt(nrm2);
+        for (int i = 0; i < nv; ++i) kb[L.H(i, a.j)] = kb[L.h(i)] + kb[L.h2(i)];
+        kb[L.H(a.j + 1, a.j)] = s;
+        kb[L.sc(a.j + 1)] = s > a.btol ? 1.0 / s : 0.0;
+      } break;
+      case FIN_AVN:
+        kb[L.misc(MISC_AVN)] = sqrt(nrm2);
+        break;
+      case FIN_BETA: {
+        const double b = sqrt(nrm2);
+        kb[L.misc(MISC_BETA)] = b;
+        kb[L.sc(0)] = b > 0.0 ? 1.0 / b : 0.0;
+      } break;
+      case FIN_SUMS:
+        for (int i = 0; i < nvals; ++i) kb[i] = tot[i];
+        break;
+      default: break;
+    }
+  }
+}
+
+template <typename BT>
+void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
+  const int nv = a.nv;
+  if (nv <= 0)
+    k_vec<BT, 0><<<grid, kBlock, 0, s>>>(a);
+  else if (nv <= 4)
+    k_vec<BT, 4><<<grid, kBlock, 0, s>>>(a);
+  else if (nv <= 8)
+    k_vec<BT, 8><<<grid, kBlock, 0, s>>>(a);
+  else if (nv <= 16)
+    k_vec<BT, 16><<<grid, kBlock, 0, s>>>(a);
+  else if (nv <= 32)
+    k_vec<BT, 32><<<grid, kBlock, 0, s>>>(a);
+  else
+    throw Error(NC_ERR_UNSUPPORTED, "Krylov dimension above 31 is not supported");
+  NC_CHECK_HIP(hipGetLastError());
+}
+
+// ---- Dormand-Prince 5(4) ------------------------------------------------------------
+__constant__ double c_dp_a[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {1.0 / 5, 0, 0, 0, 0, 0},
+    {3.0 / 40, 9.0 / 40, 0, 0, 0, 0},
+    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0},
+    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0},
+    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0},
+    {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84}};
+const double h_dp_c[7] = {0, 1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
+// b - b* (5th minus embedded 4th order weights)
+__constant__ double c_dp_e[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525,
+                                 -1.0 / 40};
+
+// Stage input y_i = y + h * sum_{j<i} a_ij k_j ; writes FP64 (ytmp) and the mat-vec copy.
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, int stage, double h, const double* y, const double* k,
+                                                     double* ytmp, VT* vin) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) {
+    double acc = 0.0;
+    for (int j = 0; j < stage; ++j) acc += c_dp_a[stage][j] * k[(int64_t)j * N + n];
+    const double v = y[n] + h * acc;
+    ytmp[n] = v;
+    vin[n] = (VT)v;
+  }
+}
+
+// Error norm sum_n (h*sum_j e_j k_j / (atol + rtol*max(|y|,|ynew|)))^2 with ynew = ytmp.
+__global__ __launch_bounds__(kBlock) void k_rk_err(int64_t N, double h, double atol, double rtol, const double* y,
+                                                   const double* ynew, const double* k, double* partials,
+                                                   unsigned int* counter, double* out) {
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) {
+    double e = 0.0;
+    for (int j = 0; j < 7; ++j) e += c_dp_e[j] * k[(int64_t)j * N + n];
+    e *= h;
+    const double sc = atol + rtol * fmax(fabs(y[n]), fabs(ynew[n]));
+    const double r = e / sc;
+    acc += r * r;
+  }
+  __shared__ double red[kBlock / 64];
+  __shared__ bool last;
+  double v = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w];
+    partials[blockIdx.x] = s;
+    __threadfence();
+    last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    double s = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b)
+      s += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *out = s;
+    *counter = 0u;
+  }
+}
+
+__global__ void k_copy_d(const double* in, double* out, int64_t N) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) out[n] = in[n];
+}
+
+template <typename VT>
+__global__ void k_to_vt(const double* in, VT* out, int64_t N) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) out[n] = (VT)in[n];
+}
+
+double round_step(double x) {
+  // Expokit rounds step sizes to 2 significant digits
+  if (!(x > 0.0)) return x;
+  const double s = std::pow(10.0, std::floor(std::log10(x)) - 1.0);
+  return std::ceil(x / s) * s;
+}
+
+}  // namespace
+
+void solver_reduce_setup(nc_solver_s* S, int nvals) {
+  S->red_blocks = std::min<unsigned>(kMaxRedBlocks, grid_for(S->N, kBlock));
+  S->partials.reserve((size_t)S->red_blocks * (size_t)(nvals + 1));
+  if (!S->counter.p) {
+    S->counter.alloc(1);
+    NC_CHECK_HIP(hipMemsetAsync(S->counter.p, 0, sizeof(unsigned int), S->s));
+  }
+}
+
+// ---- Krylov expv ---------------------------------------------------------------------
+void krylov_advance(nc_solver_s* S, double t_target) {
+  nc_matrix_s* A = S->A;
+  const int m = S->m;
+  const int64_t N = S->N;
+  const KLayout L{m};
+  hipStream_t s = S->s;
+  const bool fp32 = A->precision == NC_FP32;
+  const double tol = S->opts.krylov_tol;
+  const double btol = 1e-7, gamma = 0.9, delta = 1.2;
+  const int mxrej = 10;
+  auto Vp = [&](int i) -> unsigned char* { return S->V.p + (size_t)i * N * S->vs; };
+  if (!S->started) {
+    // first step size from the norm estimate (Expokit)
+    S->anorm = std::max(A->norm1, 1e-300);
+    const double fact = std::pow((m + 1) / std::exp(1.0), m + 1) * std::sqrt(2.0 * M_PI * (m + 1));
+    const double b = std::max(S->beta, 1e-300);
+    S->t_new = round_step((1.0 / S->anorm) * std::pow((fact * tol) / (4.0 * b * S->anorm), 1.0 / m));
+    S->started = true;
+  }
+  const double t_eps = 1e-13 * std::max(1.0, std::fabs(t_target));
+  while (S->t < t_target - t_eps) {
+    if (S->opts.max_steps && S->st.steps >= S->opts.max_steps)
+      throw Error(NC_ERR_NUMERIC, "Krylov integrator exceeded max_steps");
+    if (S->beta == 0.0) {  // zero vector stays zero
+      S->t = t_target;
+      break;
+    }
+    double t_step = std::min(t_target - S->t, S->t_new);
+    // Arnoldi: V_0 (raw) and sc[0] = 1/beta are already on the device
+    for (int j = 0; j < m; ++j) {
+      matvec_launch(A, 0.0, Vp(j), S->q.p, false, s, true);
+      S->st.matvecs++;
+      // pass 1: h = V_{0..j}^T x0, x0 = sc_j q
+      VecArgs<float> af{};
+      VecArgs<double> ad{};
+      auto common = [&](auto& a) {
+        a.N = N;
+        a.nv = j + 1;
+        a.sc = S->kbuf.p + L.sc(0);
+        a.partials = S->partials.p;
+        a.counter = S->counter.p;
+        a.kb = S->kbuf.p;
+        a.L = L;
+        a.j = j;
+        a.btol = btol;
+        a.vout_fp32 = fp32 ? 1 : 0;
+      };
+      if (fp32) {
+        common(af);
+        af.V = reinterpret_cast<const float*>(Vp(0));
+        af.xin = S->q.p;
+        af.xscale = S->kbuf.p + L.sc(j);
+        af.dots = 1;
+        af.finish = FIN_H1;
+        launch_vec(af, S->red_blocks, s);
+        // pass 2: x1 = x0 - V h ; h2 = V^T x1
+        af.coef = S->kbuf.p + L.h(0);
+        af.csign = -1.0;
+        af.xout = S->p.p;
+        af.finish = FIN_H2;
+        launch_vec(af, S->red_blocks, s);
+        // x2 = x1 - V h2 ; V_{j+1} = x2 ; s = |x2|
+        af.xin = S->p.p;
+        af.xscale = nullptr;
+        af.coef = S->kbuf.p + L.h2(0);
+        af.xout = nullptr;
+        af.vout = Vp(j + 1);
+        af.dots = 0;
+        af.finish = FIN_COL;
+        launch_vec(af, S->red_blocks, s);
+      } else {
+        common(ad);
+        ad.V = reinterpret_cast<const double*>(Vp(0));
+        ad.xin = S->q.p;
+        ad.xscale = S->kbuf.p + L.sc(j);
+        ad.dots = 1;
+        ad.finish = FIN_H1;
+        launch_vec(ad, S->red_blocks, s);
+        ad.coef = S->kbuf.p + L.h(0);
+        ad.csign = -1.0;
+        ad.xout = S->p.p;
+        ad.finish = FIN_H2;
+        launch_vec(ad, S->red_blocks, s);
+        ad.xin = S->p.p;
+        ad.xscale = nullptr;
+        ad.coef = S->kbuf.p + L.h2(0);
+        ad.xout = nullptr;
+        ad.vout = Vp(j + 1);
+        ad.dots = 0;
+        ad.finish = FIN_COL;
+        launch_vec(ad, S->red_blocks, s);
+      }
+    }
+    // avnorm = |A v_{m+1}|
+    matvec_launch(A, 0.0, Vp(m), S->q.p, false, s, true);
+    S->st.matvecs++;
+    {
+      VecArgs<double> a{};
+      a.N = N;
+      a.nv = 0;
+      a.xin = S->q.p;
+      a.xscale = S->kbuf.p + L.sc(m);
+      a.partials = S->partials.p;
+      a.counter = S->counter.p;
+      a.kb = S->kbuf.p;
+      a.L = L;
+      a.finish = FIN_AVN;
+      launch_vec(a, S->red_blocks, s);
+    }
+    NC_CHECK_HIP(hipMemcpyAsync(S->hbuf.data(), S->kbuf.p, sizeof(double) * L.size(), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    S->st.host_syncs++;
+    matvec_harvest_timing(A);
+    const double* hb = S->hbuf.data();
+    // happy breakdown detection
+    int mb = m, k1 = 2;
+    for (int j = 0; j < m; ++j) {
+      if (hb[L.H(j + 1, j)] < btol) {
+        mb = j + 1;
+        k1 = 0;
+        t_step = t_target - S->t;
+        break;
+      }
+    }
+    const double avnorm = hb[L.misc(MISC_AVN)];
+    const int mxf = mb + k1;  // size of the exponentiated matrix
+    std::vector<double> Hm((size_t)mxf * mxf, 0.0), F((size_t)mxf * mxf);
+    double err_loc = btol, xm = 1.0 / m;
+    for (int ireject = 0;; ++ireject) {
+      for (int i = 0; i < mxf; ++i)
+        for (int jj = 0; jj < mxf; ++jj) {
+          double v = 0.0;
+          if (i < mb + 1 && jj < mb && i <= jj + 1 && i < m + 1) v = hb[L.H(i, jj)];
+          if (k1 != 0 && i == m + 1 && jj == m) v = 1.0;
+          Hm[(size_t)i * mxf + jj] = t_step * v;
+        }
+      // rows beyond mb in a breakdown are zero
+      expm_dense(mxf, Hm.data(), F.data());
+      if (k1 == 0) {
+        err_loc = btol;
+        break;
+      }
+      const double phi1 = std::fabs(S->beta * F[(size_t)m * mxf + 0]);
+      const double phi2 = std::fabs(S->beta * F[(size_t)(m + 1) * mxf + 0] * avnorm);
+      if (phi1 > 10.0 * phi2) {
+        err_loc = phi2;
+        xm = 1.0 / m;
+      } else if (phi1 > phi2) {
+        err_loc = (phi1 * phi2) / (phi1 - phi2);
+        xm = 1.0 / m;
+      } else {
+        err_loc = phi1;
+        xm = 1.0 / (m - 1);
+      }
+      if (err_loc <= delta * t_step * tol) break;
+      if (ireject == mxrej) throw Error(NC_ERR_NUMERIC, "Krylov step rejected too many times; tolerance too tight");
+      t_step = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+      S->st.rejected++;
+    }
+    const int mx = mb + std::max(0, k1 - 1);
+    // w_new = c0/beta * w + sum_{i>=1} c_i sc_i V_i ; c = beta * F[:,0]
+    std::vector<double> coef(m + 2, 0.0);
+    for (int i = 1; i < mx; ++i) coef[i - 1] = S->beta * F[(size_t)i * mxf + 0];
+    coef[m + 1] = F[0];  // (beta*F00)/beta
+    NC_CHECK_HIP(hipMemcpyAsync(S->coef.p, coef.data(), sizeof(double) * (m + 2), hipMemcpyHostToDevice, s));
+    S->t += t_step;
+    const bool hit = S->t >= t_target - t_eps;
+    if (hit) S->t = t_target;
+    {
+      auto fill = [&](auto& a, auto* vbase) {
+        a.N = N;
+        a.nv = mx - 1;
+        a.V = vbase;
+        a.sc = S->kbuf.p + L.sc(1);
+        a.coef = S->coef.p;
+        a.csign = 1.0;
+        a.xin = S->u.p;
+        a.xscale = S->coef.p + (m + 1);
+        a.xout = S->u.p;
+        a.vout = Vp(0);
+        a.vout_fp32 = fp32 ? 1 : 0;
+        a.partials = S->partials.p;
+        a.counter = S->counter.p;
+        a.kb = S->kbuf.p;
+        a.L = L;
+        a.finish = FIN_BETA;
+      };
+      if (fp32) {
+        VecArgs<float> a{};
+        fill(a, reinterpret_cast<const float*>(Vp(1)));
+        launch_vec(a, S->red_blocks, s);
+      } else {
+        VecArgs<double> a{};
+        fill(a, reinterpret_cast<const double*>(Vp(1)));
+        launch_vec(a, S->red_blocks, s);
+      }
+    }
+    // the new beta is read back with the next H download; keep a host copy current
+    double bnew = 0.0;
+    NC_CHECK_HIP(hipMemcpyAsync(&bnew, S->kbuf.p + L.misc(MISC_BETA), sizeof(double), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    S->beta = bnew;
+    S->st.steps++;
+    S->st.error_estimate += std::max(err_loc, S->anorm * 2.220446049250313e-16);
+    if (k1 != 0) S->t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+    if (hit) break;
+  }
+}
+
+// ---- Dormand-Prince 5(4) ---------------------------------------------------------------
+void rk_advance(nc_solver_s* S, double t_target) {
+  nc_matrix_s* A = S->A;
+  const int64_t N = S->N;
+  hipStream_t s = S->s;
+  const bool fp32 = A->precision == NC_FP32;
+  const double rtol = S->opts.rtol, atol = S->opts.atol;
+  const unsigned g = std::min<unsigned>(4 * kMaxRedBlocks, grid_for(N, kBlock));
+  double* k = S->k.p;
+  auto kp = [&](int j) { return k + (int64_t)j * N; };
+  auto to_vin = [&](const double* src) {
+    if (fp32)
+      k_to_vt<float><<<g, kBlock, 0, s>>>(src, reinterpret_cast<float*>(S->rkv.p), N);
+    else
+      k_to_vt<double><<<g, kBlock, 0, s>>>(src, reinterpret_cast<double*>(S->rkv.p), N);
+  };
+  if (!S->fsal_valid) {
+    to_vin(S->u.p);
+    matvec_launch(A, S->t, S->rkv.p, kp(0), false, s, true);
+    S->st.matvecs++;
+    S->fsal_valid = true;
+  }
+  if (S->h <= 0.0) {
+    // initial step (Hairer, Norsett & Wanner, Solving ODEs I, II.4): based on |f(t0,y0)|
+    std::vector<double> sums(2);
+    VecArgs<double> a{};
+    a.N = N;
+    a.nv = 0;
+    a.xin = kp(0);
+    a.partials = S->partials.p;
+    a.counter = S->counter.p;
+    a.kb = S->kbuf.p;
+    a.L = KLayout{S->m};
+    a.finish = FIN_SUMS;
+    launch_vec(a, S->red_blocks, s);
+    NC_CHECK_HIP(hipMemcpyAsync(sums.data(), S->kbuf.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    const double fnorm = std::sqrt(sums[0]);
+    S->h = S->opts.h0 > 0 ? S->opts.h0 : (fnorm > 0 ? 0.01 / fnorm : 1e-3);
+    S->h = std::max(S->h, 1e-12);
+  }
+  const double t_eps = 1e-13 * std::max(1.0, std::fabs(t_target));
+  double errsum = 0.0;
+  while (S->t < t_target - t_eps) {
+    if (S->opts.max_steps && S->st.steps + S->st.rejected >= S->opts.max_steps)
+      throw Error(NC_ERR_NUMERIC, "RK integrator exceeded max_steps");
+    double h = std::min(S->h, t_target - S->t);
+    const bool clamped = h < S->h;
+    for (int stage = 1; stage < 7; ++stage) {
+      if (fp32)
+        k_rk_stage<float><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p, reinterpret_cast<float*>(S->rkv.p));
+      else
+        k_rk_stage<double><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p,
+                                                 reinterpret_cast<double*>(S->rkv.p));
+      NC_CHECK_HIP(hipGetLastError());
+      matvec_launch(A, S->t + h_dp_c[stage] * h, S->rkv.p, kp(stage), false, s, true);
+      S->st.matvecs++;
+    }
+    // after stage 6 the stage input is y + h*sum b_j k_j (row 7 of A): ytmp = y_{n+1}
+    k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->partials.p, S->counter.p,
+                                               S->kbuf.p);
+    NC_CHECK_HIP(hipGetLastError());
+    NC_CHECK_HIP(hipMemcpyAsync(&errsum, S->kbuf.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    S->st.host_syncs++;
+    matvec_harvest_timing(A);
+    const double err = std::sqrt(errsum / (double)N);
+    if (!std::isfinite(err)) throw Error(NC_ERR_NUMERIC, "non-finite RK error estimate");
+    double fac = err > 0 ? 0.9 * std::pow(err, -0.2) : 5.0;
+    if (err <= 1.0) {
+      // accept: u <- ytmp, k0 <- k6 (FSAL)
+      k_copy_d<<<g, kBlock, 0, s>>>(S->ytmp.p, S->u.p, N);
+      k_copy_d<<<g, kBlock, 0, s>>>(kp(6), kp(0), N);
+      NC_CHECK_HIP(hipGetLastError());
+      S->t += h;
+      if (S->t > t_target - t_eps) S->t = t_target;
+      S->st.steps++;
+      fac = std::min(5.0, std::max(0.2, fac));
+      if (!clamped || fac < 1.0) S->h = h * fac;
+    } else {
+      S->st.rejected++;
+      S->h = h * std::max(0.2, fac);
+    }
+    if (S->h < 1e-14 * std::max(1.0, std::fabs(S->t))) throw Error(NC_ERR_NUMERIC, "RK step size underflow");
+  }
+}
+
+}  // namespace nc
+
+using namespace nc;
+
+namespace {
+
+static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts) {
+  {
+    NC_REQUIRE(A, NC_ERR_ARG, "null matrix");
+    auto S = new nc_solver_s();
+    try {
+      if (opts)
+        S->opts = *opts;
+      else
+        nc_solve_opts_default(&S->opts);
+      if (S->opts.krylov_m <= 0) S->opts.krylov_m = 30;
+      if (S->opts.krylov_tol <= 0) S->opts.krylov_tol = 1e-12;
+      if (S->opts.rtol <= 0) S->opts.rtol = 1e-8;
+      if (S->opts.atol <= 0) S->opts.atol = 1e-14;
+      NC_REQUIRE(S->opts.krylov_m >= 2 && S->opts.krylov_m <= 31, NC_ERR_ARG, "krylov_m must be in 2..31");
+      S->A = A;
+      S->s = A->stream.s;
+      S->N = A->n_rows;
+      S->vs = A->vsize;
+      const bool tv = !A->tv_r.empty();
+      int method = S->opts.method;
+      if (method == NC_METHOD_AUTO) method = tv ? NC_METHOD_RK45 : NC_METHOD_KRYLOV;
+      NC_REQUIRE(!(method == NC_METHOD_KRYLOV && tv), NC_ERR_ARG,
+                 "the Krylov integrator needs a time-invariant generator; use NC_METHOD_RK45");
+      S->method = method;
+      S->m = S->opts.krylov_m;
+      A->timing = S->opts.time_kernels != 0;
+      S->u.alloc(S->N);
+      const KLayout L{S->m};
+      S->kbuf.alloc(L.size());
+      NC_CHECK_HIP(hipMemsetAsync(S->kbuf.p, 0, sizeof(double) * L.size(), S->s));
+      S->hbuf.assign(L.size(), 0.0);
+      // k_vec writes partials with a stride of MAXV+1 <= 33 doubles per block
+      solver_reduce_setup(S, 32);
+      if (method == NC_METHOD_KRYLOV) {
+        S->V.alloc((size_t)(S->m + 1) * S->N * S->vs);
+        S->q.alloc(S->N);
+        S->p.alloc(S->N);
+        S->coef.alloc(S->m + 2);
+      } else {
+        S->rkv.alloc(S->N * S->vs);
+        S->k.alloc((size_t)7 * S->N);
+        S->ytmp.alloc(S->N);
+      }
+      NC_CHECK_HIP(hipStreamSynchronize(S->s));
+    } catch (...) {
+      delete S;
+      throw;
+    }
+    return S;
+  }
+}
+
+// Re-initialise integrator state from the device copy u0 (time t0, norm beta0).
+static void solver_reinit(nc_solver_s* S) {
+  {
+    const unsigned gc = std::min<unsigned>(4096, grid_for(S->N, kBlock));
+    k_copy_d<<<gc, kBlock, 0, S->s>>>(S->u0.p, S->u.p, S->N);
+    S->t = S->t0;
+    S->have_state = true;
+    S->fsal_valid = false;
+    S->started = false;
+    S->h = 0.0;
+    S->st = nc_solve_stats{};
+    if (S->method == NC_METHOD_KRYLOV) {
+      S->beta = S->beta0;
+      const KLayout L{S->m};
+      const double sc0 = S->beta > 0 ? 1.0 / S->beta : 0.0;
+      NC_CHECK_HIP(hipMemcpyAsync(S->kbuf.p + L.sc(0), &sc0, sizeof(double), hipMemcpyHostToDevice, S->s));
+      const unsigned g = grid_for(S->N, kBlock);
+      if (S->A->precision == NC_FP32)
+        k_to_vt<float><<<std::min(g, 4096u), kBlock, 0, S->s>>>(S->u.p, reinterpret_cast<float*>(S->V.p), S->N);
+      else
+        k_to_vt<double><<<std::min(g, 4096u), kBlock, 0, S->s>>>(S->u.p, reinterpret_cast<double*>(S->V.p), S->N);
+    }
+    NC_CHECK_HIP(hipGetLastError());
+  }
+}
+
+static void solver_set_state_impl(nc_solver_s* S, double t0, const double* u) {
+  NC_REQUIRE(S && u, NC_ERR_ARG, "null argument");
+  if (!S->u0.p) S->u0.alloc(S->N);
+  NC_CHECK_HIP(hipMemcpyAsync(S->u0.p, u, sizeof(double) * S->N, hipMemcpyHostToDevice, S->s));
+  double b2 = 0.0;
+  for (int64_t i = 0; i < S->N; ++i) b2 += u[i] * u[i];
+  S->beta0 = std::sqrt(b2);
+  S->t0 = t0;
+  solver_reinit(S);
+  NC_CHECK_HIP(hipStreamSynchronize(S->s));
+}
+
+static void solver_run_impl(nc_solver_s* S, int32_t n_out, const double* t_out) {
+  {
+    NC_REQUIRE(S && S->have_state, NC_ERR_ARG, "solver has no state; call nc_solver_set_state first");
+    NC_REQUIRE(n_out >= 0 && (n_out == 0 || t_out), NC_ERR_ARG, "bad output times");
+    for (int k = 0; k < n_out; ++k) {
+      NC_REQUIRE(std::isfinite(t_out[k]) && t_out[k] >= S->t - 1e-12 * std::max(1.0, std::fabs(S->t)), NC_ERR_ARG,
+                 "output times must be finite and not before the current time");
+      NC_REQUIRE(k == 0 || t_out[k] >= t_out[k - 1], NC_ERR_ARG, "output times must be ascending");
+    }
+    if ((size_t)n_out * S->N > S->snaps.n) S->snaps.alloc((size_t)std::max(n_out, 1) * S->N);
+    S->n_snaps = n_out;
+    const unsigned g = std::min<unsigned>(4096, grid_for(S->N, kBlock));
+    for (int k = 0; k < n_out; ++k) {
+      if (S->method == NC_METHOD_KRYLOV)
+        krylov_advance(S, t_out[k]);
+      else
+        rk_advance(S, t_out[k]);
+      k_copy_d<<<g, kBlock, 0, S->s>>>(S->u.p, S->snaps.p + (size_t)k * S->N, S->N);
+      NC_CHECK_HIP(hipGetLastError());
+    }
+    NC_CHECK_HIP(hipStreamSynchronize(S->s));
+    matvec_harvest_timing(S->A);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int nc_solve_opts_default(nc_solve_opts* o) {
+  if (!o) return NC_ERR_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->method = NC_METHOD_AUTO;
+  o->krylov_m = 30;
+  o->krylov_tol = 1e-12;
+  o->rtol = 1e-8;
+  o->atol = 1e-14;
+  return NC_OK;
+}
+
+int nc_solver_create(nc_matrix A, const nc_solve_opts* opts, nc_solver* out) {
+  return guard([&] {
+    NC_REQUIRE(out, NC_ERR_ARG, "null argument");
+    *out = solver_create_impl(A, opts);
+  });
+}
+
+int nc_solver_destroy(nc_solver S) {
+  return guard([&] { delete S; });
+}
+
+int nc_solver_set_state(nc_solver S, double t0, const double* u) {
+  return guard([&] { solver_set_state_impl(S, t0, u); });
+}
+
+int nc_solver_restart(nc_solver S) {
+  return guard([&] {
+    NC_REQUIRE(S && S->u0.p, NC_ERR_ARG, "solver has no state; call nc_solver_set_state first");
+    solver_reinit(S);
+  });
+}
+
+int nc_solver_run(nc_solver S, int32_t n_out, const double* t_out) {
+  return guard([&] { solver_run_impl(S, n_out, t_out); });
+}
+
+int nc_solver_get_snapshot(nc_solver S, int32_t k, double* u) {
+  return guard([&] {
+    NC_REQUIRE(S && u && k >= 0 && k < S->n_snaps, NC_ERR_ARG, "snapshot index out of range");
+    NC_CHECK_HIP(hipMemcpy(u, S->snaps.p + (size_t)k * S->N, sizeof(double) * S->N, hipMemcpyDeviceToHost));
+  });
+}
+
+int nc_solver_get_state(nc_solver S, double* t, double* u) {
+  return guard([&] {
+    NC_REQUIRE(S, NC_ERR_ARG, "null solver");
+    if (t) *t = S->t;
+    if (u) NC_CHECK_HIP(hipMemcpy(u, S->u.p, sizeof(double) * S->N, hipMemcpyDeviceToHost));
+  });
+}
+
+int nc_solver_stats(nc_solver S, nc_solve_stats* st) {
+  return guard([&] {
+    NC_REQUIRE(S && st, NC_ERR_ARG, "null argument");
+    *st = S->st;
+    st->matvecs_timed = S->A->timed_count;
+    st->matvec_ms = S->A->timed_ms;
+  });
+}
+
+int nc_solve_fixed(nc_matrix A, const nc_solve_opts* opts, const double* u0, int32_t n_out, const double* t_out,
+                   double* u_out, nc_solve_stats* st) {
+  return guard([&] {
+    NC_REQUIRE(A && u0 && (n_out == 0 || (t_out && u_out)), NC_ERR_ARG, "null argument");
+    nc_solver_s* S = solver_create_impl(A, opts);
+    struct Guard {
+      nc_solver_s* S;
+      ~Guard() { delete S; }
+    } gd{S};
+    const double t0 = n_out ? t_out[0] : 0.0;
+    solver_set_state_impl(S, t0, u0);
+    solver_run_impl(S, n_out, t_out);
+    for (int k = 0; k < n_out; ++k)
+      NC_CHECK_HIP(hipMemcpy(u_out + (size_t)k * S->N, S->snaps.p + (size_t)k * S->N, sizeof(double) * S->N,
+                             hipMemcpyDeviceToHost));
+    if (st) nc_solver_stats(S, st);
+  });
+}
+
+}  // extern "C"

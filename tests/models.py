@@ -1,0 +1,147 @@
+"""Reaction networks of the test suite, written once and handed to both the oracle and
+the GPU library.
+
+Propensities use ``numcme_hip.expr`` functions, which trace symbolically for the device
+and evaluate with numpy for the oracle. Species and parameters are 0-based.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from numcme_hip import expr as nm
+
+# ---- telegraph gene model (test/test_fspmat.jl, test/test_solver.jl, config C1) ----------
+TELEGRAPH_S = np.array([[-1, 1, 0, 0], [1, -1, 0, 0], [0, 0, 1, -1]])
+
+
+def telegraph(theta=(0.05, 0.1, 5.0, 0.05)):
+    """G_off->G_on k_on, G_on->G_off k_off, G_on->G_on+mRNA k_r, mRNA->0 gamma."""
+    props = [
+        ("ti", lambda x, p: p[0] * x[0]),
+        ("ti", lambda x, p: p[1] * x[1]),
+        ("ti", lambda x, p: p[2] * x[1]),
+        ("ti", lambda x, p: p[3] * x[2]),
+    ]
+    return TELEGRAPH_S, props, np.asarray(theta, dtype=float)
+
+
+def telegraph_tv(kind, theta=(0.05, 0.1, 5.0, 1.0)):
+    """test/test_fspmat.jl:103-124: reaction 2 scaled by max(0, 1 - sin(pi t / 2))."""
+    tf = lambda t, p: nm.max(0.0, 1.0 - nm.sin(nm.pi * t / 2))
+    a2 = ("sep", tf, lambda x, p: p[1] * x[1]) if kind == "sep" else \
+         ("joint", lambda t, x, p: nm.max(0.0, 1.0 - nm.sin(nm.pi * t / 2)) * p[1] * x[1])
+    props = [("ti", lambda x, p: p[0] * x[0]), a2, ("ti", lambda x, p: p[2] * x[1]), ("ti", lambda x, p: p[3] * x[2])]
+    return TELEGRAPH_S, props, np.asarray(theta, dtype=float)
+
+
+def telegraph_sens(theta=(0.05, 0.1, 5.0, 0.5, 20.0)):
+    """test/sensmat/telegraph.jl:7-28: degradation scaled by max(0, 1 - sin(pi t / L))."""
+    props = [
+        ("ti", lambda x, p: p[0] * x[0]),
+        ("ti", lambda x, p: p[1] * x[1]),
+        ("ti", lambda x, p: p[2] * x[1]),
+        ("sep", lambda t, p: nm.max(0.0, 1.0 - nm.sin(nm.pi * t / p[4])), lambda x, p: p[3] * x[2]),
+    ]
+    return TELEGRAPH_S, props, np.asarray(theta, dtype=float)
+
+
+def telegraph_states(nmax):
+    """Gene-major rectangular telegraph space (the convention of test/sensmat/telegraph.jl:31)."""
+    return [[1, 0, m] for m in range(nmax + 1)] + [[0, 1, m] for m in range(nmax + 1)]
+
+
+# ---- toggle switch (config C2) -----------------------------------------------------------
+TOGGLE_S = np.array([[1, -1, 0, 0], [0, 0, 1, -1]])
+
+
+def toggle(theta=(50.0, 20.0, 2.5, 1.0, 1.0)):
+    """Production of S1: b + alpha/(1+(S2/K)^n); first-order degradation gamma."""
+    props = [
+        ("ti", lambda x, p: p[3] + p[0] / (1.0 + (x[1] / p[1]) ** p[2])),
+        ("ti", lambda x, p: p[4] * x[0]),
+        ("ti", lambda x, p: p[3] + p[0] / (1.0 + (x[0] / p[1]) ** p[2])),
+        ("ti", lambda x, p: p[4] * x[1]),
+    ]
+    return TOGGLE_S, props, np.asarray(theta, dtype=float)
+
+
+def box_states(bounds):
+    """All states 0 <= x_s <= bounds[s], species 0 varying fastest."""
+    grids = np.meshgrid(*[np.arange(b + 1) for b in bounds[::-1]], indexing="ij")
+    return np.stack([g.ravel() for g in grids[::-1]], axis=1).astype(np.int64)
+
+
+# ---- four-state gene, nuclear and cytoplasmic mRNA (config C3) ---------------------------
+def gene4_stoich():
+    S = np.zeros((6, 12), dtype=np.int64)
+    moves = [(0, 1), (1, 0), (1, 2), (2, 1), (2, 3), (3, 2)]
+    for r, (a, b) in enumerate(moves):
+        S[a, r] -= 1
+        S[b, r] += 1
+    for r in range(6, 10):
+        S[4, r] = 1
+    S[4, 10], S[5, 10] = -1, 1
+    S[5, 11] = -1
+    return S
+
+
+GENE4_THETA = (1.3e-3, 3.2e-3, 5.9e-3, 2e-2, 1.5e-3, 4.6e-3, 0.0, 1e-2, 0.2, 1.5, 0.05, 1e-3)
+
+
+def gene4(theta=GENE4_THETA):
+    """Gene states 0..3 (one-hot), nuclear (4) and cytoplasmic (5) mRNA; k21 pulse-modulated."""
+    sig = lambda t: (1.0 - nm.exp(-6e-3 * t)) * nm.exp(-9e-3 * t)
+    props = [
+        ("ti", lambda x, p: p[0] * x[0]),
+        ("ti", lambda x, p: p[1] * x[1]),
+        ("ti", lambda x, p: p[2] * x[1]),
+        ("sep", lambda t, p: nm.max(0.0, p[3] - 0.12 * sig(t)), lambda x, p: x[2]),
+        ("ti", lambda x, p: p[4] * x[2]),
+        ("ti", lambda x, p: p[5] * x[3]),
+        ("ti", lambda x, p: p[6] * x[0]),
+        ("ti", lambda x, p: p[7] * x[1]),
+        ("ti", lambda x, p: p[8] * x[2]),
+        ("ti", lambda x, p: p[9] * x[3]),
+        ("ti", lambda x, p: p[10] * x[4]),
+        ("ti", lambda x, p: p[11] * x[5]),
+    ]
+    return gene4_stoich(), props, np.asarray(theta, dtype=float)
+
+
+def gene4_states(nmax):
+    out = []
+    for g in range(4):
+        oh = [0, 0, 0, 0]
+        oh[g] = 1
+        for c in range(nmax + 1):
+            for n in range(nmax + 1):
+                out.append(oh + [n, c])
+    return np.asarray(out, dtype=np.int64)
+
+
+# ---- adapters ------------------------------------------------------------------------------
+def to_oracle(props):
+    from oracle.fspmatrix import Joint, Separable, TimeInvariant
+    out = []
+    for pr in props:
+        if pr[0] == "ti":
+            out.append(TimeInvariant(pr[1]))
+        elif pr[0] == "sep":
+            out.append(Separable(pr[1], pr[2]))
+        else:
+            out.append(Joint(pr[1]))
+    return out
+
+
+def to_product(props):
+    from numcme_hip import (JointTimeVaryingPropensity, SeparableTimeVaryingPropensity,
+                            StandardTimeInvariantPropensity)
+    out = []
+    for pr in props:
+        if pr[0] == "ti":
+            out.append(StandardTimeInvariantPropensity(pr[1]))
+        elif pr[0] == "sep":
+            out.append(SeparableTimeVaryingPropensity(pr[1], pr[2]))
+        else:
+            out.append(JointTimeVaryingPropensity(pr[1]))
+    return out

@@ -1,0 +1,113 @@
+"""GPU parity of the fixed-space solve against the exact FP64 oracle.
+
+Tolerances (per snapshot): FP64 path l1 <= 1e-8 and sink mass <= 1e-10; FP32-storage
+path l1 <= 1e-6. Configurations: telegraph on 800 states (t = 0:10:200), toggle on
+1024^2 states (t = 0:1:10, Krylov m = 30), the time-varying telegraph of
+test/sensmat/telegraph.jl, and the reference's fixed-solve invariants
+(test/test_solver.jl:54-77).
+"""
+import numpy as np
+import pytest
+
+import models
+import numcme_hip as nh
+from oracle.solve import solve_fixed
+
+pytestmark = pytest.mark.gpu
+
+
+def l1(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).sum())
+
+
+@pytest.mark.parametrize("method", ["krylov", "rk45"])
+def test_telegraph_c1(gpu, method):
+    S, props, th = models.telegraph()
+    states = models.telegraph_states(399)
+    p0 = np.zeros(800)
+    p0[0] = 1.0
+    ts = np.arange(0.0, 201.0, 10.0)
+    ref = solve_fixed(S, models.to_oracle(props), th, states, p0, ts)
+    model = nh.CmeModel(S, models.to_product(props), th)
+    m = nh.KrylovExpv(m=30, tol=1e-12) if method == "krylov" else nh.RK45(rtol=1e-10, atol=1e-16)
+    out = nh.solve(model, nh.FspVectorSparse(states, p0), (0.0, 200.0), m, saveat=ts)
+    assert len(out) == len(ts)
+    for k in range(len(ts)):
+        assert np.array_equal(out.p[k].states, np.asarray(states))
+        assert l1(out.p[k].values, ref.p[k]) <= 1e-8
+        assert abs(out.sinks[k].sum() - ref.sinks[k].sum()) <= 1e-10
+        assert abs(out.p[k].sum() + out.sinks[k].sum() - 1.0) < 1e-10
+
+
+def toggle_box_reference(ts, box=160):
+    """Exact toggle solution on a 0..box-1 square; by the FSP bound it equals the
+    1024^2 solution up to this box's sink mass (checked to be negligible)."""
+    S, props, th = models.toggle()
+    states = models.box_states([box - 1, box - 1])
+    p0 = np.zeros(len(states))
+    p0[0] = 1.0
+    ref = solve_fixed(S, models.to_oracle(props), th, states, p0, ts)
+    assert max(s.sum() for s in ref.sinks) < 1e-30
+    return states, ref
+
+
+@pytest.fixture(scope="module")
+def toggle_ref():
+    ts = np.arange(0.0, 11.0)
+    return ts, toggle_box_reference(ts)
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", 1e-8), ("fp32", 1e-6)])
+def test_toggle_c2_full_space(gpu, toggle_ref, precision, bar):
+    ts, (bstates, ref) = toggle_ref
+    S, props, th = models.toggle()
+    states = models.box_states([1023, 1023])
+    p0 = np.zeros(len(states))
+    p0[0] = 1.0
+    model = nh.CmeModel(S, models.to_product(props), th)
+    out = nh.solve(model, nh.FspVectorSparse(states, p0), (0.0, 10.0),
+                   nh.KrylovExpv(m=30, tol=1e-12, precision=precision), saveat=ts)
+    # embed: box state (a, b) -> index b*1024 + a
+    emb = bstates[:, 1] * 1024 + bstates[:, 0]
+    mask = np.ones(len(states), bool)
+    mask[emb] = False
+    errs = []
+    for k in range(len(ts)):
+        v = out.p[k].values
+        err = l1(v[emb], ref.p[k]) + np.abs(v[mask]).sum()
+        errs.append(err)
+        assert err <= bar, (k, err)
+        assert abs(out.sinks[k].sum()) <= 1e-10
+    print(f"toggle 1024^2 {precision}: max l1 {max(errs):.3e}, steps {out.stats['steps']}, "
+          f"matvecs {out.stats['matvecs']}")
+
+
+def test_time_varying_telegraph(gpu):
+    S, props, th = models.telegraph_sens()
+    states = models.telegraph_states(120)
+    p0 = np.zeros(len(states))
+    p0[0] = 1.0
+    ts = np.array([0.0, 5.0, 10.0, 20.0, 30.0, 45.0])
+    kinks = [10.0, 50.0]  # 1 - sin(pi t / 20) touches 0 at t = 10 + 40k
+    ref = solve_fixed(S, models.to_oracle(props), th, states, p0, ts, breakpoints=kinks)
+    model = nh.CmeModel(S, models.to_product(props), th)
+    out = nh.solve(model, nh.FspVectorSparse(states, p0), (0.0, 45.0), nh.RK45(rtol=1e-10, atol=1e-16), saveat=ts)
+    for k in range(len(ts)):
+        assert l1(out.p[k].values, ref.p[k]) <= 1e-8, k
+        assert abs(out.sinks[k].sum() - ref.sinks[k].sum()) <= 1e-10
+
+
+def test_reference_fixed_solve_invariants(gpu):
+    # test/test_solver.jl:211-234: telegraph expanded 20 levels, saveat 0:20:120
+    S, props, th = models.telegraph((0.05, 0.1, 5.0, 1.0))
+    sp = nh.StateSpaceSparse(S, [1, 0, 0])
+    sp.expand(20)
+    p0 = nh.FspVectorSparse.from_space(sp, [([1, 0, 0], 1.0)])
+    model = nh.CmeModel(S, models.to_product(props), th)
+    ts = np.arange(0.0, 121.0, 20.0)
+    for method in (nh.KrylovExpv(), nh.RK45(rtol=1e-8, atol=1e-14)):
+        sol = nh.solve(model, p0, (0.0, 120.0), method, saveat=ts)
+        assert len(sol) == len(ts)
+        for p, sinks in zip(sol.p, sol.sinks):
+            assert abs(p.sum() + sinks.sum() - 1.0) <= 1.5e-8
+        assert isinstance(sol[0], nh.FspOutputSliceSparse)
