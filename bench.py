@@ -50,13 +50,14 @@ def box_states(n):
 
 
 class Solver:
-    def __init__(self, L, A, precision, time_kernels):
+    def __init__(self, L, A, precision, time_kernels, graphs=True):
         o = L.nc_solve_opts()
         L.call("nc_solve_opts_default", C.byref(o))
         o.method = L.NC_METHOD_KRYLOV
         o.krylov_m = 30
         o.krylov_tol = 1e-12
         o.time_kernels = int(time_kernels)
+        o.no_graphs = 0 if graphs else 1
         h = C.c_void_p()
         L.call("nc_solver_create", A.handle, C.byref(o), C.byref(h))
         self.L, self.h, self.A = L, h.value, A
@@ -119,6 +120,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch Arnoldi kernels eagerly (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one FSP solve per GPU; multi-GPU runs are not part of this "
@@ -134,15 +136,17 @@ def main():
     space = nh.StateSpaceSparse(model.stoich_matrix, states)
     A = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision=args.precision, model=model)
     A64 = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision="fp64", model=model)
+    build_s = time.perf_counter() - t_build
+    rows_per_thread, _ = A.autotune(reps=30)
+    A64.autotune(reps=30)
     info = A.info()
     N = info["n_rows"]
-    build_s = time.perf_counter() - t_build
     u0 = np.zeros(N)
     u0[0] = 1.0
     ts = np.arange(0.0, 11.0)
 
     # FP64 reference solve on the GPU (accuracy check of the timed precision)
-    s64 = Solver(L, A64, "fp64", False)
+    s64 = Solver(L, A64, "fp64", False, graphs=not args.no_graphs)
     s64.set_state(u0)
     s64.solve(ts)
     ref = [s64.snapshot(k, N) for k in range(len(ts))]
@@ -152,22 +156,31 @@ def main():
     fp64_solve_s = time.perf_counter() - t0
     s64.free()
 
-    sol = Solver(L, A, args.precision, True)
+    # timed region: K complete solves, Arnoldi processes replayed as HIP graphs
+    sol = Solver(L, A, args.precision, False, graphs=not args.no_graphs)
     sol.set_state(u0)
-    for _ in range(args.warmup):
+    for _ in range(max(args.warmup, 1)):
         sol.solve(ts)
     l1 = max(float(np.abs(sol.snapshot(k, N) - ref[k]).sum()) for k in range(len(ts)))
-    sol.L.call("nc_solver_restart", sol.h)  # clears stats (and the kernel timers below)
-    A_timed_before = sol.stats()
     t0 = time.perf_counter()
     steps_stats = []
     for _ in range(args.steps):
         sol.solve(ts)
         steps_stats.append(sol.stats())
     elapsed = time.perf_counter() - t0
-    mv_ms = steps_stats[-1]["matvec_ms"] - A_timed_before["matvec_ms"]
-    mv_n = steps_stats[-1]["matvecs_timed"] - A_timed_before["matvecs_timed"]
     sol.free()
+
+    # mat-vec roofline: one more solve with HIP events recorded around every mat-vec
+    # launch on the solver's stream (graphs off, since events cannot sit inside them)
+    tim = Solver(L, A, args.precision, True)
+    tim.set_state(u0)
+    before = tim.stats()
+    tim.solve(ts)
+    after = tim.stats()
+    mv_ms = after["matvec_ms"] - before["matvec_ms"]
+    mv_n = after["matvecs_timed"] - before["matvecs_timed"]
+    tim.free()
+    back_to_back_ms = A.bench_matvec(reps=200)
 
     avg_mv_s = (mv_ms / mv_n) * 1e-3 if mv_n else float("nan")
     achieved = info["csr_bytes_per_matvec"] / avg_mv_s / 1e9
@@ -206,6 +219,8 @@ def main():
             "bytes_per_launch_own_layout": info["bytes_per_matvec"],
             "avg_launch_us": avg_mv_s * 1e6,
             "launches_timed": mv_n,
+            "back_to_back_launch_us": back_to_back_ms * 1e3,
+            "rows_per_thread": rows_per_thread,
         },
         "cpu_baseline": None,
     }

@@ -160,6 +160,8 @@ typedef struct {
   double bytes_per_matvec;   /* bytes the mat-vec kernel moves (its own layout)     */
   double csr_bytes_per_matvec; /* nnz*(s_v+4) + (N+1)*4 + 2*N*s_v, CSR-equivalent   */
   double norm1;          /* max column abs sum (2 * max |diagonal|)                 */
+  int32_t rows_per_thread; /* mat-vec kernel: consecutive state rows per thread     */
+  int32_t pad0;
 } nc_matrix_info_t;
 
 /* FspMatrixSparse{RealT}(space, propensities; parameters) — fspsparsematrix.jl:47-108.
@@ -178,6 +180,12 @@ int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t
 /* Repeat the mat-vec kernel `reps` times on device-resident vectors and report the mean
  * kernel duration from HIP events on the matrix's stream. */
 int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms);
+/* Mat-vec launch shape: consecutive state rows per thread (1, 2 or 4). No reference
+ * counterpart (the reference's CSC mul! has no launch shape). */
+int nc_matrix_set_rows_per_thread(nc_matrix A, int32_t rows);
+/* Time every launch shape on this matrix (`reps` launches each) and keep the fastest;
+ * *best_ms receives its mean duration. No reference counterpart. */
+int nc_matrix_autotune(nc_matrix A, int32_t reps, double* best_ms);
 
 /* ======================================================================================
  * Time integration — replaces the integrator loop of solve() (fspsolve.jl:10-41) with
@@ -193,7 +201,7 @@ typedef struct {
   double h0;           /* first RK step, 0 = automatic                             */
   int64_t max_steps;   /* 0 = unlimited                                            */
   int32_t time_kernels;/* record HIP events around every mat-vec launch            */
-  int32_t reserved;
+  int32_t no_graphs;  /* 1: launch kernels directly instead of replaying HIP graphs */
 } nc_solve_opts;
 
 typedef struct {
@@ -201,6 +209,7 @@ typedef struct {
   int64_t matvecs_timed;
   double matvec_ms;    /* summed event-timed mat-vec kernel durations              */
   double error_estimate;
+  int64_t reorthogonalizations; /* Arnoldi columns that needed a second Gram-Schmidt */
 } nc_solve_stats;
 
 typedef struct nc_solver_s* nc_solver;

@@ -1,12 +1,13 @@
 // FSP generator storage and mat-vec (internal).
 //
 // Layout in HBM for a space of n states and nr reactions (VT = float or double):
-//   diag    double [n]       merged time-invariant diagonal
-//   col     int32  [K][n]    merged time-invariant off-diagonal columns, slot-major
-//   val     VT     [K][n]    (rows with fewer than K predecessors are padded: col = row, val = 0)
-//   tvd     VT     [T][n]    per time-varying reaction: diagonal (statefactor or joint value)
-//   tvc     int32  [T][n]    per time-varying reaction: predecessor column (or row)
-//   tvv     VT     [T][n]    per time-varying reaction: predecessor value (or 0)
+//   diag    double [ld]      merged time-invariant diagonal
+//   col     int32  [K][ld]   merged time-invariant off-diagonal columns, slot-major
+//   val     VT     [K][ld]   (rows with fewer than K predecessors are padded: col = row, val = 0)
+//   tvd     VT     [T][ld]   per time-varying reaction: diagonal (statefactor or joint value)
+//   tvc     int32  [T][ld]   per time-varying reaction: predecessor column (or row)
+//   tvv     VT     [T][ld]   per time-varying reaction: predecessor value (or 0)
+//   (ld = n rounded up to 64; rows n..ld-1 hold zeros and column 0)
 //   sink_off int64 [nr+1]    sink row r holds entries sink_off[r] .. sink_off[r+1)
 //   sink_src int32 [E]       source state of each sink entry
 //   sink_val VT    [E]
@@ -24,7 +25,9 @@ struct nc_matrix_s {
   int ns = 0, nr = 0;
   int64_t n = 0;       // states
   int64_t n_rows = 0;  // states + sinks
+  int64_t ld = 0;      // slot stride (n rounded up to 64)
   int K = 0;
+  int rows_per_thread = 2;  // mat-vec rows per thread (1, 2 or 4; see nc_matrix_autotune)
   nc::DevBuf<double> diag;
   nc::DevBuf<int32_t> col;
   nc::DevBuf<unsigned char> val;

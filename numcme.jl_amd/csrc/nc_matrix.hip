@@ -7,8 +7,10 @@
 //    merged into one slot, exactly the duplicates SparseArrays.sparse sums (:74);
 //  * _update_sparsematrix! (:154-166) becomes a refresh kernel for joint time-varying
 //    reactions, run when t differs from the cached time (:206-213);
-//  * matvec!/matvecadd! (:196-247) become one kernel: one thread per state row (slot-major
-//    ELL, FP64 accumulation) and one workgroup per sink row (segmented reduction).
+//  * matvec!/matvecadd! (:196-247) become one kernel: R consecutive state rows per
+//    thread (slot-major ELL, 8/16-byte vector loads of the matrix streams, FP64
+//    accumulation) and one workgroup per sink row (segmented reduction). R is chosen by
+//    nc_matrix_autotune.
 #include <cfloat>
 
 #include "nc_matrix.h"
@@ -22,6 +24,7 @@ namespace nc {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int64_t kLdAlign = 64;  // slot stride in elements: keeps every slot 256-byte aligned
 
 struct Groups {
   int32_t ng;
@@ -59,13 +62,21 @@ __global__ void k_build_count(int64_t n, int nr, const int32_t* state_conn, Grou
     if (state_conn[i * nr + tv.r[q]] >= 0) atomicAdd(&tv_pred[q], 1ull);
 }
 
+// Rows i in [n, ld) are padding: zero values and column 0, so vectorised reads of a
+// thread's row group never leave the arrays and never gather out of range.
 template <typename VT>
-__global__ void k_build_fill(int64_t n, int ns, int nr, int K, const int32_t* states, const int32_t* state_conn,
-                             const uint32_t* sink_mask, const int32_t* stoich, Groups g, TvList tv, Progs pr,
-                             double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc, VT* tvv,
-                             unsigned long long* diag_max_bits) {
+__global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, const int32_t* states,
+                             const int32_t* state_conn, const uint32_t* sink_mask, const int32_t* stoich, Groups g,
+                             TvList tv, Progs pr, double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc,
+                             VT* tvv, unsigned long long* diag_max_bits) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= ld) return;
+  if (i >= n) {
+    diag[i] = 0.0;
+    for (int k = 0; k < K; ++k) col[k * ld + i] = 0, val[k * ld + i] = (VT)0;
+    for (int q = 0; q < tv.n; ++q) tvd[q * ld + i] = (VT)0, tvv[q * ld + i] = (VT)0, tvc[q * ld + i] = 0;
+    return;
+  }
   int32_t x[kMaxSpecies], y[kMaxSpecies];
   for (int s = 0; s < ns; ++s) x[s] = states[i * ns + s];
   const uint32_t smask = sink_mask[i];
@@ -79,11 +90,12 @@ __global__ void k_build_fill(int64_t n, int ns, int nr, int K, const int32_t* st
       for (int s = 0; s < ns; ++s) y[s] = x[s] - stoich[s + lead * ns];
       double a = 0.0;
       for (int m = g.start[k]; m < g.start[k + 1]; ++m) a += xprog(pr, g.members[m], 0.0, y);
-      col[(int64_t)slot * n + i] = j;
-      val[(int64_t)slot * n + i] = (VT)a;
+      col[slot * ld + i] = j;
+      val[slot * ld + i] = (VT)a;
       ++slot;
     }
-    // this row's outflow through the group: the diagonal is minus the stored column entries
+    // this row's outflow through the group: the diagonal is minus the stored column
+    // entries, so each stored column sums to zero exactly in real arithmetic
     bool nonneg = true;
     for (int s = 0; s < ns; ++s) nonneg = nonneg && (x[s] + stoich[s + lead * ns] >= 0);
     if (!nonneg) {
@@ -98,8 +110,8 @@ __global__ void k_build_fill(int64_t n, int ns, int nr, int K, const int32_t* st
     }
   }
   for (; slot < K; ++slot) {
-    col[(int64_t)slot * n + i] = (int32_t)i;
-    val[(int64_t)slot * n + i] = (VT)0;
+    col[slot * ld + i] = (int32_t)i;
+    val[slot * ld + i] = (VT)0;
   }
   diag[i] = -d;
   atomicMax(diag_max_bits, (unsigned long long)__double_as_longlong(d > 0 ? d : -d));
@@ -107,18 +119,18 @@ __global__ void k_build_fill(int64_t n, int ns, int nr, int K, const int32_t* st
     const int r = tv.r[q];
     const int32_t j = state_conn[i * nr + r];
     if (tv.kind[q] == NC_PROP_SEPARABLE) {
-      tvd[(int64_t)q * n + i] = -(VT)xprog(pr, r, 0.0, x);
+      tvd[q * ld + i] = -(VT)xprog(pr, r, 0.0, x);
       if (j >= 0) {
         for (int s = 0; s < ns; ++s) y[s] = x[s] - stoich[s + r * ns];
-        tvv[(int64_t)q * n + i] = (VT)xprog(pr, r, 0.0, y);
+        tvv[q * ld + i] = (VT)xprog(pr, r, 0.0, y);
       } else {
-        tvv[(int64_t)q * n + i] = (VT)0;
+        tvv[q * ld + i] = (VT)0;
       }
     } else {  // joint: values are written by the refresh kernel
-      tvd[(int64_t)q * n + i] = (VT)0;
-      tvv[(int64_t)q * n + i] = (VT)0;
+      tvd[q * ld + i] = (VT)0;
+      tvv[q * ld + i] = (VT)0;
     }
-    tvc[(int64_t)q * n + i] = j >= 0 ? j : (int32_t)i;
+    tvc[q * ld + i] = j >= 0 ? j : (int32_t)i;
   }
 }
 
@@ -146,8 +158,9 @@ __global__ void k_sink_fill(int64_t n, int ns, int nr, const int32_t* states, co
 
 // _update_sparsematrix! (fspsparsematrix.jl:154-166) for every joint reaction at time t.
 template <typename VT>
-__global__ void k_joint_refresh(int64_t n, int ns, int nr, const int32_t* states, const int32_t* state_conn,
-                                const int32_t* stoich, TvList tv, Progs pr, double t, VT* tvd, VT* tvv) {
+__global__ void k_joint_refresh(int64_t n, int64_t ld, int ns, int nr, const int32_t* states,
+                                const int32_t* state_conn, const int32_t* stoich, TvList tv, Progs pr, double t,
+                                VT* tvd, VT* tvv) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   int32_t x[kMaxSpecies], y[kMaxSpecies];
@@ -155,11 +168,11 @@ __global__ void k_joint_refresh(int64_t n, int ns, int nr, const int32_t* states
   for (int q = 0; q < tv.n; ++q) {
     if (tv.kind[q] != NC_PROP_JOINT) continue;
     const int r = tv.r[q];
-    tvd[(int64_t)q * n + i] = -(VT)xprog(pr, r, t, x);
+    tvd[q * ld + i] = -(VT)xprog(pr, r, t, x);
     const int32_t j = state_conn[i * nr + r];
     if (j >= 0) {
       for (int s = 0; s < ns; ++s) y[s] = x[s] - stoich[s + r * ns];
-      tvv[(int64_t)q * n + i] = (VT)xprog(pr, r, t, y);
+      tvv[q * ld + i] = (VT)xprog(pr, r, t, y);
     }
   }
 }
@@ -179,6 +192,7 @@ __global__ void k_joint_sink_refresh(int64_t e0, int64_t e1, int r, int ns, cons
 template <typename VT>
 struct MatArgs {
   int64_t n;
+  int64_t ld;
   int32_t K;
   int32_t ntv;
   int32_t nr;
@@ -195,38 +209,69 @@ struct MatArgs {
   double sink_scale[kMaxReactions];
 };
 
+template <typename T, int R>
+struct alignas(sizeof(T) * R) Pack {
+  T v[R];
+};
+
+template <typename T, int R>
+__device__ inline Pack<T, R> ldpack(const T* p) {
+  return *reinterpret_cast<const Pack<T, R>*>(p);
+}
+
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
 }
 
-template <typename VT, typename OT, int KT>
+// Thread t owns state rows [R*t, R*t+R). Matrix streams are read with one R-wide vector
+// load per slot; v[i] and the predecessor gathers are scalar.
+template <typename VT, typename OT, int KT, int R>
 __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __restrict__ v, OT* __restrict__ out,
                                                    int accumulate) {
-  const int64_t row_blocks = (a.n + kBlock - 1) / kBlock;
+  const int64_t row_threads = (a.n + R - 1) / R;
+  const int64_t row_blocks = (row_threads + kBlock - 1) / kBlock;
   if ((int64_t)blockIdx.x < row_blocks) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    const double vi = (double)v[i];
-    double acc = a.diag[i] * vi;
-    if constexpr (KT > 0) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i0 = t * R;
+    if (i0 >= a.n) return;
+    const Pack<double, R> dg = ldpack<double, R>(a.diag + i0);
+    double vi[R], acc[R];
 #pragma unroll
-      for (int k = 0; k < KT; ++k) {
-        const int64_t o = (int64_t)k * a.n + i;
-        acc += (double)a.val[o] * (double)v[a.col[o]];
-      }
-    } else {
-      for (int k = 0; k < a.K; ++k) {
-        const int64_t o = (int64_t)k * a.n + i;
-        acc += (double)a.val[o] * (double)v[a.col[o]];
+    for (int r = 0; r < R; ++r) {
+      vi[r] = (i0 + r < a.n) ? (double)v[i0 + r] : 0.0;
+      acc[r] = dg.v[r] * vi[r];
+    }
+    const int K = KT > 0 ? KT : a.K;
+#pragma unroll
+    for (int k = 0; k < (KT > 0 ? KT : 8); ++k) {
+      if (KT == 0 && k >= K) break;
+      const Pack<int32_t, R> c = ldpack<int32_t, R>(a.col + k * a.ld + i0);
+      const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
+    }
+    if (KT == 0) {
+      for (int k = 8; k < K; ++k) {
+        const Pack<int32_t, R> c = ldpack<int32_t, R>(a.col + k * a.ld + i0);
+        const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
       }
     }
     for (int q = 0; q < a.ntv; ++q) {
-      const int64_t o = (int64_t)q * a.n + i;
-      acc += a.tv_scale[q] * ((double)a.tvd[o] * vi + (double)a.tvv[o] * (double)v[a.tvc[o]]);
+      const Pack<VT, R> d = ldpack<VT, R>(a.tvd + q * a.ld + i0);
+      const Pack<int32_t, R> c = ldpack<int32_t, R>(a.tvc + q * a.ld + i0);
+      const Pack<VT, R> w = ldpack<VT, R>(a.tvv + q * a.ld + i0);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        acc[r] += a.tv_scale[q] * ((double)d.v[r] * vi[r] + (double)w.v[r] * (double)v[c.v[r]]);
     }
-    if (accumulate) acc += (double)out[i];
-    out[i] = (OT)acc;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = i0 + r;
+      if (i < a.n) out[i] = (OT)(accumulate ? (double)out[i] + acc[r] : acc[r]);
+    }
   } else {
     const int r = (int)((int64_t)blockIdx.x - row_blocks);
     const int64_t e0 = a.sink_off[r], e1 = a.sink_off[r + 1];
@@ -246,19 +291,29 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
   }
 }
 
-template <typename VT, typename OT>
+template <typename VT, typename OT, int R>
 void dispatch_k(int K, unsigned grid, hipStream_t s, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
   switch (K) {
-    case 1: k_matvec<VT, OT, 1><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 2: k_matvec<VT, OT, 2><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 3: k_matvec<VT, OT, 3><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 4: k_matvec<VT, OT, 4><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 5: k_matvec<VT, OT, 5><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 6: k_matvec<VT, OT, 6><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 7: k_matvec<VT, OT, 7><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 8: k_matvec<VT, OT, 8><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    default: k_matvec<VT, OT, 0><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 1: k_matvec<VT, OT, 1, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 2: k_matvec<VT, OT, 2, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 3: k_matvec<VT, OT, 3, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 4: k_matvec<VT, OT, 4, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 5: k_matvec<VT, OT, 5, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 6: k_matvec<VT, OT, 6, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 7: k_matvec<VT, OT, 7, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 8: k_matvec<VT, OT, 8, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    default: k_matvec<VT, OT, 0, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
   }
+}
+
+template <typename VT, typename OT>
+void dispatch_r(int R, int K, unsigned grid, hipStream_t s, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
+  if (R == 4)
+    dispatch_k<VT, OT, 4>(K, grid, s, a, vv, oo, acc);
+  else if (R == 2)
+    dispatch_k<VT, OT, 2>(K, grid, s, a, vv, oo, acc);
+  else
+    dispatch_k<VT, OT, 1>(K, grid, s, a, vv, oo, acc);
 }
 
 template <typename VT>
@@ -266,6 +321,7 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
                      bool out_fp64) {
   MatArgs<VT> a{};
   a.n = A->n;
+  a.ld = A->ld;
   a.K = A->K;
   a.ntv = (int32_t)A->tv_r.size();
   a.nr = A->nr;
@@ -282,14 +338,16 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   for (int r = 0; r < A->nr; ++r) a.sink_scale[r] = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
   for (size_t q = 0; q < A->tv_r.size(); ++q)
     a.tv_scale[q] = A->tv_kind[q] == NC_PROP_SEPARABLE ? m->tfactor(A->tv_r[q], t) : 1.0;
-  const int64_t row_blocks = (A->n + kBlock - 1) / kBlock;
+  const int R = A->rows_per_thread;
+  const int64_t row_threads = (A->n + R - 1) / R;
+  const int64_t row_blocks = (row_threads + kBlock - 1) / kBlock;
   const unsigned grid = (unsigned)(row_blocks + A->nr);
   const VT* vv = reinterpret_cast<const VT*>(v);
   const int acc = accumulate ? 1 : 0;
   if (out_fp64)
-    dispatch_k<VT, double>(A->K, grid, s, a, vv, reinterpret_cast<double*>(out), acc);
+    dispatch_r<VT, double>(R, A->K, grid, s, a, vv, reinterpret_cast<double*>(out), acc);
   else
-    dispatch_k<VT, VT>(A->K, grid, s, a, vv, reinterpret_cast<VT*>(out), acc);
+    dispatch_r<VT, VT>(R, A->K, grid, s, a, vv, reinterpret_cast<VT*>(out), acc);
 }
 
 template <typename VT>
@@ -304,9 +362,10 @@ void joint_refresh(nc_matrix_s* A, double t) {
   for (int q = 0; q < tv.n; ++q) tv.r[q] = A->tv_r[q], tv.kind[q] = A->tv_kind[q];
   Progs pr{m->d_code.p, m->d_offs.p, m->d_params.p};
   hipStream_t s = A->stream.s;
-  k_joint_refresh<VT><<<grid_for(A->n, kBlock), kBlock, 0, s>>>(
-      A->n, A->ns, A->nr, A->states.p, A->state_conn.p, m->d_stoich.p, tv, pr, t,
-      reinterpret_cast<VT*>(A->tvd.p), reinterpret_cast<VT*>(A->tvv.p));
+  if (A->n)
+    k_joint_refresh<VT><<<grid_for(A->n, kBlock), kBlock, 0, s>>>(
+        A->n, A->ld, A->ns, A->nr, A->states.p, A->state_conn.p, m->d_stoich.p, tv, pr, t,
+        reinterpret_cast<VT*>(A->tvd.p), reinterpret_cast<VT*>(A->tvv.p));
   for (int r = 0; r < A->nr; ++r) {
     if (m->kinds[r] != NC_PROP_JOINT) continue;
     const int64_t e0 = A->sink_off_h[r], e1 = A->sink_off_h[r + 1];
@@ -323,6 +382,8 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   hipStream_t s = A->stream.s;
   const int64_t n = sp->n;
   const int nr = sp->nr, ns = sp->ns;
+  const int64_t ld = ((n + kLdAlign - 1) / kLdAlign) * kLdAlign;
+  A->ld = ld;
   // groups of time-invariant reactions with identical stoichiometry (merged by sparse())
   Groups g{};
   std::vector<std::vector<int>> groups;
@@ -374,18 +435,18 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   NC_CHECK_HIP(hipStreamSynchronize(s));
   A->K = K;
   const size_t vs = sizeof(VT);
-  A->diag.alloc(std::max<int64_t>(n, 1));
-  A->col.alloc(std::max<int64_t>((int64_t)K * n, 1));
-  A->val.alloc(std::max<int64_t>((int64_t)K * n, 1) * vs);
-  A->tvd.alloc(std::max<int64_t>((int64_t)tv.n * n, 1) * vs);
-  A->tvv.alloc(std::max<int64_t>((int64_t)tv.n * n, 1) * vs);
-  A->tvc.alloc(std::max<int64_t>((int64_t)tv.n * n, 1));
+  A->diag.alloc(std::max<int64_t>(ld, 1));
+  A->col.alloc(std::max<int64_t>((int64_t)K * ld, 1));
+  A->val.alloc(std::max<int64_t>((int64_t)K * ld, 1) * vs);
+  A->tvd.alloc(std::max<int64_t>((int64_t)tv.n * ld, 1) * vs);
+  A->tvv.alloc(std::max<int64_t>((int64_t)tv.n * ld, 1) * vs);
+  A->tvc.alloc(std::max<int64_t>((int64_t)tv.n * ld, 1));
   Progs pr{m->d_code.p, m->d_offs.p, m->d_params.p};
   DevBuf<unsigned long long> d_dmax(1);
   NC_CHECK_HIP(hipMemsetAsync(d_dmax.p, 0, sizeof(unsigned long long), s));
-  if (n)
-    k_build_fill<VT><<<grid_for(n, kBlock), kBlock, 0, s>>>(
-        n, ns, nr, K, sp->states.p, sp->state_conn.p, sp->sink_mask.p, sp->d_stoich.p, g, tv, pr, A->diag.p,
+  if (ld)
+    k_build_fill<VT><<<grid_for(ld, kBlock), kBlock, 0, s>>>(
+        n, ld, ns, nr, K, sp->states.p, sp->state_conn.p, sp->sink_mask.p, sp->d_stoich.p, g, tv, pr, A->diag.p,
         A->col.p, reinterpret_cast<VT*>(A->val.p), reinterpret_cast<VT*>(A->tvd.p), A->tvc.p,
         reinterpret_cast<VT*>(A->tvv.p), d_dmax.p);
   NC_CHECK_HIP(hipGetLastError());
@@ -527,6 +588,30 @@ __global__ void k_from_vt(const VT* in, double* out, int64_t n) {
   if (i < n) out[i] = (double)in[i];
 }
 
+double bench_matvec(nc_matrix_s* A, double t, int reps) {
+  hipStream_t s = A->stream.s;
+  const int64_t N = A->n_rows;
+  DevBuf<unsigned char> v(N * A->vsize), o(N * A->vsize);
+  NC_CHECK_HIP(hipMemsetAsync(v.p, 0, N * A->vsize, s));
+  NC_CHECK_HIP(hipMemsetAsync(o.p, 0, N * A->vsize, s));
+  const bool was = A->timing;
+  A->timing = false;
+  matvec_launch(A, t, v.p, o.p, false, s);  // warm-up (and joint refresh)
+  hipEvent_t e0, e1;
+  NC_CHECK_HIP(hipEventCreate(&e0));
+  NC_CHECK_HIP(hipEventCreate(&e1));
+  NC_CHECK_HIP(hipEventRecord(e0, s));
+  for (int k = 0; k < reps; ++k) matvec_launch(A, t, (k & 1) ? o.p : v.p, (k & 1) ? v.p : o.p, false, s);
+  NC_CHECK_HIP(hipEventRecord(e1, s));
+  NC_CHECK_HIP(hipEventSynchronize(e1));
+  A->timing = was;
+  float ms = 0.f;
+  NC_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
 }  // namespace
 
 extern "C" {
@@ -580,6 +665,31 @@ int nc_matrix_info(nc_matrix A, nc_matrix_info_t* info) {
     info->csr_bytes_per_matvec =
         (double)A->nnz_ref * (sv + 4.0) + (double)(A->n_rows + 1) * 4.0 + 2.0 * (double)A->n_rows * sv;
     info->norm1 = A->norm1;
+    info->rows_per_thread = A->rows_per_thread;
+  });
+}
+
+int nc_matrix_set_rows_per_thread(nc_matrix A, int32_t rows) {
+  return guard([&] {
+    NC_REQUIRE(A, NC_ERR_ARG, "null matrix");
+    NC_REQUIRE(rows == 1 || rows == 2 || rows == 4, NC_ERR_ARG, "rows per thread must be 1, 2 or 4");
+    A->rows_per_thread = rows;
+  });
+}
+
+int nc_matrix_autotune(nc_matrix A, int32_t reps, double* best_ms) {
+  return guard([&] {
+    NC_REQUIRE(A && reps > 0, NC_ERR_ARG, "bad arguments");
+    double best = 1e300;
+    int best_r = A->rows_per_thread;
+    for (int r : {1, 2, 4}) {
+      A->rows_per_thread = r;
+      bench_matvec(A, 0.0, 2);
+      const double ms = bench_matvec(A, 0.0, reps);
+      if (ms < best) best = ms, best_r = r;
+    }
+    A->rows_per_thread = best_r;
+    if (best_ms) *best_ms = best;
   });
 }
 
@@ -610,7 +720,7 @@ int nc_matrix_matvec(nc_matrix A, double t, const double* v, double* out, int32_
 int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t* cols, double* vals) {
   return guard([&] {
     NC_REQUIRE(A && nnz, NC_ERR_ARG, "null argument");
-    const int64_t n = A->n;
+    const int64_t n = A->n, ld = A->ld;
     const int ntv = (int)A->tv_r.size();
     const int64_t total = n * (1 + A->K + 2 * ntv) + A->n_sink_entries;
     *nnz = total;
@@ -633,25 +743,25 @@ int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t
       }
       return out;
     };
-    std::vector<double> diag(n);
-    if (n) NC_CHECK_HIP(hipMemcpy(diag.data(), A->diag.p, 8 * n, hipMemcpyDeviceToHost));
-    std::vector<int32_t> col((size_t)A->K * n), tvc((size_t)ntv * n), src(A->n_sink_entries);
+    std::vector<double> diag(ld);
+    if (ld) NC_CHECK_HIP(hipMemcpy(diag.data(), A->diag.p, 8 * ld, hipMemcpyDeviceToHost));
+    std::vector<int32_t> col((size_t)A->K * ld), tvc((size_t)ntv * ld), src(A->n_sink_entries);
     if (!col.empty()) NC_CHECK_HIP(hipMemcpy(col.data(), A->col.p, 4 * col.size(), hipMemcpyDeviceToHost));
     if (!tvc.empty()) NC_CHECK_HIP(hipMemcpy(tvc.data(), A->tvc.p, 4 * tvc.size(), hipMemcpyDeviceToHost));
     if (!src.empty()) NC_CHECK_HIP(hipMemcpy(src.data(), A->sink_src.p, 4 * src.size(), hipMemcpyDeviceToHost));
-    auto val = fetch(A->val.p, (int64_t)A->K * n);
-    auto tvd = fetch(A->tvd.p, (int64_t)ntv * n);
-    auto tvv = fetch(A->tvv.p, (int64_t)ntv * n);
+    auto val = fetch(A->val.p, (int64_t)A->K * ld);
+    auto tvd = fetch(A->tvd.p, (int64_t)ntv * ld);
+    auto tvv = fetch(A->tvv.p, (int64_t)ntv * ld);
     auto sv = fetch(A->sink_val.p, A->n_sink_entries);
     const nc_model_s* m = A->model;
     int64_t k = 0;
     for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = i, vals[k] = diag[i], ++k;
     for (int s = 0; s < A->K; ++s)
-      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = col[s * n + i], vals[k] = val[s * n + i], ++k;
+      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = col[s * ld + i], vals[k] = val[s * ld + i], ++k;
     for (int q = 0; q < ntv; ++q) {
       const double sc = A->tv_kind[q] == NC_PROP_SEPARABLE ? m->tfactor(A->tv_r[q], t) : 1.0;
-      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = i, vals[k] = sc * tvd[q * n + i], ++k;
-      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = tvc[q * n + i], vals[k] = sc * tvv[q * n + i], ++k;
+      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = i, vals[k] = sc * tvd[q * ld + i], ++k;
+      for (int64_t i = 0; i < n; ++i) rows[k] = i, cols[k] = tvc[q * ld + i], vals[k] = sc * tvv[q * ld + i], ++k;
     }
     for (int r = 0; r < A->nr; ++r) {
       const double sc = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
@@ -664,27 +774,7 @@ int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t
 int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms) {
   return guard([&] {
     NC_REQUIRE(A && mean_ms && reps > 0, NC_ERR_ARG, "bad arguments");
-    hipStream_t s = A->stream.s;
-    const int64_t N = A->n_rows;
-    DevBuf<unsigned char> v(N * A->vsize), o(N * A->vsize);
-    NC_CHECK_HIP(hipMemsetAsync(v.p, 0, N * A->vsize, s));
-    NC_CHECK_HIP(hipMemsetAsync(o.p, 0, N * A->vsize, s));
-    matvec_launch(A, t, v.p, o.p, false, s);  // warm-up (and joint refresh)
-    hipEvent_t e0, e1;
-    NC_CHECK_HIP(hipEventCreate(&e0));
-    NC_CHECK_HIP(hipEventCreate(&e1));
-    const bool was = A->timing;
-    A->timing = false;
-    NC_CHECK_HIP(hipEventRecord(e0, s));
-    for (int k = 0; k < reps; ++k) matvec_launch(A, t, (k & 1) ? o.p : v.p, (k & 1) ? v.p : o.p, false, s);
-    NC_CHECK_HIP(hipEventRecord(e1, s));
-    NC_CHECK_HIP(hipEventSynchronize(e1));
-    A->timing = was;
-    float ms = 0.f;
-    NC_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *mean_ms = ms / reps;
+    *mean_ms = bench_matvec(A, t, reps);
   });
 }
 

@@ -8,6 +8,7 @@ struct nc_solver_s {
   nc_solve_opts opts{};
   int method = NC_METHOD_KRYLOV;
   int64_t N = 0;
+  int64_t Np = 0;  // padded vector length (multiple of 1024); entries N..Np-1 stay zero
   size_t vs = 8;
   double t = 0.0;
   bool have_state = false;
@@ -19,7 +20,6 @@ struct nc_solver_s {
   int32_t n_snaps = 0;
   // reduction scratch
   nc::DevBuf<double> partials;
-  nc::DevBuf<unsigned int> counter;
   unsigned red_blocks = 0;
   // Krylov
   int m = 30;
@@ -27,9 +27,15 @@ struct nc_solver_s {
   nc::DevBuf<double> q, p;      // FP64 work vectors
   nc::DevBuf<double> kbuf;      // H (m+2)^2 | h (m+2) | h2 (m+2) | sc (m+2) | misc (8)
   nc::DevBuf<double> coef;      // combination coefficients (m+2)
-  std::vector<double> hbuf;     // host copy of kbuf
+  double* hbuf = nullptr;       // pinned host copy of kbuf
+  double* hcoef = nullptr;      // pinned host staging of coef
+  double* hpart = nullptr;      // pinned host copy of per-block partial sums
   double beta = 0.0, t_new = 0.0, anorm = 0.0;
+  int64_t reorth_base = 0;  // device reorthogonalisation counter at the last restart
+  bool beta_pending = false;  // beta was updated on the device and not yet read back
   bool started = false;
+  hipGraph_t arnoldi_graph = nullptr;  // captured Arnoldi process (replayed every step)
+  hipGraphExec_t arnoldi_exec = nullptr;
   // RK
   nc::DevBuf<unsigned char> rkv;   // VT copy of the stage input
   nc::DevBuf<double> k;            // 7 x N stage derivatives (FP64)
@@ -37,6 +43,13 @@ struct nc_solver_s {
   double h = 0.0;
   bool fsal_valid = false;
   nc_solve_stats st{};
+  ~nc_solver_s() {
+    if (arnoldi_exec) (void)hipGraphExecDestroy(arnoldi_exec);
+    if (arnoldi_graph) (void)hipGraphDestroy(arnoldi_graph);
+    if (hbuf) (void)hipHostFree(hbuf);
+    if (hcoef) (void)hipHostFree(hcoef);
+    if (hpart) (void)hipHostFree(hpart);
+  }
 };
 
 namespace nc {

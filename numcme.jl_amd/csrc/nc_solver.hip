@@ -3,9 +3,11 @@
 // Replaces the integrator loop of NumCME.jl's solve (src/transientcme/sparse/fspsolve.jl:
 // 10-41, a DifferentialEquations.jl/CVODE call around matvec!) with:
 //
-//  * Krylov expv for time-invariant generators: Arnoldi with classical Gram-Schmidt and
-//    one full reorthogonalisation (CGS2), FP64 dot products and coefficients, basis
-//    vectors stored in the matrix precision; the (m+2)x(m+2) Hessenberg exponential,
+//  * Krylov expv for time-invariant generators: Arnoldi with classical Gram-Schmidt and a
+//    second Gram-Schmidt pass when the DGKS test asks for it (J. W. Daniel, W. B. Gragg,
+//    L. Kaufman & G. W. Stewart, Math. Comp. 30, 1976; decided on the device), FP64 dot
+//    products and coefficients, basis vectors stored in the matrix precision; the
+//    (m+2)x(m+2) Hessenberg exponential,
 //    the local error estimate and the step-size control run on the host, following
 //    R. B. Sidje, "Expokit: a software package for computing matrix exponentials",
 //    ACM TOMS 24(1), 1998 (algorithm of its EXPV routine). One host synchronisation
@@ -26,8 +28,11 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxRedBlocks = 1024;
+constexpr int kPartStride = 33;  // doubles of partial sums per block (up to 32 dots + a norm)
 
-enum Finish : int { FIN_NONE = 0, FIN_H1, FIN_H2, FIN_COL, FIN_AVN, FIN_BETA, FIN_SUMS };
+enum Finish : int { FIN_NONE = 0, FIN_H1, FIN_UPD, FIN_H2, FIN_COL, FIN_AVN, FIN_BETA, FIN_SUMS };
+constexpr double kBreakdownTol = 1e-7;   // Expokit's happy-breakdown threshold
+constexpr double kDgksEta = 0.70710678118654752;  // reorthogonalise when |x1| < eta |x0|
 
 // kbuf layout (doubles)
 struct KLayout {
@@ -39,7 +44,7 @@ struct KLayout {
   __host__ __device__ int misc(int i) const { return (m + 2) * (m + 2) + 3 * (m + 2) + i; }
   __host__ __device__ int size() const { return (m + 2) * (m + 2) + 3 * (m + 2) + 8; }
 };
-enum { MISC_AVN = 0, MISC_BETA = 1 };
+enum { MISC_AVN = 0, MISC_BETA = 1, MISC_S0 = 2, MISC_FLAG = 3, MISC_NREORTH = 4 };
 
 template <typename BT>
 struct VecArgs {
@@ -57,7 +62,7 @@ struct VecArgs {
   int vout_fp32;          // 1: vout is float, 0: double
   int dots, norm;
   double* partials;
-  unsigned int* counter;
+  const double* cond;     // device flag: when non-null and zero the pass is skipped
   int finish;
   double* kb;
   KLayout L;
@@ -70,64 +75,101 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Partial sums are stored value-major, partials[k * nblocks + b], so the finish kernel's
+// loads of one value across all blocks are contiguous.
+//
 // x = xscale*xin + csign * sum_i coef[i]*sc[i]*V_i ; optional stores; optional dots
-// sum_n (sc[i] V_i[n]) x[n] and norm sum_n x[n]^2, reduced across the grid by the last
-// block to finish (deterministic fixed-order reduction).
-template <typename BT, int MAXV>
+// d_i = sc[i] * sum_n V_i[n] x[n] and the norm sum_n x[n]^2 as per-block partial sums.
+// MODE picks what a pass needs: VM_DOTS (x = xscale*xin; dots) or VM_UPDATE (update, no
+// dots). Each thread owns 4 consecutive rows and reads every stream with one 16-byte
+// (FP32) or 32-byte (FP64) vector load per vector; N is a multiple of 4 and all vectors
+// are zero beyond the true length, so no tail handling is needed.
+enum VecMode : int { VM_DOTS = 0, VM_UPDATE = 1 };
+
+template <typename T, int R>
+struct alignas(sizeof(T) * R) Pack {
+  T v[R];
+};
+
+template <typename BT, int MAXV, int MODE>
 __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
+  constexpr int MV = MAXV > 0 ? MAXV : 1;
+  constexpr bool DOTS = MODE == VM_DOTS;
+  constexpr int R = 4;
+  if (a.cond && *a.cond == 0.0) return;  // uniform: the whole grid skips the pass
   const int nv = a.nv;
-  double cs[MAXV > 0 ? MAXV : 1], sci[MAXV > 0 ? MAXV : 1];
-#pragma unroll
-  for (int i = 0; i < MAXV; ++i) {
-    if (i < nv) {
-      sci[i] = a.sc ? a.sc[i] : 1.0;
-      cs[i] = a.coef ? a.csign * a.coef[i] * sci[i] : 0.0;
-    } else {
-      sci[i] = 0.0, cs[i] = 0.0;
-    }
+  // per-vector coefficients live in LDS (uniform broadcast reads)
+  __shared__ double s_cs[MV], s_sc[MV];
+  if (threadIdx.x < MV) {
+    const int i = threadIdx.x;
+    const double sc = (i < nv) ? (a.sc ? a.sc[i] : 1.0) : 0.0;
+    s_sc[i] = sc;
+    s_cs[i] = (i < nv && a.coef) ? a.csign * a.coef[i] * sc : 0.0;
   }
+  __syncthreads();
   const double xs = a.xscale ? *a.xscale : 1.0;
-  double dacc[MAXV > 0 ? MAXV : 1];
+  double dacc[DOTS ? MV : 1];
 #pragma unroll
-  for (int i = 0; i < MAXV; ++i) dacc[i] = 0.0;
+  for (int i = 0; i < (DOTS ? MV : 1); ++i) dacc[i] = 0.0;
   double nacc = 0.0;
+  const int64_t groups = a.N / R;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < a.N; n += stride) {
-    double vloc[MAXV > 0 ? MAXV : 1];
-    double x = a.xin ? xs * a.xin[n] : 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += stride) {
+    const int64_t n0 = g * R;
+    double x[R];
+    if (a.xin) {
+      const Pack<double, R> xi = *reinterpret_cast<const Pack<double, R>*>(a.xin + n0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = xs * xi.v[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = 0.0;
+    }
 #pragma unroll
     for (int i = 0; i < MAXV; ++i) {
       if (i < nv) {
-        vloc[i] = (double)a.V[(int64_t)i * a.N + n];
-        if (a.coef) x += cs[i] * vloc[i];
+        const Pack<BT, R> vv = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)i * a.N + n0);
+        if constexpr (DOTS) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) dacc[i] += (double)vv.v[r] * x[r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) x[r] += s_cs[i] * (double)vv.v[r];
+        }
       }
     }
-    if (a.xout) a.xout[n] = x;
-    if (a.xout2) a.xout2[n] = x;
-    if (a.vout) {
-      if (a.vout_fp32)
-        reinterpret_cast<float*>(a.vout)[n] = (float)x;
-      else
-        reinterpret_cast<double*>(a.vout)[n] = x;
-    }
-    if (a.dots) {
+    if (a.xout) {
+      Pack<double, R> o;
 #pragma unroll
-      for (int i = 0; i < MAXV; ++i)
-        if (i < nv) dacc[i] += sci[i] * vloc[i] * x;
+      for (int r = 0; r < R; ++r) o.v[r] = x[r];
+      *reinterpret_cast<Pack<double, R>*>(a.xout + n0) = o;
     }
-    nacc += x * x;
+    if (a.vout) {
+      if (a.vout_fp32) {
+        Pack<float, R> o;
+#pragma unroll
+        for (int r = 0; r < R; ++r) o.v[r] = (float)x[r];
+        *reinterpret_cast<Pack<float, R>*>(reinterpret_cast<float*>(a.vout) + n0) = o;
+      } else {
+        Pack<double, R> o;
+#pragma unroll
+        for (int r = 0; r < R; ++r) o.v[r] = x[r];
+        *reinterpret_cast<Pack<double, R>*>(reinterpret_cast<double*>(a.vout) + n0) = o;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) nacc += x[r] * x[r];
   }
   if (a.finish == FIN_NONE) return;
-  // block reduction of nv dots + 1 norm
-  __shared__ double red[(MAXV > 0 ? MAXV : 1) + 1][kBlock / 64];
-  __shared__ bool last;
+  // block reduction of the dots (scaled by sc[i] here) and the norm
+  __shared__ double red[MV + 1][kBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nvals = (a.dots ? nv : 0) + 1;
-  if (a.dots) {
+  const int nvals = (DOTS ? nv : 0) + 1;
+  if constexpr (DOTS) {
 #pragma unroll
     for (int i = 0; i < MAXV; ++i) {
       if (i < nv) {
-        double v = wave_sum(dacc[i]);
+        double v = wave_sum(dacc[i]) * s_sc[i];
         if (lane == 0) red[i][wave] = v;
       }
     }
@@ -137,36 +179,61 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
     if (lane == 0) red[nvals - 1][wave] = v;
   }
   __syncthreads();
-  const int pstride = MAXV + 1;
   if (threadIdx.x < nvals) {
     double s = 0.0;
     for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
-    a.partials[(int64_t)blockIdx.x * pstride + threadIdx.x] = s;
+    a.partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  __shared__ double tot[(MAXV > 0 ? MAXV : 1) + 1];
-  if (threadIdx.x < nvals) {
+  // k_vec_finish folds the partials in the next launch: the kernel boundary makes them
+  // visible across XCDs without a per-block L2 writeback fence
+}
+
+// One block: fold the per-block partials (one wave per value, fixed lane->block
+// assignment and a fixed shuffle tree, so the result is deterministic), then apply the
+// finishing step.
+template <typename BT>
+__global__ __launch_bounds__(kBlock) void k_vec_finish(VecArgs<BT> a, unsigned nblocks, int nvals) {
+  if (a.cond && *a.cond == 0.0) return;
+  const int nv = a.nv;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ double tot[kPartStride];
+  constexpr int kPer = kMaxRedBlocks / 64;  // partials per lane, all loads issued together
+  for (int k = wave; k < nvals; k += kBlock / 64) {
+    double v[kPer];
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+      const unsigned b = lane + 64u * t;
+      v[t] = b < nblocks ? a.partials[(int64_t)k * nblocks + b] : 0.0;
+    }
     double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
-      s += __hip_atomic_load(&a.partials[(int64_t)b * pstride + threadIdx.x], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    tot[threadIdx.x] = s;
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) s += v[t];
+    s = wave_sum(s);
+    if (lane == 0) tot[k] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    *a.counter = 0u;
     const KLayout& L = a.L;
     double* kb = a.kb;
     const double nrm2 = tot[nvals - 1];
     switch (a.finish) {
       case FIN_H1:
         for (int i = 0; i < nv; ++i) kb[L.h(i)] = tot[i];
+        kb[L.misc(MISC_S0)] = nrm2;
         break;
+      case FIN_UPD: {
+        // classical Gram-Schmidt done; reorthogonalise only if |x1| < eta |x0| (DGKS)
+        const double s1 = sqrt(nrm2), s0 = sqrt(kb[L.misc(MISC_S0)]);
+        const bool again = s1 >= a.btol && s1 < kDgksEta * s0;
+        kb[L.misc(MISC_FLAG)] = again ? 1.0 : 0.0;
+        if (again) {
+          kb[L.misc(MISC_NREORTH)] += 1.0;
+        } else {
+          for (int i = 0; i < nv; ++i) kb[L.H(i, a.j)] = kb[L.h(i)];
+          kb[L.H(a.j + 1, a.j)] = s1;
+          kb[L.sc(a.j + 1)] = s1 > a.btol ? 1.0 / s1 : 0.0;
+        }
+      } break;
       case FIN_H2:
         for (int i = 0; i < nv; ++i) kb[L.h2(i)] = tot[i];
         break;
@@ -175,6 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
         for (int i = 0; i < nv; ++i) kb[L.H(i, a.j)] = kb[L.h(i)] + kb[L.h2(i)];
         kb[L.H(a.j + 1, a.j)] = s;
         kb[L.sc(a.j + 1)] = s > a.btol ? 1.0 / s : 0.0;
+        kb[L.misc(MISC_FLAG)] = 0.0;
       } break;
       case FIN_AVN:
         kb[L.misc(MISC_AVN)] = sqrt(nrm2);
@@ -192,21 +260,34 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   }
 }
 
-template <typename BT>
-void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
+template <typename BT, int MODE>
+void launch_vec_mode(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
   const int nv = a.nv;
   if (nv <= 0)
-    k_vec<BT, 0><<<grid, kBlock, 0, s>>>(a);
+    k_vec<BT, 0, MODE><<<grid, kBlock, 0, s>>>(a);
   else if (nv <= 4)
-    k_vec<BT, 4><<<grid, kBlock, 0, s>>>(a);
+    k_vec<BT, 4, MODE><<<grid, kBlock, 0, s>>>(a);
   else if (nv <= 8)
-    k_vec<BT, 8><<<grid, kBlock, 0, s>>>(a);
+    k_vec<BT, 8, MODE><<<grid, kBlock, 0, s>>>(a);
   else if (nv <= 16)
-    k_vec<BT, 16><<<grid, kBlock, 0, s>>>(a);
+    k_vec<BT, 16, MODE><<<grid, kBlock, 0, s>>>(a);
+  else if (nv <= 24)
+    k_vec<BT, 24, MODE><<<grid, kBlock, 0, s>>>(a);
   else if (nv <= 32)
-    k_vec<BT, 32><<<grid, kBlock, 0, s>>>(a);
+    k_vec<BT, 32, MODE><<<grid, kBlock, 0, s>>>(a);
   else
     throw Error(NC_ERR_UNSUPPORTED, "Krylov dimension above 31 is not supported");
+}
+
+template <typename BT>
+void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
+  if (a.dots && a.coef) throw Error(NC_ERR_INTERNAL, "a vector pass cannot both update and take dot products");
+  if (a.N % 4) throw Error(NC_ERR_INTERNAL, "vector passes need a padded length");
+  if (a.dots)
+    launch_vec_mode<BT, VM_DOTS>(a, grid, s);
+  else
+    launch_vec_mode<BT, VM_UPDATE>(a, grid, s);
+  if (a.finish != FIN_NONE) k_vec_finish<BT><<<1, kBlock, 0, s>>>(a, grid, (a.dots ? a.nv : 0) + 1);
   NC_CHECK_HIP(hipGetLastError());
 }
 
@@ -238,10 +319,10 @@ __global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, int stage, doubl
   }
 }
 
-// Error norm sum_n (h*sum_j e_j k_j / (atol + rtol*max(|y|,|ynew|)))^2 with ynew = ytmp.
+// Error norm sum_n (h*sum_j e_j k_j / (atol + rtol*max(|y|,|ynew|)))^2 with ynew = ytmp;
+// one partial sum per block (folded on the host, which reads them back anyway).
 __global__ __launch_bounds__(kBlock) void k_rk_err(int64_t N, double h, double atol, double rtol, const double* y,
-                                                   const double* ynew, const double* k, double* partials,
-                                                   unsigned int* counter, double* out) {
+                                                   const double* ynew, const double* k, double* partials) {
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) {
@@ -253,7 +334,6 @@ __global__ __launch_bounds__(kBlock) void k_rk_err(int64_t N, double h, double a
     acc += r * r;
   }
   __shared__ double red[kBlock / 64];
-  __shared__ bool last;
   double v = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
@@ -261,17 +341,6 @@ __global__ __launch_bounds__(kBlock) void k_rk_err(int64_t N, double h, double a
     double s = 0.0;
     for (int w = 0; w < kBlock / 64; ++w) s += red[w];
     partials[blockIdx.x] = s;
-    __threadfence();
-    last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
-      s += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *out = s;
-    *counter = 0u;
   }
 }
 
@@ -296,24 +365,125 @@ double round_step(double x) {
 }  // namespace
 
 void solver_reduce_setup(nc_solver_s* S, int nvals) {
-  S->red_blocks = std::min<unsigned>(kMaxRedBlocks, grid_for(S->N, kBlock));
-  S->partials.reserve((size_t)S->red_blocks * (size_t)(nvals + 1));
-  if (!S->counter.p) {
-    S->counter.alloc(1);
-    NC_CHECK_HIP(hipMemsetAsync(S->counter.p, 0, sizeof(unsigned int), S->s));
-  }
+  (void)nvals;
+  S->red_blocks = std::min<unsigned>(kMaxRedBlocks, grid_for(S->Np / 4, kBlock));
+  S->partials.reserve((size_t)S->red_blocks * kPartStride);
+  if (!S->hpart)
+    NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hpart), sizeof(double) * kMaxRedBlocks * kPartStride,
+                               hipHostMallocDefault));
 }
 
 // ---- Krylov expv ---------------------------------------------------------------------
+// The device work of one Arnoldi process (m mat-vecs, CGS2 and the |A v_m| norm). Its
+// launches and arguments are identical for every Krylov step, so it is captured once
+// into a HIP graph and replayed.
+template <typename BT>
+static void arnoldi_iteration(nc_solver_s* S, int j) {
+  const int64_t N = S->Np;
+  const KLayout L{S->m};
+  hipStream_t s = S->s;
+  auto Vp = [&](int i) { return reinterpret_cast<BT*>(S->V.p + (size_t)i * N * S->vs); };
+  matvec_launch(S->A, 0.0, Vp(j), S->q.p, false, s, true);
+  VecArgs<BT> a{};
+  a.N = N;
+  a.nv = j + 1;
+  a.V = Vp(0);
+  a.sc = S->kbuf.p + L.sc(0);
+  a.partials = S->partials.p;
+  a.kb = S->kbuf.p;
+  a.L = L;
+  a.j = j;
+  a.btol = kBreakdownTol;
+  a.vout_fp32 = std::is_same<BT, float>::value ? 1 : 0;
+  // pass 1: h = V_{0..j}^T x0 and |x0| with x0 = sc_j q
+  a.xin = S->q.p;
+  a.xscale = S->kbuf.p + L.sc(j);
+  a.dots = 1;
+  a.finish = FIN_H1;
+  launch_vec(a, S->red_blocks, s);
+  // pass 2: x1 = x0 - V h, kept in FP64 (p) and as V_{j+1}; DGKS test on |x1| / |x0|
+  a.dots = 0;
+  a.coef = S->kbuf.p + L.h(0);
+  a.csign = -1.0;
+  a.xout = S->p.p;
+  a.vout = Vp(j + 1);
+  a.finish = FIN_UPD;
+  launch_vec(a, S->red_blocks, s);
+  // passes 3-4 run only when the test asked for a second orthogonalisation
+  a.cond = S->kbuf.p + L.misc(MISC_FLAG);
+  a.xin = S->p.p;
+  a.xscale = nullptr;
+  a.coef = nullptr;
+  a.xout = nullptr;
+  a.vout = nullptr;
+  a.dots = 1;
+  a.finish = FIN_H2;
+  launch_vec(a, S->red_blocks, s);
+  a.dots = 0;
+  a.coef = S->kbuf.p + L.h2(0);
+  a.vout = Vp(j + 1);
+  a.finish = FIN_COL;
+  launch_vec(a, S->red_blocks, s);
+}
+
+// The device work of one Arnoldi process (m mat-vecs, Gram-Schmidt and the |A v_m|
+// norm). Its launches and arguments are identical for every Krylov step, so it is
+// captured once into a HIP graph and replayed.
+static void arnoldi_launch(nc_solver_s* S) {
+  const KLayout L{S->m};
+  // V_0 (raw) and sc[0] = 1/beta are already on the device
+  for (int j = 0; j < S->m; ++j) {
+    if (S->A->precision == NC_FP32)
+      arnoldi_iteration<float>(S, j);
+    else
+      arnoldi_iteration<double>(S, j);
+  }
+  // avnorm = |A v_{m+1}|
+  matvec_launch(S->A, 0.0, S->V.p + (size_t)S->m * S->Np * S->vs, S->q.p, false, S->s, true);
+  VecArgs<double> a{};
+  a.N = S->Np;
+  a.nv = 0;
+  a.xin = S->q.p;
+  a.xscale = S->kbuf.p + L.sc(S->m);
+  a.partials = S->partials.p;
+  a.kb = S->kbuf.p;
+  a.L = L;
+  a.finish = FIN_AVN;
+  launch_vec(a, S->red_blocks, S->s);
+}
+
+static void arnoldi_run(nc_solver_s* S) {
+  const bool use_graph = !S->A->timing && S->opts.no_graphs == 0;
+  if (!use_graph) {
+    arnoldi_launch(S);
+    return;
+  }
+  if (!S->arnoldi_exec) {
+    hipGraph_t g = nullptr;
+    NC_CHECK_HIP(hipStreamBeginCapture(S->s, hipStreamCaptureModeThreadLocal));
+    try {
+      arnoldi_launch(S);
+    } catch (...) {
+      (void)hipStreamEndCapture(S->s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    NC_CHECK_HIP(hipStreamEndCapture(S->s, &g));
+    NC_CHECK_HIP(hipGraphInstantiate(&S->arnoldi_exec, g, nullptr, nullptr, 0));
+    S->arnoldi_graph = g;
+  }
+  NC_CHECK_HIP(hipGraphLaunch(S->arnoldi_exec, S->s));
+}
+
 void krylov_advance(nc_solver_s* S, double t_target) {
   nc_matrix_s* A = S->A;
   const int m = S->m;
-  const int64_t N = S->N;
+  const int64_t N = S->Np;
   const KLayout L{m};
   hipStream_t s = S->s;
   const bool fp32 = A->precision == NC_FP32;
   const double tol = S->opts.krylov_tol;
-  const double btol = 1e-7, gamma = 0.9, delta = 1.2;
+  const double btol = kBreakdownTol, gamma = 0.9, delta = 1.2;
   const int mxrej = 10;
   auto Vp = [&](int i) -> unsigned char* { return S->V.p + (size_t)i * N * S->vs; };
   if (!S->started) {
@@ -328,97 +498,23 @@ void krylov_advance(nc_solver_s* S, double t_target) {
   while (S->t < t_target - t_eps) {
     if (S->opts.max_steps && S->st.steps >= S->opts.max_steps)
       throw Error(NC_ERR_NUMERIC, "Krylov integrator exceeded max_steps");
-    if (S->beta == 0.0) {  // zero vector stays zero
+    if (S->beta == 0.0 && !S->beta_pending) {  // zero vector stays zero
       S->t = t_target;
       break;
     }
     double t_step = std::min(t_target - S->t, S->t_new);
-    // Arnoldi: V_0 (raw) and sc[0] = 1/beta are already on the device
-    for (int j = 0; j < m; ++j) {
-      matvec_launch(A, 0.0, Vp(j), S->q.p, false, s, true);
-      S->st.matvecs++;
-      // pass 1: h = V_{0..j}^T x0, x0 = sc_j q
-      VecArgs<float> af{};
-      VecArgs<double> ad{};
-      auto common = [&](auto& a) {
-        a.N = N;
-        a.nv = j + 1;
-        a.sc = S->kbuf.p + L.sc(0);
-        a.partials = S->partials.p;
-        a.counter = S->counter.p;
-        a.kb = S->kbuf.p;
-        a.L = L;
-        a.j = j;
-        a.btol = btol;
-        a.vout_fp32 = fp32 ? 1 : 0;
-      };
-      if (fp32) {
-        common(af);
-        af.V = reinterpret_cast<const float*>(Vp(0));
-        af.xin = S->q.p;
-        af.xscale = S->kbuf.p + L.sc(j);
-        af.dots = 1;
-        af.finish = FIN_H1;
-        launch_vec(af, S->red_blocks, s);
-        // pass 2: x1 = x0 - V h ; h2 = V^T x1
-        af.coef = S->kbuf.p + L.h(0);
-        af.csign = -1.0;
-        af.xout = S->p.p;
-        af.finish = FIN_H2;
-        launch_vec(af, S->red_blocks, s);
-        // x2 = x1 - V h2 ; V_{j+1} = x2 ; s = |x2|
-        af.xin = S->p.p;
-        af.xscale = nullptr;
-        af.coef = S->kbuf.p + L.h2(0);
-        af.xout = nullptr;
-        af.vout = Vp(j + 1);
-        af.dots = 0;
-        af.finish = FIN_COL;
-        launch_vec(af, S->red_blocks, s);
-      } else {
-        common(ad);
-        ad.V = reinterpret_cast<const double*>(Vp(0));
-        ad.xin = S->q.p;
-        ad.xscale = S->kbuf.p + L.sc(j);
-        ad.dots = 1;
-        ad.finish = FIN_H1;
-        launch_vec(ad, S->red_blocks, s);
-        ad.coef = S->kbuf.p + L.h(0);
-        ad.csign = -1.0;
-        ad.xout = S->p.p;
-        ad.finish = FIN_H2;
-        launch_vec(ad, S->red_blocks, s);
-        ad.xin = S->p.p;
-        ad.xscale = nullptr;
-        ad.coef = S->kbuf.p + L.h2(0);
-        ad.xout = nullptr;
-        ad.vout = Vp(j + 1);
-        ad.dots = 0;
-        ad.finish = FIN_COL;
-        launch_vec(ad, S->red_blocks, s);
-      }
-    }
-    // avnorm = |A v_{m+1}|
-    matvec_launch(A, 0.0, Vp(m), S->q.p, false, s, true);
-    S->st.matvecs++;
-    {
-      VecArgs<double> a{};
-      a.N = N;
-      a.nv = 0;
-      a.xin = S->q.p;
-      a.xscale = S->kbuf.p + L.sc(m);
-      a.partials = S->partials.p;
-      a.counter = S->counter.p;
-      a.kb = S->kbuf.p;
-      a.L = L;
-      a.finish = FIN_AVN;
-      launch_vec(a, S->red_blocks, s);
-    }
-    NC_CHECK_HIP(hipMemcpyAsync(S->hbuf.data(), S->kbuf.p, sizeof(double) * L.size(), hipMemcpyDeviceToHost, s));
+    arnoldi_run(S);
+    S->st.matvecs += m + 1;
+    NC_CHECK_HIP(hipMemcpyAsync(S->hbuf, S->kbuf.p, sizeof(double) * L.size(), hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
     S->st.host_syncs++;
     matvec_harvest_timing(A);
-    const double* hb = S->hbuf.data();
+    const double* hb = S->hbuf;
+    S->st.reorthogonalizations = (int64_t)hb[L.misc(MISC_NREORTH)] - S->reorth_base;
+    if (S->beta_pending) {
+      S->beta = hb[L.misc(MISC_BETA)];
+      S->beta_pending = false;
+    }
     // happy breakdown detection
     int mb = m, k1 = 2;
     for (int j = 0; j < m; ++j) {
@@ -466,10 +562,11 @@ void krylov_advance(nc_solver_s* S, double t_target) {
     }
     const int mx = mb + std::max(0, k1 - 1);
     // w_new = c0/beta * w + sum_{i>=1} c_i sc_i V_i ; c = beta * F[:,0]
-    std::vector<double> coef(m + 2, 0.0);
+    double* coef = S->hcoef;  // pinned; the previous upload completed at the last sync
+    std::fill(coef, coef + m + 2, 0.0);
     for (int i = 1; i < mx; ++i) coef[i - 1] = S->beta * F[(size_t)i * mxf + 0];
     coef[m + 1] = F[0];  // (beta*F00)/beta
-    NC_CHECK_HIP(hipMemcpyAsync(S->coef.p, coef.data(), sizeof(double) * (m + 2), hipMemcpyHostToDevice, s));
+    NC_CHECK_HIP(hipMemcpyAsync(S->coef.p, coef, sizeof(double) * (m + 2), hipMemcpyHostToDevice, s));
     S->t += t_step;
     const bool hit = S->t >= t_target - t_eps;
     if (hit) S->t = t_target;
@@ -487,7 +584,6 @@ void krylov_advance(nc_solver_s* S, double t_target) {
         a.vout = Vp(0);
         a.vout_fp32 = fp32 ? 1 : 0;
         a.partials = S->partials.p;
-        a.counter = S->counter.p;
         a.kb = S->kbuf.p;
         a.L = L;
         a.finish = FIN_BETA;
@@ -502,12 +598,11 @@ void krylov_advance(nc_solver_s* S, double t_target) {
         launch_vec(a, S->red_blocks, s);
       }
     }
-    // the new beta is read back with the next H download; keep a host copy current
-    double bnew = 0.0;
-    NC_CHECK_HIP(hipMemcpyAsync(&bnew, S->kbuf.p + L.misc(MISC_BETA), sizeof(double), hipMemcpyDeviceToHost, s));
-    NC_CHECK_HIP(hipStreamSynchronize(s));
-    S->beta = bnew;
+    // the new beta (written on the device by the combination) arrives with the next H
+    // download, so no synchronisation is needed here
+    S->beta_pending = true;
     S->st.steps++;
+    err_loc = std::max(err_loc, 1e-300);
     S->st.error_estimate += std::max(err_loc, S->anorm * 2.220446049250313e-16);
     if (k1 != 0) S->t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
     if (hit) break;
@@ -517,7 +612,7 @@ void krylov_advance(nc_solver_s* S, double t_target) {
 // ---- Dormand-Prince 5(4) ---------------------------------------------------------------
 void rk_advance(nc_solver_s* S, double t_target) {
   nc_matrix_s* A = S->A;
-  const int64_t N = S->N;
+  const int64_t N = S->Np;  // padded: rows beyond the true length stay zero
   hipStream_t s = S->s;
   const bool fp32 = A->precision == NC_FP32;
   const double rtol = S->opts.rtol, atol = S->opts.atol;
@@ -544,7 +639,6 @@ void rk_advance(nc_solver_s* S, double t_target) {
     a.nv = 0;
     a.xin = kp(0);
     a.partials = S->partials.p;
-    a.counter = S->counter.p;
     a.kb = S->kbuf.p;
     a.L = KLayout{S->m};
     a.finish = FIN_SUMS;
@@ -573,14 +667,15 @@ void rk_advance(nc_solver_s* S, double t_target) {
       S->st.matvecs++;
     }
     // after stage 6 the stage input is y + h*sum b_j k_j (row 7 of A): ytmp = y_{n+1}
-    k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->partials.p, S->counter.p,
-                                               S->kbuf.p);
+    k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->partials.p);
     NC_CHECK_HIP(hipGetLastError());
-    NC_CHECK_HIP(hipMemcpyAsync(&errsum, S->kbuf.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipMemcpyAsync(S->hpart, S->partials.p, sizeof(double) * S->red_blocks, hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
+    errsum = 0.0;
+    for (unsigned b = 0; b < S->red_blocks; ++b) errsum += S->hpart[b];
     S->st.host_syncs++;
     matvec_harvest_timing(A);
-    const double err = std::sqrt(errsum / (double)N);
+    const double err = std::sqrt(errsum / (double)S->N);
     if (!std::isfinite(err)) throw Error(NC_ERR_NUMERIC, "non-finite RK error estimate");
     double fac = err > 0 ? 0.9 * std::pow(err, -0.2) : 5.0;
     if (err <= 1.0) {
@@ -633,22 +728,31 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
       S->method = method;
       S->m = S->opts.krylov_m;
       A->timing = S->opts.time_kernels != 0;
-      S->u.alloc(S->N);
+      S->Np = (S->N + 1023) / 1024 * 1024;
+      S->u.alloc(S->Np);
+      NC_CHECK_HIP(hipMemsetAsync(S->u.p, 0, sizeof(double) * S->Np, S->s));
       const KLayout L{S->m};
       S->kbuf.alloc(L.size());
       NC_CHECK_HIP(hipMemsetAsync(S->kbuf.p, 0, sizeof(double) * L.size(), S->s));
-      S->hbuf.assign(L.size(), 0.0);
+      NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hbuf), sizeof(double) * L.size(), hipHostMallocDefault));
+      NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hcoef), sizeof(double) * (S->m + 2), hipHostMallocDefault));
       // k_vec writes partials with a stride of MAXV+1 <= 33 doubles per block
       solver_reduce_setup(S, 32);
       if (method == NC_METHOD_KRYLOV) {
-        S->V.alloc((size_t)(S->m + 1) * S->N * S->vs);
-        S->q.alloc(S->N);
-        S->p.alloc(S->N);
+        S->V.alloc((size_t)(S->m + 1) * S->Np * S->vs);
+        S->q.alloc(S->Np);
+        S->p.alloc(S->Np);
+        NC_CHECK_HIP(hipMemsetAsync(S->V.p, 0, (size_t)(S->m + 1) * S->Np * S->vs, S->s));
+        NC_CHECK_HIP(hipMemsetAsync(S->q.p, 0, sizeof(double) * S->Np, S->s));
+        NC_CHECK_HIP(hipMemsetAsync(S->p.p, 0, sizeof(double) * S->Np, S->s));
         S->coef.alloc(S->m + 2);
       } else {
-        S->rkv.alloc(S->N * S->vs);
-        S->k.alloc((size_t)7 * S->N);
-        S->ytmp.alloc(S->N);
+        S->rkv.alloc(S->Np * S->vs);
+        S->k.alloc((size_t)7 * S->Np);
+        S->ytmp.alloc(S->Np);
+        NC_CHECK_HIP(hipMemsetAsync(S->rkv.p, 0, S->Np * S->vs, S->s));
+        NC_CHECK_HIP(hipMemsetAsync(S->k.p, 0, sizeof(double) * 7 * S->Np, S->s));
+        NC_CHECK_HIP(hipMemsetAsync(S->ytmp.p, 0, sizeof(double) * S->Np, S->s));
       }
       NC_CHECK_HIP(hipStreamSynchronize(S->s));
     } catch (...) {
@@ -662,24 +766,26 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
 // Re-initialise integrator state from the device copy u0 (time t0, norm beta0).
 static void solver_reinit(nc_solver_s* S) {
   {
-    const unsigned gc = std::min<unsigned>(4096, grid_for(S->N, kBlock));
-    k_copy_d<<<gc, kBlock, 0, S->s>>>(S->u0.p, S->u.p, S->N);
+    const unsigned gc = std::min<unsigned>(4096, grid_for(S->Np, kBlock));
+    k_copy_d<<<gc, kBlock, 0, S->s>>>(S->u0.p, S->u.p, S->Np);
     S->t = S->t0;
     S->have_state = true;
     S->fsal_valid = false;
     S->started = false;
     S->h = 0.0;
+    S->reorth_base += S->st.reorthogonalizations;
     S->st = nc_solve_stats{};
     if (S->method == NC_METHOD_KRYLOV) {
       S->beta = S->beta0;
+      S->beta_pending = false;
       const KLayout L{S->m};
       const double sc0 = S->beta > 0 ? 1.0 / S->beta : 0.0;
       NC_CHECK_HIP(hipMemcpyAsync(S->kbuf.p + L.sc(0), &sc0, sizeof(double), hipMemcpyHostToDevice, S->s));
-      const unsigned g = grid_for(S->N, kBlock);
+      const unsigned g = grid_for(S->Np, kBlock);
       if (S->A->precision == NC_FP32)
-        k_to_vt<float><<<std::min(g, 4096u), kBlock, 0, S->s>>>(S->u.p, reinterpret_cast<float*>(S->V.p), S->N);
+        k_to_vt<float><<<std::min(g, 4096u), kBlock, 0, S->s>>>(S->u.p, reinterpret_cast<float*>(S->V.p), S->Np);
       else
-        k_to_vt<double><<<std::min(g, 4096u), kBlock, 0, S->s>>>(S->u.p, reinterpret_cast<double*>(S->V.p), S->N);
+        k_to_vt<double><<<std::min(g, 4096u), kBlock, 0, S->s>>>(S->u.p, reinterpret_cast<double*>(S->V.p), S->Np);
     }
     NC_CHECK_HIP(hipGetLastError());
   }
@@ -687,7 +793,10 @@ static void solver_reinit(nc_solver_s* S) {
 
 static void solver_set_state_impl(nc_solver_s* S, double t0, const double* u) {
   NC_REQUIRE(S && u, NC_ERR_ARG, "null argument");
-  if (!S->u0.p) S->u0.alloc(S->N);
+  if (!S->u0.p) {
+    S->u0.alloc(S->Np);
+    NC_CHECK_HIP(hipMemsetAsync(S->u0.p, 0, sizeof(double) * S->Np, S->s));
+  }
   NC_CHECK_HIP(hipMemcpyAsync(S->u0.p, u, sizeof(double) * S->N, hipMemcpyHostToDevice, S->s));
   double b2 = 0.0;
   for (int64_t i = 0; i < S->N; ++i) b2 += u[i] * u[i];

@@ -64,6 +64,8 @@ class nc_matrix_info_t(C.Structure):
         ("bytes_per_matvec", C.c_double),
         ("csr_bytes_per_matvec", C.c_double),
         ("norm1", C.c_double),
+        ("rows_per_thread", C.c_int32),
+        ("pad0", C.c_int32),
     ]
 
 
@@ -77,7 +79,7 @@ class nc_solve_opts(C.Structure):
         ("h0", C.c_double),
         ("max_steps", C.c_int64),
         ("time_kernels", C.c_int32),
-        ("reserved", C.c_int32),
+        ("no_graphs", C.c_int32),
     ]
 
 
@@ -90,6 +92,7 @@ class nc_solve_stats(C.Structure):
         ("matvecs_timed", C.c_int64),
         ("matvec_ms", C.c_double),
         ("error_estimate", C.c_double),
+        ("reorthogonalizations", C.c_int64),
     ]
 
 
@@ -126,6 +129,8 @@ SIGNATURES = {
     "nc_matrix_matvec": [P, D, PD, PD, I32],
     "nc_matrix_export": [P, D, PI64, PI64, PI64, PD],
     "nc_matrix_bench_matvec": [P, D, I32, PD],
+    "nc_matrix_set_rows_per_thread": [P, I32],
+    "nc_matrix_autotune": [P, I32, PD],
     "nc_solve_opts_default": [C.POINTER(nc_solve_opts)],
     "nc_solver_create": [P, C.POINTER(nc_solve_opts), C.POINTER(P)],
     "nc_solver_destroy": [P],

@@ -98,6 +98,15 @@ class FspMatrixSparse(L.Handle):
         A.sum_duplicates()
         return A
 
+    def set_rows_per_thread(self, rows):
+        L.call("nc_matrix_set_rows_per_thread", self.handle, int(rows))
+
+    def autotune(self, reps=50):
+        """Pick the fastest mat-vec launch shape for this matrix; returns (rows, ms)."""
+        ms = C.c_double()
+        L.call("nc_matrix_autotune", self.handle, int(reps), C.byref(ms))
+        return self.info()["rows_per_thread"], ms.value
+
     def bench_matvec(self, reps=100, t=0.0):
         """Mean mat-vec kernel time in ms (HIP events on the matrix stream)."""
         ms = C.c_double()

@@ -30,6 +30,7 @@ class KrylovExpv:
     tol: float = 1e-12
     precision: str = "fp64"
     time_kernels: bool = False
+    graphs: bool = True  # replay each Arnoldi process as a captured HIP graph
 
     def opts(self, odeatol, odertol):
         o = L.nc_solve_opts()
@@ -38,6 +39,7 @@ class KrylovExpv:
         o.krylov_m = self.m
         o.krylov_tol = self.tol
         o.time_kernels = int(self.time_kernels)
+        o.no_graphs = 0 if self.graphs else 1
         return o
 
 

@@ -194,7 +194,8 @@ class StateSpace:
         """1-based index of each row of ``X`` in the space, 0 if absent or negative."""
         X = np.asarray(X, dtype=np.int64).reshape(-1, self.ns)
         out = np.zeros(X.shape[0], dtype=np.int64)
-        ok = np.all(X >= 0, axis=1)
+        # states beyond the packing range cannot have been added (adding them raises)
+        ok = np.all(X >= 0, axis=1) & np.all(X < (1 << (self.bits - 1)), axis=1)
         if not ok.any() or self._keys_sorted.size == 0:
             return out
         k = self._pack(X[ok])

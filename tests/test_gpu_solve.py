@@ -111,3 +111,24 @@ def test_reference_fixed_solve_invariants(gpu):
         for p, sinks in zip(sol.p, sol.sinks):
             assert abs(p.sum() + sinks.sum() - 1.0) <= 1.5e-8
         assert isinstance(sol[0], nh.FspOutputSliceSparse)
+
+
+def test_graph_replays_match_eager(gpu):
+    # The Arnoldi process is captured once and replayed every Krylov step; each replay
+    # (across two whole solves, with eager host work between them) must equal the
+    # eagerly launched solve bit for bit.
+    S, props, th = models.toggle()
+    states = models.box_states([70, 70])
+    p0 = np.zeros(len(states))
+    p0[0] = 1.0
+    model = nh.CmeModel(S, models.to_product(props), th)
+    ts = np.array([0.0, 0.5, 2.0])
+    runs = []
+    for graphs in (False, True, True):
+        out = nh.solve(model, nh.FspVectorSparse(states, p0), (0.0, 2.0),
+                       nh.KrylovExpv(precision="fp32", graphs=graphs), saveat=ts)
+        runs.append(out)
+        assert out.stats["steps"] > 3
+    for k in range(len(ts)):
+        assert np.array_equal(runs[0].p[k].values, runs[1].p[k].values)
+        assert np.array_equal(runs[0].p[k].values, runs[2].p[k].values)

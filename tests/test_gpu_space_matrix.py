@@ -197,3 +197,25 @@ def test_sens_matrix_zero_sum_poisson(gpu):
     A = nh.FspMatrixSparse(d, models.to_product(props), [10.0, 5.0])
     w = nh.matvec(0.0, A, np.ones(A.rowcount))
     assert abs(w.sum()) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_rows_per_thread_variants_identical(gpu, precision):
+    # 803 states (not a multiple of 4): every launch shape gives the same bits
+    S, props, th = models.toggle()
+    states = models.box_states([72, 10])[:803]
+    d = nh.StateSpaceSparse(S, states)
+    o = StateSpace(S, states)
+    A = nh.FspMatrixSparse(d, models.to_product(props), th, precision=precision)
+    Ao = FspMatrix(o, models.to_oracle(props), th)
+    v = np.random.default_rng(1).random(A.rowcount)
+    outs = []
+    for r in (1, 2, 4):
+        A.set_rows_per_thread(r)
+        outs.append(nh.matvec(0.0, A, v))
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    ref = Ao.matvec(0.0, v if precision == "fp64" else v.astype(np.float32).astype(float))
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    assert np.abs(outs[0] - ref).max() <= tol * np.abs(ref).max()
+    rows, ms = A.autotune(reps=5)
+    assert rows in (1, 2, 4) and ms > 0
