@@ -234,6 +234,63 @@ int nc_solver_stats(nc_solver s, nc_solve_stats* st);
 int nc_solve_fixed(nc_matrix A, const nc_solve_opts* opts, const double* u0, int32_t n_out,
                    const double* t_out, double* u_out, nc_solve_stats* st);
 
+/* ======================================================================================
+ * Adaptive FSP — replaces solve(model, p0, tspan, ::AdaptiveFspSparse; saveat, fsptol,
+ * odeatol, odertol) (fspsolve.jl:59-62,105-197) and the space adapters RStepAdapter /
+ * SelectiveRStepAdapter with their init! / adapt! (rstepadapters.jl:12-110).
+ * ==================================================================================== */
+enum { NC_ADAPTER_RSTEP = 0, NC_ADAPTER_SELECTIVE_RSTEP = 1 };
+typedef struct {
+  int32_t kind;               /* NC_ADAPTER_* (rstepadapters.jl:12-16, 61-65)          */
+  int32_t initial_step_count; /* init! expansion depth                                 */
+  int32_t max_step_count;     /* adapt! expansion depth                                */
+  int32_t dropstates;         /* drop low-probability states before expanding         */
+} nc_adapter;
+
+/* init!(space, adapter, p, t, fsptol) — rstepadapters.jl:23-28, 74-79: expand! over all
+ * reactions by initial_step_count levels. The caller appends zeros to its p for the new
+ * states (nc_space_count gives the new size). */
+int nc_adapter_init(nc_space space, const nc_adapter* ad);
+
+/* adapt!(space, adapter, p, sinks, t, tend, fsptol; integrator) — rstepadapters.jl:35-52,
+ * 86-110. p: the n probabilities of the space's states. dsinks: the nr sink entries of
+ * du = A(t) u at time t (get_du!, :101-104), required by the selective adapter and
+ * ignored otherwise. On return drop_ids[0..*n_drop) holds the sorted 0-based states that
+ * were deleted (capacity n); the caller deletes them from p and appends zeros for the
+ * states that expand! added. Errors: NC_ERR_ARG for an empty p with the selective adapter
+ * (the reference's ArgumentError, :90-92). */
+int nc_adapter_adapt(nc_space space, const nc_adapter* ad, int64_t n, const double* p, const double* dsinks,
+                     double t, double tend, double fsptol, int64_t* n_drop, int64_t* drop_ids);
+
+typedef struct {
+  nc_adapter adapter;         /* space_adapter of AdaptiveFspSparse (fspsolve.jl:59-62)  */
+  int32_t precision;          /* generator precision, NC_FP64 or NC_FP32                 */
+  int32_t max_adaptations;    /* error out after this many adaptations (0 = unlimited)   */
+  double fsptol;              /* FSP tolerance (default 1e-6, fspsolve.jl:108)           */
+  nc_solve_opts ode;          /* ode_method and its tolerances                           */
+} nc_adaptive_opts;
+
+/* FspOutputSparse (fspoutput.jl:19-74): snapshots with their own state lists, kept on
+ * the device until copied out. */
+typedef struct nc_output_s* nc_output;
+int nc_adaptive_opts_default(nc_adaptive_opts* opts);
+/* Adaptive solve from the initial distribution (n0 states, n0 x ns row-major, unique and
+ * non-negative, with probabilities p0) over tspan [t0, t1]. saveat[0..n_save) ascending;
+ * n_save == 0 saves the start of every space and the end of every integrator step, as
+ * DifferentialEquations does for an empty saveat. The final snapshot appears twice, as
+ * in the reference (fspsolve.jl:163-167 and :189-194). */
+int nc_solve_adaptive(nc_model model, int64_t n0, const int32_t* states0, const double* p0, double t0, double t1,
+                      int32_t n_save, const double* saveat, const nc_adaptive_opts* opts, nc_output* out,
+                      nc_solve_stats* st);
+int nc_output_destroy(nc_output o);
+/* Number of snapshots, and of state spaces (adaptations + 1). */
+int nc_output_count(nc_output o, int32_t* n_snapshots, int32_t* n_spaces);
+/* Time, state count and state-space index of snapshot k. */
+int nc_output_info(nc_output o, int32_t k, double* t, int64_t* n_states, int32_t* space_id);
+/* Copy snapshot k: states (n_states x ns int32), p (n_states) and sinks (nr); any output
+ * may be null. */
+int nc_output_get(nc_output o, int32_t k, int32_t* states, double* p, double* sinks);
+
 /* ---- helpers exposed for testing --------------------------------------------------- */
 /* Dense matrix exponential of an n x n row-major matrix (Pade 13 scaling and squaring,
  * Higham 2005) — the host-side Hessenberg exponential of the Krylov integrator. */

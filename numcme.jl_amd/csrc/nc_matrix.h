@@ -58,6 +58,8 @@ struct nc_matrix_s {
 };
 
 namespace nc {
+// FspMatrixSparse{RealT}(space, propensities; parameters) for the space's current states
+nc_matrix_s* matrix_create(nc_space_s* sp, nc_model_s* m, int precision);
 // out = A(t) v (+ out when accumulate); v is a device vector of n_rows VT values, out holds
 // n_rows VT values, or FP64 values when out_fp64 is set.
 void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,

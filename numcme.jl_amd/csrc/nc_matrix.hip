@@ -614,34 +614,40 @@ double bench_matvec(nc_matrix_s* A, double t, int reps) {
 
 }  // namespace
 
+namespace nc {
+nc_matrix_s* matrix_create(nc_space_s* sp, nc_model_s* m, int precision) {
+  NC_REQUIRE(precision == NC_FP64 || precision == NC_FP32, NC_ERR_ARG, "precision must be NC_FP64 or NC_FP32");
+  NC_REQUIRE(sp->ns == m->ns && sp->nr == m->nr, NC_ERR_ARG, "model and state space sizes differ");
+  NC_REQUIRE(sp->stoich == m->stoich, NC_ERR_ARG, "model and state space stoichiometry differ");
+  auto A = new nc_matrix_s();
+  try {
+    A->precision = precision;
+    A->vsize = precision == NC_FP32 ? 4 : 8;
+    A->ns = sp->ns;
+    A->nr = sp->nr;
+    A->n = sp->n;
+    A->n_rows = sp->n + sp->nr;
+    A->model = m;
+    NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
+    if (precision == NC_FP32)
+      build_t<float>(A, sp, m);
+    else
+      build_t<double>(A, sp, m);
+    NC_CHECK_HIP(hipStreamSynchronize(A->stream.s));
+  } catch (...) {
+    delete A;
+    throw;
+  }
+  return A;
+}
+}  // namespace nc
+
 extern "C" {
 
 int nc_matrix_create(nc_space sp, nc_model m, int32_t precision, nc_matrix* out) {
   return guard([&] {
     NC_REQUIRE(sp && m && out, NC_ERR_ARG, "null argument");
-    NC_REQUIRE(precision == NC_FP64 || precision == NC_FP32, NC_ERR_ARG, "precision must be NC_FP64 or NC_FP32");
-    NC_REQUIRE(sp->ns == m->ns && sp->nr == m->nr, NC_ERR_ARG, "model and state space sizes differ");
-    NC_REQUIRE(sp->stoich == m->stoich, NC_ERR_ARG, "model and state space stoichiometry differ");
-    auto A = new nc_matrix_s();
-    try {
-      A->precision = precision;
-      A->vsize = precision == NC_FP32 ? 4 : 8;
-      A->ns = sp->ns;
-      A->nr = sp->nr;
-      A->n = sp->n;
-      A->n_rows = sp->n + sp->nr;
-      A->model = m;
-      NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
-      if (precision == NC_FP32)
-        build_t<float>(A, sp, m);
-      else
-        build_t<double>(A, sp, m);
-      NC_CHECK_HIP(hipStreamSynchronize(A->stream.s));
-    } catch (...) {
-      delete A;
-      throw;
-    }
-    *out = A;
+    *out = nc::matrix_create(sp, m, precision);
   });
 }
 

@@ -53,7 +53,18 @@ struct nc_solver_s {
 };
 
 namespace nc {
-void krylov_advance(nc_solver_s* S, double t_target);
-void rk_advance(nc_solver_s* S, double t_target);
+// FSP sink-mass budget of the adaptive solve: integration stops where
+// g(t) = sum(sinks) - fsptol * t / tend crosses zero from below (fspsolve.jl:141-156).
+struct EventSpec {
+  double fsptol = 0.0, tend = 1.0;
+  bool fired = false;
+};
+// Advance to t_target (or to the budget crossing when ev is given); with one_step the
+// call returns after the first accepted step.
+void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev = nullptr, bool one_step = false);
+void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev = nullptr, bool one_step = false);
+// Internal entry points shared with the adaptive driver.
+nc_solver_s* solver_create(nc_matrix_s* A, const nc_solve_opts* opts);
+void solver_set_state_device(nc_solver_s* S, double t0, const double* u_dev);
 void solver_reduce_setup(nc_solver_s* S, int nvals);
 }  // namespace nc

@@ -19,6 +19,7 @@
 // Vectors: the master state u is FP64; mat-vec inputs are stored in the matrix precision.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 #include "nc_solver.h"
 
@@ -42,7 +43,10 @@ struct KLayout {
   __host__ __device__ int h2(int i) const { return (m + 2) * (m + 2) + (m + 2) + i; }
   __host__ __device__ int sc(int i) const { return (m + 2) * (m + 2) + 2 * (m + 2) + i; }
   __host__ __device__ int misc(int i) const { return (m + 2) * (m + 2) + 3 * (m + 2) + i; }
+  // sink rows of basis vector j (j <= m) and of the state (j = m+1), R sinks each
+  __host__ __device__ int sinkv(int j, int r, int R) const { return misc(8) + j * R + r; }
   __host__ __device__ int size() const { return (m + 2) * (m + 2) + 3 * (m + 2) + 8; }
+  __host__ __device__ int size_with_sinks(int R) const { return size() + (m + 2) * R; }
 };
 enum { MISC_AVN = 0, MISC_BETA = 1, MISC_S0 = 2, MISC_FLAG = 3, MISC_NREORTH = 4 };
 
@@ -344,6 +348,30 @@ __global__ __launch_bounds__(kBlock) void k_rk_err(int64_t N, double h, double a
   }
 }
 
+// Cubic Hermite dense output on an accepted step, written over y:
+// y(t + theta h) from y, y1 = y(t + h) and the derivatives k0 = f(y), k6 = f(y1).
+__global__ __launch_bounds__(kBlock) void k_rk_hermite(int64_t N, double theta, double h, double* y, const double* y1,
+                                                       const double* k0, const double* k6) {
+  const double t2 = theta * theta, t3 = t2 * theta;
+  const double a0 = 2 * t3 - 3 * t2 + 1, b0 = (t3 - 2 * t2 + theta) * h;
+  const double a1 = -2 * t3 + 3 * t2, b1 = (t3 - t2) * h;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride)
+    y[n] = a0 * y[n] + b0 * k0[n] + a1 * y1[n] + b1 * k6[n];
+}
+
+// Copy the sink rows (indices n_states .. n_states+R-1) of the m+1 basis vectors and of
+// the state u into the kbuf sink region, so the host can evaluate the sink-mass budget of
+// any combination of the basis.
+template <typename BT>
+__global__ void k_gather_sinks(const BT* V, int64_t ld, int nvec, const double* u, int64_t n_states, int R,
+                               double* dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (nvec + 1) * R) return;
+  const int j = t / R, r = t % R;
+  dst[t] = j < nvec ? (double)V[(int64_t)j * ld + n_states + r] : u[n_states + r];
+}
+
 __global__ void k_copy_d(const double* in, double* out, int64_t N) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) out[n] = in[n];
@@ -369,7 +397,7 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
   S->red_blocks = std::min<unsigned>(kMaxRedBlocks, grid_for(S->Np / 4, kBlock));
   S->partials.reserve((size_t)S->red_blocks * kPartStride);
   if (!S->hpart)
-    NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hpart), sizeof(double) * kMaxRedBlocks * kPartStride,
+    NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hpart), sizeof(double) * (kMaxRedBlocks * kPartStride + 4 * kMaxReactions),
                                hipHostMallocDefault));
 }
 
@@ -450,6 +478,15 @@ static void arnoldi_launch(nc_solver_s* S) {
   a.L = L;
   a.finish = FIN_AVN;
   launch_vec(a, S->red_blocks, S->s);
+  const int R = S->A->nr;
+  double* dst = S->kbuf.p + L.sinkv(0, 0, R);
+  if (S->A->precision == NC_FP32)
+    k_gather_sinks<float><<<grid_for((int64_t)(S->m + 2) * R, kBlock), kBlock, 0, S->s>>>(reinterpret_cast<const float*>(S->V.p), S->Np, S->m + 1,
+                                                       S->u.p, S->A->n, R, dst);
+  else
+    k_gather_sinks<double><<<grid_for((int64_t)(S->m + 2) * R, kBlock), kBlock, 0, S->s>>>(reinterpret_cast<const double*>(S->V.p), S->Np, S->m + 1,
+                                                        S->u.p, S->A->n, R, dst);
+  NC_CHECK_HIP(hipGetLastError());
 }
 
 static void arnoldi_run(nc_solver_s* S) {
@@ -475,11 +512,17 @@ static void arnoldi_run(nc_solver_s* S) {
   NC_CHECK_HIP(hipGraphLaunch(S->arnoldi_exec, S->s));
 }
 
-void krylov_advance(nc_solver_s* S, double t_target) {
+// Sink-mass budget of the adaptive solve (fspsolve.jl:145-148): g(t) = sum(sinks) - c t.
+static double budget(const EventSpec* ev, double sink_sum, double t) {
+  return sink_sum - ev->fsptol * t / ev->tend;
+}
+
+void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
   nc_matrix_s* A = S->A;
   const int m = S->m;
   const int64_t N = S->Np;
   const KLayout L{m};
+  const int R = A->nr;
   hipStream_t s = S->s;
   const bool fp32 = A->precision == NC_FP32;
   const double tol = S->opts.krylov_tol;
@@ -505,7 +548,7 @@ void krylov_advance(nc_solver_s* S, double t_target) {
     double t_step = std::min(t_target - S->t, S->t_new);
     arnoldi_run(S);
     S->st.matvecs += m + 1;
-    NC_CHECK_HIP(hipMemcpyAsync(S->hbuf, S->kbuf.p, sizeof(double) * L.size(), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipMemcpyAsync(S->hbuf, S->kbuf.p, sizeof(double) * L.size_with_sinks(R), hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
     S->st.host_syncs++;
     matvec_harvest_timing(A);
@@ -528,17 +571,19 @@ void krylov_advance(nc_solver_s* S, double t_target) {
     const double avnorm = hb[L.misc(MISC_AVN)];
     const int mxf = mb + k1;  // size of the exponentiated matrix
     std::vector<double> Hm((size_t)mxf * mxf, 0.0), F((size_t)mxf * mxf);
-    double err_loc = btol, xm = 1.0 / m;
-    for (int ireject = 0;; ++ireject) {
+    auto exp_tau = [&](double tau, std::vector<double>& out) {
       for (int i = 0; i < mxf; ++i)
         for (int jj = 0; jj < mxf; ++jj) {
           double v = 0.0;
           if (i < mb + 1 && jj < mb && i <= jj + 1 && i < m + 1) v = hb[L.H(i, jj)];
           if (k1 != 0 && i == m + 1 && jj == m) v = 1.0;
-          Hm[(size_t)i * mxf + jj] = t_step * v;
+          Hm[(size_t)i * mxf + jj] = tau * v;
         }
-      // rows beyond mb in a breakdown are zero
-      expm_dense(mxf, Hm.data(), F.data());
+      expm_dense(mxf, Hm.data(), out.data());
+    };
+    double err_loc = btol, xm = 1.0 / m;
+    for (int ireject = 0;; ++ireject) {
+      exp_tau(t_step, F);
       if (k1 == 0) {
         err_loc = btol;
         break;
@@ -561,6 +606,36 @@ void krylov_advance(nc_solver_s* S, double t_target) {
       S->st.rejected++;
     }
     const int mx = mb + std::max(0, k1 - 1);
+    // sink mass of w(tau) = beta V exp(tau H) e1, with the exact state for the first term
+    auto sink_sum = [&](const std::vector<double>& Fm) {
+      double tot = 0.0;
+      for (int r = 0; r < R; ++r) {
+        double v = Fm[0] * hb[L.sinkv(m + 1, r, R)];
+        for (int i = 1; i < mx; ++i) v += S->beta * Fm[(size_t)i * mxf] * hb[L.sc(i)] * hb[L.sinkv(i, r, R)];
+        tot += v;
+      }
+      return tot;
+    };
+    bool fire = false;
+    double sinks0 = 0.0;  // sink mass of the state at the start of the step
+    for (int r = 0; r < R; ++r) sinks0 += hb[L.sinkv(m + 1, r, R)];
+    if (ev && R > 0 && budget(ev, sinks0, S->t) <= 0.0 && budget(ev, sink_sum(F), S->t + t_step) > 0.0) {
+      // the budget is crossed inside this step: locate the crossing on the exact Krylov
+      // solution w(tau) by bisection (the reference's callback root-finds to eps)
+      std::vector<double> Ft((size_t)mxf * mxf);
+      double lo = 0.0, hi = t_step;
+      for (int it = 0; it < 200 && hi - lo > 4.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, S->t + hi); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        exp_tau(mid, Ft);
+        if (budget(ev, sink_sum(Ft), S->t + mid) > 0.0)
+          hi = mid;
+        else
+          lo = mid;
+      }
+      t_step = hi;
+      exp_tau(t_step, F);
+      fire = true;
+    }
     // w_new = c0/beta * w + sum_{i>=1} c_i sc_i V_i ; c = beta * F[:,0]
     double* coef = S->hcoef;  // pinned; the previous upload completed at the last sync
     std::fill(coef, coef + m + 2, 0.0);
@@ -568,7 +643,7 @@ void krylov_advance(nc_solver_s* S, double t_target) {
     coef[m + 1] = F[0];  // (beta*F00)/beta
     NC_CHECK_HIP(hipMemcpyAsync(S->coef.p, coef, sizeof(double) * (m + 2), hipMemcpyHostToDevice, s));
     S->t += t_step;
-    const bool hit = S->t >= t_target - t_eps;
+    const bool hit = !fire && S->t >= t_target - t_eps;
     if (hit) S->t = t_target;
     {
       auto fill = [&](auto& a, auto* vbase) {
@@ -604,15 +679,21 @@ void krylov_advance(nc_solver_s* S, double t_target) {
     S->st.steps++;
     err_loc = std::max(err_loc, 1e-300);
     S->st.error_estimate += std::max(err_loc, S->anorm * 2.220446049250313e-16);
-    if (k1 != 0) S->t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
-    if (hit) break;
+    if (k1 != 0 && !fire) S->t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
+    if (fire) {
+      ev->fired = true;
+      break;
+    }
+    if (hit || one_step) break;
   }
 }
 
 // ---- Dormand-Prince 5(4) ---------------------------------------------------------------
-void rk_advance(nc_solver_s* S, double t_target) {
+void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
   nc_matrix_s* A = S->A;
   const int64_t N = S->Np;  // padded: rows beyond the true length stay zero
+  const int R = A->nr;
+  const int64_t ns = A->n;
   hipStream_t s = S->s;
   const bool fp32 = A->precision == NC_FP32;
   const double rtol = S->opts.rtol, atol = S->opts.atol;
@@ -624,6 +705,18 @@ void rk_advance(nc_solver_s* S, double t_target) {
       k_to_vt<float><<<g, kBlock, 0, s>>>(src, reinterpret_cast<float*>(S->rkv.p), N);
     else
       k_to_vt<double><<<g, kBlock, 0, s>>>(src, reinterpret_cast<double*>(S->rkv.p), N);
+  };
+  auto stages = [&](double h) {
+    for (int stage = 1; stage < 7; ++stage) {
+      if (fp32)
+        k_rk_stage<float><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p, reinterpret_cast<float*>(S->rkv.p));
+      else
+        k_rk_stage<double><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p,
+                                                 reinterpret_cast<double*>(S->rkv.p));
+      NC_CHECK_HIP(hipGetLastError());
+      matvec_launch(A, S->t + h_dp_c[stage] * h, S->rkv.p, kp(stage), false, s, true);
+      S->st.matvecs++;
+    }
   };
   if (!S->fsal_valid) {
     to_vin(S->u.p);
@@ -649,6 +742,11 @@ void rk_advance(nc_solver_s* S, double t_target) {
     S->h = S->opts.h0 > 0 ? S->opts.h0 : (fnorm > 0 ? 0.01 / fnorm : 1e-3);
     S->h = std::max(S->h, 1e-12);
   }
+  double* hs = S->hpart + kMaxRedBlocks;  // pinned: sinks of u, y_{n+1}, k0, k6 (R each)
+  if (ev && R > 0) {
+    NC_CHECK_HIP(hipMemcpyAsync(hs, S->u.p + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+  }
   const double t_eps = 1e-13 * std::max(1.0, std::fabs(t_target));
   double errsum = 0.0;
   while (S->t < t_target - t_eps) {
@@ -656,20 +754,16 @@ void rk_advance(nc_solver_s* S, double t_target) {
       throw Error(NC_ERR_NUMERIC, "RK integrator exceeded max_steps");
     double h = std::min(S->h, t_target - S->t);
     const bool clamped = h < S->h;
-    for (int stage = 1; stage < 7; ++stage) {
-      if (fp32)
-        k_rk_stage<float><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p, reinterpret_cast<float*>(S->rkv.p));
-      else
-        k_rk_stage<double><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p,
-                                                 reinterpret_cast<double*>(S->rkv.p));
-      NC_CHECK_HIP(hipGetLastError());
-      matvec_launch(A, S->t + h_dp_c[stage] * h, S->rkv.p, kp(stage), false, s, true);
-      S->st.matvecs++;
-    }
+    stages(h);
     // after stage 6 the stage input is y + h*sum b_j k_j (row 7 of A): ytmp = y_{n+1}
     k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->partials.p);
     NC_CHECK_HIP(hipGetLastError());
     NC_CHECK_HIP(hipMemcpyAsync(S->hpart, S->partials.p, sizeof(double) * S->red_blocks, hipMemcpyDeviceToHost, s));
+    if (ev && R > 0) {
+      NC_CHECK_HIP(hipMemcpyAsync(hs + R, S->ytmp.p + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+      NC_CHECK_HIP(hipMemcpyAsync(hs + 2 * R, kp(0) + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+      NC_CHECK_HIP(hipMemcpyAsync(hs + 3 * R, kp(6) + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+    }
     NC_CHECK_HIP(hipStreamSynchronize(s));
     errsum = 0.0;
     for (unsigned b = 0; b < S->red_blocks; ++b) errsum += S->hpart[b];
@@ -679,15 +773,48 @@ void rk_advance(nc_solver_s* S, double t_target) {
     if (!std::isfinite(err)) throw Error(NC_ERR_NUMERIC, "non-finite RK error estimate");
     double fac = err > 0 ? 0.9 * std::pow(err, -0.2) : 5.0;
     if (err <= 1.0) {
+      if (ev && R > 0) {
+        double s0 = 0, s1 = 0, d0 = 0, d1 = 0;
+        for (int r = 0; r < R; ++r) s0 += hs[r], s1 += hs[R + r], d0 += hs[2 * R + r], d1 += hs[3 * R + r];
+        const double c = ev->fsptol / ev->tend;
+        const double g0 = s0 - c * S->t, g1 = s1 - c * (S->t + h);
+        if (g0 <= 0.0 && g1 > 0.0) {
+          // cubic Hermite dense output of g on the step (derivatives from the FSAL stages),
+          // crossing located by bisection; the event state is the same Hermite
+          // interpolant of the whole state, so its sink mass is the root-found one
+          const double dg0 = (d0 - c) * h, dg1 = (d1 - c) * h;
+          auto herm = [&](double th) {
+            const double t2 = th * th, t3 = t2 * th;
+            return (2 * t3 - 3 * t2 + 1) * g0 + (t3 - 2 * t2 + th) * dg0 + (-2 * t3 + 3 * t2) * g1 + (t3 - t2) * dg1;
+          };
+          double lo = 0.0, hi = 1.0;
+          for (int it = 0; it < 200 && (hi - lo) * h > 4.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, S->t + h); ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (herm(mid) > 0.0)
+              hi = mid;
+            else
+              lo = mid;
+          }
+          k_rk_hermite<<<g, kBlock, 0, s>>>(N, hi, h, S->u.p, S->ytmp.p, kp(0), kp(6));
+          NC_CHECK_HIP(hipGetLastError());
+          S->t += hi * h;
+          S->st.steps++;
+          S->fsal_valid = false;  // k0 no longer matches u
+          ev->fired = true;
+          break;
+        }
+      }
       // accept: u <- ytmp, k0 <- k6 (FSAL)
       k_copy_d<<<g, kBlock, 0, s>>>(S->ytmp.p, S->u.p, N);
       k_copy_d<<<g, kBlock, 0, s>>>(kp(6), kp(0), N);
       NC_CHECK_HIP(hipGetLastError());
       S->t += h;
-      if (S->t > t_target - t_eps) S->t = t_target;
       S->st.steps++;
+      if (S->t > t_target - t_eps) S->t = t_target;
+      for (int r = 0; ev && r < R; ++r) hs[r] = hs[R + r];
       fac = std::min(5.0, std::max(0.2, fac));
       if (!clamped || fac < 1.0) S->h = h * fac;
+      if (one_step) break;
     } else {
       S->st.rejected++;
       S->h = h * std::max(0.2, fac);
@@ -732,9 +859,10 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
       S->u.alloc(S->Np);
       NC_CHECK_HIP(hipMemsetAsync(S->u.p, 0, sizeof(double) * S->Np, S->s));
       const KLayout L{S->m};
-      S->kbuf.alloc(L.size());
-      NC_CHECK_HIP(hipMemsetAsync(S->kbuf.p, 0, sizeof(double) * L.size(), S->s));
-      NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hbuf), sizeof(double) * L.size(), hipHostMallocDefault));
+      const int ksize = L.size_with_sinks(A->nr);
+      S->kbuf.alloc(ksize);
+      NC_CHECK_HIP(hipMemsetAsync(S->kbuf.p, 0, sizeof(double) * ksize, S->s));
+      NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hbuf), sizeof(double) * ksize, hipHostMallocDefault));
       NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hcoef), sizeof(double) * (S->m + 2), hipHostMallocDefault));
       // k_vec writes partials with a stride of MAXV+1 <= 33 doubles per block
       solver_reduce_setup(S, 32);
@@ -916,3 +1044,35 @@ int nc_solve_fixed(nc_matrix A, const nc_solve_opts* opts, const double* u0, int
 }
 
 }  // extern "C"
+
+namespace nc {
+
+nc_solver_s* solver_create(nc_matrix_s* A, const nc_solve_opts* opts) { return solver_create_impl(A, opts); }
+
+// Set the state from a device vector of the true length N (t0 is its time).
+void solver_set_state_device(nc_solver_s* S, double t0, const double* u_dev) {
+  if (!S->u0.p) {
+    S->u0.alloc(S->Np);
+    NC_CHECK_HIP(hipMemsetAsync(S->u0.p, 0, sizeof(double) * S->Np, S->s));
+  }
+  NC_CHECK_HIP(hipMemcpyAsync(S->u0.p, u_dev, sizeof(double) * S->N, hipMemcpyDeviceToDevice, S->s));
+  const KLayout L{S->m};
+  VecArgs<double> a{};
+  a.N = S->Np;
+  a.nv = 0;
+  a.xin = S->u0.p;
+  a.partials = S->partials.p;
+  a.kb = S->kbuf.p + L.misc(5);
+  a.L = L;
+  a.finish = FIN_SUMS;
+  launch_vec(a, S->red_blocks, S->s);
+  double b2 = 0.0;
+  NC_CHECK_HIP(hipMemcpyAsync(&b2, S->kbuf.p + L.misc(5), sizeof(double), hipMemcpyDeviceToHost, S->s));
+  NC_CHECK_HIP(hipStreamSynchronize(S->s));
+  S->beta0 = std::sqrt(b2);
+  S->t0 = t0;
+  solver_reinit(S);
+  NC_CHECK_HIP(hipStreamSynchronize(S->s));
+}
+
+}  // namespace nc

@@ -314,22 +314,26 @@ static void space_expand(nc_space_s* sp, int levels, const std::vector<int32_t>&
   }
 }
 
-static void space_delete(nc_space_s* sp, int64_t m, const int64_t* ids_host) {
+__global__ void k_compact_vec(const uint32_t* keep, const uint32_t* ranks, int64_t n, const double* in, double* out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n && keep[i]) out[ranks[i]] = in[i];
+}
+
+// deleteat! with the (sorted, unique, in-range) ids already on the device. The vectors
+// vin[k] (length n before the call) are compacted the same way into vout[k].
+void space_delete_device(nc_space_s* sp, const int64_t* d_ids, int64_t m, int nvec, const double* const* vin,
+                         double* const* vout) {
   hipStream_t s = sp->stream.s;
-  for (int64_t k = 0; k < m; ++k) {
-    NC_REQUIRE(ids_host[k] >= 0 && ids_host[k] < sp->n, NC_ERR_ARG, "deleteat!: index out of range");
-    NC_REQUIRE(k == 0 || ids_host[k] > ids_host[k - 1], NC_ERR_ARG, "deleteat!: indices must be unique and sorted");
-  }
   if (sp->n == 0) return;
   const int64_t n = sp->n;
-  DevBuf<int64_t> d_ids(m ? m : 1);
-  if (m) NC_CHECK_HIP(hipMemcpyAsync(d_ids.p, ids_host, m * sizeof(int64_t), hipMemcpyHostToDevice, s));
   sp->flags.reserve(n);
   sp->ranks.reserve(n);
   k_fill_u32<<<grid_for(n, kBlock), kBlock, 0, s>>>(sp->flags.p, n, 1u);
-  if (m) k_mark_ids<<<grid_for(m, kBlock), kBlock, 0, s>>>(d_ids.p, m, sp->flags.p);
+  if (m) k_mark_ids<<<grid_for(m, kBlock), kBlock, 0, s>>>(d_ids, m, sp->flags.p);
   uint32_t kept = 0;
   exclusive_scan_u32(sp->flags.p, sp->ranks.p, n, sp->scan_tmp, s, &kept);
+  for (int k = 0; k < nvec; ++k)
+    k_compact_vec<<<grid_for(n, kBlock), kBlock, 0, s>>>(sp->flags.p, sp->ranks.p, n, vin[k], vout[k]);
   DevBuf<int32_t> st2((size_t)sp->cap * sp->ns), cn2((size_t)sp->cap * sp->nr);
   DevBuf<uint32_t> sk2((size_t)sp->cap);
   k_compact<<<grid_for(n, kBlock), kBlock, 0, s>>>(sp->flags.p, sp->ranks.p, n, sp->ns, sp->nr, sp->states.p,
@@ -349,6 +353,50 @@ static void space_delete(nc_space_s* sp, int64_t m, const int64_t* ids_host) {
   NC_CHECK_HIP(hipGetLastError());
 }
 
+static void space_delete(nc_space_s* sp, int64_t m, const int64_t* ids_host) {
+  for (int64_t k = 0; k < m; ++k) {
+    NC_REQUIRE(ids_host[k] >= 0 && ids_host[k] < sp->n, NC_ERR_ARG, "deleteat!: index out of range");
+    NC_REQUIRE(k == 0 || ids_host[k] > ids_host[k - 1], NC_ERR_ARG, "deleteat!: indices must be unique and sorted");
+  }
+  if (sp->n == 0) return;
+  DevBuf<int64_t> d_ids(m ? m : 1);
+  if (m) NC_CHECK_HIP(hipMemcpyAsync(d_ids.p, ids_host, m * sizeof(int64_t), hipMemcpyHostToDevice, sp->stream.s));
+  space_delete_device(sp, d_ids.p, m, 0, nullptr, nullptr);
+}
+
+nc_space_s* space_create(int ns, int nr, const int32_t* stoich, int64_t n_init, const int32_t* init_states) {
+  NC_REQUIRE(ns >= 1 && ns <= kMaxSpecies, NC_ERR_UNSUPPORTED, "species count must be in 1..16");
+  NC_REQUIRE(nr >= 1 && nr <= kMaxReactions, NC_ERR_UNSUPPORTED, "reaction count must be in 1..32");
+  NC_REQUIRE(stoich, NC_ERR_ARG, "null stoichiometry matrix");
+  NC_REQUIRE(n_init >= 0 && (n_init == 0 || init_states), NC_ERR_ARG, "bad initial state list");
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+    throw Error(NC_ERR_NO_DEVICE, "no HIP device available");
+  auto sp = new nc_space_s();
+  try {
+    sp->ns = ns;
+    sp->nr = nr;
+    sp->stoich.assign(stoich, stoich + (size_t)ns * nr);
+    sp->d_stoich.alloc((size_t)ns * nr);
+    NC_CHECK_HIP(hipMemcpy(sp->d_stoich.p, stoich, sizeof(int32_t) * ns * nr, hipMemcpyHostToDevice));
+    space_reserve(sp, std::max<int64_t>(n_init, 1));
+    space_rebuild_table(sp, std::max<int64_t>(n_init, 1));
+    if (n_init) {
+      sp->cands.reserve((size_t)n_init * ns);
+      NC_CHECK_HIP(hipMemcpyAsync(sp->cands.p, init_states, sizeof(int32_t) * ns * n_init, hipMemcpyHostToDevice,
+                                  sp->stream.s));
+      space_add_candidates(sp, n_init);  // _addstates!(space, initstates): :122
+    }
+    NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
+  } catch (...) {
+    delete sp;
+    throw;
+  }
+  return sp;
+}
+
+void space_expand_list(nc_space_s* sp, int levels, const std::vector<int32_t>& rlist) { space_expand(sp, levels, rlist); }
+
 }  // namespace nc
 
 using namespace nc;
@@ -359,34 +407,7 @@ int nc_space_create(int32_t ns, int32_t nr, const int32_t* stoich, int64_t n_ini
                     nc_space* out) {
   return guard([&] {
     NC_REQUIRE(out, NC_ERR_ARG, "null output handle");
-    NC_REQUIRE(ns >= 1 && ns <= kMaxSpecies, NC_ERR_UNSUPPORTED, "species count must be in 1..16");
-    NC_REQUIRE(nr >= 1 && nr <= kMaxReactions, NC_ERR_UNSUPPORTED, "reaction count must be in 1..32");
-    NC_REQUIRE(stoich, NC_ERR_ARG, "null stoichiometry matrix");
-    NC_REQUIRE(n_init >= 0 && (n_init == 0 || init_states), NC_ERR_ARG, "bad initial state list");
-    int dev_count = 0;
-    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
-      throw Error(NC_ERR_NO_DEVICE, "no HIP device available");
-    auto sp = new nc_space_s();
-    try {
-      sp->ns = ns;
-      sp->nr = nr;
-      sp->stoich.assign(stoich, stoich + (size_t)ns * nr);
-      sp->d_stoich.alloc((size_t)ns * nr);
-      NC_CHECK_HIP(hipMemcpy(sp->d_stoich.p, stoich, sizeof(int32_t) * ns * nr, hipMemcpyHostToDevice));
-      space_reserve(sp, std::max<int64_t>(n_init, 1));
-      space_rebuild_table(sp, std::max<int64_t>(n_init, 1));
-      if (n_init) {
-        sp->cands.reserve((size_t)n_init * ns);
-        NC_CHECK_HIP(hipMemcpyAsync(sp->cands.p, init_states, sizeof(int32_t) * ns * n_init, hipMemcpyHostToDevice,
-                                    sp->stream.s));
-        space_add_candidates(sp, n_init);  // _addstates!(space, initstates): :122
-      }
-      NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
-    } catch (...) {
-      delete sp;
-      throw;
-    }
-    *out = sp;
+    *out = space_create(ns, nr, stoich, n_init, init_states);
   });
 }
 

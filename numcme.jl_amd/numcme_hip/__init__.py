@@ -12,6 +12,7 @@ from .model import (CmeModel, CmeModelWithSensitivity, JointTimeVaryingPropensit
                     SeparableTimeVaryingPropensity, StandardTimeInvariantPropensity, istimeseparable,
                     istimevarying, propensity)
 from .solve import RK45, FspOutputSliceSparse, FspOutputSparse, KrylovExpv, solve
+from .adaptive import AdaptiveFspSparse, RStepAdapter, SelectiveRStepAdapter, adapt_, init_
 from .statespace import (StateSpaceSparse, deleteat, expand, get_sink_connectivity, get_sink_count,
                          get_state_connectivity, get_state_count, get_statedict, get_states, get_stoich_matrix)
 from . import expr as nmath
@@ -22,6 +23,7 @@ __all__ = [
     "CmeModel", "CmeModelWithSensitivity", "JointTimeVaryingPropensity", "Propensity",
     "SeparableTimeVaryingPropensity", "StandardTimeInvariantPropensity", "istimeseparable", "istimevarying",
     "propensity", "RK45", "FspOutputSliceSparse", "FspOutputSparse", "KrylovExpv", "solve",
+    "AdaptiveFspSparse", "RStepAdapter", "SelectiveRStepAdapter", "adapt_", "init_",
     "StateSpaceSparse", "deleteat", "expand", "get_sink_connectivity", "get_sink_count",
     "get_state_connectivity", "get_state_count", "get_statedict", "get_states", "get_stoich_matrix",
 ]

@@ -34,6 +34,9 @@ NC_METHOD_AUTO = 0
 NC_METHOD_KRYLOV = 1
 NC_METHOD_RK45 = 2
 
+NC_ADAPTER_RSTEP = 0
+NC_ADAPTER_SELECTIVE_RSTEP = 1
+
 
 class NumCmeHipError(RuntimeError):
     """A failed library call; ``code`` is the C status."""
@@ -96,6 +99,25 @@ class nc_solve_stats(C.Structure):
     ]
 
 
+class nc_adapter(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("initial_step_count", C.c_int32),
+        ("max_step_count", C.c_int32),
+        ("dropstates", C.c_int32),
+    ]
+
+
+class nc_adaptive_opts(C.Structure):
+    _fields_ = [
+        ("adapter", nc_adapter),
+        ("precision", C.c_int32),
+        ("max_adaptations", C.c_int32),
+        ("fsptol", C.c_double),
+        ("ode", nc_solve_opts),
+    ]
+
+
 P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -141,6 +163,15 @@ SIGNATURES = {
     "nc_solver_get_state": [P, PD, PD],
     "nc_solver_stats": [P, C.POINTER(nc_solve_stats)],
     "nc_solve_fixed": [P, C.POINTER(nc_solve_opts), PD, I32, PD, PD, C.POINTER(nc_solve_stats)],
+    "nc_adapter_init": [P, C.POINTER(nc_adapter)],
+    "nc_adapter_adapt": [P, C.POINTER(nc_adapter), I64, PD, PD, D, D, D, PI64, PI64],
+    "nc_adaptive_opts_default": [C.POINTER(nc_adaptive_opts)],
+    "nc_solve_adaptive": [P, I64, PI32, PD, D, D, I32, PD, C.POINTER(nc_adaptive_opts), C.POINTER(P),
+                          C.POINTER(nc_solve_stats)],
+    "nc_output_destroy": [P],
+    "nc_output_count": [P, PI32, PI32],
+    "nc_output_info": [P, I32, PD, PI64, PI32],
+    "nc_output_get": [P, I32, PI32, PD, PD],
     "nc_expm_dense": [I32, PD, PD],
 }
 

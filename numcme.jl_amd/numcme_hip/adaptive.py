@@ -1,17 +1,174 @@
 """Adaptive FSP solve and R-step space adapters — mirrors ``src/transientcme/sparse/``.
 
-Placeholder module: the adaptive driver is added in a later commit.
+* ``RStepAdapter`` / ``SelectiveRStepAdapter`` (rstepadapters.jl:12-16, 61-65) with
+  ``init_`` / ``adapt_`` standing for ``init!`` / ``adapt!`` (:23-52, :74-110). The state
+  space is changed in place on the GPU; the probability vector is returned (the reference
+  resizes its ``p`` in place, which numpy arrays cannot do).
+* ``AdaptiveFspSparse(ode_method, space_adapter)`` (fspsolve.jl:59-62) and the adaptive
+  ``solve`` (fspsolve.jl:105-197). The whole adapt-and-restart loop runs in the library
+  (``nc_solve_adaptive``): the budget event, the drop selection, ``expand!``, the generator
+  rebuild and the restart never leave the device except for a handful of scalars.
 """
 from __future__ import annotations
 
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
 from . import _lib as L
+from ._lib import ArgumentError
+from .fspvector import FspVectorSparse
 
 
+@dataclass(frozen=True)
+class RStepAdapter:
+    """Reachability adapter (rstepadapters.jl:12-16)."""
+
+    initial_step_count: int
+    max_step_count: int
+    dropstates: bool
+
+    kind = L.NC_ADAPTER_RSTEP
+
+    def c_struct(self):
+        return L.nc_adapter(self.kind, int(self.initial_step_count), int(self.max_step_count), int(bool(self.dropstates)))
+
+
+@dataclass(frozen=True)
+class SelectiveRStepAdapter(RStepAdapter):
+    """Reachability adapter over the reactions whose sinks grow (rstepadapters.jl:61-65)."""
+
+    kind = L.NC_ADAPTER_SELECTIVE_RSTEP
+
+
+def init_(space, adapter: RStepAdapter, p, t=0.0, fsptol=1e-6):
+    """``init!`` (rstepadapters.jl:23-28, 74-79); returns p extended with zeros."""
+    p = np.asarray(p, dtype=np.float64)
+    nold = space.get_state_count()
+    ad = adapter.c_struct()
+    L.call("nc_adapter_init", space.handle, C.byref(ad))
+    return np.concatenate([p, np.zeros(space.get_state_count() - nold)])
+
+
+def adapt_(space, adapter: RStepAdapter, p, sinks, t, tend, fsptol, dsinks=None):
+    """``adapt!`` (rstepadapters.jl:35-52, 86-110).
+
+    ``dsinks`` are the sink entries of ``du = A(t) u`` (what ``get_du!`` gives the
+    reference's selective adapter). Returns ``(p_new, dropids)``: the probabilities of the
+    adapted space and the sorted 1-based indices that were deleted.
+    """
+    p = L.f64(p)
+    n = p.size
+    ad = adapter.c_struct()
+    ds = None
+    if adapter.kind == L.NC_ADAPTER_SELECTIVE_RSTEP:
+        if n == 0:
+            raise ArgumentError(L.NC_ERR_ARG, "Empty `p` input in `adapt!`.")
+        if dsinks is None:
+            raise ArgumentError(L.NC_ERR_ARG, "the selective adapter needs the sink derivatives (dsinks)")
+        ds = L.f64(dsinks)
+        if ds.size != space.get_sink_count():
+            raise ArgumentError(L.NC_ERR_ARG, "dsinks must have one entry per reaction")
+    ids = np.empty(max(n, 1), dtype=np.int64)
+    m = C.c_int64()
+    pp = p if n else np.zeros(1)
+    L.call("nc_adapter_adapt", space.handle, C.byref(ad), n, L.ptr(pp, C.c_double),
+           None if ds is None else L.ptr(ds, C.c_double), float(t), float(tend), float(fsptol), C.byref(m),
+           L.ptr(ids, C.c_int64))
+    drop = ids[: m.value].copy()
+    keep = np.ones(n, dtype=bool)
+    keep[drop] = False
+    p_new = np.concatenate([p[keep], np.zeros(space.get_state_count() - int(keep.sum()))])
+    return p_new, drop + 1
+
+
+@dataclass
 class AdaptiveFspSparse:
-    def __init__(self, ode_method=None, space_adapter=None):
-        self.ode_method = ode_method
-        self.space_adapter = space_adapter
+    """``AdaptiveFspSparse`` (fspsolve.jl:59-62). ``ode_method`` is ``KrylovExpv``,
+    ``RK45`` or None (Krylov for time-invariant models, RK45 otherwise)."""
+
+    ode_method: object = None
+    space_adapter: RStepAdapter = None
+    max_adaptations: int = 100000
 
 
-def solve_adaptive(*args, **kwargs):
-    raise L.NumCmeHipError(L.NC_ERR_UNSUPPORTED, "adaptive solve is not available yet")
+def _saveat_points(saveat, tspan):
+    if isinstance(saveat, (int, float)) and not isinstance(saveat, bool):
+        # Julia's tspan[1]:saveat:tspan[2]
+        a, b, h = float(tspan[0]), float(tspan[1]), float(saveat)
+        if h <= 0:
+            raise ArgumentError(L.NC_ERR_ARG, "saveat step must be positive")
+        k = math.floor((b - a) / h * (1 + 1e-12) + 1e-12)
+        return np.array([a + i * h for i in range(k + 1)], dtype=np.float64)
+    return np.asarray(list(saveat), dtype=np.float64)
+
+
+def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: AdaptiveFspSparse, saveat=(),
+                   fsptol=1.0e-6, odeatol=1.0e-6, odertol=1.0e-4, verbose=False):
+    from .solve import RK45, FspOutputSparse, KrylovExpv
+
+    if not isinstance(alg.space_adapter, RStepAdapter):
+        raise ArgumentError(L.NC_ERR_ARG, "space_adapter must be RStepAdapter or SelectiveRStepAdapter")
+    o = L.nc_adaptive_opts()
+    L.call("nc_adaptive_opts_default", C.byref(o))
+    o.adapter = alg.space_adapter.c_struct()
+    o.fsptol = float(fsptol)
+    o.max_adaptations = int(alg.max_adaptations)
+    ode = alg.ode_method
+    if ode is None:
+        L.call("nc_solve_opts_default", C.byref(o.ode))
+        o.ode.method = L.NC_METHOD_AUTO
+        o.ode.rtol, o.ode.atol = float(odertol), float(odeatol)
+        o.precision = L.NC_FP64
+    elif isinstance(ode, (KrylovExpv, RK45)):
+        o.ode = ode.opts(odeatol, odertol)
+        o.precision = L.NC_FP32 if ode.precision == "fp32" else L.NC_FP64
+    else:
+        raise ArgumentError(L.NC_ERR_ARG, "ode_method must be KrylovExpv, RK45 or None")
+
+    ts = _saveat_points(saveat, tspan)
+    X = L.i32(np.asarray(initial_distribution.states).reshape(-1, model.get_species_count()))
+    p0 = L.f64(initial_distribution.values)
+    if X.shape[0] != p0.size:
+        raise ArgumentError(L.NC_ERR_ARG, "initial distribution states and values differ in length")
+    h = C.c_void_p()
+    st = L.nc_solve_stats()
+    tsp = ts if ts.size else np.zeros(1)
+    L.call("nc_solve_adaptive", model.device_model().handle, X.shape[0], L.ptr(X, C.c_int32), L.ptr(p0, C.c_double),
+           float(tspan[0]), float(tspan[1]), int(ts.size), L.ptr(L.f64(tsp), C.c_double), C.byref(o), C.byref(h),
+           C.byref(st))
+    res = _Output(h.value)
+    out = FspOutputSparse()
+    out.space_ids = []  # index of the state space each snapshot lives on
+    nsnap, nspace = C.c_int32(), C.c_int32()
+    L.call("nc_output_count", res.handle, C.byref(nsnap), C.byref(nspace))
+    ns, nr = model.get_species_count(), model.get_reaction_count()
+    spaces = {}
+    for k in range(nsnap.value):
+        t, n, sid = C.c_double(), C.c_int64(), C.c_int32()
+        L.call("nc_output_info", res.handle, k, C.byref(t), C.byref(n), C.byref(sid))
+        p = np.empty(max(n.value, 1))
+        sinks = np.empty(nr)
+        states = None
+        if sid.value not in spaces:
+            states = np.empty((n.value, ns), dtype=np.int32)
+        L.call("nc_output_get", res.handle, k, None if states is None else L.ptr(states, C.c_int32),
+               L.ptr(p, C.c_double), L.ptr(sinks, C.c_double))
+        if states is not None:
+            spaces[sid.value] = states.astype(np.int64)
+        out.t.append(t.value)
+        out.space_ids.append(sid.value)
+        out.p.append(FspVectorSparse(spaces[sid.value], p[: n.value], copy=False))
+        out.sinks.append(sinks)
+        if verbose and states is not None and sid.value > 0:
+            print(f"t = {round(t.value, 2)}. Update state space. New size: {n.value}.")
+    out.stats = {f: getattr(st, f) for f, _ in L.nc_solve_stats._fields_}
+    out.stats["adaptations"] = nspace.value - 1
+    res.free()
+    return out
+
+
+class _Output(L.Handle):
+    _destroy = "nc_output_destroy"

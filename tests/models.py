@@ -145,3 +145,20 @@ def to_product(props):
         else:
             out.append(JointTimeVaryingPropensity(pr[1]))
     return out
+
+
+# ---- three-species cascade (config C5) ----------------------------------------------------
+CASCADE_S = np.array([[1, 0, 0, -1, 0, 0], [0, 1, 0, 0, -1, 0], [0, 0, 1, 0, 0, -1]])
+
+
+def cascade(theta=(10.0, 2.0, 1.0, 0.5, 0.2, 0.1)):
+    """0->A k0; A->A+B k1*A; B->B+C k2*B; A->0, B->0, C->0 first order."""
+    props = [
+        ("ti", lambda x, p: p[0] + 0.0 * x[0]),
+        ("ti", lambda x, p: p[1] * x[0]),
+        ("ti", lambda x, p: p[2] * x[1]),
+        ("ti", lambda x, p: p[3] * x[0]),
+        ("ti", lambda x, p: p[4] * x[1]),
+        ("ti", lambda x, p: p[5] * x[2]),
+    ]
+    return CASCADE_S, props, np.asarray(theta, dtype=float)
