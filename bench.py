@@ -113,6 +113,27 @@ def cpu_baseline(model_theta, gpu_fp64_steps, max_steps=3):
     }
 
 
+def spmv_probe(nh, precision, reps):
+    """Mat-vec launches plus a copy kernel of known size, for rocprofv3 --pmc runs."""
+    import torch
+
+    model = toggle_model(nh)
+    space = nh.StateSpaceSparse(model.stoich_matrix, box_states(1024))
+    A = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision=precision, model=model)
+    rows, _ = A.autotune(reps=10)
+    ms = A.bench_matvec(reps=reps)
+    x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)  # reads 256 MiB, writes 256 MiB
+    torch.cuda.synchronize()
+    info = A.info()
+    print(json.dumps({"probe": "spmv", "precision": precision, "reps": reps, "rows_per_thread": rows,
+                      "mean_launch_us": ms * 1e3, "csr_bytes_per_matvec": info["csr_bytes_per_matvec"],
+                      "own_bytes_per_matvec": info["bytes_per_matvec"], "copy_bytes_each_way": x.numel() * 4}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +142,9 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch Arnoldi kernels eagerly (for profilers)")
+    ap.add_argument("--spmv-probe", type=int, default=0, metavar="REPS",
+                    help="only build the matrix, launch the mat-vec REPS times back to back and run a plain "
+                         "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one FSP solve per GPU; multi-GPU runs are not part of this "
@@ -129,6 +153,10 @@ def main():
 
     import numcme_hip as nh
     from numcme_hip import _lib as L
+
+    if args.spmv_probe:
+        spmv_probe(nh, args.precision, args.spmv_probe)
+        return
 
     t_build = time.perf_counter()
     model = toggle_model(nh)

@@ -13,6 +13,8 @@
 //    nc_matrix_autotune.
 #include <cfloat>
 
+#include <hip/hip_ext.h>
+
 #include "nc_matrix.h"
 
 nc_matrix_s::~nc_matrix_s() {
@@ -232,7 +234,11 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
   const int64_t row_threads = (a.n + R - 1) / R;
   const int64_t row_blocks = (row_threads + kBlock - 1) / kBlock;
   if ((int64_t)blockIdx.x < row_blocks) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    // Workgroups are dealt to the 8 XCDs round-robin; give each XCD one contiguous range
+    // of row blocks so the neighbouring v entries a row block gathers sit in its own L2.
+    const int64_t b = blockIdx.x, q = row_blocks >> 3, r8 = row_blocks & 7, x = b & 7;
+    const int64_t lb = x * q + (x < r8 ? x : r8) + (b >> 3);
+    const int64_t t = lb * kBlock + threadIdx.x;
     const int64_t i0 = t * R;
     if (i0 >= a.n) return;
     const Pack<double, R> dg = ldpack<double, R>(a.diag + i0);
@@ -291,34 +297,51 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
   }
 }
 
+// Launch timing: when e0/e1 are given the kernel is launched with hipExtLaunchKernelGGL,
+// which stamps the events from the dispatch itself (the kernel's own start and end), so
+// the recorded durations carry no event-packet or dispatch gaps.
+struct LaunchCfg {
+  unsigned grid;
+  hipStream_t s;
+  hipEvent_t e0, e1;
+};
+
+template <typename VT, typename OT, int K, int R>
+void launch_one(const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
+  if (c.e0)
+    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo, acc);
+  else
+    k_matvec<VT, OT, K, R><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+}
+
 template <typename VT, typename OT, int R>
-void dispatch_k(int K, unsigned grid, hipStream_t s, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
+void dispatch_k(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
   switch (K) {
-    case 1: k_matvec<VT, OT, 1, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 2: k_matvec<VT, OT, 2, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 3: k_matvec<VT, OT, 3, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 4: k_matvec<VT, OT, 4, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 5: k_matvec<VT, OT, 5, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 6: k_matvec<VT, OT, 6, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 7: k_matvec<VT, OT, 7, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    case 8: k_matvec<VT, OT, 8, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
-    default: k_matvec<VT, OT, 0, R><<<grid, kBlock, 0, s>>>(a, vv, oo, acc); break;
+    case 1: launch_one<VT, OT, 1, R>(c, a, vv, oo, acc); break;
+    case 2: launch_one<VT, OT, 2, R>(c, a, vv, oo, acc); break;
+    case 3: launch_one<VT, OT, 3, R>(c, a, vv, oo, acc); break;
+    case 4: launch_one<VT, OT, 4, R>(c, a, vv, oo, acc); break;
+    case 5: launch_one<VT, OT, 5, R>(c, a, vv, oo, acc); break;
+    case 6: launch_one<VT, OT, 6, R>(c, a, vv, oo, acc); break;
+    case 7: launch_one<VT, OT, 7, R>(c, a, vv, oo, acc); break;
+    case 8: launch_one<VT, OT, 8, R>(c, a, vv, oo, acc); break;
+    default: launch_one<VT, OT, 0, R>(c, a, vv, oo, acc); break;
   }
 }
 
 template <typename VT, typename OT>
-void dispatch_r(int R, int K, unsigned grid, hipStream_t s, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
+void dispatch_r(int R, int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
   if (R == 4)
-    dispatch_k<VT, OT, 4>(K, grid, s, a, vv, oo, acc);
+    dispatch_k<VT, OT, 4>(K, c, a, vv, oo, acc);
   else if (R == 2)
-    dispatch_k<VT, OT, 2>(K, grid, s, a, vv, oo, acc);
+    dispatch_k<VT, OT, 2>(K, c, a, vv, oo, acc);
   else
-    dispatch_k<VT, OT, 1>(K, grid, s, a, vv, oo, acc);
+    dispatch_k<VT, OT, 1>(K, c, a, vv, oo, acc);
 }
 
 template <typename VT>
 void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
-                     bool out_fp64) {
+                     bool out_fp64, hipEvent_t e0, hipEvent_t e1) {
   MatArgs<VT> a{};
   a.n = A->n;
   a.ld = A->ld;
@@ -341,13 +364,13 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   const int R = A->rows_per_thread;
   const int64_t row_threads = (A->n + R - 1) / R;
   const int64_t row_blocks = (row_threads + kBlock - 1) / kBlock;
-  const unsigned grid = (unsigned)(row_blocks + A->nr);
+  const LaunchCfg c{(unsigned)(row_blocks + A->nr), s, e0, e1};
   const VT* vv = reinterpret_cast<const VT*>(v);
   const int acc = accumulate ? 1 : 0;
   if (out_fp64)
-    dispatch_r<VT, double>(R, A->K, grid, s, a, vv, reinterpret_cast<double*>(out), acc);
+    dispatch_r<VT, double>(R, A->K, c, a, vv, reinterpret_cast<double*>(out), acc);
   else
-    dispatch_r<VT, VT>(R, A->K, grid, s, a, vv, reinterpret_cast<VT*>(out), acc);
+    dispatch_r<VT, VT>(R, A->K, c, a, vv, reinterpret_cast<VT*>(out), acc);
 }
 
 template <typename VT>
@@ -557,17 +580,14 @@ void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accu
         matvec_harvest_timing(A);
       }
     }
-    NC_CHECK_HIP(hipEventRecord(A->ev[2 * A->ev_used], s));
   }
+  hipEvent_t e0 = timed ? A->ev[2 * A->ev_used] : nullptr, e1 = timed ? A->ev[2 * A->ev_used + 1] : nullptr;
   if (A->precision == NC_FP32)
-    launch_matvec_t<float>(A, t, v, out, accumulate, s, out_fp64);
+    launch_matvec_t<float>(A, t, v, out, accumulate, s, out_fp64, e0, e1);
   else
-    launch_matvec_t<double>(A, t, v, out, accumulate, s, out_fp64);
+    launch_matvec_t<double>(A, t, v, out, accumulate, s, out_fp64, e0, e1);
   NC_CHECK_HIP(hipGetLastError());
-  if (timed) {
-    NC_CHECK_HIP(hipEventRecord(A->ev[2 * A->ev_used + 1], s));
-    A->ev_used++;
-  }
+  if (timed) A->ev_used++;
   A->launches++;
 }
 

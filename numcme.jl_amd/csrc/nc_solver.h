@@ -24,7 +24,7 @@ struct nc_solver_s {
   // Krylov
   int m = 30;
   nc::DevBuf<unsigned char> V;  // (m+1) x N, VT, unnormalised
-  nc::DevBuf<double> q, p;      // FP64 work vectors
+  nc::DevBuf<double> q;         // FP64 mat-vec output
   nc::DevBuf<double> kbuf;      // H (m+2)^2 | h (m+2) | h2 (m+2) | sc (m+2) | misc (8)
   nc::DevBuf<double> coef;      // combination coefficients (m+2)
   double* hbuf = nullptr;       // pinned host copy of kbuf

@@ -31,9 +31,10 @@ constexpr int kBlock = 256;
 constexpr int kMaxRedBlocks = 1024;
 constexpr int kPartStride = 33;  // doubles of partial sums per block (up to 32 dots + a norm)
 
-enum Finish : int { FIN_NONE = 0, FIN_H1, FIN_UPD, FIN_H2, FIN_COL, FIN_AVN, FIN_BETA, FIN_SUMS };
+enum Finish : int { FIN_NONE = 0, FIN_H1, FIN_H2, FIN_COL, FIN_AVN, FIN_BETA, FIN_SUMS };
 constexpr double kBreakdownTol = 1e-7;   // Expokit's happy-breakdown threshold
-constexpr double kDgksEta = 0.70710678118654752;  // reorthogonalise when |x1| < eta |x0|
+constexpr int kFinBlock = 1024;  // threads of the one-block partial-sum fold
+constexpr unsigned kVecBlocks = 512;  // grid of the vector passes
 
 // kbuf layout (doubles)
 struct KLayout {
@@ -48,7 +49,7 @@ struct KLayout {
   __host__ __device__ int size() const { return (m + 2) * (m + 2) + 3 * (m + 2) + 8; }
   __host__ __device__ int size_with_sinks(int R) const { return size() + (m + 2) * R; }
 };
-enum { MISC_AVN = 0, MISC_BETA = 1, MISC_S0 = 2, MISC_FLAG = 3, MISC_NREORTH = 4 };
+enum { MISC_AVN = 0, MISC_BETA = 1, MISC_NREORTH = 4 };
 
 template <typename BT>
 struct VecArgs {
@@ -84,11 +85,13 @@ __device__ inline double wave_sum(double v) {
 //
 // x = xscale*xin + csign * sum_i coef[i]*sc[i]*V_i ; optional stores; optional dots
 // d_i = sc[i] * sum_n V_i[n] x[n] and the norm sum_n x[n]^2 as per-block partial sums.
-// MODE picks what a pass needs: VM_DOTS (x = xscale*xin; dots) or VM_UPDATE (update, no
-// dots). Each thread owns 4 consecutive rows and reads every stream with one 16-byte
-// (FP32) or 32-byte (FP64) vector load per vector; N is a multiple of 4 and all vectors
-// are zero beyond the true length, so no tail handling is needed.
-enum VecMode : int { VM_DOTS = 0, VM_UPDATE = 1 };
+// MODE picks what a pass needs: VM_DOTS (x = xscale*xin; dots), VM_UPDATE (update, no
+// dots) or VM_DOTS2 (update, then dots of the updated x with the same vectors: the
+// second Gram-Schmidt projection fused with the first update, x never stored). Each
+// thread owns 4 consecutive rows and reads every stream with one 16-byte (FP32) or
+// 32-byte (FP64) vector load per vector; N is a multiple of 4 and all vectors are zero
+// beyond the true length, so no tail handling is needed.
+enum VecMode : int { VM_DOTS = 0, VM_UPDATE = 1, VM_DOTS2 = 2, VM_DOTS2_KEEP = 3 };
 
 template <typename T, int R>
 struct alignas(sizeof(T) * R) Pack {
@@ -98,7 +101,12 @@ struct alignas(sizeof(T) * R) Pack {
 template <typename BT, int MAXV, int MODE>
 __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   constexpr int MV = MAXV > 0 ? MAXV : 1;
-  constexpr bool DOTS = MODE == VM_DOTS;
+  constexpr bool DOTS = MODE != VM_UPDATE;
+  constexpr bool UPD = MODE != VM_DOTS;
+  // the fused pass either keeps the basis values in registers between its two sweeps
+  // (VM_DOTS2_KEEP) or loads them again (the second load of the same lines hits the caches)
+  constexpr bool FUSED = MODE == VM_DOTS2 || MODE == VM_DOTS2_KEEP;
+  constexpr bool KEEP = MODE == VM_DOTS2_KEEP;
   constexpr int R = 4;
   if (a.cond && *a.cond == 0.0) return;  // uniform: the whole grid skips the pass
   const int nv = a.nv;
@@ -129,16 +137,32 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
 #pragma unroll
       for (int r = 0; r < R; ++r) x[r] = 0.0;
     }
+    Pack<BT, R> keep[KEEP ? MV : 1];
 #pragma unroll
     for (int i = 0; i < MAXV; ++i) {
       if (i < nv) {
         const Pack<BT, R> vv = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)i * a.N + n0);
-        if constexpr (DOTS) {
-#pragma unroll
-          for (int r = 0; r < R; ++r) dacc[i] += (double)vv.v[r] * x[r];
-        } else {
+        if constexpr (KEEP) keep[i] = vv;
+        if constexpr (UPD) {
 #pragma unroll
           for (int r = 0; r < R; ++r) x[r] += s_cs[i] * (double)vv.v[r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) dacc[i] += (double)vv.v[r] * x[r];
+        }
+      }
+    }
+    if constexpr (FUSED) {
+#pragma unroll
+      for (int i = 0; i < MAXV; ++i) {
+        if (i < nv) {
+          Pack<BT, R> vv;
+          if constexpr (KEEP)
+            vv = keep[i];
+          else
+            vv = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)i * a.N + n0);
+#pragma unroll
+          for (int r = 0; r < R; ++r) dacc[i] += (double)vv.v[r] * x[r];
         }
       }
     }
@@ -196,13 +220,13 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
 // assignment and a fixed shuffle tree, so the result is deterministic), then apply the
 // finishing step.
 template <typename BT>
-__global__ __launch_bounds__(kBlock) void k_vec_finish(VecArgs<BT> a, unsigned nblocks, int nvals) {
+__global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigned nblocks, int nvals) {
   if (a.cond && *a.cond == 0.0) return;
   const int nv = a.nv;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __shared__ double tot[kPartStride];
   constexpr int kPer = kMaxRedBlocks / 64;  // partials per lane, all loads issued together
-  for (int k = wave; k < nvals; k += kBlock / 64) {
+  for (int k = wave; k < nvals; k += kFinBlock / 64) {
     double v[kPer];
 #pragma unroll
     for (int t = 0; t < kPer; ++t) {
@@ -221,32 +245,21 @@ __global__ __launch_bounds__(kBlock) void k_vec_finish(VecArgs<BT> a, unsigned n
     double* kb = a.kb;
     const double nrm2 = tot[nvals - 1];
     switch (a.finish) {
-      case FIN_H1:
+      case FIN_H1:  // first projection h1
         for (int i = 0; i < nv; ++i) kb[L.h(i)] = tot[i];
-        kb[L.misc(MISC_S0)] = nrm2;
         break;
-      case FIN_UPD: {
-        // classical Gram-Schmidt done; reorthogonalise only if |x1| < eta |x0| (DGKS)
-        const double s1 = sqrt(nrm2), s0 = sqrt(kb[L.misc(MISC_S0)]);
-        const bool again = s1 >= a.btol && s1 < kDgksEta * s0;
-        kb[L.misc(MISC_FLAG)] = again ? 1.0 : 0.0;
-        if (again) {
-          kb[L.misc(MISC_NREORTH)] += 1.0;
-        } else {
-          for (int i = 0; i < nv; ++i) kb[L.H(i, a.j)] = kb[L.h(i)];
-          kb[L.H(a.j + 1, a.j)] = s1;
-          kb[L.sc(a.j + 1)] = s1 > a.btol ? 1.0 / s1 : 0.0;
+      case FIN_H2:  // second projection: column j of H is h1 + h2, also the update coefficients
+        for (int i = 0; i < nv; ++i) {
+          const double h = kb[L.h(i)] + tot[i];
+          kb[L.H(i, a.j)] = h;
+          kb[L.h(i)] = h;
         }
-      } break;
-      case FIN_H2:
-        for (int i = 0; i < nv; ++i) kb[L.h2(i)] = tot[i];
+        kb[L.misc(MISC_NREORTH)] += 1.0;
         break;
-      case FIN_COL: {
+      case FIN_COL: {  // norm of the new vector
         const double s = sqrt(nrm2);
-        for (int i = 0; i < nv; ++i) kb[L.H(i, a.j)] = kb[L.h(i)] + kb[L.h2(i)];
         kb[L.H(a.j + 1, a.j)] = s;
         kb[L.sc(a.j + 1)] = s > a.btol ? 1.0 / s : 0.0;
-        kb[L.misc(MISC_FLAG)] = 0.0;
       } break;
       case FIN_AVN:
         kb[L.misc(MISC_AVN)] = sqrt(nrm2);
@@ -283,15 +296,20 @@ void launch_vec_mode(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
     throw Error(NC_ERR_UNSUPPORTED, "Krylov dimension above 31 is not supported");
 }
 
+constexpr int kKeepMax = 32;  // largest bucket whose fused pass keeps FP32 basis values in registers
+
 template <typename BT>
 void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
-  if (a.dots && a.coef) throw Error(NC_ERR_INTERNAL, "a vector pass cannot both update and take dot products");
   if (a.N % 4) throw Error(NC_ERR_INTERNAL, "vector passes need a padded length");
-  if (a.dots)
+  if (a.dots && a.coef && sizeof(BT) == 4 && a.nv <= kKeepMax)
+    launch_vec_mode<BT, VM_DOTS2_KEEP>(a, grid, s);
+  else if (a.dots && a.coef)
+    launch_vec_mode<BT, VM_DOTS2>(a, grid, s);
+  else if (a.dots)
     launch_vec_mode<BT, VM_DOTS>(a, grid, s);
   else
     launch_vec_mode<BT, VM_UPDATE>(a, grid, s);
-  if (a.finish != FIN_NONE) k_vec_finish<BT><<<1, kBlock, 0, s>>>(a, grid, (a.dots ? a.nv : 0) + 1);
+  if (a.finish != FIN_NONE) k_vec_finish<BT><<<1, kFinBlock, 0, s>>>(a, grid, (a.dots ? a.nv : 0) + 1);
   NC_CHECK_HIP(hipGetLastError());
 }
 
@@ -394,7 +412,8 @@ double round_step(double x) {
 
 void solver_reduce_setup(nc_solver_s* S, int nvals) {
   (void)nvals;
-  S->red_blocks = std::min<unsigned>(kMaxRedBlocks, grid_for(S->Np / 4, kBlock));
+  // 512 blocks of 256 threads (two per CU) measured best for the vector passes
+  S->red_blocks = std::min<unsigned>(kVecBlocks, grid_for(S->Np / 4, kBlock));
   S->partials.reserve((size_t)S->red_blocks * kPartStride);
   if (!S->hpart)
     NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hpart), sizeof(double) * (kMaxRedBlocks * kPartStride + 4 * kMaxReactions),
@@ -423,32 +442,20 @@ static void arnoldi_iteration(nc_solver_s* S, int j) {
   a.j = j;
   a.btol = kBreakdownTol;
   a.vout_fp32 = std::is_same<BT, float>::value ? 1 : 0;
-  // pass 1: h = V_{0..j}^T x0 and |x0| with x0 = sc_j q
+  // Classical Gram-Schmidt applied twice, in three passes over V_{0..j}:
+  // pass 1: h1 = V^T x0 with x0 = sc_j q
   a.xin = S->q.p;
   a.xscale = S->kbuf.p + L.sc(j);
   a.dots = 1;
   a.finish = FIN_H1;
   launch_vec(a, S->red_blocks, s);
-  // pass 2: x1 = x0 - V h, kept in FP64 (p) and as V_{j+1}; DGKS test on |x1| / |x0|
-  a.dots = 0;
+  // pass 2: x1 = x0 - V h1 formed on the fly and h2 = V^T x1; H(:, j) = h1 + h2
   a.coef = S->kbuf.p + L.h(0);
   a.csign = -1.0;
-  a.xout = S->p.p;
-  a.vout = Vp(j + 1);
-  a.finish = FIN_UPD;
-  launch_vec(a, S->red_blocks, s);
-  // passes 3-4 run only when the test asked for a second orthogonalisation
-  a.cond = S->kbuf.p + L.misc(MISC_FLAG);
-  a.xin = S->p.p;
-  a.xscale = nullptr;
-  a.coef = nullptr;
-  a.xout = nullptr;
-  a.vout = nullptr;
-  a.dots = 1;
   a.finish = FIN_H2;
   launch_vec(a, S->red_blocks, s);
+  // pass 3: V_{j+1} = x0 - V (h1 + h2) and its norm H(j+1, j)
   a.dots = 0;
-  a.coef = S->kbuf.p + L.h2(0);
   a.vout = Vp(j + 1);
   a.finish = FIN_COL;
   launch_vec(a, S->red_blocks, s);
@@ -869,10 +876,8 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
       if (method == NC_METHOD_KRYLOV) {
         S->V.alloc((size_t)(S->m + 1) * S->Np * S->vs);
         S->q.alloc(S->Np);
-        S->p.alloc(S->Np);
         NC_CHECK_HIP(hipMemsetAsync(S->V.p, 0, (size_t)(S->m + 1) * S->Np * S->vs, S->s));
         NC_CHECK_HIP(hipMemsetAsync(S->q.p, 0, sizeof(double) * S->Np, S->s));
-        NC_CHECK_HIP(hipMemsetAsync(S->p.p, 0, sizeof(double) * S->Np, S->s));
         S->coef.alloc(S->m + 2);
       } else {
         S->rkv.alloc(S->Np * S->vs);
