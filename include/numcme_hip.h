@@ -209,7 +209,6 @@ typedef struct {
   int64_t matvecs_timed;
   double matvec_ms;    /* summed event-timed mat-vec kernel durations              */
   double error_estimate;
-  int64_t reorthogonalizations; /* Arnoldi columns that needed a second Gram-Schmidt */
 } nc_solve_stats;
 
 typedef struct nc_solver_s* nc_solver;

@@ -252,7 +252,6 @@ void accumulate_stats(nc_solve_stats& tot, const nc_solver_s* S) {
   tot.matvecs += S->st.matvecs;
   tot.host_syncs += S->st.host_syncs;
   tot.error_estimate += S->st.error_estimate;
-  tot.reorthogonalizations += S->st.reorthogonalizations;
   tot.matvecs_timed += S->A->timed_count;
   tot.matvec_ms += S->A->timed_ms;
 }

@@ -25,13 +25,13 @@ struct nc_solver_s {
   int m = 30;
   nc::DevBuf<unsigned char> V;  // (m+1) x N, VT, unnormalised
   nc::DevBuf<double> q;         // FP64 mat-vec output
-  nc::DevBuf<double> kbuf;      // H (m+2)^2 | h (m+2) | h2 (m+2) | sc (m+2) | misc (8)
+  nc::DevBuf<double> kbuf;      // H (m+2)^2 | h (m+2) | sc (m+2) | misc (8) | sink rows
+  nc::DevBuf<double> gram;      // (m+1)^2 Gram matrix of the scaled basis
   nc::DevBuf<double> coef;      // combination coefficients (m+2)
   double* hbuf = nullptr;       // pinned host copy of kbuf
   double* hcoef = nullptr;      // pinned host staging of coef
   double* hpart = nullptr;      // pinned host copy of per-block partial sums
   double beta = 0.0, t_new = 0.0, anorm = 0.0;
-  int64_t reorth_base = 0;  // device reorthogonalisation counter at the last restart
   bool beta_pending = false;  // beta was updated on the device and not yet read back
   bool started = false;
   hipGraph_t arnoldi_graph = nullptr;  // captured Arnoldi process (replayed every step)

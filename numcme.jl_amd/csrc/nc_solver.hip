@@ -29,27 +29,27 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxRedBlocks = 1024;
-constexpr int kPartStride = 33;  // doubles of partial sums per block (up to 32 dots + a norm)
+constexpr int kPartStrideMax = 65;  // partial sums per block: up to 2 x 32 dots + a norm
 
-enum Finish : int { FIN_NONE = 0, FIN_H1, FIN_H2, FIN_COL, FIN_AVN, FIN_BETA, FIN_SUMS };
+enum Finish : int { FIN_NONE = 0, FIN_ARN, FIN_LAST, FIN_BETA, FIN_SUMS };
 constexpr double kBreakdownTol = 1e-7;   // Expokit's happy-breakdown threshold
 constexpr int kFinBlock = 1024;  // threads of the one-block partial-sum fold
 constexpr unsigned kVecBlocks = 512;  // grid of the vector passes
+constexpr int kJacobiSweeps = 3;  // G h = a with G = I + O(1e-7): the error falls 1e-6 per sweep
 
 // kbuf layout (doubles)
 struct KLayout {
   int m;
   __host__ __device__ int H(int i, int j) const { return i * (m + 2) + j; }
   __host__ __device__ int h(int i) const { return (m + 2) * (m + 2) + i; }
-  __host__ __device__ int h2(int i) const { return (m + 2) * (m + 2) + (m + 2) + i; }
-  __host__ __device__ int sc(int i) const { return (m + 2) * (m + 2) + 2 * (m + 2) + i; }
-  __host__ __device__ int misc(int i) const { return (m + 2) * (m + 2) + 3 * (m + 2) + i; }
+  __host__ __device__ int sc(int i) const { return (m + 2) * (m + 2) + (m + 2) + i; }
+  __host__ __device__ int misc(int i) const { return (m + 2) * (m + 2) + 2 * (m + 2) + i; }
   // sink rows of basis vector j (j <= m) and of the state (j = m+1), R sinks each
   __host__ __device__ int sinkv(int j, int r, int R) const { return misc(8) + j * R + r; }
-  __host__ __device__ int size() const { return (m + 2) * (m + 2) + 3 * (m + 2) + 8; }
+  __host__ __device__ int size() const { return (m + 2) * (m + 2) + 2 * (m + 2) + 8; }
   __host__ __device__ int size_with_sinks(int R) const { return size() + (m + 2) * R; }
 };
-enum { MISC_AVN = 0, MISC_BETA = 1, MISC_NREORTH = 4 };
+enum { MISC_AVN = 0, MISC_BETA = 1 };
 
 template <typename BT>
 struct VecArgs {
@@ -62,14 +62,13 @@ struct VecArgs {
   const double* xin;      // FP64 input (may be null = 0)
   const double* xscale;   // device scalar multiplying xin (may be null = 1)
   double* xout;           // FP64 outputs (nullable)
-  double* xout2;
   void* vout;             // VT output (nullable)
   int vout_fp32;          // 1: vout is float, 0: double
-  int dots, norm;
+  int mode;               // VM_*
   double* partials;
-  const double* cond;     // device flag: when non-null and zero the pass is skipped
   int finish;
   double* kb;
+  double* gram;           // (m+1) x (m+1) Gram matrix of the scaled basis (FIN_ARN)
   KLayout L;
   int j;
   double btol;
@@ -83,15 +82,17 @@ __device__ inline double wave_sum(double v) {
 // Partial sums are stored value-major, partials[k * nblocks + b], so the finish kernel's
 // loads of one value across all blocks are contiguous.
 //
-// x = xscale*xin + csign * sum_i coef[i]*sc[i]*V_i ; optional stores; optional dots
-// d_i = sc[i] * sum_n V_i[n] x[n] and the norm sum_n x[n]^2 as per-block partial sums.
-// MODE picks what a pass needs: VM_DOTS (x = xscale*xin; dots), VM_UPDATE (update, no
-// dots) or VM_DOTS2 (update, then dots of the updated x with the same vectors: the
-// second Gram-Schmidt projection fused with the first update, x never stored). Each
-// thread owns 4 consecutive rows and reads every stream with one 16-byte (FP32) or
-// 32-byte (FP64) vector load per vector; N is a multiple of 4 and all vectors are zero
-// beyond the true length, so no tail handling is needed.
-enum VecMode : int { VM_DOTS = 0, VM_UPDATE = 1, VM_DOTS2 = 2, VM_DOTS2_KEEP = 3 };
+// x = xscale*xin + csign * sum_i coef[i]*sc[i]*V_i, with optional FP64 / VT stores of x,
+// and per-block partial sums of what the pass measures:
+//   VM_DOTS    x = xscale*xin; d_i = sc[i] * sum_n V_i[n] x[n]        (nv dots)
+//   VM_UPDATE  the update, no dots
+//   VM_DUAL    x = xin; p_i = sum_n V_i[n] x[n] and g_i = sum_n V_i[n] V_last[n]
+//              with V_last = V_{nv-1}                                  (2 nv dots)
+// and always the norm sum_n x[n]^2 last. Each thread owns 4 consecutive rows and reads
+// every stream with one 16-byte (FP32) or 32-byte (FP64) vector load per vector; N is a
+// multiple of 4 and all vectors are zero beyond the true length, so no tail handling is
+// needed.
+enum VecMode : int { VM_DOTS = 0, VM_UPDATE = 1, VM_DUAL = 2 };
 
 template <typename T, int R>
 struct alignas(sizeof(T) * R) Pack {
@@ -102,13 +103,9 @@ template <typename BT, int MAXV, int MODE>
 __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   constexpr int MV = MAXV > 0 ? MAXV : 1;
   constexpr bool DOTS = MODE != VM_UPDATE;
-  constexpr bool UPD = MODE != VM_DOTS;
-  // the fused pass either keeps the basis values in registers between its two sweeps
-  // (VM_DOTS2_KEEP) or loads them again (the second load of the same lines hits the caches)
-  constexpr bool FUSED = MODE == VM_DOTS2 || MODE == VM_DOTS2_KEEP;
-  constexpr bool KEEP = MODE == VM_DOTS2_KEEP;
+  constexpr bool UPD = MODE == VM_UPDATE;
+  constexpr bool DUAL = MODE == VM_DUAL;
   constexpr int R = 4;
-  if (a.cond && *a.cond == 0.0) return;  // uniform: the whole grid skips the pass
   const int nv = a.nv;
   // per-vector coefficients live in LDS (uniform broadcast reads)
   __shared__ double s_cs[MV], s_sc[MV];
@@ -120,9 +117,11 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   }
   __syncthreads();
   const double xs = a.xscale ? *a.xscale : 1.0;
-  double dacc[DOTS ? MV : 1];
+  double dacc[DOTS ? MV : 1], gacc[DUAL ? MV : 1];
 #pragma unroll
   for (int i = 0; i < (DOTS ? MV : 1); ++i) dacc[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < (DUAL ? MV : 1); ++i) gacc[i] = 0.0;
   double nacc = 0.0;
   const int64_t groups = a.N / R;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -137,12 +136,12 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
 #pragma unroll
       for (int r = 0; r < R; ++r) x[r] = 0.0;
     }
-    Pack<BT, R> keep[KEEP ? MV : 1];
+    Pack<BT, R> vl;
+    if constexpr (DUAL) vl = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)(nv - 1) * a.N + n0);
 #pragma unroll
     for (int i = 0; i < MAXV; ++i) {
       if (i < nv) {
         const Pack<BT, R> vv = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)i * a.N + n0);
-        if constexpr (KEEP) keep[i] = vv;
         if constexpr (UPD) {
 #pragma unroll
           for (int r = 0; r < R; ++r) x[r] += s_cs[i] * (double)vv.v[r];
@@ -150,19 +149,9 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
 #pragma unroll
           for (int r = 0; r < R; ++r) dacc[i] += (double)vv.v[r] * x[r];
         }
-      }
-    }
-    if constexpr (FUSED) {
+        if constexpr (DUAL) {
 #pragma unroll
-      for (int i = 0; i < MAXV; ++i) {
-        if (i < nv) {
-          Pack<BT, R> vv;
-          if constexpr (KEEP)
-            vv = keep[i];
-          else
-            vv = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)i * a.N + n0);
-#pragma unroll
-          for (int r = 0; r < R; ++r) dacc[i] += (double)vv.v[r] * x[r];
+          for (int r = 0; r < R; ++r) gacc[i] += (double)vv.v[r] * (double)vl.v[r];
         }
       }
     }
@@ -190,24 +179,28 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   }
   if (a.finish == FIN_NONE) return;
   // block reduction of the dots (scaled by sc[i] here) and the norm
-  __shared__ double red[MV + 1][kBlock / 64];
+  __shared__ double red[2 * MV + 1][kBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nvals = (DOTS ? nv : 0) + 1;
+  const int ndots = DOTS ? (DUAL ? 2 * nv : nv) : 0;
   if constexpr (DOTS) {
 #pragma unroll
     for (int i = 0; i < MAXV; ++i) {
       if (i < nv) {
         double v = wave_sum(dacc[i]) * s_sc[i];
         if (lane == 0) red[i][wave] = v;
+        if constexpr (DUAL) {
+          double w = wave_sum(gacc[i]);
+          if (lane == 0) red[nv + i][wave] = w;
+        }
       }
     }
   }
   {
     double v = wave_sum(nacc);
-    if (lane == 0) red[nvals - 1][wave] = v;
+    if (lane == 0) red[ndots][wave] = v;
   }
   __syncthreads();
-  if (threadIdx.x < nvals) {
+  if (threadIdx.x <= ndots) {
     double s = 0.0;
     for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
     a.partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
@@ -216,15 +209,64 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   // visible across XCDs without a per-block L2 writeback fence
 }
 
+// Finish of the first pass of Arnoldi column j (VM_DUAL over V_0..V_j with x = q = A V_j):
+// tot[i] = V_i.q, tot[nv+i] = V_i.V_j. Sets the norm of V_j (H(j, j-1), sc_j), the new
+// column of the Gram matrix G of the scaled basis, and solves G h = a with
+// a_i = sc_i sc_j V_i.q by Jacobi sweeps (G is the identity up to the FP32 rounding of
+// the stored basis). h is column j of H and the update coefficients of the second pass;
+// the new vector V_{j+1} = sc_j q - sum_i h_i sc_i V_i is then orthogonal to the stored
+// basis up to FP64 rounding, without a second projection pass.
+__device__ void arnoldi_finish(const KLayout& L, double* kb, double* gram, int j, double btol, const double* tot) {
+  const int nv = j + 1, ld = L.m + 1;
+  __shared__ double s_scj, s_a[kPartStrideMax], s_h[kPartStrideMax], s_G[32][33];
+  if (threadIdx.x == 0) {
+    if (j >= 1) {
+      const double nj = sqrt(tot[nv + j]);
+      kb[L.H(j, j - 1)] = nj;
+      kb[L.sc(j)] = nj > btol ? 1.0 / nj : 0.0;
+    }
+    s_scj = kb[L.sc(j)];
+  }
+  __syncthreads();
+  const int i = threadIdx.x;
+  const double scj = s_scj;
+  if (i < nv) {
+    const double sci = i == j ? scj : kb[L.sc(i)];
+    double gij = sci * scj * tot[nv + i];
+    if (i == j && scj == 0.0) gij = 1.0;  // a vanished vector: keep G invertible
+    gram[i * ld + j] = gij;
+    gram[j * ld + i] = gij;
+    for (int k = 0; k < j && i < j; ++k) s_G[i][k] = gram[i * ld + k];
+    s_G[i][j] = gij;
+    s_G[j][i] = gij;
+    const double ai = sci * scj * tot[i];
+    s_a[i] = ai;
+    s_h[i] = ai;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < kJacobiSweeps; ++sweep) {
+    double r = 0.0;
+    if (i < nv) {
+      r = s_a[i];
+      for (int k = 0; k < nv; ++k) r -= (k == i ? s_G[i][k] - 1.0 : s_G[i][k]) * s_h[k];
+    }
+    __syncthreads();
+    if (i < nv) s_h[i] = r;
+    __syncthreads();
+  }
+  if (i < nv) {
+    kb[L.H(i, j)] = s_h[i];
+    kb[L.h(i)] = s_h[i];
+  }
+}
+
 // One block: fold the per-block partials (one wave per value, fixed lane->block
 // assignment and a fixed shuffle tree, so the result is deterministic), then apply the
 // finishing step.
 template <typename BT>
 __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigned nblocks, int nvals) {
-  if (a.cond && *a.cond == 0.0) return;
-  const int nv = a.nv;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __shared__ double tot[kPartStride];
+  __shared__ double tot[kPartStrideMax];
   constexpr int kPer = kMaxRedBlocks / 64;  // partials per lane, all loads issued together
   for (int k = wave; k < nvals; k += kFinBlock / 64) {
     double v[kPer];
@@ -240,30 +282,25 @@ __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigne
     if (lane == 0) tot[k] = s;
   }
   __syncthreads();
+  const KLayout& L = a.L;
+  double* kb = a.kb;
+  if (a.finish == FIN_ARN) {
+    arnoldi_finish(L, kb, a.gram, a.j, a.btol, tot);
+    return;
+  }
   if (threadIdx.x == 0) {
-    const KLayout& L = a.L;
-    double* kb = a.kb;
     const double nrm2 = tot[nvals - 1];
     switch (a.finish) {
-      case FIN_H1:  // first projection h1
-        for (int i = 0; i < nv; ++i) kb[L.h(i)] = tot[i];
-        break;
-      case FIN_H2:  // second projection: column j of H is h1 + h2, also the update coefficients
-        for (int i = 0; i < nv; ++i) {
-          const double h = kb[L.h(i)] + tot[i];
-          kb[L.H(i, a.j)] = h;
-          kb[L.h(i)] = h;
-        }
-        kb[L.misc(MISC_NREORTH)] += 1.0;
-        break;
-      case FIN_COL: {  // norm of the new vector
-        const double s = sqrt(nrm2);
-        kb[L.H(a.j + 1, a.j)] = s;
-        kb[L.sc(a.j + 1)] = s > a.btol ? 1.0 / s : 0.0;
+      case FIN_LAST: {
+        // VM_DUAL over V_m alone with x = A V_m: norm of V_m and |A v_m| for the error
+        // estimate (avnorm of Expokit)
+        const int m = L.m;
+        const double nm = sqrt(tot[1]);
+        const double scm = nm > a.btol ? 1.0 / nm : 0.0;
+        kb[L.H(m, m - 1)] = nm;
+        kb[L.sc(m)] = scm;
+        kb[L.misc(MISC_AVN)] = scm * sqrt(nrm2);
       } break;
-      case FIN_AVN:
-        kb[L.misc(MISC_AVN)] = sqrt(nrm2);
-        break;
       case FIN_BETA: {
         const double b = sqrt(nrm2);
         kb[L.misc(MISC_BETA)] = b;
@@ -296,20 +333,16 @@ void launch_vec_mode(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
     throw Error(NC_ERR_UNSUPPORTED, "Krylov dimension above 31 is not supported");
 }
 
-constexpr int kKeepMax = 32;  // largest bucket whose fused pass keeps FP32 basis values in registers
-
 template <typename BT>
 void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
   if (a.N % 4) throw Error(NC_ERR_INTERNAL, "vector passes need a padded length");
-  if (a.dots && a.coef && sizeof(BT) == 4 && a.nv <= kKeepMax)
-    launch_vec_mode<BT, VM_DOTS2_KEEP>(a, grid, s);
-  else if (a.dots && a.coef)
-    launch_vec_mode<BT, VM_DOTS2>(a, grid, s);
-  else if (a.dots)
-    launch_vec_mode<BT, VM_DOTS>(a, grid, s);
-  else
-    launch_vec_mode<BT, VM_UPDATE>(a, grid, s);
-  if (a.finish != FIN_NONE) k_vec_finish<BT><<<1, kFinBlock, 0, s>>>(a, grid, (a.dots ? a.nv : 0) + 1);
+  int nvals = 1;
+  switch (a.mode) {
+    case VM_DOTS: launch_vec_mode<BT, VM_DOTS>(a, grid, s); nvals = a.nv + 1; break;
+    case VM_DUAL: launch_vec_mode<BT, VM_DUAL>(a, grid, s); nvals = 2 * a.nv + 1; break;
+    default: launch_vec_mode<BT, VM_UPDATE>(a, grid, s); break;
+  }
+  if (a.finish != FIN_NONE) k_vec_finish<BT><<<1, kFinBlock, 0, s>>>(a, grid, nvals);
   NC_CHECK_HIP(hipGetLastError());
 }
 
@@ -414,16 +447,21 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
   (void)nvals;
   // 512 blocks of 256 threads (two per CU) measured best for the vector passes
   S->red_blocks = std::min<unsigned>(kVecBlocks, grid_for(S->Np / 4, kBlock));
-  S->partials.reserve((size_t)S->red_blocks * kPartStride);
+  S->partials.reserve((size_t)S->red_blocks * kPartStrideMax);
   if (!S->hpart)
-    NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hpart), sizeof(double) * (kMaxRedBlocks * kPartStride + 4 * kMaxReactions),
+    NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hpart), sizeof(double) * (kMaxRedBlocks + 4 * kMaxReactions),
                                hipHostMallocDefault));
 }
 
 // ---- Krylov expv ---------------------------------------------------------------------
-// The device work of one Arnoldi process (m mat-vecs, CGS2 and the |A v_m| norm). Its
-// launches and arguments are identical for every Krylov step, so it is captured once
-// into a HIP graph and replayed.
+// Arnoldi column j in two passes over the stored basis V_0..V_j (FP32 or FP64, kept
+// unnormalised with scales sc_i):
+//   pass 1 (VM_DUAL):   V_i.q and V_i.V_j for q = A V_j; the finish sets |V_j|, the Gram
+//                       column of V_j and solves G h = a, a_i = sc_i sc_j V_i.q
+//   pass 2 (VM_UPDATE): V_{j+1} = sc_j q - sum_i h_i sc_i V_i
+// Projecting with the Gram matrix of the stored vectors makes V_{j+1} orthogonal to them
+// to FP64 rounding in one projection: the work a second classical Gram-Schmidt pass
+// would do, without a third pass over V.
 template <typename BT>
 static void arnoldi_iteration(nc_solver_s* S, int j) {
   const int64_t N = S->Np;
@@ -435,59 +473,67 @@ static void arnoldi_iteration(nc_solver_s* S, int j) {
   a.N = N;
   a.nv = j + 1;
   a.V = Vp(0);
-  a.sc = S->kbuf.p + L.sc(0);
   a.partials = S->partials.p;
   a.kb = S->kbuf.p;
+  a.gram = S->gram.p;
   a.L = L;
   a.j = j;
   a.btol = kBreakdownTol;
   a.vout_fp32 = std::is_same<BT, float>::value ? 1 : 0;
-  // Classical Gram-Schmidt applied twice, in three passes over V_{0..j}:
-  // pass 1: h1 = V^T x0 with x0 = sc_j q
+  a.mode = VM_DUAL;
   a.xin = S->q.p;
-  a.xscale = S->kbuf.p + L.sc(j);
-  a.dots = 1;
-  a.finish = FIN_H1;
+  a.finish = FIN_ARN;
   launch_vec(a, S->red_blocks, s);
-  // pass 2: x1 = x0 - V h1 formed on the fly and h2 = V^T x1; H(:, j) = h1 + h2
+  a.mode = VM_UPDATE;
+  a.sc = S->kbuf.p + L.sc(0);
+  a.xscale = S->kbuf.p + L.sc(j);
   a.coef = S->kbuf.p + L.h(0);
   a.csign = -1.0;
-  a.finish = FIN_H2;
-  launch_vec(a, S->red_blocks, s);
-  // pass 3: V_{j+1} = x0 - V (h1 + h2) and its norm H(j+1, j)
-  a.dots = 0;
   a.vout = Vp(j + 1);
-  a.finish = FIN_COL;
+  a.finish = FIN_NONE;
   launch_vec(a, S->red_blocks, s);
 }
 
-// The device work of one Arnoldi process (m mat-vecs, Gram-Schmidt and the |A v_m|
-// norm). Its launches and arguments are identical for every Krylov step, so it is
+template <typename BT>
+static void arnoldi_last(nc_solver_s* S) {
+  // |V_m| (H(m, m-1), sc_m) and avnorm = |A v_m| in one VM_DUAL pass over V_m
+  const KLayout L{S->m};
+  BT* vm = reinterpret_cast<BT*>(S->V.p + (size_t)S->m * S->Np * S->vs);
+  matvec_launch(S->A, 0.0, vm, S->q.p, false, S->s, true);
+  VecArgs<BT> a{};
+  a.N = S->Np;
+  a.nv = 1;
+  a.V = vm;
+  a.mode = VM_DUAL;
+  a.xin = S->q.p;
+  a.partials = S->partials.p;
+  a.kb = S->kbuf.p;
+  a.L = L;
+  a.btol = kBreakdownTol;
+  a.finish = FIN_LAST;
+  launch_vec(a, S->red_blocks, S->s);
+}
+
+// The device work of one Arnoldi process (m + 1 mat-vecs, the projections and the
+// |A v_m| norm). Its launches and arguments are identical for every Krylov step, so it is
 // captured once into a HIP graph and replayed.
 static void arnoldi_launch(nc_solver_s* S) {
   const KLayout L{S->m};
+  const bool fp32 = S->A->precision == NC_FP32;
   // V_0 (raw) and sc[0] = 1/beta are already on the device
   for (int j = 0; j < S->m; ++j) {
-    if (S->A->precision == NC_FP32)
+    if (fp32)
       arnoldi_iteration<float>(S, j);
     else
       arnoldi_iteration<double>(S, j);
   }
-  // avnorm = |A v_{m+1}|
-  matvec_launch(S->A, 0.0, S->V.p + (size_t)S->m * S->Np * S->vs, S->q.p, false, S->s, true);
-  VecArgs<double> a{};
-  a.N = S->Np;
-  a.nv = 0;
-  a.xin = S->q.p;
-  a.xscale = S->kbuf.p + L.sc(S->m);
-  a.partials = S->partials.p;
-  a.kb = S->kbuf.p;
-  a.L = L;
-  a.finish = FIN_AVN;
-  launch_vec(a, S->red_blocks, S->s);
+  if (fp32)
+    arnoldi_last<float>(S);
+  else
+    arnoldi_last<double>(S);
   const int R = S->A->nr;
   double* dst = S->kbuf.p + L.sinkv(0, 0, R);
-  if (S->A->precision == NC_FP32)
+  if (fp32)
     k_gather_sinks<float><<<grid_for((int64_t)(S->m + 2) * R, kBlock), kBlock, 0, S->s>>>(reinterpret_cast<const float*>(S->V.p), S->Np, S->m + 1,
                                                        S->u.p, S->A->n, R, dst);
   else
@@ -560,7 +606,6 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
     S->st.host_syncs++;
     matvec_harvest_timing(A);
     const double* hb = S->hbuf;
-    S->st.reorthogonalizations = (int64_t)hb[L.misc(MISC_NREORTH)] - S->reorth_base;
     if (S->beta_pending) {
       S->beta = hb[L.misc(MISC_BETA)];
       S->beta_pending = false;
@@ -669,6 +714,7 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
         a.kb = S->kbuf.p;
         a.L = L;
         a.finish = FIN_BETA;
+        a.mode = VM_UPDATE;
       };
       if (fp32) {
         VecArgs<float> a{};
@@ -879,6 +925,8 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
         NC_CHECK_HIP(hipMemsetAsync(S->V.p, 0, (size_t)(S->m + 1) * S->Np * S->vs, S->s));
         NC_CHECK_HIP(hipMemsetAsync(S->q.p, 0, sizeof(double) * S->Np, S->s));
         S->coef.alloc(S->m + 2);
+        S->gram.alloc((size_t)(S->m + 1) * (S->m + 1));
+        NC_CHECK_HIP(hipMemsetAsync(S->gram.p, 0, sizeof(double) * (S->m + 1) * (S->m + 1), S->s));
       } else {
         S->rkv.alloc(S->Np * S->vs);
         S->k.alloc((size_t)7 * S->Np);
@@ -906,7 +954,6 @@ static void solver_reinit(nc_solver_s* S) {
     S->fsal_valid = false;
     S->started = false;
     S->h = 0.0;
-    S->reorth_base += S->st.reorthogonalizations;
     S->st = nc_solve_stats{};
     if (S->method == NC_METHOD_KRYLOV) {
       S->beta = S->beta0;

@@ -95,7 +95,6 @@ class nc_solve_stats(C.Structure):
         ("matvecs_timed", C.c_int64),
         ("matvec_ms", C.c_double),
         ("error_estimate", C.c_double),
-        ("reorthogonalizations", C.c_int64),
     ]
 
 
