@@ -217,9 +217,13 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
 // the new vector V_{j+1} = sc_j q - sum_i h_i sc_i V_i is then orthogonal to the stored
 // basis up to FP64 rounding, without a second projection pass.
 __device__ void arnoldi_finish(const KLayout& L, double* kb, double* gram, int j, double btol, const double* tot) {
+  // 1024 threads as a 32 x 32 grid (i, k) over the Gram block; the Jacobi sweeps reduce
+  // each row over k with shuffles inside its 32-lane group
   const int nv = j + 1, ld = L.m + 1;
-  __shared__ double s_scj, s_a[kPartStrideMax], s_h[kPartStrideMax], s_G[32][33];
-  if (threadIdx.x == 0) {
+  __shared__ double s_scj, s_a[32], s_h[32], s_G[32][33];
+  const int tid = threadIdx.x, i = tid >> 5, k = tid & 31;
+  if (i < j && k < j) s_G[i][k] = gram[i * ld + k];
+  if (tid == 0) {
     if (j >= 1) {
       const double nj = sqrt(tot[nv + j]);
       kb[L.H(j, j - 1)] = nj;
@@ -228,35 +232,32 @@ __device__ void arnoldi_finish(const KLayout& L, double* kb, double* gram, int j
     s_scj = kb[L.sc(j)];
   }
   __syncthreads();
-  const int i = threadIdx.x;
   const double scj = s_scj;
-  if (i < nv) {
-    const double sci = i == j ? scj : kb[L.sc(i)];
-    double gij = sci * scj * tot[nv + i];
-    if (i == j && scj == 0.0) gij = 1.0;  // a vanished vector: keep G invertible
-    gram[i * ld + j] = gij;
-    gram[j * ld + i] = gij;
-    for (int k = 0; k < j && i < j; ++k) s_G[i][k] = gram[i * ld + k];
-    s_G[i][j] = gij;
-    s_G[j][i] = gij;
-    const double ai = sci * scj * tot[i];
-    s_a[i] = ai;
-    s_h[i] = ai;
+  if (tid < nv) {
+    const int r = tid;
+    const double scr = r == j ? scj : kb[L.sc(r)];
+    double grj = scr * scj * tot[nv + r];
+    if (r == j && scj == 0.0) grj = 1.0;  // a vanished vector: keep G invertible
+    gram[r * ld + j] = grj;
+    gram[j * ld + r] = grj;
+    s_G[r][j] = grj;
+    s_G[j][r] = grj;
+    const double ar = scr * scj * tot[r];
+    s_a[r] = ar;
+    s_h[r] = ar;
   }
   __syncthreads();
   for (int sweep = 0; sweep < kJacobiSweeps; ++sweep) {
-    double r = 0.0;
-    if (i < nv) {
-      r = s_a[i];
-      for (int k = 0; k < nv; ++k) r -= (k == i ? s_G[i][k] - 1.0 : s_G[i][k]) * s_h[k];
-    }
+    double t = 0.0;
+    if (i < nv && k < nv) t = (k == i ? s_G[i][k] - 1.0 : s_G[i][k]) * s_h[k];
+    for (int off = 16; off > 0; off >>= 1) t += __shfl_down(t, off, 32);
     __syncthreads();
-    if (i < nv) s_h[i] = r;
+    if (k == 0 && i < nv) s_h[i] = s_a[i] - t;
     __syncthreads();
   }
-  if (i < nv) {
-    kb[L.H(i, j)] = s_h[i];
-    kb[L.h(i)] = s_h[i];
+  if (tid < nv) {
+    kb[L.H(tid, j)] = s_h[tid];
+    kb[L.h(tid)] = s_h[tid];
   }
 }
 

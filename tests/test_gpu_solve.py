@@ -36,7 +36,10 @@ def test_telegraph_c1(gpu, method):
         assert np.array_equal(out.p[k].states, np.asarray(states))
         assert l1(out.p[k].values, ref.p[k]) <= 1e-8
         assert abs(out.sinks[k].sum() - ref.sinks[k].sum()) <= 1e-10
-        assert abs(out.p[k].sum() + out.sinks[k].sum() - 1.0) < 1e-10
+        # mass is conserved up to the integrator's accumulated local error (the Krylov
+        # truncation leaks O(tol) per unit time); the reference's test asks for isapprox,
+        # i.e. 1.5e-8 (test/test_solver.jl:63-65)
+        assert abs(out.p[k].sum() + out.sinks[k].sum() - 1.0) < 1e-9
 
 
 def toggle_box_reference(ts, box=160):
