@@ -234,6 +234,33 @@ int nc_solve_fixed(nc_matrix A, const nc_solve_opts* opts, const double* u0, int
                    const double* t_out, double* u_out, nc_solve_stats* st);
 
 /* ======================================================================================
+ * Forward sensitivities — replaces ForwardSensFspMatrixSparse
+ * (src/forwardsensfspmatrix/forwardsensfspmatrixsparse/sensfspmatrixsparse.jl:31-142) and
+ * the fixed-space form of the sensitivity solve (src/forwardsenscme/sparse/
+ * forwardsenscmesparse.jl:99-215). Vectors hold n_params+1 blocks of n_rows entries,
+ * block-major: [p; sinks; S_1; dsinks_1; ...] as the reference's u (:132-138).
+ * ==================================================================================== */
+typedef struct nc_sensmatrix_s* nc_sensmatrix;
+/* ForwardSensFspMatrixSparse{RealT}(model, statespace) — sensfspmatrixsparse.jl:31-95. The
+ * model needs its gradients (nc_model_set_gradients); at most 31 parameters and 128
+ * (reaction, parameter) pairs. */
+int nc_sensmatrix_create(nc_space space, nc_model model, int32_t precision, nc_sensmatrix* out);
+int nc_sensmatrix_destroy(nc_sensmatrix SA);
+/* Rows per block, parameter count, derivative pairs and the bytes one mat-vec moves. */
+int nc_sensmatrix_info(nc_sensmatrix SA, int64_t* n_rows, int32_t* n_params, int32_t* n_pairs,
+                       double* bytes_per_matvec);
+/* matvec!(out, t, SA, vs) — sensfspmatrixsparse.jl:97-142. v, out: (n_params+1) x n_rows. */
+int nc_sensmatrix_matvec(nc_sensmatrix SA, double t, const double* v, double* out);
+/* Mean duration (ms) of `reps` back-to-back launches. No reference counterpart. */
+int nc_sensmatrix_bench_matvec(nc_sensmatrix SA, double t, int32_t reps, double* mean_ms);
+/* A solver over the sensitivity system; nc_solver_set_state / run / get_snapshot then take
+ * (n_params+1) x n_rows vectors. */
+int nc_solver_create_sens(nc_sensmatrix SA, const nc_solve_opts* opts, nc_solver* out);
+/* Fixed-space sensitivity solve. u0: (n_params+1) x n_rows; u_out: n_out of those. */
+int nc_solve_sens_fixed(nc_sensmatrix SA, const nc_solve_opts* opts, const double* u0, int32_t n_out,
+                        const double* t_out, double* u_out, nc_solve_stats* st);
+
+/* ======================================================================================
  * Adaptive FSP — replaces solve(model, p0, tspan, ::AdaptiveFspSparse; saveat, fsptol,
  * odeatol, odertol) (fspsolve.jl:59-62,105-197) and the space adapters RStepAdapter /
  * SelectiveRStepAdapter with their init! / adapt! (rstepadapters.jl:12-110).
@@ -281,6 +308,14 @@ int nc_adaptive_opts_default(nc_adaptive_opts* opts);
 int nc_solve_adaptive(nc_model model, int64_t n0, const int32_t* states0, const double* p0, double t0, double t1,
                       int32_t n_save, const double* saveat, const nc_adaptive_opts* opts, nc_output* out,
                       nc_solve_stats* st);
+/* Adaptive sensitivity solve — solve(model::CmeModelWithSensitivity, initial_condition,
+ * tspan, ::AdaptiveForwardSensFspSparse; ...) (forwardsenscmesparse.jl:99-215) with the
+ * ForwardSensRStepAdapter (fsspaceadapterssparse.jl:10-60; opts->adapter.kind must be
+ * NC_ADAPTER_RSTEP). S0: n_params x n0 initial sensitivities, row-major. The model needs
+ * its gradients. Snapshots then also carry S and dsinks (nc_output_get_sens). */
+int nc_solve_sens_adaptive(nc_model model, int64_t n0, const int32_t* states0, const double* p0,
+                           const double* S0, double t0, double t1, int32_t n_save, const double* saveat,
+                           const nc_adaptive_opts* opts, nc_output* out, nc_solve_stats* st);
 int nc_output_destroy(nc_output o);
 /* Number of snapshots, and of state spaces (adaptations + 1). */
 int nc_output_count(nc_output o, int32_t* n_snapshots, int32_t* n_spaces);
@@ -289,6 +324,9 @@ int nc_output_info(nc_output o, int32_t k, double* t, int64_t* n_states, int32_t
 /* Copy snapshot k: states (n_states x ns int32), p (n_states) and sinks (nr); any output
  * may be null. */
 int nc_output_get(nc_output o, int32_t k, int32_t* states, double* p, double* sinks);
+/* Copy the sensitivity of snapshot k to parameter `param` (0-based): S (n_states) and
+ * dsinks (nr); either may be null. */
+int nc_output_get_sens(nc_output o, int32_t k, int32_t param, double* S, double* dsinks);
 
 /* ---- helpers exposed for testing --------------------------------------------------- */
 /* Dense matrix exponential of an n x n row-major matrix (Pade 13 scaling and squaring,

@@ -208,6 +208,7 @@ std::vector<int32_t> selective_reactions(const double* dsinks, int nr) {
 // ---- adaptive solve output ----------------------------------------------------------------
 struct nc_output_s {
   int ns = 0, nr = 0;
+  int P = 0;  // sensitivity parameters (each snapshot holds P+1 blocks of n + nr)
   struct Space {
     int64_t n = 0;
     nc::DevBuf<int32_t> states;
@@ -215,7 +216,7 @@ struct nc_output_s {
   struct Snap {
     double t = 0.0;
     int32_t space = 0;
-    nc::DevBuf<double> u;  // [p; sinks]
+    nc::DevBuf<double> u;  // [p; sinks] (and [S_i; dsinks_i] blocks for sensitivities)
   };
   std::vector<std::unique_ptr<Space>> spaces;
   std::vector<std::unique_ptr<Snap>> snaps;
@@ -237,12 +238,14 @@ void output_add_space(nc_output_s* o, nc_space_s* sp) {
   o->spaces.push_back(std::move(S));
 }
 
-void output_add_snap(nc_output_s* o, double t, const double* u_dev, int64_t len, hipStream_t s) {
+// Snapshot of the solver's current state, stored with block stride Nb (unpadded).
+void output_add_snap(nc_output_s* o, double t, const nc_solver_s* S) {
   auto P = std::make_unique<nc_output_s::Snap>();
   P->t = t;
   P->space = (int32_t)o->spaces.size() - 1;
-  P->u.alloc((size_t)std::max<int64_t>(len, 1));
-  NC_CHECK_HIP(hipMemcpyAsync(P->u.p, u_dev, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
+  P->u.alloc((size_t)std::max<int64_t>(S->N, 1));
+  NC_CHECK_HIP(hipMemcpy2DAsync(P->u.p, sizeof(double) * S->Nb, S->u.p, sizeof(double) * S->Npb,
+                                sizeof(double) * S->Nb, S->blocks, hipMemcpyDeviceToDevice, S->s));
   o->snaps.push_back(std::move(P));
 }
 
@@ -279,22 +282,28 @@ std::vector<double> sink_derivative(nc_matrix_s* A, double t, const double* u, h
 // adapt!: optional drop (the kept probabilities are compacted into p_kept), then expand!.
 // p is a device vector of the space's n states. Returns the number dropped; the sorted
 // dropped ids are left in `ids` when requested.
-int64_t adapt_space(nc_space_s* sp, const nc_adapter& ad, const double* p, const double* dsinks, double t, double tend,
-                    double fsptol, DropWork& w, DevBuf<int64_t>& ids, DevBuf<double>& p_kept) {
+// vecs[0] is p; further vectors (sensitivities) are dropped alongside it
+// (fsspaceadapterssparse.jl:42-58). kept[b] receives the compacted vecs[b].
+int64_t adapt_space(nc_space_s* sp, const nc_adapter& ad, const std::vector<const double*>& vecs,
+                    const double* dsinks, double t, double tend, double fsptol, DropWork& w, DevBuf<int64_t>& ids,
+                    std::vector<DevBuf<double>>& kept) {
   hipStream_t s = sp->stream.s;
   const int64_t n = sp->n;
+  const double* p = vecs[0];
   int64_t m = 0;
   if (ad.dropstates) {
     const double thr = 1.0 - t * fsptol / tend;
     m = select_drop(p, n, thr, ad.kind == NC_ADAPTER_SELECTIVE_RSTEP, s, w, ids);
   }
-  p_kept.reserve((size_t)std::max<int64_t>(n - m, 1));
+  kept.resize(vecs.size());
+  for (auto& k : kept) k.reserve((size_t)std::max<int64_t>(n - m, 1));
   if (m) {
-    double* outs[1] = {p_kept.p};
-    const double* ins[1] = {p};
-    space_delete_device(sp, ids.p, m, 1, ins, outs);
+    std::vector<double*> outs;
+    for (auto& k : kept) outs.push_back(k.p);
+    space_delete_device(sp, ids.p, m, (int)vecs.size(), vecs.data(), outs.data());
   } else if (n) {
-    NC_CHECK_HIP(hipMemcpyAsync(p_kept.p, p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    for (size_t b = 0; b < vecs.size(); ++b)
+      NC_CHECK_HIP(hipMemcpyAsync(kept[b].p, vecs[b], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   }
   const std::vector<int32_t> rl =
       ad.kind == NC_ADAPTER_SELECTIVE_RSTEP ? selective_reactions(dsinks, sp->nr) : all_reactions(sp->nr);
@@ -309,19 +318,25 @@ bool is_time_varying(const nc_model_s* m) {
   return false;
 }
 
-void solve_adaptive_impl(nc_model_s* model, int64_t n0, const int32_t* states0, const double* p0, double ta,
-                         double tb, int32_t n_save, const double* saveat, const nc_adaptive_opts& o,
-                         nc_output_s* out, nc_solve_stats* st_out) {
-  const int ns = model->ns, nr = model->nr;
+// The adaptive solve (fspsolve.jl:105-197), and with P > 0 the adaptive sensitivity solve
+// (forwardsenscmesparse.jl:99-215) whose state holds P+1 blocks [p; sinks], [S_i; dsinks_i].
+void solve_adaptive_impl(nc_model_s* model, int P, int64_t n0, const int32_t* states0, const double* p0,
+                         const double* S0, double ta, double tb, int32_t n_save, const double* saveat,
+                         const nc_adaptive_opts& o, nc_output_s* out, nc_solve_stats* st_out) {
+  const int ns = model->ns, nr = model->nr, B = P + 1;
   const double tstart = std::min(ta, tb), tend = std::max(ta, tb);
   NC_REQUIRE(std::isfinite(tstart) && std::isfinite(tend), NC_ERR_ARG, "tspan must be finite");
   NC_REQUIRE(o.fsptol > 0.0 && std::isfinite(o.fsptol), NC_ERR_ARG, "fsptol must be positive");
   NC_REQUIRE(n_save >= 0 && (n_save == 0 || saveat), NC_ERR_ARG, "bad saveat");
   for (int k = 1; k < n_save; ++k) NC_REQUIRE(saveat[k] >= saveat[k - 1], NC_ERR_ARG, "saveat must be ascending");
   NC_REQUIRE(n0 >= 1 && states0 && p0, NC_ERR_ARG, "empty initial distribution");
+  NC_REQUIRE(P == 0 || S0, NC_ERR_ARG, "null initial sensitivities");
   check_adapter(&o.adapter);
+  NC_REQUIRE(P == 0 || o.adapter.kind == NC_ADAPTER_RSTEP, NC_ERR_ARG,
+             "sensitivity solves adapt with the R-step rule (ForwardSensRStepAdapter)");
   out->ns = ns;
   out->nr = nr;
+  out->P = P;
 
   // statespace = StateSpaceSparse(stoich, p0.states); init!(...) (fspsolve.jl:122-125)
   std::unique_ptr<nc_space_s> sp(space_create(ns, nr, model->stoich.data(), n0, states0));
@@ -329,10 +344,14 @@ void solve_adaptive_impl(nc_model_s* model, int64_t n0, const int32_t* states0, 
   space_expand_list(sp.get(), o.adapter.initial_step_count, all_reactions(nr));
   NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
 
-  // unow = [p0; zeros; zeros(sink_count)]
-  DevBuf<double> ucur((size_t)(sp->n + nr));
-  NC_CHECK_HIP(hipMemsetAsync(ucur.p, 0, sizeof(double) * (sp->n + nr), sp->stream.s));
+  // unow = [p0; zeros; zeros(sink_count)] per block (forwardsenscmesparse.jl:132-138)
+  int64_t Nb = sp->n + nr;
+  DevBuf<double> ucur((size_t)(B * Nb));
+  NC_CHECK_HIP(hipMemsetAsync(ucur.p, 0, sizeof(double) * B * Nb, sp->stream.s));
   NC_CHECK_HIP(hipMemcpyAsync(ucur.p, p0, sizeof(double) * n0, hipMemcpyHostToDevice, sp->stream.s));
+  for (int b = 1; b < B; ++b)
+    NC_CHECK_HIP(hipMemcpyAsync(ucur.p + b * Nb, S0 + (size_t)(b - 1) * n0, sizeof(double) * n0,
+                                hipMemcpyHostToDevice, sp->stream.s));
   NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
 
   nc_solve_opts ode = o.ode;
@@ -341,16 +360,24 @@ void solve_adaptive_impl(nc_model_s* model, int64_t n0, const int32_t* states0, 
   nc_solve_stats tot{};
   DropWork work;
   DevBuf<int64_t> ids;
-  DevBuf<double> p_kept;
+  std::vector<DevBuf<double>> kept;
   double tnow = tstart;
   int64_t adaptations = 0;
   int save_idx = 0;  // first saveat point not yet passed
   while (tnow < tend) {
-    std::unique_ptr<nc_matrix_s> A(matrix_create(sp.get(), model, o.precision));
-    std::unique_ptr<nc_solver_s> S(solver_create(A.get(), &ode));
+    std::unique_ptr<nc_matrix_s> Aown;
+    std::unique_ptr<nc_sensmatrix_s> SA;
+    nc_matrix_s* A = nullptr;
+    if (P) {
+      SA.reset(sensmatrix_create(sp.get(), model, o.precision));
+      A = SA->A;
+    } else {
+      Aown.reset(matrix_create(sp.get(), model, o.precision));
+      A = Aown.get();
+    }
+    std::unique_ptr<nc_solver_s> S(solver_create(A, &ode, SA.get()));
     solver_set_state_device(S.get(), tnow, ucur.p);
     output_add_space(out, sp.get());
-    const int64_t N = A->n_rows;
     EventSpec ev{o.fsptol, tend, false};
     auto advance = [&](double target, bool one_step) {
       if (S->method == NC_METHOD_KRYLOV)
@@ -362,7 +389,7 @@ void solve_adaptive_impl(nc_model_s* model, int64_t n0, const int32_t* states0, 
       // saveat semantics of DifferentialEquations: the start is saved when it is a
       // saveat point, then every point in (tnow, tend] the integration passes
       for (int k = 0; k < n_save; ++k)
-        if (saveat[k] == tnow) output_add_snap(out, tnow, S->u.p, N, S->s);
+        if (saveat[k] == tnow) output_add_snap(out, tnow, S.get());
       while (save_idx < n_save && saveat[save_idx] <= tnow) ++save_idx;
       for (;;) {
         const bool to_point = save_idx < n_save && saveat[save_idx] <= tend;
@@ -370,10 +397,10 @@ void solve_adaptive_impl(nc_model_s* model, int64_t n0, const int32_t* states0, 
         advance(target, false);
         if (ev.fired) break;
         if (to_point) {
-          output_add_snap(out, S->t, S->u.p, N, S->s);
+          output_add_snap(out, S->t, S.get());
           ++save_idx;
           while (save_idx < n_save && saveat[save_idx] == saveat[save_idx - 1]) {
-            output_add_snap(out, S->t, S->u.p, N, S->s);
+            output_add_snap(out, S->t, S.get());
             ++save_idx;
           }
         }
@@ -381,10 +408,10 @@ void solve_adaptive_impl(nc_model_s* model, int64_t n0, const int32_t* states0, 
       }
     } else {
       // empty saveat: every integrator step is saved, the start included
-      output_add_snap(out, tnow, S->u.p, N, S->s);
+      output_add_snap(out, tnow, S.get());
       for (;;) {
         advance(tend, true);
-        output_add_snap(out, S->t, S->u.p, N, S->s);
+        output_add_snap(out, S->t, S.get());
         if (ev.fired || S->t >= tend) break;
       }
     }
@@ -396,31 +423,40 @@ void solve_adaptive_impl(nc_model_s* model, int64_t n0, const int32_t* states0, 
                  "adaptive FSP exceeded max_adaptations");
       ++adaptations;
       const int64_t n_old = A->n;
-      std::vector<double> sinks(nr);
-      NC_CHECK_HIP(hipMemcpy(sinks.data(), S->u.p + n_old, sizeof(double) * nr, hipMemcpyDeviceToHost));
+      // sinks of every block (block 0: sink probabilities, others: their derivatives)
+      std::vector<double> sinks((size_t)B * nr);
+      NC_CHECK_HIP(hipMemcpy2D(sinks.data(), sizeof(double) * nr, S->u.p + n_old, sizeof(double) * S->Npb,
+                               sizeof(double) * nr, B, hipMemcpyDeviceToHost));
       std::vector<double> dsinks;
-      if (o.adapter.kind == NC_ADAPTER_SELECTIVE_RSTEP) dsinks = sink_derivative(A.get(), tnow, S->u.p, S->s);
+      if (o.adapter.kind == NC_ADAPTER_SELECTIVE_RSTEP) dsinks = sink_derivative(A, tnow, S->u.p, S->s);
+      std::vector<const double*> vecs;
+      for (int b = 0; b < B; ++b) vecs.push_back(S->u.p + b * S->Npb);
       const int64_t n_kept =
-          n_old - adapt_space(sp.get(), o.adapter, S->u.p, dsinks.data(), tnow, tend, o.fsptol, work, ids, p_kept);
+          n_old - adapt_space(sp.get(), o.adapter, vecs, dsinks.data(), tnow, tend, o.fsptol, work, ids, kept);
       // numerical trick of fspsolve.jl:178-181: restart strictly below the budget
       double ssum = 0.0;
       for (int r = 0; r < nr; ++r) ssum += sinks[r];
       if (ssum >= tnow * o.fsptol / tend)
         for (int r = 0; r < nr; ++r) sinks[r] -= 2.220446049250313e-16;
       S.reset();
-      A.reset();
-      // unow = [p; zeros(new states); sinks]
+      SA.reset();
+      Aown.reset();
+      // unow = [p; zeros(new states); sinks] per block
       const int64_t n_new = sp->n;
-      ucur.alloc((size_t)(n_new + nr));
-      NC_CHECK_HIP(hipMemsetAsync(ucur.p, 0, sizeof(double) * (n_new + nr), sp->stream.s));
-      if (n_kept)
-        NC_CHECK_HIP(hipMemcpyAsync(ucur.p, p_kept.p, sizeof(double) * n_kept, hipMemcpyDeviceToDevice, sp->stream.s));
-      NC_CHECK_HIP(hipMemcpyAsync(ucur.p + n_new, sinks.data(), sizeof(double) * nr, hipMemcpyHostToDevice,
-                                  sp->stream.s));
+      Nb = n_new + nr;
+      ucur.alloc((size_t)(B * Nb));
+      NC_CHECK_HIP(hipMemsetAsync(ucur.p, 0, sizeof(double) * B * Nb, sp->stream.s));
+      for (int b = 0; b < B; ++b) {
+        if (n_kept)
+          NC_CHECK_HIP(hipMemcpyAsync(ucur.p + b * Nb, kept[b].p, sizeof(double) * n_kept, hipMemcpyDeviceToDevice,
+                                      sp->stream.s));
+        NC_CHECK_HIP(hipMemcpyAsync(ucur.p + b * Nb + n_new, sinks.data() + (size_t)b * nr, sizeof(double) * nr,
+                                    hipMemcpyHostToDevice, sp->stream.s));
+      }
       NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
     } else {
       // the final slice is added once more (fspsolve.jl:189-194)
-      output_add_snap(out, tnow, S->u.p, N, S->s);
+      output_add_snap(out, tnow, S.get());
       NC_CHECK_HIP(hipStreamSynchronize(S->s));
     }
   }
@@ -456,8 +492,8 @@ int nc_adapter_adapt(nc_space sp, const nc_adapter* ad, int64_t n, const double*
     if (n) NC_CHECK_HIP(hipMemcpyAsync(pd.p, p, sizeof(double) * n, hipMemcpyHostToDevice, sp->stream.s));
     DropWork w;
     DevBuf<int64_t> ids;
-    DevBuf<double> kept;
-    const int64_t m = adapt_space(sp, *ad, pd.p, dsinks, t, tend, fsptol, w, ids, kept);
+    std::vector<DevBuf<double>> kept;
+    const int64_t m = adapt_space(sp, *ad, {pd.p}, dsinks, t, tend, fsptol, w, ids, kept);
     if (m) NC_CHECK_HIP(hipMemcpy(drop_ids, ids.p, sizeof(int64_t) * m, hipMemcpyDeviceToHost));
     *n_drop = m;
   });
@@ -487,8 +523,38 @@ int nc_solve_adaptive(nc_model model, int64_t n0, const int32_t* states0, const 
     else
       nc_adaptive_opts_default(&o);
     auto O = std::make_unique<nc_output_s>();
-    solve_adaptive_impl(model, n0, states0, p0, t0, t1, n_save, saveat, o, O.get(), st);
+    solve_adaptive_impl(model, 0, n0, states0, p0, nullptr, t0, t1, n_save, saveat, o, O.get(), st);
     *out = O.release();
+  });
+}
+
+int nc_solve_sens_adaptive(nc_model model, int64_t n0, const int32_t* states0, const double* p0, const double* S0,
+                           double t0, double t1, int32_t n_save, const double* saveat, const nc_adaptive_opts* opts,
+                           nc_output* out, nc_solve_stats* st) {
+  return guard([&] {
+    NC_REQUIRE(model && out, NC_ERR_ARG, "null argument");
+    NC_REQUIRE(model->has_grad, NC_ERR_ARG, "the model has no propensity gradients (CmeModelWithSensitivity)");
+    NC_REQUIRE(model->np >= 1, NC_ERR_ARG, "the model has no parameters");
+    nc_adaptive_opts o{};
+    if (opts)
+      o = *opts;
+    else
+      nc_adaptive_opts_default(&o);
+    auto O = std::make_unique<nc_output_s>();
+    solve_adaptive_impl(model, model->np, n0, states0, p0, S0, t0, t1, n_save, saveat, o, O.get(), st);
+    *out = O.release();
+  });
+}
+
+int nc_output_get_sens(nc_output o, int32_t k, int32_t param, double* S, double* dsinks) {
+  return guard([&] {
+    NC_REQUIRE(o && k >= 0 && k < (int32_t)o->snaps.size(), NC_ERR_ARG, "snapshot index out of range");
+    NC_REQUIRE(param >= 0 && param < o->P, NC_ERR_ARG, "parameter index out of range");
+    const auto& P = *o->snaps[k];
+    const int64_t n = o->spaces[P.space]->n, Nb = n + o->nr;
+    const double* base = P.u.p + (size_t)(param + 1) * Nb;
+    if (S && n) NC_CHECK_HIP(hipMemcpy(S, base, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (dsinks) NC_CHECK_HIP(hipMemcpy(dsinks, base + n, sizeof(double) * o->nr, hipMemcpyDeviceToHost));
   });
 }
 

@@ -60,10 +60,17 @@ struct nc_matrix_s {
 namespace nc {
 // FspMatrixSparse{RealT}(space, propensities; parameters) for the space's current states
 nc_matrix_s* matrix_create(nc_space_s* sp, nc_model_s* m, int precision);
+// Bring time-dependent values up to date for time t (joint refresh, on the matrix stream).
+void matrix_prepare(nc_matrix_s* A, double t);
+// Launch-time scales at t: sink rows (nr) and time-varying slots (tv_r.size()).
+void matrix_scales(const nc_matrix_s* A, double t, double* sink_scale, double* tv_scale);
 // out = A(t) v (+ out when accumulate); v is a device vector of n_rows VT values, out holds
 // n_rows VT values, or FP64 values when out_fp64 is set.
 void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
                    bool out_fp64 = false);
+// When kernel timing is on, hand out the next pair of HIP events for a dispatch-stamped
+// launch (the caller increments ev_used after the launch); false when timing is off.
+bool matvec_timing_slot(nc_matrix_s* A, hipEvent_t* e0, hipEvent_t* e1);
 // Fold recorded kernel timings into timed_ms (synchronises on the recorded events).
 void matvec_harvest_timing(nc_matrix_s* A);
 double matrix_bytes_per_matvec(const nc_matrix_s* A);

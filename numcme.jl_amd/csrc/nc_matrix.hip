@@ -569,19 +569,8 @@ void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accu
     NC_CHECK_HIP(hipStreamWaitEvent(s, e, 0));
     NC_CHECK_HIP(hipEventDestroy(e));
   }
-  const bool timed = A->timing;
-  if (timed) {
-    if (A->ev_used * 2 >= A->ev.size()) {
-      if (A->ev.size() < 8192) {
-        size_t old = A->ev.size();
-        A->ev.resize(old ? old * 2 : 256);
-        for (size_t k = old; k < A->ev.size(); ++k) NC_CHECK_HIP(hipEventCreate(&A->ev[k]));
-      } else {
-        matvec_harvest_timing(A);
-      }
-    }
-  }
-  hipEvent_t e0 = timed ? A->ev[2 * A->ev_used] : nullptr, e1 = timed ? A->ev[2 * A->ev_used + 1] : nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = matvec_timing_slot(A, &e0, &e1);
   if (A->precision == NC_FP32)
     launch_matvec_t<float>(A, t, v, out, accumulate, s, out_fp64, e0, e1);
   else
@@ -589,6 +578,23 @@ void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accu
   NC_CHECK_HIP(hipGetLastError());
   if (timed) A->ev_used++;
   A->launches++;
+}
+
+bool matvec_timing_slot(nc_matrix_s* A, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
+  if (!A->timing) return false;
+  if (A->ev_used * 2 >= A->ev.size()) {
+    if (A->ev.size() < 8192) {
+      size_t old = A->ev.size();
+      A->ev.resize(old ? old * 2 : 256);
+      for (size_t k = old; k < A->ev.size(); ++k) NC_CHECK_HIP(hipEventCreate(&A->ev[k]));
+    } else {
+      matvec_harvest_timing(A);
+    }
+  }
+  *e0 = A->ev[2 * A->ev_used];
+  *e1 = A->ev[2 * A->ev_used + 1];
+  return true;
 }
 
 }  // namespace nc
@@ -635,6 +641,20 @@ double bench_matvec(nc_matrix_s* A, double t, int reps) {
 }  // namespace
 
 namespace nc {
+void matrix_prepare(nc_matrix_s* A, double t) {
+  if (A->precision == NC_FP32)
+    joint_refresh<float>(A, t);
+  else
+    joint_refresh<double>(A, t);
+}
+
+void matrix_scales(const nc_matrix_s* A, double t, double* sink_scale, double* tv_scale) {
+  const nc_model_s* m = A->model;
+  for (int r = 0; r < A->nr; ++r) sink_scale[r] = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
+  for (size_t q = 0; q < A->tv_r.size(); ++q)
+    tv_scale[q] = A->tv_kind[q] == NC_PROP_SEPARABLE ? m->tfactor(A->tv_r[q], t) : 1.0;
+}
+
 nc_matrix_s* matrix_create(nc_space_s* sp, nc_model_s* m, int precision) {
   NC_REQUIRE(precision == NC_FP64 || precision == NC_FP32, NC_ERR_ARG, "precision must be NC_FP64 or NC_FP32");
   NC_REQUIRE(sp->ns == m->ns && sp->nr == m->nr, NC_ERR_ARG, "model and state space sizes differ");

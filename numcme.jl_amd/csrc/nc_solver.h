@@ -1,14 +1,17 @@
 // Time integrators (internal): device-resident Krylov expv and Dormand-Prince 5(4).
 #pragma once
 
-#include "nc_matrix.h"
+#include "nc_sens.h"
 
 struct nc_solver_s {
-  nc_matrix_s* A = nullptr;  // borrowed; must outlive the solver
+  nc_matrix_s* A = nullptr;  // borrowed; must outlive the solver (the base generator for SA)
+  nc_sensmatrix_s* SA = nullptr;  // sensitivity operator (borrowed), or null
+  int blocks = 1;                 // P+1 for the sensitivity system
+  int64_t Nb = 0, Npb = 0;        // true and padded rows per block
   nc_solve_opts opts{};
   int method = NC_METHOD_KRYLOV;
-  int64_t N = 0;
-  int64_t Np = 0;  // padded vector length (multiple of 1024); entries N..Np-1 stay zero
+  int64_t N = 0;   // true vector length (blocks * Nb)
+  int64_t Np = 0;  // padded length (blocks * Npb, Npb a multiple of 1024); padding stays zero
   size_t vs = 8;
   double t = 0.0;
   bool have_state = false;
@@ -16,7 +19,7 @@ struct nc_solver_s {
   nc::DevBuf<double> u;     // FP64 master state [p; sinks]
   nc::DevBuf<double> u0;    // state uploaded by the last set_state (for restart)
   double t0 = 0.0, beta0 = 0.0;
-  nc::DevBuf<double> snaps; // n_out x N
+  nc::DevBuf<double> snaps; // n_out x Np (padded layout)
   int32_t n_snaps = 0;
   // reduction scratch
   nc::DevBuf<double> partials;
@@ -64,7 +67,7 @@ struct EventSpec {
 void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev = nullptr, bool one_step = false);
 void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev = nullptr, bool one_step = false);
 // Internal entry points shared with the adaptive driver.
-nc_solver_s* solver_create(nc_matrix_s* A, const nc_solve_opts* opts);
+nc_solver_s* solver_create(nc_matrix_s* A, const nc_solve_opts* opts, nc_sensmatrix_s* SA = nullptr);
 void solver_set_state_device(nc_solver_s* S, double t0, const double* u_dev);
 void solver_reduce_setup(nc_solver_s* S, int nvals);
 }  // namespace nc

@@ -454,6 +454,31 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
                                hipHostMallocDefault));
 }
 
+// The right-hand-side operator: the FSP generator, or the block sensitivity operator.
+static void op_apply(nc_solver_s* S, double t, const void* v, double* out) {
+  if (S->SA)
+    sens_matvec_launch(S->SA, t, v, S->Npb, out, S->Npb, S->s);
+  else
+    matvec_launch(S->A, t, v, out, false, S->s, true);
+}
+
+// Device (padded, block stride Npb) <-> host (contiguous, block stride Nb) copies.
+static void copy_from_host(nc_solver_s* S, double* dev, const double* host) {
+  if (S->blocks == 1)
+    NC_CHECK_HIP(hipMemcpyAsync(dev, host, sizeof(double) * S->N, hipMemcpyHostToDevice, S->s));
+  else
+    NC_CHECK_HIP(hipMemcpy2DAsync(dev, sizeof(double) * S->Npb, host, sizeof(double) * S->Nb, sizeof(double) * S->Nb,
+                                  S->blocks, hipMemcpyHostToDevice, S->s));
+}
+static void copy_to_host(nc_solver_s* S, double* host, const double* dev) {
+  if (S->blocks == 1)
+    NC_CHECK_HIP(hipMemcpyAsync(host, dev, sizeof(double) * S->N, hipMemcpyDeviceToHost, S->s));
+  else
+    NC_CHECK_HIP(hipMemcpy2DAsync(host, sizeof(double) * S->Nb, dev, sizeof(double) * S->Npb, sizeof(double) * S->Nb,
+                                  S->blocks, hipMemcpyDeviceToHost, S->s));
+  NC_CHECK_HIP(hipStreamSynchronize(S->s));
+}
+
 // ---- Krylov expv ---------------------------------------------------------------------
 // Arnoldi column j in two passes over the stored basis V_0..V_j (FP32 or FP64, kept
 // unnormalised with scales sc_i):
@@ -469,7 +494,7 @@ static void arnoldi_iteration(nc_solver_s* S, int j) {
   const KLayout L{S->m};
   hipStream_t s = S->s;
   auto Vp = [&](int i) { return reinterpret_cast<BT*>(S->V.p + (size_t)i * N * S->vs); };
-  matvec_launch(S->A, 0.0, Vp(j), S->q.p, false, s, true);
+  op_apply(S, 0.0, Vp(j), S->q.p);
   VecArgs<BT> a{};
   a.N = N;
   a.nv = j + 1;
@@ -500,7 +525,7 @@ static void arnoldi_last(nc_solver_s* S) {
   // |V_m| (H(m, m-1), sc_m) and avnorm = |A v_m| in one VM_DUAL pass over V_m
   const KLayout L{S->m};
   BT* vm = reinterpret_cast<BT*>(S->V.p + (size_t)S->m * S->Np * S->vs);
-  matvec_launch(S->A, 0.0, vm, S->q.p, false, S->s, true);
+  op_apply(S, 0.0, vm, S->q.p);
   VecArgs<BT> a{};
   a.N = S->Np;
   a.nv = 1;
@@ -585,7 +610,7 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
   auto Vp = [&](int i) -> unsigned char* { return S->V.p + (size_t)i * N * S->vs; };
   if (!S->started) {
     // first step size from the norm estimate (Expokit)
-    S->anorm = std::max(A->norm1, 1e-300);
+    S->anorm = std::max(S->SA ? S->SA->norm1 : A->norm1, 1e-300);
     const double fact = std::pow((m + 1) / std::exp(1.0), m + 1) * std::sqrt(2.0 * M_PI * (m + 1));
     const double b = std::max(S->beta, 1e-300);
     S->t_new = round_step((1.0 / S->anorm) * std::pow((fact * tol) / (4.0 * b * S->anorm), 1.0 / m));
@@ -768,13 +793,13 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
         k_rk_stage<double><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p,
                                                  reinterpret_cast<double*>(S->rkv.p));
       NC_CHECK_HIP(hipGetLastError());
-      matvec_launch(A, S->t + h_dp_c[stage] * h, S->rkv.p, kp(stage), false, s, true);
+      op_apply(S, S->t + h_dp_c[stage] * h, S->rkv.p, kp(stage));
       S->st.matvecs++;
     }
   };
   if (!S->fsal_valid) {
     to_vin(S->u.p);
-    matvec_launch(A, S->t, S->rkv.p, kp(0), false, s, true);
+    op_apply(S, S->t, S->rkv.p, kp(0));
     S->st.matvecs++;
     S->fsal_valid = true;
   }
@@ -883,7 +908,7 @@ using namespace nc;
 
 namespace {
 
-static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts) {
+static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts, nc_sensmatrix_s* SA = nullptr) {
   {
     NC_REQUIRE(A, NC_ERR_ARG, "null matrix");
     auto S = new nc_solver_s();
@@ -899,7 +924,11 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
       NC_REQUIRE(S->opts.krylov_m >= 2 && S->opts.krylov_m <= 31, NC_ERR_ARG, "krylov_m must be in 2..31");
       S->A = A;
       S->s = A->stream.s;
-      S->N = A->n_rows;
+      S->SA = SA;
+      S->blocks = SA ? SA->P + 1 : 1;
+      S->Nb = A->n_rows;
+      S->Npb = (S->Nb + 1023) / 1024 * 1024;
+      S->N = S->blocks * S->Nb;
       S->vs = A->vsize;
       const bool tv = !A->tv_r.empty();
       int method = S->opts.method;
@@ -909,7 +938,7 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
       S->method = method;
       S->m = S->opts.krylov_m;
       A->timing = S->opts.time_kernels != 0;
-      S->Np = (S->N + 1023) / 1024 * 1024;
+      S->Np = S->blocks * S->Npb;
       S->u.alloc(S->Np);
       NC_CHECK_HIP(hipMemsetAsync(S->u.p, 0, sizeof(double) * S->Np, S->s));
       const KLayout L{S->m};
@@ -978,7 +1007,7 @@ static void solver_set_state_impl(nc_solver_s* S, double t0, const double* u) {
     S->u0.alloc(S->Np);
     NC_CHECK_HIP(hipMemsetAsync(S->u0.p, 0, sizeof(double) * S->Np, S->s));
   }
-  NC_CHECK_HIP(hipMemcpyAsync(S->u0.p, u, sizeof(double) * S->N, hipMemcpyHostToDevice, S->s));
+  copy_from_host(S, S->u0.p, u);
   double b2 = 0.0;
   for (int64_t i = 0; i < S->N; ++i) b2 += u[i] * u[i];
   S->beta0 = std::sqrt(b2);
@@ -996,15 +1025,15 @@ static void solver_run_impl(nc_solver_s* S, int32_t n_out, const double* t_out) 
                  "output times must be finite and not before the current time");
       NC_REQUIRE(k == 0 || t_out[k] >= t_out[k - 1], NC_ERR_ARG, "output times must be ascending");
     }
-    if ((size_t)n_out * S->N > S->snaps.n) S->snaps.alloc((size_t)std::max(n_out, 1) * S->N);
+    if ((size_t)n_out * S->Np > S->snaps.n) S->snaps.alloc((size_t)std::max(n_out, 1) * S->Np);
     S->n_snaps = n_out;
-    const unsigned g = std::min<unsigned>(4096, grid_for(S->N, kBlock));
+    const unsigned g = std::min<unsigned>(4096, grid_for(S->Np, kBlock));
     for (int k = 0; k < n_out; ++k) {
       if (S->method == NC_METHOD_KRYLOV)
         krylov_advance(S, t_out[k]);
       else
         rk_advance(S, t_out[k]);
-      k_copy_d<<<g, kBlock, 0, S->s>>>(S->u.p, S->snaps.p + (size_t)k * S->N, S->N);
+      k_copy_d<<<g, kBlock, 0, S->s>>>(S->u.p, S->snaps.p + (size_t)k * S->Np, S->Np);
       NC_CHECK_HIP(hipGetLastError());
     }
     NC_CHECK_HIP(hipStreamSynchronize(S->s));
@@ -1056,7 +1085,7 @@ int nc_solver_run(nc_solver S, int32_t n_out, const double* t_out) {
 int nc_solver_get_snapshot(nc_solver S, int32_t k, double* u) {
   return guard([&] {
     NC_REQUIRE(S && u && k >= 0 && k < S->n_snaps, NC_ERR_ARG, "snapshot index out of range");
-    NC_CHECK_HIP(hipMemcpy(u, S->snaps.p + (size_t)k * S->N, sizeof(double) * S->N, hipMemcpyDeviceToHost));
+    copy_to_host(S, u, S->snaps.p + (size_t)k * S->Np);
   });
 }
 
@@ -1064,7 +1093,7 @@ int nc_solver_get_state(nc_solver S, double* t, double* u) {
   return guard([&] {
     NC_REQUIRE(S, NC_ERR_ARG, "null solver");
     if (t) *t = S->t;
-    if (u) NC_CHECK_HIP(hipMemcpy(u, S->u.p, sizeof(double) * S->N, hipMemcpyDeviceToHost));
+    if (u) copy_to_host(S, u, S->u.p);
   });
 }
 
@@ -1077,22 +1106,38 @@ int nc_solver_stats(nc_solver S, nc_solve_stats* st) {
   });
 }
 
+static void solve_fixed_impl(nc_matrix_s* A, nc_sensmatrix_s* SA, const nc_solve_opts* opts, const double* u0,
+                             int32_t n_out, const double* t_out, double* u_out, nc_solve_stats* st) {
+  NC_REQUIRE(A && u0 && (n_out == 0 || (t_out && u_out)), NC_ERR_ARG, "null argument");
+  nc_solver_s* S = solver_create_impl(A, opts, SA);
+  struct Guard {
+    nc_solver_s* S;
+    ~Guard() { delete S; }
+  } gd{S};
+  const double t0 = n_out ? t_out[0] : 0.0;
+  solver_set_state_impl(S, t0, u0);
+  solver_run_impl(S, n_out, t_out);
+  for (int k = 0; k < n_out; ++k) copy_to_host(S, u_out + (size_t)k * S->N, S->snaps.p + (size_t)k * S->Np);
+  if (st) nc_solver_stats(S, st);
+}
+
 int nc_solve_fixed(nc_matrix A, const nc_solve_opts* opts, const double* u0, int32_t n_out, const double* t_out,
                    double* u_out, nc_solve_stats* st) {
+  return guard([&] { solve_fixed_impl(A, nullptr, opts, u0, n_out, t_out, u_out, st); });
+}
+
+int nc_solver_create_sens(nc_sensmatrix SA, const nc_solve_opts* opts, nc_solver* out) {
   return guard([&] {
-    NC_REQUIRE(A && u0 && (n_out == 0 || (t_out && u_out)), NC_ERR_ARG, "null argument");
-    nc_solver_s* S = solver_create_impl(A, opts);
-    struct Guard {
-      nc_solver_s* S;
-      ~Guard() { delete S; }
-    } gd{S};
-    const double t0 = n_out ? t_out[0] : 0.0;
-    solver_set_state_impl(S, t0, u0);
-    solver_run_impl(S, n_out, t_out);
-    for (int k = 0; k < n_out; ++k)
-      NC_CHECK_HIP(hipMemcpy(u_out + (size_t)k * S->N, S->snaps.p + (size_t)k * S->N, sizeof(double) * S->N,
-                             hipMemcpyDeviceToHost));
-    if (st) nc_solver_stats(S, st);
+    NC_REQUIRE(SA && out, NC_ERR_ARG, "null argument");
+    *out = solver_create_impl(SA->A, opts, SA);
+  });
+}
+
+int nc_solve_sens_fixed(nc_sensmatrix SA, const nc_solve_opts* opts, const double* u0, int32_t n_out,
+                        const double* t_out, double* u_out, nc_solve_stats* st) {
+  return guard([&] {
+    NC_REQUIRE(SA, NC_ERR_ARG, "null argument");
+    solve_fixed_impl(SA->A, SA, opts, u0, n_out, t_out, u_out, st);
   });
 }
 
@@ -1100,15 +1145,20 @@ int nc_solve_fixed(nc_matrix A, const nc_solve_opts* opts, const double* u0, int
 
 namespace nc {
 
-nc_solver_s* solver_create(nc_matrix_s* A, const nc_solve_opts* opts) { return solver_create_impl(A, opts); }
+nc_solver_s* solver_create(nc_matrix_s* A, const nc_solve_opts* opts, nc_sensmatrix_s* SA) {
+  return solver_create_impl(A, opts, SA);
+}
 
-// Set the state from a device vector of the true length N (t0 is its time).
+// Set the state from a device vector of the true length N, blocks contiguous (t0 is its
+// time).
 void solver_set_state_device(nc_solver_s* S, double t0, const double* u_dev) {
   if (!S->u0.p) {
     S->u0.alloc(S->Np);
     NC_CHECK_HIP(hipMemsetAsync(S->u0.p, 0, sizeof(double) * S->Np, S->s));
   }
-  NC_CHECK_HIP(hipMemcpyAsync(S->u0.p, u_dev, sizeof(double) * S->N, hipMemcpyDeviceToDevice, S->s));
+  // u_dev holds the blocks contiguously (stride Nb); the solver pads each to Npb
+  NC_CHECK_HIP(hipMemcpy2DAsync(S->u0.p, sizeof(double) * S->Npb, u_dev, sizeof(double) * S->Nb,
+                                sizeof(double) * S->Nb, S->blocks, hipMemcpyDeviceToDevice, S->s));
   const KLayout L{S->m};
   VecArgs<double> a{};
   a.N = S->Np;

@@ -13,6 +13,9 @@ from .model import (CmeModel, CmeModelWithSensitivity, JointTimeVaryingPropensit
                     istimevarying, propensity)
 from .solve import RK45, FspOutputSliceSparse, FspOutputSparse, KrylovExpv, solve
 from .adaptive import AdaptiveFspSparse, RStepAdapter, SelectiveRStepAdapter, adapt_, init_
+from .sens import (AdaptiveForwardSensFspSparse, ForwardSensFspInitialConditionSparse, ForwardSensFspMatrixSparse,
+                   ForwardSensFspOutputSliceSparse, ForwardSensFspOutputSparse, ForwardSensRStepAdapter,
+                   forwardsens_initial_condition)
 from .statespace import (StateSpaceSparse, deleteat, expand, get_sink_connectivity, get_sink_count,
                          get_state_connectivity, get_state_count, get_statedict, get_states, get_stoich_matrix)
 from . import expr as nmath
@@ -24,6 +27,9 @@ __all__ = [
     "SeparableTimeVaryingPropensity", "StandardTimeInvariantPropensity", "istimeseparable", "istimevarying",
     "propensity", "RK45", "FspOutputSliceSparse", "FspOutputSparse", "KrylovExpv", "solve",
     "AdaptiveFspSparse", "RStepAdapter", "SelectiveRStepAdapter", "adapt_", "init_",
+    "AdaptiveForwardSensFspSparse", "ForwardSensFspInitialConditionSparse", "ForwardSensFspMatrixSparse",
+    "ForwardSensFspOutputSliceSparse", "ForwardSensFspOutputSparse", "ForwardSensRStepAdapter",
+    "forwardsens_initial_condition",
     "StateSpaceSparse", "deleteat", "expand", "get_sink_connectivity", "get_sink_count",
     "get_state_connectivity", "get_state_count", "get_statedict", "get_states", "get_stoich_matrix",
 ]

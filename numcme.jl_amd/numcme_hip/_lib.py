@@ -162,11 +162,21 @@ SIGNATURES = {
     "nc_solver_get_state": [P, PD, PD],
     "nc_solver_stats": [P, C.POINTER(nc_solve_stats)],
     "nc_solve_fixed": [P, C.POINTER(nc_solve_opts), PD, I32, PD, PD, C.POINTER(nc_solve_stats)],
+    "nc_sensmatrix_create": [P, P, I32, C.POINTER(P)],
+    "nc_sensmatrix_destroy": [P],
+    "nc_sensmatrix_info": [P, PI64, PI32, PI32, PD],
+    "nc_sensmatrix_matvec": [P, D, PD, PD],
+    "nc_sensmatrix_bench_matvec": [P, D, I32, PD],
+    "nc_solver_create_sens": [P, C.POINTER(nc_solve_opts), C.POINTER(P)],
+    "nc_solve_sens_fixed": [P, C.POINTER(nc_solve_opts), PD, I32, PD, PD, C.POINTER(nc_solve_stats)],
     "nc_adapter_init": [P, C.POINTER(nc_adapter)],
     "nc_adapter_adapt": [P, C.POINTER(nc_adapter), I64, PD, PD, D, D, D, PI64, PI64],
     "nc_adaptive_opts_default": [C.POINTER(nc_adaptive_opts)],
     "nc_solve_adaptive": [P, I64, PI32, PD, D, D, I32, PD, C.POINTER(nc_adaptive_opts), C.POINTER(P),
                           C.POINTER(nc_solve_stats)],
+    "nc_solve_sens_adaptive": [P, I64, PI32, PD, PD, D, D, I32, PD, C.POINTER(nc_adaptive_opts), C.POINTER(P),
+                               C.POINTER(nc_solve_stats)],
+    "nc_output_get_sens": [P, I32, I32, PD, PD],
     "nc_output_destroy": [P],
     "nc_output_count": [P, PI32, PI32],
     "nc_output_info": [P, I32, PD, PI64, PI32],

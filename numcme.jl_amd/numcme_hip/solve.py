@@ -93,7 +93,12 @@ def solve(model: CmeModel, initial_distribution: FspVectorSparse, tspan, method,
     """FSP solve. ``method`` is an integrator (fixed space, fspsolve.jl:10-41) or an
     ``AdaptiveFspSparse`` (adaptive space, fspsolve.jl:105-197)."""
     from .adaptive import AdaptiveFspSparse, solve_adaptive
+    from .sens import AdaptiveForwardSensFspSparse, solve_sens
 
+    if isinstance(method, AdaptiveForwardSensFspSparse):
+        # solve(model::CmeModelWithSensitivity, initial_condition, tspan, alg) (forwardsenscmesparse.jl:99)
+        return solve_sens(model, initial_distribution, tspan, method, saveat=saveat, fsptol=fsptol,
+                          odeatol=odeatol, odertol=odertol, verbose=verbose)
     if isinstance(method, AdaptiveFspSparse):
         return solve_adaptive(model, initial_distribution, tspan, method, saveat=saveat, fsptol=fsptol,
                               odeatol=odeatol, odertol=odertol, verbose=verbose)

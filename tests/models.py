@@ -162,3 +162,70 @@ def cascade(theta=(10.0, 2.0, 1.0, 0.5, 0.2, 0.1)):
         ("ti", lambda x, p: p[5] * x[2]),
     ]
     return CASCADE_S, props, np.asarray(theta, dtype=float)
+
+
+# ---- hand-derived propensity partial derivatives (the oracle's gradient input) ----------
+# Keys (r, p) form the gradient sparsity pattern. Julia's max(0, y) is y unless y < 0, so
+# its derivative is dy where y >= 0 and 0 elsewhere.
+def _dmax0(y, dy):
+    return np.where(np.asarray(y) >= 0, dy, 0.0)
+
+
+def telegraph_sens_grads():
+    from oracle.sensmatrix import DSeparable, DTimeInvariant
+    tf_arg = lambda t, p: np.pi * t / p[4]
+    return {
+        (0, 0): DTimeInvariant(lambda x, p: 1.0 * x[0]),
+        (1, 1): DTimeInvariant(lambda x, p: 1.0 * x[1]),
+        (2, 2): DTimeInvariant(lambda x, p: 1.0 * x[1]),
+        (3, 3): DSeparable(None, lambda x, p: 1.0 * x[2]),
+        (3, 4): DSeparable(lambda t, p: _dmax0(1.0 - np.sin(tf_arg(t, p)), np.cos(tf_arg(t, p)) * np.pi * t / p[4] ** 2),
+                           None),
+    }
+
+
+def gene4_grads():
+    from oracle.sensmatrix import DSeparable, DTimeInvariant
+    sig = lambda t: (1.0 - np.exp(-6e-3 * t)) * np.exp(-9e-3 * t)
+    species = {0: 0, 1: 1, 2: 1, 4: 2, 5: 3, 6: 0, 7: 1, 8: 2, 9: 3, 10: 4, 11: 5}
+    g = {(r, r): DTimeInvariant((lambda s: lambda x, p: 1.0 * x[s])(s)) for r, s in species.items()}
+    g[(3, 3)] = DSeparable(lambda t, p: _dmax0(p[3] - 0.12 * sig(t), 1.0), None)
+    return g
+
+
+def toggle_grads():
+    from oracle.sensmatrix import DTimeInvariant
+
+    def hill_terms(y, p):
+        y = np.asarray(y, dtype=np.float64)
+        u = y / p[1]
+        un = np.power(u, p[2])
+        den = 1.0 + un
+        with np.errstate(divide="ignore", invalid="ignore"):
+            logu = np.where(u > 0, np.log(np.where(u > 0, u, 1.0)), 0.0)
+        return un, den, logu
+
+    def d_alpha(s):
+        return lambda x, p: 1.0 / hill_terms(x[s], p)[1]
+
+    def d_K(s):
+        def f(x, p):
+            un, den, _ = hill_terms(x[s], p)
+            return p[0] * un * p[2] / p[1] / den ** 2
+        return f
+
+    def d_n(s):
+        def f(x, p):
+            un, den, logu = hill_terms(x[s], p)
+            return -p[0] * un * logu / den ** 2
+        return f
+
+    g = {}
+    for r, other in ((0, 1), (2, 0)):
+        g[(r, 0)] = DTimeInvariant(d_alpha(other))
+        g[(r, 1)] = DTimeInvariant(d_K(other))
+        g[(r, 2)] = DTimeInvariant(d_n(other))
+        g[(r, 3)] = DTimeInvariant(lambda x, p: 1.0 + 0.0 * x[0])
+    g[(1, 4)] = DTimeInvariant(lambda x, p: 1.0 * x[0])
+    g[(3, 4)] = DTimeInvariant(lambda x, p: 1.0 * x[1])
+    return g
