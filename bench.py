@@ -127,6 +127,82 @@ def cpu_baseline(model_theta, gpu_fp64_steps, max_steps=3):
     }
 
 
+def gene4_model(nh):
+    """Four gene states (one-hot G0..G3), nuclear and cytoplasmic mRNA; k21 pulse-modulated:
+    k21(t) = max(0, k21 - 0.12 s(t)), s(t) = (1 - e^{-0.006 t}) e^{-0.009 t}; 12 parameters
+    (k01, k10, k12, k21, k23, k32, four transcription rates, export, cytoplasmic decay)."""
+    S = np.zeros((6, 12), dtype=np.int64)
+    for r, (a, b) in enumerate([(0, 1), (1, 0), (1, 2), (2, 1), (2, 3), (3, 2)]):
+        S[a, r], S[b, r] = -1, 1
+    S[4, 6:10] = 1
+    S[4, 10], S[5, 10] = -1, 1
+    S[5, 11] = -1
+    nm = nh.nmath
+    sig = lambda t: (1.0 - nm.exp(-6e-3 * t)) * nm.exp(-9e-3 * t)
+    props = [nh.propensity(lambda x, p: p[0] * x[0]), nh.propensity(lambda x, p: p[1] * x[1]),
+             nh.propensity(lambda x, p: p[2] * x[1]),
+             nh.propensity(lambda x, p: x[2], tfactor=lambda t, p: nm.max(0.0, p[3] - 0.12 * sig(t))),
+             nh.propensity(lambda x, p: p[4] * x[2]), nh.propensity(lambda x, p: p[5] * x[3]),
+             nh.propensity(lambda x, p: p[6] * x[0]), nh.propensity(lambda x, p: p[7] * x[1]),
+             nh.propensity(lambda x, p: p[8] * x[2]), nh.propensity(lambda x, p: p[9] * x[3]),
+             nh.propensity(lambda x, p: p[10] * x[4]), nh.propensity(lambda x, p: p[11] * x[5])]
+    theta = [1.3e-3, 3.2e-3, 5.9e-3, 2e-2, 1.5e-3, 4.6e-3, 0.0, 1e-2, 0.2, 1.5, 0.05, 1e-3]
+    return nh.CmeModelWithSensitivity(nh.CmeModel(S, props, theta))
+
+
+def gene4_states(nmax):
+    g = np.repeat(np.eye(4, dtype=np.int64), (nmax + 1) ** 2, axis=0)
+    c, n = np.meshgrid(np.arange(nmax + 1), np.arange(nmax + 1), indexing="ij")
+    mrna = np.tile(np.stack([n.ravel(), c.ravel()], axis=1), (4, 1))
+    return np.concatenate([g, mrna], axis=1)
+
+
+def c3_sensitivity(nh, L):
+    """Config C3: 12-parameter forward sensitivities on 262,144 states, t = 0:60:900, RK45 on
+    the device (the generator is time-varying), FP32 storage against FP64."""
+    model = gene4_model(nh)
+    states = gene4_states(255)
+    space = nh.StateSpaceSparse(model.get_stoich_matrix(), states)
+    ts = np.arange(0.0, 901.0, 60.0)
+    res = {"workload": "4 gene states x 256 nuclear x 256 cytoplasmic mRNA (262,144 states + 12 sinks), "
+                       "12-parameter forward sensitivities (13 blocks), t = 0:60:900, device RK45 rtol 1e-8"}
+    finals = {}
+    for prec in ("fp64", "fp32"):
+        SA = nh.ForwardSensFspMatrixSparse(model, space, precision=prec)
+        n, P = SA.rowcount, SA.parameter_count
+        u0 = np.zeros(n * (P + 1))
+        u0[0] = 1.0  # delta at (G0, 0, 0): the first state of the list
+        o = L.nc_solve_opts()
+        L.call("nc_solve_opts_default", C.byref(o))
+        o.method, o.rtol, o.atol = L.NC_METHOD_RK45, 1e-8, 1e-14
+        h = C.c_void_p()
+        L.call("nc_solver_create_sens", SA.handle, C.byref(o), C.byref(h))
+        L.call("nc_solver_set_state", h, 0.0, L.ptr(u0, C.c_double))
+        t0 = time.perf_counter()
+        L.call("nc_solver_run", h, len(ts), L.ptr(ts, C.c_double))
+        res[f"{prec}_solve_s"] = time.perf_counter() - t0
+        st = L.nc_solve_stats()
+        L.call("nc_solver_stats", h, C.byref(st))
+        res[f"{prec}_rk_steps"] = st.steps
+        res[f"{prec}_sens_matvecs"] = st.matvecs
+        fin = np.empty(n * (P + 1))
+        L.call("nc_solver_get_snapshot", h, len(ts) - 1, L.ptr(fin, C.c_double))
+        finals[prec] = fin.reshape(P + 1, n)
+        L.call("nc_solver_destroy", h)
+        ms = SA.bench_matvec(reps=50, t=300.0)
+        res[f"{prec}_sens_matvec_us"] = ms * 1e3
+        res[f"{prec}_sens_bytes_per_matvec"] = SA.bytes_per_matvec
+        res[f"{prec}_sens_matvec_GBps"] = SA.bytes_per_matvec / (ms * 1e-3) / 1e9
+        SA.free()
+    a, b = finals["fp64"], finals["fp32"]
+    res["l1_p_fp32_vs_fp64"] = float(np.abs(a[0] - b[0]).sum())
+    res["max_rel_S_fp32_vs_fp64"] = float(max(np.abs(a[k] - b[k]).max() / max(np.abs(a[k]).max(), 1e-300)
+                                              for k in range(1, a.shape[0])))
+    # d/dtheta of the total mass is zero: sum of each sensitivity block (states + sinks)
+    res["max_abs_mass_derivative_fp64"] = float(np.abs(a[1:].sum(axis=1)).max())
+    return res
+
+
 def spmv_probe(nh, precision, reps):
     """Mat-vec launches plus a copy kernel of known size, for rocprofv3 --pmc runs."""
     import torch
@@ -156,6 +232,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch Arnoldi kernels eagerly (for profilers)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the forward-sensitivity (C3) measurement")
     ap.add_argument("--spmv-probe", type=int, default=0, metavar="REPS",
                     help="only build the matrix, launch the mat-vec REPS times back to back and run a plain "
                          "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
@@ -269,6 +346,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_c3:
+        line["c3_sensitivity"] = c3_sensitivity(nh, L)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(model.parameters, fp64_steps)
     print(json.dumps(line), flush=True)
