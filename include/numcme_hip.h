@@ -261,6 +261,26 @@ int nc_solve_sens_fixed(nc_sensmatrix SA, const nc_solve_opts* opts, const doubl
                         const double* t_out, double* u_out, nc_solve_stats* st);
 
 /* ======================================================================================
+ * Batched solves — no reference counterpart (the reference solves one model per call).
+ * Many time-invariant generators of uneven size are integrated together with
+ * Dormand-Prince 5(4), per-instance step control, one replayed HIP graph per iteration.
+ * ==================================================================================== */
+typedef struct nc_batch_s* nc_batch;
+/* Build a batch from n generators (all of one precision, time-invariant). The matrices are
+ * copied; they may be destroyed afterwards. */
+int nc_batch_create(int32_t n, const nc_matrix* mats, nc_batch* out);
+int nc_batch_destroy(nc_batch B);
+/* Total device rows (padded) and each instance's n_rows (instance_rows: n entries). */
+int nc_batch_info(nc_batch B, int64_t* total_rows, int64_t* instance_rows);
+/* Integrate every instance from u0 (the instances' n_rows vectors, concatenated) through
+ * the shared ascending output times t_out[0..n_out) (t_out[0] is the start); opts gives
+ * rtol, atol, h0 and no_graphs. Snapshots stay on the device. */
+int nc_batch_solve(nc_batch B, const nc_solve_opts* opts, const double* u0, int32_t n_out,
+                   const double* t_out, nc_solve_stats* st);
+/* Copy snapshot k of instance inst (its n_rows doubles). */
+int nc_batch_get_snapshot(nc_batch B, int32_t inst, int32_t k, double* u);
+
+/* ======================================================================================
  * Adaptive FSP — replaces solve(model, p0, tspan, ::AdaptiveFspSparse; saveat, fsptol,
  * odeatol, odertol) (fspsolve.jl:59-62,105-197) and the space adapters RStepAdapter /
  * SelectiveRStepAdapter with their init! / adapt! (rstepadapters.jl:12-110).

@@ -1,0 +1,572 @@
+// Batched FSP solves: many independent generators of uneven size integrated together.
+//
+// Layout: every instance's slot-major generator is copied into one concatenated ELL
+// (ld = total rows). Instance i owns rows [row_off[i], row_off[i] + rows_pad[i]), padded
+// to whole 1024-row tiles, so a 256-thread block of 4 rows per thread never straddles two
+// instances: tile_inst[tile] names the owner. Rows n_i .. n_i + nr_i - 1 of an instance are
+// its sinks (written by one workgroup per (instance, reaction) segment); padding rows stay
+// zero. Vectors use the same row space.
+//
+// Integrator: Dormand-Prince 5(4) with per-instance t, h and error control. One iteration
+// (accept/snapshot of the previous step, six stages with batched mat-vecs, the error
+// partials and their per-instance fold) is captured once as a HIP graph and replayed; the
+// per-instance step sizes and decisions live in device arrays the host rewrites between
+// replays, so a replay needs no argument changes. One host synchronisation per iteration
+// reads every instance's error norm.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "nc_matrix.h"
+
+namespace nc {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 1024;  // rows per block (4 per thread)
+constexpr int R4 = 4;
+
+template <typename T, int R>
+struct alignas(sizeof(T) * R) Pack {
+  T v[R];
+};
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__constant__ double c_a[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {1.0 / 5, 0, 0, 0, 0, 0},
+    {3.0 / 40, 9.0 / 40, 0, 0, 0, 0},
+    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0},
+    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0},
+    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0},
+    {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84}};
+__constant__ double c_e[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525, -1.0 / 40};
+
+struct Dev {
+  const int32_t* tile_inst;  // owner instance of each tile
+  const int64_t* row_off;    // first row of each instance
+  const int64_t* nstates;    // states of each instance
+  const double* h;           // step size of the current iteration (0: inactive)
+  const int32_t* accept;     // 1: accept the last step
+  const int32_t* snap;       // snapshot slot to fill after accepting, or -1
+};
+
+// y_s = u + h_i sum_{j<s} a_sj k_j over the instance's rows, and its VT copy
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_bstage(Dev d, int64_t ld, int stage, const double* u, const double* k,
+                                                   double* ytmp, VT* vin) {
+  const int inst = d.tile_inst[blockIdx.x];
+  const double h = d.h[inst];
+  if (h == 0.0) return;
+  const int64_t n0 = (int64_t)blockIdx.x * kTile + threadIdx.x * R4;
+  Pack<double, R4> y = *reinterpret_cast<const Pack<double, R4>*>(u + n0);
+  double acc[R4] = {0, 0, 0, 0};
+  for (int j = 0; j < stage; ++j) {
+    const double a = c_a[stage][j];
+    const Pack<double, R4> kj = *reinterpret_cast<const Pack<double, R4>*>(k + (int64_t)j * ld + n0);
+#pragma unroll
+    for (int r = 0; r < R4; ++r) acc[r] += a * kj.v[r];
+  }
+  Pack<VT, R4> o;
+#pragma unroll
+  for (int r = 0; r < R4; ++r) {
+    y.v[r] += h * acc[r];
+    o.v[r] = (VT)y.v[r];
+  }
+  *reinterpret_cast<Pack<double, R4>*>(ytmp + n0) = y;
+  *reinterpret_cast<Pack<VT, R4>*>(vin + n0) = o;
+}
+
+// out = A v over the state rows of active instances (4 rows per thread)
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, const double* diag, const int32_t* col,
+                                                    const VT* val, const VT* v, double* out) {
+  const int inst = d.tile_inst[blockIdx.x];
+  if (d.h[inst] == 0.0) return;
+  const int64_t n0 = (int64_t)blockIdx.x * kTile + threadIdx.x * R4;
+  const int64_t nend = d.row_off[inst] + d.nstates[inst];
+  if (n0 >= nend) return;
+  const Pack<double, R4> dg = *reinterpret_cast<const Pack<double, R4>*>(diag + n0);
+  const Pack<VT, R4> vi = *reinterpret_cast<const Pack<VT, R4>*>(v + n0);
+  double acc[R4];
+#pragma unroll
+  for (int r = 0; r < R4; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
+  for (int s = 0; s < K; ++s) {
+    const Pack<int32_t, R4> c = *reinterpret_cast<const Pack<int32_t, R4>*>(col + (int64_t)s * ld + n0);
+    const Pack<VT, R4> w = *reinterpret_cast<const Pack<VT, R4>*>(val + (int64_t)s * ld + n0);
+#pragma unroll
+    for (int r = 0; r < R4; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
+  }
+#pragma unroll
+  for (int r = 0; r < R4; ++r)
+    if (n0 + r < nend) out[n0 + r] = acc[r];
+}
+
+// sink row of one (instance, reaction) segment
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_bsinks(Dev d, const int32_t* seg_inst, const int64_t* seg_row,
+                                                   const int64_t* seg_off, const int32_t* src, const VT* sval,
+                                                   const VT* v, double* out) {
+  const int seg = blockIdx.x;
+  if (d.h[seg_inst[seg]] == 0.0) return;
+  double acc = 0.0;
+  for (int64_t e = seg_off[seg] + threadIdx.x; e < seg_off[seg + 1]; e += kBlock) acc += (double)sval[e] * (double)v[src[e]];
+  __shared__ double part[kBlock / 64];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[seg_row[seg]] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Error partials per tile: sum ((h sum_j e_j k_j) / (atol + rtol max(|u|, |y|)))^2
+__global__ __launch_bounds__(kBlock) void k_berr(Dev d, int64_t ld, double atol, double rtol, const double* u,
+                                                 const double* y, const double* k, double* partials) {
+  const int inst = d.tile_inst[blockIdx.x];
+  const double h = d.h[inst];
+  double acc = 0.0;
+  if (h != 0.0) {
+    const int64_t n0 = (int64_t)blockIdx.x * kTile + threadIdx.x * R4;
+    const Pack<double, R4> uu = *reinterpret_cast<const Pack<double, R4>*>(u + n0);
+    const Pack<double, R4> yy = *reinterpret_cast<const Pack<double, R4>*>(y + n0);
+    double e[R4] = {0, 0, 0, 0};
+    for (int j = 0; j < 7; ++j) {
+      if (c_e[j] == 0.0) continue;
+      const Pack<double, R4> kj = *reinterpret_cast<const Pack<double, R4>*>(k + (int64_t)j * ld + n0);
+#pragma unroll
+      for (int r = 0; r < R4; ++r) e[r] += c_e[j] * kj.v[r];
+    }
+#pragma unroll
+    for (int r = 0; r < R4; ++r) {
+      const double sc = atol + rtol * fmax(fabs(uu.v[r]), fabs(yy.v[r]));
+      const double q = h * e[r] / sc;
+      acc += q * q;
+    }
+  }
+  __shared__ double part[kBlock / 64];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Per-instance fold of the tile partials (tiles of an instance are contiguous)
+__global__ void k_bfold(int n_inst, const int64_t* tile_start, const double* partials, double* err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inst) return;
+  double s = 0.0;
+  for (int64_t t = tile_start[i]; t < tile_start[i + 1]; ++t) s += partials[t];
+  err[i] = s;
+}
+
+// Accept the last step where asked (u <- y, k0 <- k6) and fill requested snapshots.
+__global__ __launch_bounds__(kBlock) void k_baccept(Dev d, int64_t ld, double* u, const double* y, double* k,
+                                                    double* snaps) {
+  const int inst = d.tile_inst[blockIdx.x];
+  const int64_t n0 = (int64_t)blockIdx.x * kTile + threadIdx.x * R4;
+  if (d.accept[inst]) {
+    *reinterpret_cast<Pack<double, R4>*>(u + n0) = *reinterpret_cast<const Pack<double, R4>*>(y + n0);
+    *reinterpret_cast<Pack<double, R4>*>(k + n0) = *reinterpret_cast<const Pack<double, R4>*>(k + 6 * ld + n0);
+  }
+  const int sl = d.snap[inst];
+  if (sl >= 0)
+    *reinterpret_cast<Pack<double, R4>*>(snaps + (int64_t)sl * ld + n0) = *reinterpret_cast<const Pack<double, R4>*>(u + n0);
+}
+
+// Copy one instance's slot-major ELL into the concatenated arrays.
+template <typename VT>
+__global__ void k_bcopy(int64_t n, int64_t ld_src, int K_src, int K, int64_t off, int64_t ld, const double* diag_s,
+                        const int32_t* col_s, const VT* val_s, double* diag, int32_t* col, VT* val) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  diag[off + i] = diag_s[i];
+  for (int s = 0; s < K; ++s) {
+    if (s < K_src) {
+      col[s * ld + off + i] = col_s[s * ld_src + i] + (int32_t)off;
+      val[s * ld + off + i] = val_s[s * ld_src + i];
+    } else {
+      col[s * ld + off + i] = (int32_t)(off + i);
+      val[s * ld + off + i] = (VT)0;
+    }
+  }
+}
+
+__global__ void k_badd_off(int64_t n, const int32_t* src_s, int64_t off, int32_t* src) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) src[i] = src_s[i] + (int32_t)off;
+}
+
+}  // namespace
+}  // namespace nc
+
+struct nc_batch_s {
+  int n = 0, precision = NC_FP64;
+  size_t vs = 8;
+  int K = 0;
+  int64_t ld = 0, tiles = 0;
+  std::vector<int64_t> row_off, nstates, nrows, tile_start;
+  nc::Stream stream;
+  nc::DevBuf<double> diag;
+  nc::DevBuf<int32_t> col;
+  nc::DevBuf<unsigned char> val;
+  int nseg = 0;
+  nc::DevBuf<int32_t> seg_inst, sink_src;
+  nc::DevBuf<int64_t> seg_row, seg_off;
+  nc::DevBuf<unsigned char> sink_val;
+  nc::DevBuf<int32_t> tile_inst;
+  nc::DevBuf<int64_t> d_row_off, d_nstates, d_tile_start;
+  // integrator state
+  nc::DevBuf<double> u, ytmp, k, partials, err, snaps;
+  nc::DevBuf<unsigned char> vin;
+  nc::DevBuf<double> d_h;
+  nc::DevBuf<int32_t> d_accept, d_snap;
+  double* h_pin = nullptr;     // pinned mirrors of the per-instance controls
+  int32_t* acc_pin = nullptr;
+  int32_t* snap_pin = nullptr;
+  double* err_pin = nullptr;
+  int n_out = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~nc_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (h_pin) (void)hipHostFree(h_pin);
+    if (acc_pin) (void)hipHostFree(acc_pin);
+    if (snap_pin) (void)hipHostFree(snap_pin);
+    if (err_pin) (void)hipHostFree(err_pin);
+  }
+};
+
+using namespace nc;
+
+namespace {
+
+template <typename VT>
+void batch_build(nc_batch_s* B, const nc_matrix_s* const* mats) {
+  hipStream_t s = B->stream.s;
+  for (int i = 0; i < B->n; ++i) {
+    const nc_matrix_s* A = mats[i];
+    NC_CHECK_HIP(hipStreamSynchronize(A->stream.s));
+    if (A->n)
+      k_bcopy<VT><<<grid_for(A->n, kBlock), kBlock, 0, s>>>(
+          A->n, A->ld, A->K, B->K, B->row_off[i], B->ld, A->diag.p, A->col.p, reinterpret_cast<const VT*>(A->val.p),
+          B->diag.p, B->col.p, reinterpret_cast<VT*>(B->val.p));
+    NC_CHECK_HIP(hipGetLastError());
+  }
+  // sink segments: (instance, reaction), entries concatenated
+  std::vector<int32_t> seg_inst;
+  std::vector<int64_t> seg_row, seg_off{0};
+  int64_t total = 0;
+  for (int i = 0; i < B->n; ++i)
+    for (int r = 0; r < mats[i]->nr; ++r) {
+      seg_inst.push_back(i);
+      seg_row.push_back(B->row_off[i] + mats[i]->n + r);
+      total += mats[i]->sink_off_h[r + 1] - mats[i]->sink_off_h[r];
+      seg_off.push_back(total);
+    }
+  B->nseg = (int)seg_inst.size();
+  B->seg_inst.alloc(std::max<size_t>(seg_inst.size(), 1));
+  B->seg_row.alloc(std::max<size_t>(seg_row.size(), 1));
+  B->seg_off.alloc(seg_off.size());
+  B->sink_src.alloc(std::max<int64_t>(total, 1));
+  B->sink_val.alloc(std::max<int64_t>(total, 1) * B->vs);
+  NC_CHECK_HIP(hipMemcpyAsync(B->seg_inst.p, seg_inst.data(), sizeof(int32_t) * seg_inst.size(), hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->seg_row.p, seg_row.data(), sizeof(int64_t) * seg_row.size(), hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->seg_off.p, seg_off.data(), sizeof(int64_t) * seg_off.size(), hipMemcpyHostToDevice, s));
+  int64_t e = 0;
+  for (int i = 0; i < B->n; ++i) {
+    const nc_matrix_s* A = mats[i];
+    const int64_t m = A->n_sink_entries;
+    if (m) {
+      k_badd_off<<<grid_for(m, kBlock), kBlock, 0, s>>>(m, A->sink_src.p, B->row_off[i], B->sink_src.p + e);
+      NC_CHECK_HIP(hipMemcpyAsync(B->sink_val.p + e * B->vs, A->sink_val.p, m * B->vs, hipMemcpyDeviceToDevice, s));
+    }
+    e += m;
+  }
+  NC_CHECK_HIP(hipGetLastError());
+  NC_CHECK_HIP(hipStreamSynchronize(s));
+}
+
+nc_batch_s* batch_create(int n, const nc_matrix* mats) {
+  NC_REQUIRE(n >= 1 && mats, NC_ERR_ARG, "empty batch");
+  auto B = std::make_unique<nc_batch_s>();
+  B->n = n;
+  B->precision = mats[0]->precision;
+  B->vs = mats[0]->vsize;
+  int64_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    const nc_matrix_s* A = mats[i];
+    NC_REQUIRE(A, NC_ERR_ARG, "null matrix in batch");
+    NC_REQUIRE(A->precision == B->precision, NC_ERR_ARG, "all matrices of a batch must share one precision");
+    NC_REQUIRE(A->tv_r.empty(), NC_ERR_UNSUPPORTED, "batched solves take time-invariant generators");
+    B->K = std::max(B->K, A->K);
+    const int64_t rows = (A->n_rows + kTile - 1) / kTile * kTile;
+    B->row_off.push_back(off);
+    B->nstates.push_back(A->n);
+    B->nrows.push_back(A->n_rows);
+    off += std::max<int64_t>(rows, kTile);
+  }
+  NC_REQUIRE(off < (int64_t)0x7FFFFFF0, NC_ERR_OVERFLOW, "batch exceeds 2^31 rows");
+  B->ld = off;
+  B->tiles = off / kTile;
+  std::vector<int32_t> tile_inst(B->tiles);
+  B->tile_start.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const int64_t t0 = B->row_off[i] / kTile, t1 = (i + 1 < n ? B->row_off[i + 1] : off) / kTile;
+    B->tile_start[i] = t0;
+    for (int64_t t = t0; t < t1; ++t) tile_inst[t] = i;
+  }
+  B->tile_start[n] = B->tiles;
+  hipStream_t s = B->stream.s;
+  const int64_t ld = B->ld;
+  B->diag.alloc(ld);
+  B->col.alloc(std::max<int64_t>((int64_t)B->K * ld, 1));
+  B->val.alloc(std::max<int64_t>((int64_t)B->K * ld, 1) * B->vs);
+  NC_CHECK_HIP(hipMemsetAsync(B->diag.p, 0, sizeof(double) * ld, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->col.p, 0, sizeof(int32_t) * B->K * ld, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->val.p, 0, B->vs * B->K * ld, s));
+  B->tile_inst.alloc(B->tiles);
+  B->d_row_off.alloc(n);
+  B->d_nstates.alloc(n);
+  B->d_tile_start.alloc(n + 1);
+  NC_CHECK_HIP(hipMemcpyAsync(B->tile_inst.p, tile_inst.data(), sizeof(int32_t) * B->tiles, hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->d_row_off.p, B->row_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->d_nstates.p, B->nstates.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->d_tile_start.p, B->tile_start.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+  if (B->precision == NC_FP32)
+    batch_build<float>(B.get(), mats);
+  else
+    batch_build<double>(B.get(), mats);
+  // integrator buffers
+  B->u.alloc(ld);
+  B->ytmp.alloc(ld);
+  B->k.alloc((size_t)7 * ld);
+  B->vin.alloc(ld * B->vs);
+  B->partials.alloc(B->tiles);
+  B->err.alloc(n);
+  B->d_h.alloc(n);
+  B->d_accept.alloc(n);
+  B->d_snap.alloc(n);
+  NC_CHECK_HIP(hipMemsetAsync(B->u.p, 0, sizeof(double) * ld, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->ytmp.p, 0, sizeof(double) * ld, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->k.p, 0, sizeof(double) * 7 * ld, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->vin.p, 0, B->vs * ld, s));
+  NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->h_pin), sizeof(double) * n, hipHostMallocDefault));
+  NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->acc_pin), sizeof(int32_t) * n, hipHostMallocDefault));
+  NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->snap_pin), sizeof(int32_t) * n, hipHostMallocDefault));
+  NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->err_pin), sizeof(double) * n, hipHostMallocDefault));
+  NC_CHECK_HIP(hipStreamSynchronize(s));
+  return B.release();
+}
+
+Dev dev_view(nc_batch_s* B) {
+  return Dev{B->tile_inst.p, B->d_row_off.p, B->d_nstates.p, B->d_h.p, B->d_accept.p, B->d_snap.p};
+}
+
+template <typename VT>
+void launch_matvec(nc_batch_s* B, const VT* v, double* out) {
+  const Dev d = dev_view(B);
+  hipStream_t s = B->stream.s;
+  k_bmatvec<VT><<<(unsigned)B->tiles, kBlock, 0, s>>>(d, B->ld, B->K, B->diag.p, B->col.p,
+                                                      reinterpret_cast<const VT*>(B->val.p), v, out);
+  if (B->nseg)
+    k_bsinks<VT><<<(unsigned)B->nseg, kBlock, 0, s>>>(d, B->seg_inst.p, B->seg_row.p, B->seg_off.p, B->sink_src.p,
+                                                      reinterpret_cast<const VT*>(B->sink_val.p), v, out);
+  NC_CHECK_HIP(hipGetLastError());
+}
+
+// accept/snapshot of the previous step, then one full DP5 attempt and the error fold
+template <typename VT>
+void iteration(nc_batch_s* B, double atol, double rtol) {
+  const Dev d = dev_view(B);
+  hipStream_t s = B->stream.s;
+  const unsigned g = (unsigned)B->tiles;
+  k_baccept<<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+  for (int st = 1; st < 7; ++st) {
+    k_bstage<VT><<<g, kBlock, 0, s>>>(d, B->ld, st, B->u.p, B->k.p, B->ytmp.p, reinterpret_cast<VT*>(B->vin.p));
+    launch_matvec<VT>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p + (int64_t)st * B->ld);
+  }
+  k_berr<<<g, kBlock, 0, s>>>(d, B->ld, atol, rtol, B->u.p, B->ytmp.p, B->k.p, B->partials.p);
+  k_bfold<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(B->n, B->d_tile_start.p, B->partials.p, B->err.p);
+  NC_CHECK_HIP(hipGetLastError());
+}
+
+template <typename VT>
+void batch_solve_t(nc_batch_s* B, const nc_solve_opts& o, const double* u0, int n_out, const double* t_out,
+                   nc_solve_stats* st) {
+  hipStream_t s = B->stream.s;
+  const int n = B->n;
+  const double rtol = o.rtol > 0 ? o.rtol : 1e-8, atol = o.atol > 0 ? o.atol : 1e-14;
+  // initial state and snapshot storage
+  NC_CHECK_HIP(hipMemsetAsync(B->u.p, 0, sizeof(double) * B->ld, s));
+  int64_t hoff = 0;
+  for (int i = 0; i < n; ++i) {
+    NC_CHECK_HIP(hipMemcpyAsync(B->u.p + B->row_off[i], u0 + hoff, sizeof(double) * B->nrows[i], hipMemcpyHostToDevice, s));
+    hoff += B->nrows[i];
+  }
+  if ((size_t)std::max(n_out, 1) * B->ld > B->snaps.n) {
+    // the captured iteration holds the snapshot pointer: drop it with the old buffer
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->exec = nullptr;
+    B->graph = nullptr;
+    B->snaps.alloc((size_t)std::max(n_out, 1) * B->ld);
+  }
+  B->n_out = n_out;
+  // k0 = A u for every instance (all marked active with h = 1 for the launch)
+  for (int i = 0; i < n; ++i) B->h_pin[i] = 1.0, B->acc_pin[i] = 0, B->snap_pin[i] = n_out > 0 ? 0 : -1;
+  NC_CHECK_HIP(hipMemcpyAsync(B->d_h.p, B->h_pin, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->d_accept.p, B->acc_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->d_snap.p, B->snap_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  {
+    const unsigned g = (unsigned)B->tiles;
+    k_bstage<VT><<<g, kBlock, 0, s>>>(dev_view(B), B->ld, 0, B->u.p, B->k.p, B->ytmp.p,
+                                      reinterpret_cast<VT*>(B->vin.p));
+    launch_matvec<VT>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p);
+  }
+  // per-instance controller state (the first output time is the start)
+  std::vector<double> t(n, n_out ? t_out[0] : 0.0), h(n, 0.0), h_try(n, 0.0);
+  std::vector<int> next(n, 1), done(n, 0), clamped(n, 0);
+  std::vector<int64_t> steps(n, 0), rejected(n, 0);
+  const double tend = n_out ? t_out[n_out - 1] : 0.0;
+  for (int i = 0; i < n; ++i) {
+    done[i] = n_out <= 1 ? 1 : 0;
+    // first step: a fraction of the span (the controller adapts it within a few steps)
+    h[i] = o.h0 > 0 ? o.h0 : 1e-3 * std::max(1e-12, tend - t[i]);
+  }
+  int64_t iters = 0, host_syncs = 0;
+  // snapshot slot 0 (the start) is filled by the first accept/snapshot kernel
+  for (;;) {
+    int active = 0;
+    for (int i = 0; i < n; ++i) {
+      if (done[i]) {
+        h_try[i] = 0.0;
+        continue;
+      }
+      const double target = t_out[next[i]];
+      double hh = std::min(h[i], target - t[i]);
+      clamped[i] = hh < h[i];
+      h_try[i] = hh;
+      ++active;
+    }
+    for (int i = 0; i < n; ++i) B->h_pin[i] = h_try[i];
+    NC_CHECK_HIP(hipMemcpyAsync(B->d_h.p, B->h_pin, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    NC_CHECK_HIP(hipMemcpyAsync(B->d_accept.p, B->acc_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    NC_CHECK_HIP(hipMemcpyAsync(B->d_snap.p, B->snap_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    if (!active) {
+      // flush the last accepts and snapshots
+      k_baccept<<<(unsigned)B->tiles, kBlock, 0, s>>>(dev_view(B), B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+      NC_CHECK_HIP(hipGetLastError());
+      NC_CHECK_HIP(hipStreamSynchronize(s));
+      break;
+    }
+    NC_REQUIRE(!o.max_steps || iters < o.max_steps, NC_ERR_NUMERIC, "batched RK exceeded max_steps");
+    if (o.no_graphs) {
+      iteration<VT>(B, atol, rtol);
+    } else {
+      if (!B->exec) {
+        NC_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          iteration<VT>(B, atol, rtol);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        NC_CHECK_HIP(hipStreamEndCapture(s, &B->graph));
+        NC_CHECK_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+      }
+      NC_CHECK_HIP(hipGraphLaunch(B->exec, s));
+    }
+    NC_CHECK_HIP(hipMemcpyAsync(B->err_pin, B->err.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    ++iters;
+    ++host_syncs;
+    for (int i = 0; i < n; ++i) {
+      B->acc_pin[i] = 0;
+      B->snap_pin[i] = -1;
+      if (h_try[i] == 0.0) continue;
+      const double err = std::sqrt(B->err_pin[i] / (double)B->nrows[i]);
+      NC_REQUIRE(std::isfinite(err), NC_ERR_NUMERIC, "non-finite error estimate in a batched instance");
+      double fac = err > 0 ? 0.9 * std::pow(err, -0.2) : 5.0;
+      if (err <= 1.0) {
+        B->acc_pin[i] = 1;
+        ++steps[i];
+        t[i] += h_try[i];
+        const double target = t_out[next[i]];
+        if (t[i] >= target - 1e-13 * std::max(1.0, std::fabs(target))) {
+          t[i] = target;
+          B->snap_pin[i] = next[i];
+          if (++next[i] >= n_out) done[i] = 1;
+        }
+        fac = std::min(5.0, std::max(0.2, fac));
+        if (!clamped[i] || fac < 1.0) h[i] = h_try[i] * fac;
+      } else {
+        ++rejected[i];
+        h[i] = h_try[i] * std::max(0.2, fac);
+      }
+      NC_REQUIRE(h[i] > 1e-14 * std::max(1.0, std::fabs(t[i])), NC_ERR_NUMERIC, "batched RK step size underflow");
+    }
+  }
+  if (st) {
+    *st = nc_solve_stats{};
+    for (int i = 0; i < n; ++i) st->steps += steps[i], st->rejected += rejected[i];
+    st->host_syncs = host_syncs;
+    st->matvecs = iters * 6;  // batched mat-vec launches
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int nc_batch_create(int32_t n, const nc_matrix* mats, nc_batch* out) {
+  return guard([&] {
+    NC_REQUIRE(out, NC_ERR_ARG, "null argument");
+    *out = batch_create(n, mats);
+  });
+}
+
+int nc_batch_destroy(nc_batch B) {
+  return guard([&] { delete B; });
+}
+
+int nc_batch_info(nc_batch B, int64_t* total_rows, int64_t* instance_rows) {
+  return guard([&] {
+    NC_REQUIRE(B, NC_ERR_ARG, "null batch");
+    if (total_rows) *total_rows = B->ld;
+    if (instance_rows)
+      for (int i = 0; i < B->n; ++i) instance_rows[i] = B->nrows[i];
+  });
+}
+
+int nc_batch_solve(nc_batch B, const nc_solve_opts* opts, const double* u0, int32_t n_out, const double* t_out,
+                   nc_solve_stats* st) {
+  return guard([&] {
+    NC_REQUIRE(B && u0 && n_out >= 1 && t_out, NC_ERR_ARG, "bad arguments");
+    for (int k = 1; k < n_out; ++k) NC_REQUIRE(t_out[k] > t_out[k - 1], NC_ERR_ARG, "output times must increase");
+    nc_solve_opts o{};
+    if (opts)
+      o = *opts;
+    else
+      nc_solve_opts_default(&o);
+    if (B->precision == NC_FP32)
+      batch_solve_t<float>(B, o, u0, n_out, t_out, st);
+    else
+      batch_solve_t<double>(B, o, u0, n_out, t_out, st);
+  });
+}
+
+int nc_batch_get_snapshot(nc_batch B, int32_t inst, int32_t k, double* u) {
+  return guard([&] {
+    NC_REQUIRE(B && u && inst >= 0 && inst < B->n && k >= 0 && k < B->n_out, NC_ERR_ARG, "index out of range");
+    NC_CHECK_HIP(hipMemcpy(u, B->snaps.p + (size_t)k * B->ld + B->row_off[inst], sizeof(double) * B->nrows[inst],
+                           hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

@@ -13,6 +13,7 @@ from .model import (CmeModel, CmeModelWithSensitivity, JointTimeVaryingPropensit
                     istimevarying, propensity)
 from .solve import RK45, FspOutputSliceSparse, FspOutputSparse, KrylovExpv, solve
 from .adaptive import AdaptiveFspSparse, RStepAdapter, SelectiveRStepAdapter, adapt_, init_
+from .batch import BatchFsp
 from .sens import (AdaptiveForwardSensFspSparse, ForwardSensFspInitialConditionSparse, ForwardSensFspMatrixSparse,
                    ForwardSensFspOutputSliceSparse, ForwardSensFspOutputSparse, ForwardSensRStepAdapter,
                    forwardsens_initial_condition)
@@ -27,7 +28,7 @@ __all__ = [
     "SeparableTimeVaryingPropensity", "StandardTimeInvariantPropensity", "istimeseparable", "istimevarying",
     "propensity", "RK45", "FspOutputSliceSparse", "FspOutputSparse", "KrylovExpv", "solve",
     "AdaptiveFspSparse", "RStepAdapter", "SelectiveRStepAdapter", "adapt_", "init_",
-    "AdaptiveForwardSensFspSparse", "ForwardSensFspInitialConditionSparse", "ForwardSensFspMatrixSparse",
+    "BatchFsp", "AdaptiveForwardSensFspSparse", "ForwardSensFspInitialConditionSparse", "ForwardSensFspMatrixSparse",
     "ForwardSensFspOutputSliceSparse", "ForwardSensFspOutputSparse", "ForwardSensRStepAdapter",
     "forwardsens_initial_condition",
     "StateSpaceSparse", "deleteat", "expand", "get_sink_connectivity", "get_sink_count",

@@ -169,6 +169,11 @@ SIGNATURES = {
     "nc_sensmatrix_bench_matvec": [P, D, I32, PD],
     "nc_solver_create_sens": [P, C.POINTER(nc_solve_opts), C.POINTER(P)],
     "nc_solve_sens_fixed": [P, C.POINTER(nc_solve_opts), PD, I32, PD, PD, C.POINTER(nc_solve_stats)],
+    "nc_batch_create": [I32, C.POINTER(P), C.POINTER(P)],
+    "nc_batch_destroy": [P],
+    "nc_batch_info": [P, PI64, PI64],
+    "nc_batch_solve": [P, C.POINTER(nc_solve_opts), PD, I32, PD, C.POINTER(nc_solve_stats)],
+    "nc_batch_get_snapshot": [P, I32, I32, PD],
     "nc_adapter_init": [P, C.POINTER(nc_adapter)],
     "nc_adapter_adapt": [P, C.POINTER(nc_adapter), I64, PD, PD, D, D, D, PI64, PI64],
     "nc_adaptive_opts_default": [C.POINTER(nc_adaptive_opts)],

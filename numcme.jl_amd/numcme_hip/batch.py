@@ -1,0 +1,62 @@
+"""Batched FSP solves (no reference counterpart: NumCME.jl solves one model per call).
+
+``BatchFsp(matrices)`` copies many time-invariant generators (``FspMatrixSparse``) of
+uneven size into one device layout; ``solve(u0s, ts)`` integrates all of them together
+with Dormand-Prince 5(4), each instance with its own step control, and returns one list of
+snapshots (``len(ts)`` arrays of ``rowcount`` entries) per instance.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import ArgumentError
+
+
+class BatchFsp(L.Handle):
+    _destroy = "nc_batch_destroy"
+
+    def __init__(self, matrices):
+        self.matrices = list(matrices)
+        arr = (C.c_void_p * len(self.matrices))(*[m.handle for m in self.matrices])
+        h = C.c_void_p()
+        L.call("nc_batch_create", len(self.matrices), arr, C.byref(h))
+        super().__init__(h.value)
+        total = C.c_int64()
+        rows = np.empty(len(self.matrices), dtype=np.int64)
+        L.call("nc_batch_info", self.handle, C.byref(total), L.ptr(rows, C.c_int64))
+        self.total_rows = total.value
+        self.rowcounts = rows
+        self.stats = None
+
+    def __len__(self):
+        return len(self.matrices)
+
+    def solve(self, u0s, ts, rtol=1e-8, atol=1e-14, h0=0.0, graphs=True, fetch=True):
+        """Integrate instance i from ``u0s[i]`` through the shared output times ``ts``."""
+        if len(u0s) != len(self):
+            raise ArgumentError(L.NC_ERR_ARG, "one initial vector per instance")
+        for u, n in zip(u0s, self.rowcounts):
+            if len(u) != n:
+                raise ArgumentError(L.NC_ERR_ARG, "initial vector length must equal the instance's rowcount")
+        u0 = L.f64(np.concatenate([np.asarray(u, dtype=np.float64) for u in u0s]))
+        ts = L.f64(ts)
+        o = L.nc_solve_opts()
+        L.call("nc_solve_opts_default", C.byref(o))
+        o.method = L.NC_METHOD_RK45
+        o.rtol, o.atol, o.h0 = float(rtol), float(atol), float(h0)
+        o.no_graphs = 0 if graphs else 1
+        st = L.nc_solve_stats()
+        L.call("nc_batch_solve", self.handle, C.byref(o), L.ptr(u0, C.c_double), len(ts), L.ptr(ts, C.c_double),
+               C.byref(st))
+        self.stats = {f: getattr(st, f) for f, _ in L.nc_solve_stats._fields_}
+        if not fetch:
+            return None
+        return [[self.snapshot(i, k) for k in range(len(ts))] for i in range(len(self))]
+
+    def snapshot(self, inst, k):
+        out = np.empty(int(self.rowcounts[inst]))
+        L.call("nc_batch_get_snapshot", self.handle, int(inst), int(k), L.ptr(out, C.c_double))
+        return out
