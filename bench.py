@@ -203,6 +203,83 @@ def c3_sensitivity(nh, L):
     return res
 
 
+C4_TABLE = ROOT / "tests" / "golden" / "c4_instances.json"
+
+
+def c4_instance(nh, spec, precision):
+    """Space, generator and initial vector of one batched instance (telegraph or toggle)."""
+    th = spec["theta"]
+    nmax = spec["nmax"]
+    if spec["kind"] == "telegraph":
+        S = np.array([[-1, 1, 0, 0], [1, -1, 0, 0], [0, 0, 1, -1]])
+        props = [nh.propensity(lambda x, p: p[0] * x[0]), nh.propensity(lambda x, p: p[1] * x[1]),
+                 nh.propensity(lambda x, p: p[2] * x[1]), nh.propensity(lambda x, p: p[3] * x[2])]
+        states = np.array([[1, 0, m] for m in range(nmax + 1)] + [[0, 1, m] for m in range(nmax + 1)])
+    else:
+        model = toggle_model(nh)
+        S, props = model.stoich_matrix, model.propensities
+        states = box_states(nmax + 1)
+    sp = nh.StateSpaceSparse(S, states)
+    A = nh.FspMatrixSparse(sp, props, th, precision=precision)
+    u0 = np.zeros(len(states) + S.shape[1])
+    u0[0] = 1.0  # telegraph (G_off, 0) / toggle (0, 0): the first state of the list
+    return A, u0
+
+
+def c4_batch(nh, L, n_plain=32):
+    """Config C4: 512 uneven instances in one batched FP32 solve against the plain FP64
+    single-solve path (timed on an evenly spaced sample by size, scaled to 512)."""
+    table = json.loads(C4_TABLE.read_text())
+    ts = np.arange(0.0, 11.0)
+    mats, u0s = [], []
+    for spec in table:
+        A, u0 = c4_instance(nh, spec, "fp32")
+        mats.append(A)
+        u0s.append(u0)
+    B = nh.BatchFsp(mats)
+    B.solve(u0s, ts, fetch=False)  # warm-up: captures the iteration graph
+    t0 = time.perf_counter()
+    B.solve(u0s, ts, fetch=False)
+    batch_s = time.perf_counter() - t0
+    stats = dict(B.stats)
+    order = np.argsort([t["states"] for t in table])
+    sample = order[np.linspace(0, len(order) - 1, n_plain).round().astype(int)]
+    plain = {"rk45": 0.0, "krylov": 0.0}
+    l1 = 0.0
+    for i in sample:
+        A64, u0 = c4_instance(nh, table[i], "fp64")
+        for method in ("rk45", "krylov"):
+            o = L.nc_solve_opts()
+            L.call("nc_solve_opts_default", C.byref(o))
+            o.method = L.NC_METHOD_RK45 if method == "rk45" else L.NC_METHOD_KRYLOV
+            o.rtol, o.atol = 1e-8, 1e-14
+            out = np.empty(len(ts) * len(u0))
+            t1 = time.perf_counter()
+            L.call("nc_solve_fixed", A64.handle, C.byref(o), L.ptr(u0, C.c_double), len(ts), L.ptr(ts, C.c_double),
+                   L.ptr(out, C.c_double), None)
+            plain[method] += time.perf_counter() - t1
+            out = out.reshape(len(ts), len(u0))
+            for k in range(len(ts)):
+                l1 = max(l1, float(np.abs(B.snapshot(int(i), k) - out[k]).sum()))
+    best = min(plain, key=plain.get)
+    plain_est = plain[best] * len(table) / n_plain
+    sizes = [t["states"] for t in table]
+    return {
+        "workload": f"512 instances (256 telegraph + 256 toggle, seeded table {C4_TABLE.name}), "
+                    f"{min(sizes)}..{max(sizes)} states, {sum(sizes)} in total, t = 0:1:10, DP5(4) rtol 1e-8",
+        "batched_fp32_solve_s": batch_s,
+        "batched_solves_per_s": len(table) / batch_s,
+        "batched_iterations": stats["host_syncs"],
+        "plain_fp64_solves_per_s": len(table) / plain_est,
+        "plain_fp64_rk45_solves_per_s": len(table) * n_plain / (plain["rk45"] * len(table)),
+        "plain_fp64_krylov_solves_per_s": len(table) * n_plain / (plain["krylov"] * len(table)),
+        "plain_method": best,
+        "plain_sample": f"{n_plain} instances evenly spaced by size, timed one by one, scaled to 512",
+        "speedup_batched_vs_plain": plain_est / batch_s,
+        "max_l1_batched_fp32_vs_plain_fp64": l1,
+    }
+
+
 def spmv_probe(nh, precision, reps):
     """Mat-vec launches plus a copy kernel of known size, for rocprofv3 --pmc runs."""
     import torch
@@ -233,6 +310,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch Arnoldi kernels eagerly (for profilers)")
     ap.add_argument("--no-c3", action="store_true", help="skip the forward-sensitivity (C3) measurement")
+    ap.add_argument("--no-c4", action="store_true", help="skip the batched (C4) measurement")
     ap.add_argument("--spmv-probe", type=int, default=0, metavar="REPS",
                     help="only build the matrix, launch the mat-vec REPS times back to back and run a plain "
                          "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
@@ -348,6 +426,8 @@ def main():
     }
     if not args.no_c3:
         line["c3_sensitivity"] = c3_sensitivity(nh, L)
+    if not args.no_c4:
+        line["c4_batch"] = c4_batch(nh, L)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(model.parameters, fp64_steps)
     print(json.dumps(line), flush=True)
