@@ -348,6 +348,27 @@ int nc_output_get(nc_output o, int32_t k, int32_t* states, double* p, double* si
  * dsinks (nr); either may be null. */
 int nc_output_get_sens(nc_output o, int32_t k, int32_t param, double* S, double* dsinks);
 
+/* ======================================================================================
+ * Learned space adapter network — no reference counterpart. An 8 -> 64 -> 64 -> 1 MLP
+ * (tanh-form GELU hidden layers, keep/add logit) on BF16 MFMA with FP32 accumulation,
+ * trained on the device (binary cross-entropy, Adam on FP32 master weights). Parameters
+ * (4801 floats): W1 [8][64], b1 [64], W2 [64][64], b2 [64], w3 [64], b3.
+ * ==================================================================================== */
+typedef struct nc_mlp_s* nc_mlp;
+#define NC_MLP_FEATURES 8
+#define NC_MLP_PARAMS 4801
+int nc_mlp_create(uint64_t seed, nc_mlp* out);
+int nc_mlp_destroy(nc_mlp M);
+int nc_mlp_get_params(nc_mlp M, float* params);
+int nc_mlp_set_params(nc_mlp M, const float* params);
+/* logits[i] for the n x 8 feature rows x */
+int nc_mlp_forward(nc_mlp M, int64_t n, const float* x, float* logits);
+/* Gradient of the mean binary cross-entropy over (x, y) (labels 0/1), and the loss. */
+int nc_mlp_grad(nc_mlp M, int64_t n, const float* x, const float* y, float* grad, double* loss);
+/* Adam training over shuffled mini-batches; *final_loss is the mean loss of the last epoch. */
+int nc_mlp_train(nc_mlp M, int64_t n, const float* x, const float* y, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, double* final_loss);
+
 /* ---- helpers exposed for testing --------------------------------------------------- */
 /* Dense matrix exponential of an n x n row-major matrix (Pade 13 scaling and squaring,
  * Higham 2005) — the host-side Hessenberg exponential of the Krylov integrator. */

@@ -14,6 +14,7 @@ from .model import (CmeModel, CmeModelWithSensitivity, JointTimeVaryingPropensit
 from .solve import RK45, FspOutputSliceSparse, FspOutputSparse, KrylovExpv, solve
 from .adaptive import AdaptiveFspSparse, RStepAdapter, SelectiveRStepAdapter, adapt_, init_
 from .batch import BatchFsp
+from .mlp import AdapterMlp
 from .sens import (AdaptiveForwardSensFspSparse, ForwardSensFspInitialConditionSparse, ForwardSensFspMatrixSparse,
                    ForwardSensFspOutputSliceSparse, ForwardSensFspOutputSparse, ForwardSensRStepAdapter,
                    forwardsens_initial_condition)

@@ -174,6 +174,13 @@ SIGNATURES = {
     "nc_batch_info": [P, PI64, PI64],
     "nc_batch_solve": [P, C.POINTER(nc_solve_opts), PD, I32, PD, C.POINTER(nc_solve_stats)],
     "nc_batch_get_snapshot": [P, I32, I32, PD],
+    "nc_mlp_create": [C.c_uint64, C.POINTER(P)],
+    "nc_mlp_destroy": [P],
+    "nc_mlp_get_params": [P, C.POINTER(C.c_float)],
+    "nc_mlp_set_params": [P, C.POINTER(C.c_float)],
+    "nc_mlp_forward": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+    "nc_mlp_grad": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), PD],
+    "nc_mlp_train": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float), I32, I32, C.c_float, C.c_uint64, PD],
     "nc_adapter_init": [P, C.POINTER(nc_adapter)],
     "nc_adapter_adapt": [P, C.POINTER(nc_adapter), I64, PD, PD, D, D, D, PI64, PI64],
     "nc_adaptive_opts_default": [C.POINTER(nc_adaptive_opts)],
