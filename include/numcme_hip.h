@@ -209,6 +209,10 @@ typedef struct {
   int64_t matvecs_timed;
   double matvec_ms;    /* summed event-timed mat-vec kernel durations              */
   double error_estimate;
+  int64_t adaptations; /* adaptive solves: state-space adaptations                 */
+  int64_t learned_adaptations;  /* of which the learned adapter selected the states */
+  int64_t fallback_adaptations; /* of which fell back to plain SelectiveRStep       */
+  int64_t max_states;  /* adaptive solves: largest state count reached             */
 } nc_solve_stats;
 
 typedef struct nc_solver_s* nc_solver;
@@ -285,12 +289,23 @@ int nc_batch_get_snapshot(nc_batch B, int32_t inst, int32_t k, double* u);
  * odeatol, odertol) (fspsolve.jl:59-62,105-197) and the space adapters RStepAdapter /
  * SelectiveRStepAdapter with their init! / adapt! (rstepadapters.jl:12-110).
  * ==================================================================================== */
-enum { NC_ADAPTER_RSTEP = 0, NC_ADAPTER_SELECTIVE_RSTEP = 1 };
+/* NC_ADAPTER_LEARNED has no reference counterpart: it is SelectiveRStepAdapter whose
+ * first expansion level starts only from the boundary states a trained nc_mlp selects
+ * (see the learned space adapter section below). */
+enum { NC_ADAPTER_RSTEP = 0, NC_ADAPTER_SELECTIVE_RSTEP = 1, NC_ADAPTER_LEARNED = 2 };
+typedef struct nc_mlp_s* nc_mlp;
 typedef struct {
   int32_t kind;               /* NC_ADAPTER_* (rstepadapters.jl:12-16, 61-65)          */
   int32_t initial_step_count; /* init! expansion depth                                 */
   int32_t max_step_count;     /* adapt! expansion depth                                */
   int32_t dropstates;         /* drop low-probability states before expanding         */
+  nc_mlp mlp;                 /* NC_ADAPTER_LEARNED: the trained network (borrowed)    */
+  double threshold;           /* NC_ADAPTER_LEARNED: expand from a boundary state when
+                                 sigmoid(logit) >= threshold                          */
+  double fallback_dt;         /* NC_ADAPTER_LEARNED: when the sink bound fires again
+                                 sooner than this after a learned adaptation, the next
+                                 adaptation is plain SelectiveRStep; <= 0 means
+                                 1e-3 (t1 - t0)                                       */
 } nc_adapter;
 
 /* init!(space, adapter, p, t, fsptol) — rstepadapters.jl:23-28, 74-79: expand! over all
@@ -304,16 +319,20 @@ int nc_adapter_init(nc_space space, const nc_adapter* ad);
  * ignored otherwise. On return drop_ids[0..*n_drop) holds the sorted 0-based states that
  * were deleted (capacity n); the caller deletes them from p and appends zeros for the
  * states that expand! added. Errors: NC_ERR_ARG for an empty p with the selective adapter
- * (the reference's ArgumentError, :90-92). */
+ * (the reference's ArgumentError, :90-92), and for NC_ADAPTER_LEARNED, whose features need
+ * the model: it adapts inside nc_solve_adaptive. */
 int nc_adapter_adapt(nc_space space, const nc_adapter* ad, int64_t n, const double* p, const double* dsinks,
                      double t, double tend, double fsptol, int64_t* n_drop, int64_t* drop_ids);
 
+typedef struct nc_records_s* nc_records; /* learned-adapter training records (below) */
 typedef struct {
   nc_adapter adapter;         /* space_adapter of AdaptiveFspSparse (fspsolve.jl:59-62)  */
   int32_t precision;          /* generator precision, NC_FP64 or NC_FP32                 */
   int32_t max_adaptations;    /* error out after this many adaptations (0 = unlimited)   */
   double fsptol;              /* FSP tolerance (default 1e-6, fspsolve.jl:108)           */
   nc_solve_opts ode;          /* ode_method and its tolerances                           */
+  int32_t verbose;            /* print each space update to stdout (fspsolve.jl:184-186) */
+  nc_records records;         /* optional: collect learned-adapter training records      */
 } nc_adaptive_opts;
 
 /* FspOutputSparse (fspoutput.jl:19-74): snapshots with their own state lists, kept on
@@ -354,7 +373,6 @@ int nc_output_get_sens(nc_output o, int32_t k, int32_t param, double* S, double*
  * trained on the device (binary cross-entropy, Adam on FP32 master weights). Parameters
  * (4801 floats): W1 [8][64], b1 [64], W2 [64][64], b2 [64], w3 [64], b3.
  * ==================================================================================== */
-typedef struct nc_mlp_s* nc_mlp;
 #define NC_MLP_FEATURES 8
 #define NC_MLP_PARAMS 4801
 int nc_mlp_create(uint64_t seed, nc_mlp* out);
@@ -368,6 +386,34 @@ int nc_mlp_grad(nc_mlp M, int64_t n, const float* x, const float* y, float* grad
 /* Adam training over shuffled mini-batches; *final_loss is the mean loss of the last epoch. */
 int nc_mlp_train(nc_mlp M, int64_t n, const float* x, const float* y, int32_t epochs, int32_t batch, float lr,
                  uint64_t seed, double* final_loss);
+
+/* Learned space adapter — no reference counterpart. At a sink-bound event the reactions
+ * to expand are chosen as SelectiveRStepAdapter chooses them (rstepadapters.jl:100-108);
+ * the boundary states are the states with a sink on one of them. Each boundary state x of
+ * the space (n states, probabilities p) at time t gets eight features:
+ *   f0..f2  x_s / (1 + max_s) for species s = 0, 1, 2 (max over the space; 0 for s >= ns)
+ *   f3      log10(max(p(x), 1e-30)) / 30
+ *   f4      log10(1 + sum_r a_r(t, x)) / 4                       (total propensity)
+ *   f5      log10(max(share, 1e-30)) / 30, share = p(x) sum_{r in sinks(x)} a_r(t, x) / sum_r dsinks_r
+ *   f6      t / tend
+ *   f7      |sinks(x)| / nr                                         (all reactions)
+ * The network's logit keeps a boundary state when sigmoid(logit) >= threshold; expand!
+ * then starts its first level from the kept boundary states only. Nothing kept, or a sink
+ * event sooner than fallback_dt after the previous learned adaptation, makes the
+ * adaptation plain SelectiveRStep; the sink bound still judges every step. */
+/* Boundary states (0-based ids, ascending; capacity n) and their feature rows (n x 8) for
+ * the given reactions-to-expand choice (dsinks, nr values). */
+int nc_adapter_features(nc_space space, nc_model model, int64_t n, const double* p, const double* dsinks, double t,
+                        double tend, int64_t* n_boundary, int64_t* ids, float* features);
+/* Training records: during an adaptive solve whose opts->records is set, every adaptation
+ * records the features of up to per_adaptation boundary states (an even stride over them),
+ * and labels each one at the next adaptation (or at the end of the solve): 1 when the
+ * probability then held by its successors along its sink reactions is >= label_mass. */
+int nc_records_create(int64_t capacity, int64_t per_adaptation, double label_mass, nc_records* out);
+int nc_records_destroy(nc_records R);
+int nc_records_count(nc_records R, int64_t* n);
+int nc_records_get(nc_records R, float* features, float* labels);
+int nc_records_clear(nc_records R);
 
 /* ---- helpers exposed for testing --------------------------------------------------- */
 /* Dense matrix exponential of an n x n row-major matrix (Pade 13 scaling and squaring,

@@ -18,8 +18,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <memory>
+#include <string>
 
+#include "nc_learned.h"
 #include "nc_solver.h"
 
 namespace nc {
@@ -181,10 +184,20 @@ int64_t select_drop(const double* p, int64_t n, double thr, bool strict, hipStre
 
 void check_adapter(const nc_adapter* ad) {
   NC_REQUIRE(ad, NC_ERR_ARG, "null adapter");
-  NC_REQUIRE(ad->kind == NC_ADAPTER_RSTEP || ad->kind == NC_ADAPTER_SELECTIVE_RSTEP, NC_ERR_ARG,
-             "unknown adapter kind");
+  NC_REQUIRE(ad->kind == NC_ADAPTER_RSTEP || ad->kind == NC_ADAPTER_SELECTIVE_RSTEP ||
+                 ad->kind == NC_ADAPTER_LEARNED,
+             NC_ERR_ARG, "unknown adapter kind");
   NC_REQUIRE(ad->initial_step_count >= 0 && ad->max_step_count >= 0, NC_ERR_ARG, "step counts must be non-negative");
+  if (ad->kind == NC_ADAPTER_LEARNED) {
+    NC_REQUIRE(ad->mlp, NC_ERR_ARG, "the learned adapter needs a network");
+    NC_REQUIRE(std::isfinite(ad->threshold) && std::isfinite(ad->fallback_dt), NC_ERR_ARG,
+               "threshold and fallback_dt must be finite");
+  }
 }
+
+// SelectiveRStep and the learned adapter built on it expand only along reactions with a
+// positive sink derivative and drop with ">".
+bool selective_kind(int kind) { return kind != NC_ADAPTER_RSTEP; }
 
 std::vector<int32_t> all_reactions(int nr) {
   std::vector<int32_t> rl(nr);
@@ -284,16 +297,26 @@ std::vector<double> sink_derivative(nc_matrix_s* A, double t, const double* u, h
 // dropped ids are left in `ids` when requested.
 // vecs[0] is p; further vectors (sensitivities) are dropped alongside it
 // (fsspaceadapterssparse.jl:42-58). kept[b] receives the compacted vecs[b].
+// Extras of an adaptation inside the adaptive solve: the learned adapter's network input
+// and the training records.
+struct AdaptCtx {
+  const nc_model_s* model = nullptr;
+  nc_records_s* records = nullptr;
+  LearnWork* lw = nullptr;
+  bool force_plain = false;  // learned kind: adapt as plain SelectiveRStep this time
+  bool learned = false;      // out: the network chose the first-level states
+};
+
 int64_t adapt_space(nc_space_s* sp, const nc_adapter& ad, const std::vector<const double*>& vecs,
                     const double* dsinks, double t, double tend, double fsptol, DropWork& w, DevBuf<int64_t>& ids,
-                    std::vector<DevBuf<double>>& kept) {
+                    std::vector<DevBuf<double>>& kept, AdaptCtx* ctx = nullptr) {
   hipStream_t s = sp->stream.s;
   const int64_t n = sp->n;
   const double* p = vecs[0];
   int64_t m = 0;
   if (ad.dropstates) {
     const double thr = 1.0 - t * fsptol / tend;
-    m = select_drop(p, n, thr, ad.kind == NC_ADAPTER_SELECTIVE_RSTEP, s, w, ids);
+    m = select_drop(p, n, thr, selective_kind(ad.kind), s, w, ids);
   }
   kept.resize(vecs.size());
   for (auto& k : kept) k.reserve((size_t)std::max<int64_t>(n - m, 1));
@@ -306,10 +329,34 @@ int64_t adapt_space(nc_space_s* sp, const nc_adapter& ad, const std::vector<cons
       NC_CHECK_HIP(hipMemcpyAsync(kept[b].p, vecs[b], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   }
   const std::vector<int32_t> rl =
-      ad.kind == NC_ADAPTER_SELECTIVE_RSTEP ? selective_reactions(dsinks, sp->nr) : all_reactions(sp->nr);
-  space_expand_list(sp, ad.max_step_count, rl);
+      selective_kind(ad.kind) ? selective_reactions(dsinks, sp->nr) : all_reactions(sp->nr);
+  const uint32_t* select = nullptr;
+  const bool want_learned = ctx && ad.kind == NC_ADAPTER_LEARNED && !ctx->force_plain;
+  const bool want_records = ctx && ctx->records;
+  if ((want_learned || want_records) && sp->n > 0) {
+    uint32_t rmask = 0;
+    for (int32_t r : rl) rmask |= 1u << r;
+    const int64_t nE = learned_features(sp, ctx->model, kept[0].p, dsinks, rmask, t, tend, *ctx->lw, s);
+    if (want_records) records_add(ctx->records, sp, nE, rmask, *ctx->lw, s);
+    if (want_learned && nE > 0 && learned_select(sp, ad.mlp, nE, ad.threshold, *ctx->lw, s) > 0) {
+      select = ctx->lw->select.p;
+      ctx->learned = true;
+    }
+  }
+  space_expand_list(sp, ad.max_step_count, rl, select);
   NC_CHECK_HIP(hipStreamSynchronize(s));
   return m;
+}
+
+// println("t = $(round(tnow, digits=2)). Update state space. New size: $(n).")
+// (fspsolve.jl:184-186); Julia prints the rounded Float64 without trailing zeros.
+void print_update(double t, int64_t n) {
+  char buf[64];
+  std::snprintf(buf, sizeof(buf), "%.2f", std::round(t * 100.0) / 100.0);
+  std::string v(buf);
+  while (v.size() > 2 && v.back() == '0' && v[v.size() - 2] != '.') v.pop_back();
+  std::printf("t = %s. Update state space. New size: %lld.\n", v.c_str(), (long long)n);
+  std::fflush(stdout);
 }
 
 bool is_time_varying(const nc_model_s* m) {
@@ -364,6 +411,17 @@ void solve_adaptive_impl(nc_model_s* model, int P, int64_t n0, const int32_t* st
   double tnow = tstart;
   int64_t adaptations = 0;
   int save_idx = 0;  // first saveat point not yet passed
+  NC_REQUIRE(P == 0 || !o.records, NC_ERR_ARG, "training records come from plain (P = 0) solves");
+  LearnWork lw;
+  AdaptCtx ctx;
+  ctx.model = model;
+  ctx.records = o.records;
+  ctx.lw = &lw;
+  const double fallback_dt = o.adapter.fallback_dt > 0.0 ? o.adapter.fallback_dt : 1e-3 * (tend - tstart);
+  double last_adapt_t = -INFINITY;
+  bool last_learned = false;
+  int64_t n_learned = 0, n_fallback = 0, max_states = sp->n;
+  double step_hint = 0.0;
   while (tnow < tend) {
     std::unique_ptr<nc_matrix_s> Aown;
     std::unique_ptr<nc_sensmatrix_s> SA;
@@ -377,6 +435,8 @@ void solve_adaptive_impl(nc_model_s* model, int P, int64_t n0, const int32_t* st
     }
     std::unique_ptr<nc_solver_s> S(solver_create(A, &ode, SA.get()));
     solver_set_state_device(S.get(), tnow, ucur.p);
+    // a restart continues with the step size the previous space's integrator reached
+    if (ode.h0 <= 0.0) solver_seed_step(S.get(), step_hint);
     output_add_space(out, sp.get());
     EventSpec ev{o.fsptol, tend, false};
     auto advance = [&](double target, bool one_step) {
@@ -418,6 +478,7 @@ void solve_adaptive_impl(nc_model_s* model, int P, int64_t n0, const int32_t* st
     NC_CHECK_HIP(hipStreamSynchronize(S->s));
     accumulate_stats(tot, S.get());
     tnow = S->t;
+    step_hint = solver_step_hint(S.get());
     if (ev.fired && tnow < tend) {
       NC_REQUIRE(o.max_adaptations <= 0 || adaptations < o.max_adaptations, NC_ERR_NUMERIC,
                  "adaptive FSP exceeded max_adaptations");
@@ -427,12 +488,23 @@ void solve_adaptive_impl(nc_model_s* model, int P, int64_t n0, const int32_t* st
       std::vector<double> sinks((size_t)B * nr);
       NC_CHECK_HIP(hipMemcpy2D(sinks.data(), sizeof(double) * nr, S->u.p + n_old, sizeof(double) * S->Npb,
                                sizeof(double) * nr, B, hipMemcpyDeviceToHost));
+      // the records of the previous adaptation are labelled by the mass reached now
+      if (o.records) records_label(o.records, sp.get(), S->u.p, sp->stream.s);
       std::vector<double> dsinks;
-      if (o.adapter.kind == NC_ADAPTER_SELECTIVE_RSTEP) dsinks = sink_derivative(A, tnow, S->u.p, S->s);
+      if (selective_kind(o.adapter.kind) || o.records) dsinks = sink_derivative(A, tnow, S->u.p, S->s);
       std::vector<const double*> vecs;
       for (int b = 0; b < B; ++b) vecs.push_back(S->u.p + b * S->Npb);
-      const int64_t n_kept =
-          n_old - adapt_space(sp.get(), o.adapter, vecs, dsinks.data(), tnow, tend, o.fsptol, work, ids, kept);
+      // an event soon after a learned adaptation means it added too little: adapt plainly
+      ctx.force_plain =
+          o.adapter.kind == NC_ADAPTER_LEARNED && last_learned && tnow - last_adapt_t < fallback_dt;
+      ctx.learned = false;
+      const int64_t n_kept = n_old - adapt_space(sp.get(), o.adapter, vecs, dsinks.data(), tnow, tend, o.fsptol,
+                                                 work, ids, kept, &ctx);
+      if (o.adapter.kind == NC_ADAPTER_LEARNED) ++(ctx.learned ? n_learned : n_fallback);
+      last_learned = ctx.learned;
+      last_adapt_t = tnow;
+      max_states = std::max<int64_t>(max_states, sp->n);
+      if (o.verbose) print_update(tnow, sp->n);
       // numerical trick of fspsolve.jl:178-181: restart strictly below the budget
       double ssum = 0.0;
       for (int r = 0; r < nr; ++r) ssum += sinks[r];
@@ -458,8 +530,13 @@ void solve_adaptive_impl(nc_model_s* model, int P, int64_t n0, const int32_t* st
       // the final slice is added once more (fspsolve.jl:189-194)
       output_add_snap(out, tnow, S.get());
       NC_CHECK_HIP(hipStreamSynchronize(S->s));
+      if (o.records) records_label(o.records, sp.get(), S->u.p, sp->stream.s);
     }
   }
+  tot.adaptations = adaptations;
+  tot.learned_adaptations = n_learned;
+  tot.fallback_adaptations = n_fallback;
+  tot.max_states = max_states;
   if (st_out) *st_out = tot;
 }
 
@@ -481,6 +558,8 @@ int nc_adapter_adapt(nc_space sp, const nc_adapter* ad, int64_t n, const double*
   return guard([&] {
     NC_REQUIRE(sp && n_drop, NC_ERR_ARG, "null argument");
     check_adapter(ad);
+    NC_REQUIRE(ad->kind != NC_ADAPTER_LEARNED, NC_ERR_ARG,
+               "the learned adapter's features need the model: it adapts inside nc_solve_adaptive");
     NC_REQUIRE(n == sp->n, NC_ERR_ARG, "p must hold one probability per state");
     NC_REQUIRE(n == 0 || p, NC_ERR_ARG, "null p");
     if (ad->kind == NC_ADAPTER_SELECTIVE_RSTEP) {
@@ -508,6 +587,11 @@ int nc_adaptive_opts_default(nc_adaptive_opts* o) {
   o->adapter.dropstates = 0;
   o->precision = NC_FP64;
   o->max_adaptations = 100000;
+  o->adapter.mlp = nullptr;
+  o->adapter.threshold = 0.5;
+  o->adapter.fallback_dt = 0.0;
+  o->verbose = 0;
+  o->records = nullptr;
   o->fsptol = 1e-6;
   return nc_solve_opts_default(&o->ode);
 }

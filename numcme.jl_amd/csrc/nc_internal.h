@@ -55,6 +55,21 @@ int guard(F&& f) {
   }
 }
 
+// ---- memory and stream caches ---------------------------------------------------------
+// Adaptive solves rebuild their generator and integrator at every space adaptation, so
+// device blocks, pinned host blocks and streams are cached for reuse instead of going back
+// to the driver (hipMalloc / hipHostMalloc / hipStreamCreate cost tens to hundreds of
+// microseconds each). A freed device block is reused for a request of at most its size and
+// at least half of it; a failed allocation empties the cache and retries. Callers release
+// a block only when no queued work uses it.
+void* pool_alloc(size_t bytes);
+void pool_free(void* p);
+void pool_trim();  // return every cached device block to the driver
+void* pinned_alloc(size_t bytes);
+void pinned_free(void* p);
+hipStream_t stream_acquire();        // non-blocking stream, default priority
+void stream_release(hipStream_t s);  // synchronises it, then keeps it for reuse
+
 // ---- device buffers -------------------------------------------------------------------
 template <class T>
 struct DevBuf {
@@ -73,11 +88,7 @@ struct DevBuf {
   void alloc(size_t count) {
     release();
     if (count == 0) return;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      throw Error(NC_ERR_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
-    }
+    p = static_cast<T*>(pool_alloc(count * sizeof(T)));
     n = count;
   }
   // grow to at least `count` elements; contents are NOT preserved
@@ -85,9 +96,7 @@ struct DevBuf {
   // grow to at least `count` elements, preserving the first `keep` elements
   void grow_keep(size_t count, size_t keep, hipStream_t s) {
     if (count <= n) return;
-    T* q = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&q), count * sizeof(T));
-    if (e != hipSuccess) throw Error(NC_ERR_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    T* q = static_cast<T*>(pool_alloc(count * sizeof(T)));
     if (keep && p) NC_CHECK_HIP(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
     release();
@@ -98,7 +107,7 @@ struct DevBuf {
     if (p) {
       // work queued on any stream may still read the buffer
       (void)hipDeviceSynchronize();
-      (void)hipFree(p);
+      pool_free(p);
     }
     p = nullptr;
     n = 0;
@@ -122,8 +131,8 @@ void expm_dense(int n, const double* a, double* out);
 // Stream with library-owned lifetime; default priority only.
 struct Stream {
   hipStream_t s = nullptr;
-  Stream() { NC_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
-  ~Stream() { if (s) (void)hipStreamDestroy(s); }
+  Stream() { s = stream_acquire(); }
+  ~Stream() { if (s) stream_release(s); }
   Stream(const Stream&) = delete;
   Stream& operator=(const Stream&) = delete;
 };

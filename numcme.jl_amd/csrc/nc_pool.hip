@@ -1,0 +1,177 @@
+// Caches of device blocks, pinned host blocks and streams (host code only).
+//
+// Device blocks: a freed block goes into a per-device size-ordered free list and serves
+// any later request of at most its size and at least half of it (best fit). Requests are
+// rounded up to 4 KiB, or 2 MiB above 1 MiB, so that the repeated rebuilds of an adaptive
+// solve, whose sizes creep upwards, find their blocks again. The cache holds at most
+// kCacheCap bytes; a block freed beyond that goes back to the driver. When hipMalloc
+// fails, every cached block is returned and the allocation is retried once.
+//
+// The caches live until the process ends and are never torn down (the HIP runtime may
+// already be gone when static destructors run).
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "nc_internal.h"
+
+namespace nc {
+namespace {
+
+constexpr size_t kSmallGrain = 4096;
+constexpr size_t kLargeGrain = size_t(2) << 20;
+constexpr size_t kCacheCap = size_t(16) << 30;
+
+size_t round_bytes(size_t b) {
+  const size_t g = b > (size_t(1) << 20) ? kLargeGrain : kSmallGrain;
+  return (b + g - 1) / g * g;
+}
+
+struct Cache {
+  std::mutex mu;
+  std::map<int, std::multimap<size_t, void*>> free_dev;  // device -> size -> block
+  std::unordered_map<void*, std::pair<int, size_t>> live_dev;
+  size_t cached_dev = 0;
+  std::multimap<size_t, void*> free_host;
+  std::unordered_map<void*, size_t> live_host;
+  std::map<int, std::vector<hipStream_t>> streams;
+  std::unordered_map<hipStream_t, int> stream_dev;
+};
+
+Cache& cache() {
+  static Cache* c = new Cache();  // intentionally never destroyed
+  return *c;
+}
+
+int current_device() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  return d;
+}
+
+void trim_locked(Cache& c) {
+  for (auto& [dev, lst] : c.free_dev) {
+    if (lst.empty()) continue;
+    int cur = current_device();
+    (void)hipSetDevice(dev);
+    for (auto& [sz, p] : lst) (void)hipFree(p);
+    lst.clear();
+    (void)hipSetDevice(cur);
+  }
+  c.cached_dev = 0;
+}
+
+}  // namespace
+
+void* pool_alloc(size_t bytes) {
+  Cache& c = cache();
+  const size_t want = round_bytes(bytes);
+  const int dev = current_device();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto& lst = c.free_dev[dev];
+  auto it = lst.lower_bound(want);
+  if (it != lst.end() && it->first <= 2 * want) {
+    void* p = it->second;
+    const size_t sz = it->first;
+    lst.erase(it);
+    c.cached_dev -= sz;
+    c.live_dev[p] = {dev, sz};
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim_locked(c);
+    e = hipMalloc(&p, want);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(NC_ERR_ALLOC, "hipMalloc of " + std::to_string(want) + " bytes failed: " + hipGetErrorString(e));
+  }
+  c.live_dev[p] = {dev, want};
+  return p;
+}
+
+void pool_free(void* p) {
+  if (!p) return;
+  Cache& c = cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.live_dev.find(p);
+  if (it == c.live_dev.end()) return;  // not ours
+  const auto [dev, sz] = it->second;
+  c.live_dev.erase(it);
+  if (c.cached_dev + sz > kCacheCap) {
+    (void)hipFree(p);
+    return;
+  }
+  c.free_dev[dev].emplace(sz, p);
+  c.cached_dev += sz;
+}
+
+void pool_trim() {
+  Cache& c = cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  trim_locked(c);
+}
+
+void* pinned_alloc(size_t bytes) {
+  Cache& c = cache();
+  const size_t want = round_bytes(bytes);
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.free_host.lower_bound(want);
+    if (it != c.free_host.end() && it->first <= 2 * want) {
+      void* p = it->second;
+      c.live_host[p] = it->first;
+      c.free_host.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  NC_CHECK_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.live_host[p] = want;
+  return p;
+}
+
+void pinned_free(void* p) {
+  if (!p) return;
+  Cache& c = cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.live_host.find(p);
+  if (it == c.live_host.end()) return;
+  c.free_host.emplace(it->second, p);
+  c.live_host.erase(it);
+}
+
+hipStream_t stream_acquire() {
+  Cache& c = cache();
+  const int dev = current_device();
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto& v = c.streams[dev];
+    if (!v.empty()) {
+      hipStream_t s = v.back();
+      v.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s = nullptr;
+  NC_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.stream_dev[s] = dev;
+  return s;
+}
+
+void stream_release(hipStream_t s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s);
+  Cache& c = cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.stream_dev.find(s);
+  if (it == c.stream_dev.end()) return;
+  c.streams[it->second].push_back(s);
+}
+
+}  // namespace nc

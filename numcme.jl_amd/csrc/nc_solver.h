@@ -49,9 +49,10 @@ struct nc_solver_s {
   ~nc_solver_s() {
     if (arnoldi_exec) (void)hipGraphExecDestroy(arnoldi_exec);
     if (arnoldi_graph) (void)hipGraphDestroy(arnoldi_graph);
-    if (hbuf) (void)hipHostFree(hbuf);
-    if (hcoef) (void)hipHostFree(hcoef);
-    if (hpart) (void)hipHostFree(hpart);
+    if (s) (void)hipStreamSynchronize(s);  // copies into the pinned blocks have landed
+    if (hbuf) nc::pinned_free(hbuf);
+    if (hcoef) nc::pinned_free(hcoef);
+    if (hpart) nc::pinned_free(hpart);
   }
 };
 
@@ -69,5 +70,9 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev = nullptr, bool o
 // Internal entry points shared with the adaptive driver.
 nc_solver_s* solver_create(nc_matrix_s* A, const nc_solve_opts* opts, nc_sensmatrix_s* SA = nullptr);
 void solver_set_state_device(nc_solver_s* S, double t0, const double* u_dev);
+// Step size the integrator would try next (RK: h, Krylov: tau), 0 before its first step.
+double solver_step_hint(const nc_solver_s* S);
+// Start the next step from h instead of the automatic first step (h <= 0: no change).
+void solver_seed_step(nc_solver_s* S, double h);
 void solver_reduce_setup(nc_solver_s* S, int nvals);
 }  // namespace nc

@@ -450,8 +450,7 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
   S->red_blocks = std::min<unsigned>(kVecBlocks, grid_for(S->Np / 4, kBlock));
   S->partials.reserve((size_t)S->red_blocks * kPartStrideMax);
   if (!S->hpart)
-    NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hpart), sizeof(double) * (kMaxRedBlocks + 4 * kMaxReactions),
-                               hipHostMallocDefault));
+    S->hpart = static_cast<double*>(pinned_alloc(sizeof(double) * (kMaxRedBlocks + 4 * kMaxReactions)));
 }
 
 // The right-hand-side operator: the FSP generator, or the block sensitivity operator.
@@ -945,8 +944,8 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
       const int ksize = L.size_with_sinks(A->nr);
       S->kbuf.alloc(ksize);
       NC_CHECK_HIP(hipMemsetAsync(S->kbuf.p, 0, sizeof(double) * ksize, S->s));
-      NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hbuf), sizeof(double) * ksize, hipHostMallocDefault));
-      NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->hcoef), sizeof(double) * (S->m + 2), hipHostMallocDefault));
+      S->hbuf = static_cast<double*>(pinned_alloc(sizeof(double) * ksize));
+      S->hcoef = static_cast<double*>(pinned_alloc(sizeof(double) * (S->m + 2)));
       // k_vec writes partials with a stride of MAXV+1 <= 33 doubles per block
       solver_reduce_setup(S, 32);
       if (method == NC_METHOD_KRYLOV) {
@@ -1151,6 +1150,22 @@ nc_solver_s* solver_create(nc_matrix_s* A, const nc_solve_opts* opts, nc_sensmat
 
 // Set the state from a device vector of the true length N, blocks contiguous (t0 is its
 // time).
+double solver_step_hint(const nc_solver_s* S) {
+  if (S->method == NC_METHOD_KRYLOV) return S->started ? S->t_new : 0.0;
+  return S->h;
+}
+
+void solver_seed_step(nc_solver_s* S, double h) {
+  if (!(h > 0.0) || !std::isfinite(h)) return;
+  if (S->method == NC_METHOD_KRYLOV) {
+    S->anorm = std::max(S->SA ? S->SA->norm1 : S->A->norm1, 1e-300);
+    S->t_new = h;
+    S->started = true;
+  } else {
+    S->h = h;
+  }
+}
+
 void solver_set_state_device(nc_solver_s* S, double t0, const double* u_dev) {
   if (!S->u0.p) {
     S->u0.alloc(S->Np);

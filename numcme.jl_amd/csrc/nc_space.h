@@ -77,8 +77,11 @@ nc_space_s* space_create(int ns, int nr, const int32_t* stoich, int64_t n_init, 
 void space_reserve(nc_space_s* sp, int64_t rows);
 void space_rebuild_table(nc_space_s* sp, int64_t min_entries);
 void space_add_candidates(nc_space_s* sp, int64_t n_cand);  // candidates already in sp->cands
-// expand! over the reactions in rlist (0-based; callers pass every reaction for "all")
-void space_expand_list(nc_space_s* sp, int levels, const std::vector<int32_t>& rlist);
+// expand! over the reactions in rlist (0-based; callers pass every reaction for "all").
+// With `select` (device flags, one per state) the first level starts only from the
+// explorable states whose flag is nonzero; later levels explore every state added.
+void space_expand_list(nc_space_s* sp, int levels, const std::vector<int32_t>& rlist,
+                       const uint32_t* select = nullptr);
 // deleteat! with sorted unique device ids; compacts nvec companion vectors alongside
 void space_delete_device(nc_space_s* sp, const int64_t* d_ids, int64_t m, int nvec, const double* const* vin,
                          double* const* vout);

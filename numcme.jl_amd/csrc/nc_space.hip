@@ -40,10 +40,12 @@ __global__ void k_table_insert_all(uint32_t* table, uint64_t mask, const int32_t
   }
 }
 
-// Explorables of the first level: states with a sink on one of the listed reactions.
-__global__ void k_flag_explorable(const uint32_t* sink_mask, int64_t n, uint32_t rmask, uint32_t* flags) {
+// Explorables of the first level: states with a sink on one of the listed reactions (and,
+// when `select` is given, a nonzero select flag).
+__global__ void k_flag_explorable(const uint32_t* sink_mask, int64_t n, uint32_t rmask, const uint32_t* select,
+                                  uint32_t* flags) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) flags[i] = (sink_mask[i] & rmask) ? 1u : 0u;
+  if (i < n) flags[i] = ((sink_mask[i] & rmask) && (!select || select[i])) ? 1u : 0u;
 }
 
 __global__ void k_scatter_flagged(const uint32_t* flags, const uint32_t* ranks, int64_t n, int32_t* out) {
@@ -272,7 +274,8 @@ void space_add_candidates(nc_space_s* sp, int64_t n_cand) {
   }
 }
 
-static void space_expand(nc_space_s* sp, int levels, const std::vector<int32_t>& rlist) {
+static void space_expand(nc_space_s* sp, int levels, const std::vector<int32_t>& rlist,
+                         const uint32_t* select = nullptr) {
   if (levels <= 0) return;  // sparsestatespace.jl:154-156
   hipStream_t s = sp->stream.s;
   RList rl{};
@@ -288,7 +291,8 @@ static void space_expand(nc_space_s* sp, int levels, const std::vector<int32_t>&
     sp->flags.reserve(sp->n);
     sp->ranks.reserve(sp->n);
     sp->explorables.reserve(sp->n);
-    k_flag_explorable<<<grid_for(sp->n, kBlock), kBlock, 0, s>>>(sp->sink_mask.p, sp->n, rmask, sp->flags.p);
+    k_flag_explorable<<<grid_for(sp->n, kBlock), kBlock, 0, s>>>(sp->sink_mask.p, sp->n, rmask, select,
+                                                                   sp->flags.p);
     uint32_t cnt = 0;
     exclusive_scan_u32(sp->flags.p, sp->ranks.p, sp->n, sp->scan_tmp, s, &cnt);
     k_scatter_flagged<<<grid_for(sp->n, kBlock), kBlock, 0, s>>>(sp->flags.p, sp->ranks.p, sp->n, sp->explorables.p);
@@ -395,7 +399,9 @@ nc_space_s* space_create(int ns, int nr, const int32_t* stoich, int64_t n_init, 
   return sp;
 }
 
-void space_expand_list(nc_space_s* sp, int levels, const std::vector<int32_t>& rlist) { space_expand(sp, levels, rlist); }
+void space_expand_list(nc_space_s* sp, int levels, const std::vector<int32_t>& rlist, const uint32_t* select) {
+  space_expand(sp, levels, rlist, select);
+}
 
 }  // namespace nc
 

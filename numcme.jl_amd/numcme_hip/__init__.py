@@ -12,9 +12,10 @@ from .model import (CmeModel, CmeModelWithSensitivity, JointTimeVaryingPropensit
                     SeparableTimeVaryingPropensity, StandardTimeInvariantPropensity, istimeseparable,
                     istimevarying, propensity)
 from .solve import RK45, FspOutputSliceSparse, FspOutputSparse, KrylovExpv, solve
-from .adaptive import AdaptiveFspSparse, RStepAdapter, SelectiveRStepAdapter, adapt_, init_
+from .adaptive import AdaptiveFspSparse, LearnedAdapter, RStepAdapter, SelectiveRStepAdapter, adapt_, init_
 from .batch import BatchFsp
 from .mlp import AdapterMlp
+from .learned import TrainingRecords, boundary_features, train_adapter
 from .sens import (AdaptiveForwardSensFspSparse, ForwardSensFspInitialConditionSparse, ForwardSensFspMatrixSparse,
                    ForwardSensFspOutputSliceSparse, ForwardSensFspOutputSparse, ForwardSensRStepAdapter,
                    forwardsens_initial_condition)

@@ -36,6 +36,7 @@ NC_METHOD_RK45 = 2
 
 NC_ADAPTER_RSTEP = 0
 NC_ADAPTER_SELECTIVE_RSTEP = 1
+NC_ADAPTER_LEARNED = 2
 
 
 class NumCmeHipError(RuntimeError):
@@ -95,6 +96,10 @@ class nc_solve_stats(C.Structure):
         ("matvecs_timed", C.c_int64),
         ("matvec_ms", C.c_double),
         ("error_estimate", C.c_double),
+        ("adaptations", C.c_int64),
+        ("learned_adaptations", C.c_int64),
+        ("fallback_adaptations", C.c_int64),
+        ("max_states", C.c_int64),
     ]
 
 
@@ -104,6 +109,9 @@ class nc_adapter(C.Structure):
         ("initial_step_count", C.c_int32),
         ("max_step_count", C.c_int32),
         ("dropstates", C.c_int32),
+        ("mlp", C.c_void_p),
+        ("threshold", C.c_double),
+        ("fallback_dt", C.c_double),
     ]
 
 
@@ -114,6 +122,8 @@ class nc_adaptive_opts(C.Structure):
         ("max_adaptations", C.c_int32),
         ("fsptol", C.c_double),
         ("ode", nc_solve_opts),
+        ("verbose", C.c_int32),
+        ("records", C.c_void_p),
     ]
 
 
@@ -181,6 +191,13 @@ SIGNATURES = {
     "nc_mlp_forward": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float)],
     "nc_mlp_grad": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), PD],
     "nc_mlp_train": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float), I32, I32, C.c_float, C.c_uint64, PD],
+    "nc_adapter_features": [P, P, I64, PD, PD, C.c_double, C.c_double, C.POINTER(I64), C.POINTER(I64),
+                            C.POINTER(C.c_float)],
+    "nc_records_create": [I64, I64, C.c_double, C.POINTER(P)],
+    "nc_records_destroy": [P],
+    "nc_records_count": [P, C.POINTER(I64)],
+    "nc_records_get": [P, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+    "nc_records_clear": [P],
     "nc_adapter_init": [P, C.POINTER(nc_adapter)],
     "nc_adapter_adapt": [P, C.POINTER(nc_adapter), I64, PD, PD, D, D, D, PI64, PI64],
     "nc_adaptive_opts_default": [C.POINTER(nc_adaptive_opts)],

@@ -33,7 +33,8 @@ class RStepAdapter:
     kind = L.NC_ADAPTER_RSTEP
 
     def c_struct(self):
-        return L.nc_adapter(self.kind, int(self.initial_step_count), int(self.max_step_count), int(bool(self.dropstates)))
+        return L.nc_adapter(self.kind, int(self.initial_step_count), int(self.max_step_count), int(bool(self.dropstates)),
+                            None, 0.5, 0.0)
 
 
 @dataclass(frozen=True)
@@ -41,6 +42,29 @@ class SelectiveRStepAdapter(RStepAdapter):
     """Reachability adapter over the reactions whose sinks grow (rstepadapters.jl:61-65)."""
 
     kind = L.NC_ADAPTER_SELECTIVE_RSTEP
+
+
+@dataclass(frozen=True)
+class LearnedAdapter:
+    """Learned space adapter (no reference counterpart): ``SelectiveRStepAdapter`` whose
+    first expansion level starts only from the boundary states that ``mlp`` (an
+    ``AdapterMlp``) keeps, i.e. whose predicted probability is at least ``threshold``.
+    Nothing kept, or a sink event sooner than ``fallback_dt`` (0: 1e-3 of the time span)
+    after a learned adaptation, makes that adaptation plain ``SelectiveRStepAdapter``. The
+    sink bound judges every step exactly as with the plain adapters."""
+
+    mlp: object
+    initial_step_count: int
+    max_step_count: int
+    dropstates: bool = False
+    threshold: float = 0.5
+    fallback_dt: float = 0.0
+
+    kind = L.NC_ADAPTER_LEARNED
+
+    def c_struct(self):
+        return L.nc_adapter(self.kind, int(self.initial_step_count), int(self.max_step_count), int(bool(self.dropstates)),
+                            self.mlp.handle, float(self.threshold), float(self.fallback_dt))
 
 
 def init_(space, adapter: RStepAdapter, p, t=0.0, fsptol=1e-6):
@@ -106,14 +130,19 @@ def _saveat_points(saveat, tspan):
 
 
 def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: AdaptiveFspSparse, saveat=(),
-                   fsptol=1.0e-6, odeatol=1.0e-6, odertol=1.0e-4, verbose=False):
+                   fsptol=1.0e-6, odeatol=1.0e-6, odertol=1.0e-4, verbose=False, records=None):
+    """The adaptive solve. ``verbose`` prints every space update as the reference does;
+    ``records`` (a ``TrainingRecords``) collects learned-adapter training records."""
     from .solve import RK45, FspOutputSparse, KrylovExpv
 
-    if not isinstance(alg.space_adapter, RStepAdapter):
-        raise ArgumentError(L.NC_ERR_ARG, "space_adapter must be RStepAdapter or SelectiveRStepAdapter")
+    if not isinstance(alg.space_adapter, (RStepAdapter, LearnedAdapter)):
+        raise ArgumentError(L.NC_ERR_ARG,
+                            "space_adapter must be RStepAdapter, SelectiveRStepAdapter or LearnedAdapter")
     o = L.nc_adaptive_opts()
     L.call("nc_adaptive_opts_default", C.byref(o))
     o.adapter = alg.space_adapter.c_struct()
+    o.verbose = 1 if verbose else 0
+    o.records = None if records is None else records.handle
     o.fsptol = float(fsptol)
     o.max_adaptations = int(alg.max_adaptations)
     ode = alg.ode_method
@@ -162,10 +191,7 @@ def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: Ada
         out.space_ids.append(sid.value)
         out.p.append(FspVectorSparse(spaces[sid.value], p[: n.value], copy=False))
         out.sinks.append(sinks)
-        if verbose and states is not None and sid.value > 0:
-            print(f"t = {round(t.value, 2)}. Update state space. New size: {n.value}.")
     out.stats = {f: getattr(st, f) for f, _ in L.nc_solve_stats._fields_}
-    out.stats["adaptations"] = nspace.value - 1
     res.free()
     return out
 

@@ -89,9 +89,10 @@ class FspOutputSparse:
 
 
 def solve(model: CmeModel, initial_distribution: FspVectorSparse, tspan, method, saveat=(), fsptol=1.0e-6,
-          odeatol=1.0e-6, odertol=1.0e-4, verbose=False):
+          odeatol=1.0e-6, odertol=1.0e-4, verbose=False, records=None):
     """FSP solve. ``method`` is an integrator (fixed space, fspsolve.jl:10-41) or an
-    ``AdaptiveFspSparse`` (adaptive space, fspsolve.jl:105-197)."""
+    ``AdaptiveFspSparse`` (adaptive space, fspsolve.jl:105-197). ``records`` (adaptive
+    solves only) collects learned-adapter training records."""
     from .adaptive import AdaptiveFspSparse, solve_adaptive
     from .sens import AdaptiveForwardSensFspSparse, solve_sens
 
@@ -101,7 +102,9 @@ def solve(model: CmeModel, initial_distribution: FspVectorSparse, tspan, method,
                           odeatol=odeatol, odertol=odertol, verbose=verbose)
     if isinstance(method, AdaptiveFspSparse):
         return solve_adaptive(model, initial_distribution, tspan, method, saveat=saveat, fsptol=fsptol,
-                              odeatol=odeatol, odertol=odertol, verbose=verbose)
+                              odeatol=odeatol, odertol=odertol, verbose=verbose, records=records)
+    if records is not None:
+        raise ArgumentError(L.NC_ERR_ARG, "training records come from adaptive solves")
     return solve_fixed(model, initial_distribution, tspan, method, saveat=saveat, odeatol=odeatol, odertol=odertol)
 
 
