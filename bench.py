@@ -280,6 +280,109 @@ def c4_batch(nh, L, n_plain=32):
     }
 
 
+C5_THETA = (10.0, 2.0, 1.0, 0.5, 0.2, 0.1)
+MLP_FLOP_FORWARD = 2 * (8 * 64 + 64 * 64 + 64)  # 9,344 per sample
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def cascade_model(nh, theta):
+    """0 -> A k0; A -> A + B k1 A; B -> B + C k2 B; A, B, C -> 0 at k3 A, k4 B, k5 C."""
+    S = np.array([[1, 0, 0, -1, 0, 0], [0, 1, 0, 0, -1, 0], [0, 0, 1, 0, 0, -1]])
+    props = [nh.propensity(lambda x, p: p[0] + 0.0 * x[0]), nh.propensity(lambda x, p: p[1] * x[0]),
+             nh.propensity(lambda x, p: p[2] * x[1]), nh.propensity(lambda x, p: p[3] * x[0]),
+             nh.propensity(lambda x, p: p[4] * x[1]), nh.propensity(lambda x, p: p[5] * x[2])]
+    return nh.CmeModel(S, props, [float(v) for v in theta])
+
+
+def _l1_between(a, b):
+    da = {tuple(x): v for x, v in zip(a.states.tolist(), a.values)}
+    db = {tuple(x): v for x, v in zip(b.states.tolist(), b.values)}
+    return float(sum(abs(da.get(x, 0.0) - db.get(x, 0.0)) for x in set(da) | set(db)))
+
+
+def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_methods=("krylov",), log=None):
+    """Config C5: the cascade from (0,0,0) with fsptol 1e-4 over [0, T], adaptive FSP with
+    the learned adapter against plain SelectiveRStepAdapter(2, 2). The network is trained
+    on the device on records from plain runs over n_scalings seeded parameter scalings."""
+    def solve(model, adapter, ode, tend, records=None):
+        alg = nh.AdaptiveFspSparse(ode_method=ode, space_adapter=adapter)
+        t0 = time.perf_counter()
+        out = nh.solve(model, nh.FspVectorSparse([[0, 0, 0]], [1.0]), (0.0, tend), alg,
+                       saveat=np.linspace(0.0, tend, 6), fsptol=1e-4, records=records)
+        return out, time.perf_counter() - t0
+
+    rk = lambda prec: nh.RK45(rtol=1e-6, atol=1e-10, precision=prec)
+    kr = lambda prec: nh.KrylovExpv(m=30, tol=1e-8, precision=prec)
+    say = log or (lambda m: None)
+    rng = np.random.default_rng(0)
+    rec = nh.TrainingRecords(capacity=1_000_000, per_adaptation=1024, label_mass=1e-10)
+    t0 = time.perf_counter()
+    for _ in range(n_scalings):
+        th = np.asarray(C5_THETA) * np.exp(rng.uniform(np.log(0.5), np.log(2.0), size=6))
+        solve(cascade_model(nh, th), nh.SelectiveRStepAdapter(2, 2, False), rk("fp32"), T_rec, rec)
+    gen_s = time.perf_counter() - t0
+    X, Y = rec.arrays()
+    say(f"c5: {len(X)} records in {gen_s:.1f}s")
+    t0 = time.perf_counter()
+    net, loss = nh.train_adapter(X, Y, epochs=20, batch=4096, lr=2e-3, seed=0)
+    train_s = time.perf_counter() - t0
+    acc = float(np.mean((net.forward(X) > 0) == (Y > 0.5)))
+    say(f"c5: trained in {train_s:.1f}s, loss {loss:.4f}, accuracy {acc:.4f}")
+
+    model = cascade_model(nh, C5_THETA)
+    learned = nh.LearnedAdapter(net, 2, 2, False, threshold=0.3)
+    plain = nh.SelectiveRStepAdapter(2, 2, False)
+    solve(model, learned, rk("fp32"), T)  # warm-up
+    out_l, t_l = solve(model, learned, rk("fp32"), T)
+    out_l64, t_l64 = solve(model, learned, rk("fp64"), T)
+    say(f"c5: learned {t_l:.2f}s (fp64 {t_l64:.2f}s)")
+    plain_runs = {}
+    for m in plain_methods:
+        out_p, t_p = solve(model, plain, kr("fp64") if m == "krylov" else rk("fp64"), T)
+        plain_runs[m] = (out_p, t_p)
+        say(f"c5: plain fp64 {m} {t_p:.2f}s")
+    best = min(plain_runs, key=lambda m: plain_runs[m][1])
+    out_p, t_p = plain_runs[best]
+    l1_prec = max(_l1_between(a, b) for a, b in zip(out_l.p, out_l64.p))
+    l1_vs_plain = max(_l1_between(a, b) for a, b in zip(out_l64.p, out_p.p))
+    fwd_ms = net.bench(n=1 << 21, reps=20, train=False)
+    trn_ms = net.bench(n=1 << 21, reps=10, train=True)
+    fwd_tf = MLP_FLOP_FORWARD * (1 << 21) / (fwd_ms * 1e-3) / 1e12
+    trn_tf = 3 * MLP_FLOP_FORWARD * (1 << 21) / (trn_ms * 1e-3) / 1e12
+    st_l, st_p = out_l.stats, out_p.stats
+    return {
+        "workload": f"cascade (0->A 10, A->A+B 2A, B->B+C B, decay 0.5A 0.2B 0.1C) from (0,0,0), fsptol 1e-4, "
+                    f"t in [0, {T:g}] with 6 outputs; learned adapter (threshold 0.3) vs plain "
+                    f"SelectiveRStepAdapter(2, 2, false)",
+        "training": {"records": int(len(X)), "positive_fraction": float(Y.mean()), "scalings": n_scalings,
+                     "record_horizon": T_rec, "label_mass": 1e-10, "record_s": gen_s, "train_s": train_s,
+                     "epochs": 20, "final_loss": loss, "train_accuracy": acc},
+        "learned_fp32_rk45_s": t_l,
+        "learned_fp64_rk45_s": t_l64,
+        "learned_solves_per_s": 1.0 / t_l,
+        "plain_fp64_solves_per_s": 1.0 / t_p,
+        "plain_fp64_method": best,
+        "plain_fp64_times_s": {m: v[1] for m, v in plain_runs.items()},
+        "speedup_learned_vs_plain_fp64": t_p / t_l,
+        "learned_max_states": st_l["max_states"],
+        "plain_max_states": st_p["max_states"],
+        "learned_adaptations": st_l["adaptations"],
+        "learned_fallbacks": st_l["fallback_adaptations"],
+        "plain_adaptations": st_p["adaptations"],
+        "final_sink_mass_learned": float(out_l.sinks[-1].sum()),
+        "final_sink_mass_plain": float(out_p.sinks[-1].sum()),
+        "max_l1_learned_fp32_vs_fp64": l1_prec,
+        "max_l1_learned_vs_plain_fp64": l1_vs_plain,
+        "mlp_roofline": {
+            "bound": "mfma", "unit": "TFLOP/s", "peak": BF16_DENSE_PEAK_TFLOPS,
+            "forward_achieved": fwd_tf, "forward_frac": fwd_tf / BF16_DENSE_PEAK_TFLOPS,
+            "train_achieved": trn_tf, "train_frac": trn_tf / BF16_DENSE_PEAK_TFLOPS,
+            "flop_per_sample_forward": MLP_FLOP_FORWARD, "samples_per_launch": 1 << 21,
+            "forward_launch_ms": fwd_ms, "train_launch_ms": trn_ms,
+        },
+    }
+
+
 def spmv_probe(nh, precision, reps):
     """Mat-vec launches plus a copy kernel of known size, for rocprofv3 --pmc runs."""
     import torch
@@ -311,6 +414,7 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch Arnoldi kernels eagerly (for profilers)")
     ap.add_argument("--no-c3", action="store_true", help="skip the forward-sensitivity (C3) measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched (C4) measurement")
+    ap.add_argument("--no-c5", action="store_true", help="skip the learned-adapter (C5) measurement")
     ap.add_argument("--spmv-probe", type=int, default=0, metavar="REPS",
                     help="only build the matrix, launch the mat-vec REPS times back to back and run a plain "
                          "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
@@ -428,6 +532,8 @@ def main():
         line["c3_sensitivity"] = c3_sensitivity(nh, L)
     if not args.no_c4:
         line["c4_batch"] = c4_batch(nh, L)
+    if not args.no_c5:
+        line["c5_learned"] = c5_learned(nh, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(model.parameters, fp64_steps)
     print(json.dumps(line), flush=True)

@@ -387,6 +387,11 @@ int nc_mlp_grad(nc_mlp M, int64_t n, const float* x, const float* y, float* grad
 int nc_mlp_train(nc_mlp M, int64_t n, const float* x, const float* y, int32_t epochs, int32_t batch, float lr,
                  uint64_t seed, double* final_loss);
 
+/* Average duration (ms) of one launch of the forward (train = 0) or forward+backward
+ * (train = 1) kernel over n device-resident synthetic feature rows, timed with HIP events
+ * around reps back-to-back launches (the MLP's roofline measurement). */
+int nc_mlp_bench(nc_mlp M, int64_t n, int32_t reps, int32_t train, double* ms_per_launch);
+
 /* Learned space adapter — no reference counterpart. At a sink-bound event the reactions
  * to expand are chosen as SelectiveRStepAdapter chooses them (rstepadapters.jl:100-108);
  * the boundary states are the states with a sink on one of them. Each boundary state x of

@@ -453,4 +453,48 @@ int nc_mlp_train(nc_mlp M, int64_t n, const float* x, const float* y, int32_t ep
   });
 }
 
+int nc_mlp_bench(nc_mlp M, int64_t n, int32_t reps, int32_t train, double* ms_per_launch) {
+  return guard([&] {
+    NC_REQUIRE(M && n >= 1 && reps >= 1 && ms_per_launch, NC_ERR_ARG, "bad arguments");
+    hipStream_t s = M->stream.s;
+    // features uniform in [-1, 1), labels 0/1 (a fixed 64-bit LCG, so runs are repeatable)
+    std::vector<float> hx((size_t)n * kF), hy((size_t)n);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto next = [&] {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      return (float)((st >> 40) * (1.0 / 16777216.0));
+    };
+    for (auto& v : hx) v = 2.0f * next() - 1.0f;
+    for (auto& v : hy) v = next() < 0.3f ? 1.0f : 0.0f;
+    DevBuf<float> dx(hx.size()), dy(hy.size()), dl((size_t)n);
+    NC_CHECK_HIP(hipMemcpyAsync(dx.p, hx.data(), sizeof(float) * hx.size(), hipMemcpyHostToDevice, s));
+    NC_CHECK_HIP(hipMemcpyAsync(dy.p, hy.data(), sizeof(float) * hy.size(), hipMemcpyHostToDevice, s));
+    const unsigned g = (unsigned)((n + kSamples - 1) / kSamples);
+    if (train) {
+      M->part.reserve((size_t)g * kNP);
+      M->loss_part.reserve(g);
+    }
+    auto launch = [&] {
+      if (train)
+        k_mlp<true><<<g, kThreads, 0, s>>>(dx.p, dy.p, n, M->params.p, nullptr, M->part.p, M->loss_part.p);
+      else
+        k_mlp<false><<<g, kThreads, 0, s>>>(dx.p, nullptr, n, M->params.p, dl.p, nullptr, nullptr);
+    };
+    launch();  // warm-up
+    hipEvent_t e0, e1;
+    NC_CHECK_HIP(hipEventCreate(&e0));
+    NC_CHECK_HIP(hipEventCreate(&e1));
+    NC_CHECK_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    NC_CHECK_HIP(hipEventRecord(e1, s));
+    NC_CHECK_HIP(hipGetLastError());
+    NC_CHECK_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    NC_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_launch = ms / reps;
+  });
+}
+
 }  // extern "C"

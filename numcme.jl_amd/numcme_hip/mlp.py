@@ -76,3 +76,10 @@ class AdapterMlp(L.Handle):
         L.call("nc_mlp_train", self.handle, X.shape[0], L.ptr(X, C.c_float), L.ptr(Y, C.c_float), int(epochs),
                int(batch), C.c_float(lr), C.c_uint64(int(seed)), C.byref(loss))
         return loss.value
+
+    def bench(self, n=1 << 21, reps=20, train=False):
+        """Average kernel duration (ms) of one forward (or forward+backward) launch over n
+        synthetic device-resident rows."""
+        ms = C.c_double()
+        L.call("nc_mlp_bench", self.handle, int(n), int(reps), 1 if train else 0, C.byref(ms))
+        return ms.value
