@@ -47,16 +47,24 @@ __global__ void k_scatter_ids(const uint32_t* flags, const uint32_t* ranks, int6
   if (i < n && flags[i]) E[ranks[i]] = (int32_t)i;
 }
 
-// per-species maximum of the first three species: grid-stride maxima, a wavefront
-// shuffle reduction and one atomic per wave (states are non-negative, xmax starts at 0)
-__global__ void k_species_max(const int32_t* states, int64_t n, int ns, int32_t* xmax) {
+// per-species maximum of the first three species: grid-stride maxima, wavefront shuffles,
+// the block's waves through LDS, and one atomic per block and species (states are
+// non-negative, xmax starts at 0; the grid is kept small so few atomics meet one address)
+__global__ __launch_bounds__(kBlock) void k_species_max(const int32_t* states, int64_t n, int ns, int32_t* xmax) {
+  __shared__ int32_t wm[kBlock / 64][3];
   int32_t m[3] = {0, 0, 0};
   const int nsm = ns < 3 ? ns : 3;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     for (int s = 0; s < nsm; ++s) m[s] = max(m[s], states[i * ns + s]);
-  for (int s = 0; s < nsm; ++s) {
+  for (int s = 0; s < 3; ++s) {
     for (int off = 32; off > 0; off >>= 1) m[s] = max(m[s], __shfl_xor(m[s], off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(xmax + s, m[s]);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6][s] = m[s];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nsm) {
+    int32_t v = 0;
+    for (int w = 0; w < kBlock / 64; ++w) v = max(v, wm[w][threadIdx.x]);
+    atomicMax(xmax + threadIdx.x, v);
   }
 }
 
@@ -153,7 +161,7 @@ int64_t learned_features(nc_space_s* sp, const nc_model_s* m, const double* p, c
   k_scatter_ids<<<grid_for(n, kBlock), kBlock, 0, s>>>(w.flags.p, w.ranks.p, n, w.E.p);
   w.xmax.reserve(3);
   NC_CHECK_HIP(hipMemsetAsync(w.xmax.p, 0, sizeof(int32_t) * 3, s));
-  k_species_max<<<std::min<unsigned>(grid_for(n, kBlock), 1024), kBlock, 0, s>>>(sp->states.p, n, sp->ns, w.xmax.p);
+  k_species_max<<<std::min<unsigned>(grid_for(n, kBlock), 256), kBlock, 0, s>>>(sp->states.p, n, sp->ns, w.xmax.p);
   PropDesc d{};
   d.code = m->d_code.p;
   d.offs = m->d_offs.p;

@@ -44,6 +44,7 @@ struct nc_solver_s {
   nc::DevBuf<double> k;            // 7 x N stage derivatives (FP64)
   nc::DevBuf<double> ytmp;         // N
   double h = 0.0;
+  int ksw = 0;  // 1: RK stages 0 and 6 have traded slots in k
   bool fsal_valid = false;
   nc_solve_stats st{};
   ~nc_solver_s() {

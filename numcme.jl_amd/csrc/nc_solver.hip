@@ -361,14 +361,18 @@ const double h_dp_c[7] = {0, 1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
 __constant__ double c_dp_e[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525,
                                  -1.0 / 40};
 
+// Stage j lives in slot j of k, except that with sw = 1 stages 0 and 6 trade slots (an
+// accepted step hands k6 = f(y_{n+1}) on as the next k0 by flipping sw, not by copying).
+__device__ __host__ inline int rk_slot(int j, int sw) { return (sw && (j == 0 || j == 6)) ? 6 - j : j; }
+
 // Stage input y_i = y + h * sum_{j<i} a_ij k_j ; writes FP64 (ytmp) and the mat-vec copy.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, int stage, double h, const double* y, const double* k,
-                                                     double* ytmp, VT* vin) {
+                                                     int sw, double* ytmp, VT* vin) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) {
     double acc = 0.0;
-    for (int j = 0; j < stage; ++j) acc += c_dp_a[stage][j] * k[(int64_t)j * N + n];
+    for (int j = 0; j < stage; ++j) acc += c_dp_a[stage][j] * k[(int64_t)rk_slot(j, sw) * N + n];
     const double v = y[n] + h * acc;
     ytmp[n] = v;
     vin[n] = (VT)v;
@@ -376,14 +380,24 @@ __global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, int stage, doubl
 }
 
 // Error norm sum_n (h*sum_j e_j k_j / (atol + rtol*max(|y|,|ynew|)))^2 with ynew = ytmp;
-// one partial sum per block (folded on the host, which reads them back anyway).
+// one partial sum per block (folded on the host, which reads them back anyway). With
+// R > 0, block 0 also gathers the sink rows ns..ns+R-1 of ynew, k0 and k6 behind the
+// partials, so that one copy brings back everything the step controller needs.
 __global__ __launch_bounds__(kBlock) void k_rk_err(int64_t N, double h, double atol, double rtol, const double* y,
-                                                   const double* ynew, const double* k, double* partials) {
+                                                   const double* ynew, const double* k, int sw, int64_t ns, int R,
+                                                   double* partials) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < R) {
+    double* ex = partials + gridDim.x;
+    const int r = threadIdx.x;
+    ex[r] = ynew[ns + r];
+    ex[R + r] = k[(int64_t)rk_slot(0, sw) * N + ns + r];
+    ex[2 * R + r] = k[(int64_t)rk_slot(6, sw) * N + ns + r];
+  }
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) {
     double e = 0.0;
-    for (int j = 0; j < 7; ++j) e += c_dp_e[j] * k[(int64_t)j * N + n];
+    for (int j = 0; j < 7; ++j) e += c_dp_e[j] * k[(int64_t)rk_slot(j, sw) * N + n];
     e *= h;
     const double sc = atol + rtol * fmax(fabs(y[n]), fabs(ynew[n]));
     const double r = e / sc;
@@ -448,7 +462,8 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
   (void)nvals;
   // 512 blocks of 256 threads (two per CU) measured best for the vector passes
   S->red_blocks = std::min<unsigned>(kVecBlocks, grid_for(S->Np / 4, kBlock));
-  S->partials.reserve((size_t)S->red_blocks * kPartStrideMax);
+  // the RK error pass puts 3 R gathered sink values behind its partials
+  S->partials.reserve(std::max<size_t>((size_t)S->red_blocks * kPartStrideMax, S->red_blocks + 3 * kMaxReactions));
   if (!S->hpart)
     S->hpart = static_cast<double*>(pinned_alloc(sizeof(double) * (kMaxRedBlocks + 4 * kMaxReactions)));
 }
@@ -777,7 +792,7 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
   const double rtol = S->opts.rtol, atol = S->opts.atol;
   const unsigned g = std::min<unsigned>(4 * kMaxRedBlocks, grid_for(N, kBlock));
   double* k = S->k.p;
-  auto kp = [&](int j) { return k + (int64_t)j * N; };
+  auto kp = [&](int j) { return k + (int64_t)rk_slot(j, S->ksw) * N; };
   auto to_vin = [&](const double* src) {
     if (fp32)
       k_to_vt<float><<<g, kBlock, 0, s>>>(src, reinterpret_cast<float*>(S->rkv.p), N);
@@ -787,9 +802,10 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
   auto stages = [&](double h) {
     for (int stage = 1; stage < 7; ++stage) {
       if (fp32)
-        k_rk_stage<float><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p, reinterpret_cast<float*>(S->rkv.p));
+        k_rk_stage<float><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ksw, S->ytmp.p,
+                                                reinterpret_cast<float*>(S->rkv.p));
       else
-        k_rk_stage<double><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ytmp.p,
+        k_rk_stage<double><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ksw, S->ytmp.p,
                                                  reinterpret_cast<double*>(S->rkv.p));
       NC_CHECK_HIP(hipGetLastError());
       op_apply(S, S->t + h_dp_c[stage] * h, S->rkv.p, kp(stage));
@@ -834,15 +850,17 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
     const bool clamped = h < S->h;
     stages(h);
     // after stage 6 the stage input is y + h*sum b_j k_j (row 7 of A): ytmp = y_{n+1}
-    k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->partials.p);
+    const int Rg = (ev && R > 0) ? R : 0;
+    k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->ksw, ns, Rg, S->partials.p);
     NC_CHECK_HIP(hipGetLastError());
-    NC_CHECK_HIP(hipMemcpyAsync(S->hpart, S->partials.p, sizeof(double) * S->red_blocks, hipMemcpyDeviceToHost, s));
-    if (ev && R > 0) {
-      NC_CHECK_HIP(hipMemcpyAsync(hs + R, S->ytmp.p + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
-      NC_CHECK_HIP(hipMemcpyAsync(hs + 2 * R, kp(0) + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
-      NC_CHECK_HIP(hipMemcpyAsync(hs + 3 * R, kp(6) + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
-    }
+    NC_CHECK_HIP(hipMemcpyAsync(S->hpart, S->partials.p, sizeof(double) * (S->red_blocks + 3 * Rg),
+                                hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
+    for (int r = 0; r < Rg; ++r) {  // sinks of y_{n+1}, k0 and k6 behind the partials
+      hs[R + r] = S->hpart[S->red_blocks + r];
+      hs[2 * R + r] = S->hpart[S->red_blocks + R + r];
+      hs[3 * R + r] = S->hpart[S->red_blocks + 2 * R + r];
+    }
     errsum = 0.0;
     for (unsigned b = 0; b < S->red_blocks; ++b) errsum += S->hpart[b];
     S->st.host_syncs++;
@@ -882,10 +900,9 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
           break;
         }
       }
-      // accept: u <- ytmp, k0 <- k6 (FSAL)
-      k_copy_d<<<g, kBlock, 0, s>>>(S->ytmp.p, S->u.p, N);
-      k_copy_d<<<g, kBlock, 0, s>>>(kp(6), kp(0), N);
-      NC_CHECK_HIP(hipGetLastError());
+      // accept: u <- ytmp, k0 <- k6 (FSAL), by exchanging buffers
+      std::swap(S->u, S->ytmp);
+      S->ksw ^= 1;
       S->t += h;
       S->st.steps++;
       if (S->t > t_target - t_eps) S->t = t_target;
