@@ -531,6 +531,7 @@ def main():
             "launches_timed": mv_n,
             "back_to_back_launch_us": back_to_back_ms * 1e3,
             "rows_per_thread": rows_per_thread,
+            "computed_col_slots": info["computed_col_slots"],
         },
         "cpu_baseline": None,
     }

@@ -161,7 +161,8 @@ typedef struct {
   double csr_bytes_per_matvec; /* nnz*(s_v+4) + (N+1)*4 + 2*N*s_v, CSR-equivalent   */
   double norm1;          /* max column abs sum (2 * max |diagonal|)                 */
   int32_t rows_per_thread; /* mat-vec kernel: consecutive state rows per thread     */
-  int32_t pad0;
+  int32_t computed_col_slots; /* slots whose columns are row + constant offset in
+                            every nonzero row, computed instead of loaded          */
 } nc_matrix_info_t;
 
 /* FspMatrixSparse{RealT}(space, propensities; parameters) — fspsparsematrix.jl:47-108.

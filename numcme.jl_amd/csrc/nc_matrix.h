@@ -39,6 +39,10 @@ struct nc_matrix_s {
   nc::DevBuf<int64_t> sink_off;
   nc::DevBuf<int32_t> sink_src;
   nc::DevBuf<unsigned char> sink_val;
+  // slots whose stored column is i + dia_off[k] in every row with a nonzero value (box
+  // spaces in lattice order): the mat-vec computes those columns instead of loading them
+  uint32_t dia_mask = 0;
+  std::vector<int32_t> dia_off;
   nc::DevBuf<int32_t> states;      // kept for joint refreshes
   nc::DevBuf<int32_t> state_conn;  // kept for joint refreshes
   nc_model_s* model = nullptr;     // borrowed; must outlive the matrix

@@ -66,8 +66,13 @@ __global__ void k_build_count(int64_t n, int nr, const int32_t* state_conn, Grou
 
 // Rows i in [n, ld) are padding: zero values and column 0, so vectorised reads of a
 // thread's row group never leave the arrays and never gather out of range.
+// With `positional` (set when some row has every group's predecessor, so K equals the
+// group count) group k always occupies slot k and a missing predecessor is a zero entry
+// in column i; the products then run in the same order with zeros added, so the row sums
+// are unchanged, and on lattice-ordered spaces each slot's column is row + constant.
+// Otherwise a row's entries are packed into its first slots.
 template <typename VT>
-__global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, const int32_t* states,
+__global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int positional, const int32_t* states,
                              const int32_t* state_conn, const uint32_t* sink_mask, const int32_t* stoich, Groups g,
                              TvList tv, Progs pr, double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc,
                              VT* tvv, unsigned long long* diag_max_bits) {
@@ -94,6 +99,10 @@ __global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, const
       for (int m = g.start[k]; m < g.start[k + 1]; ++m) a += xprog(pr, g.members[m], 0.0, y);
       col[slot * ld + i] = j;
       val[slot * ld + i] = (VT)a;
+      ++slot;
+    } else if (positional) {
+      col[slot * ld + i] = (int32_t)i;
+      val[slot * ld + i] = (VT)0;
       ++slot;
     }
     // this row's outflow through the group: the diagonal is minus the stored column
@@ -133,6 +142,35 @@ __global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, const
       tvv[q * ld + i] = (VT)0;
     }
     tvc[q * ld + i] = j >= 0 ? j : (int32_t)i;
+  }
+}
+
+// Range of col[i] - i over the rows of one slot whose value is nonzero: mnmx[0] = min,
+// mnmx[1] = max (a grid-stride pass, wave and workgroup reductions, one atomic pair per
+// workgroup).
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t n, const int32_t* col, const VT* val, int32_t* mnmx) {
+  __shared__ int32_t slo[kBlock / 64], shi[kBlock / 64];
+  int32_t lo = INT32_MAX, hi = INT32_MIN;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    if (val[i] != (VT)0) {
+      const int32_t d = col[i] - (int32_t)i;
+      lo = min(lo, d);
+      hi = max(hi, d);
+    }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off, 64));
+    hi = max(hi, __shfl_xor(hi, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    slo[threadIdx.x >> 6] = lo;
+    shi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) lo = min(lo, slo[w]), hi = max(hi, shi[w]);
+    atomicMin(mnmx, lo);
+    atomicMax(mnmx + 1, hi);
   }
 }
 
@@ -209,6 +247,8 @@ struct MatArgs {
   const VT* sink_val;
   double tv_scale[kMaxReactions];
   double sink_scale[kMaxReactions];
+  uint32_t dia_mask;              // bit k: slot k's column is i + dia_off[k]
+  int32_t dia_off[kMaxReactions];
 };
 
 template <typename T, int R>
@@ -249,17 +289,33 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
       acc[r] = dg.v[r] * vi[r];
     }
     const int K = KT > 0 ? KT : a.K;
+    // column of slot k for rows i0..i0+R-1: loaded, or i + dia_off[k] clamped to the
+    // states (rows whose value is 0 may point anywhere; they add nothing)
+    auto slot_cols = [&](int k) {
+      Pack<int32_t, R> c;
+      if ((a.dia_mask >> k) & 1u) {
+        const int64_t hi = a.n - 1;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int64_t j = i0 + r + a.dia_off[k];
+          c.v[r] = (int32_t)(j < 0 ? 0 : (j > hi ? hi : j));
+        }
+      } else {
+        c = ldpack<int32_t, R>(a.col + k * a.ld + i0);
+      }
+      return c;
+    };
 #pragma unroll
     for (int k = 0; k < (KT > 0 ? KT : 8); ++k) {
       if (KT == 0 && k >= K) break;
-      const Pack<int32_t, R> c = ldpack<int32_t, R>(a.col + k * a.ld + i0);
+      const Pack<int32_t, R> c = slot_cols(k);
       const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
 #pragma unroll
       for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
     }
     if (KT == 0) {
       for (int k = 8; k < K; ++k) {
-        const Pack<int32_t, R> c = ldpack<int32_t, R>(a.col + k * a.ld + i0);
+        const Pack<int32_t, R> c = slot_cols(k);
         const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
@@ -357,6 +413,8 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   a.sink_off = A->sink_off.p;
   a.sink_src = A->sink_src.p;
   a.sink_val = reinterpret_cast<const VT*>(A->sink_val.p);
+  a.dia_mask = A->dia_mask;
+  for (size_t k = 0; k < A->dia_off.size() && k < (size_t)kMaxReactions; ++k) a.dia_off[k] = A->dia_off[k];
   const nc_model_s* m = A->model;
   for (int r = 0; r < A->nr; ++r) a.sink_scale[r] = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
   for (size_t q = 0; q < A->tv_r.size(); ++q)
@@ -469,10 +527,23 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   NC_CHECK_HIP(hipMemsetAsync(d_dmax.p, 0, sizeof(unsigned long long), s));
   if (ld)
     k_build_fill<VT><<<grid_for(ld, kBlock), kBlock, 0, s>>>(
-        n, ld, ns, nr, K, sp->states.p, sp->state_conn.p, sp->sink_mask.p, sp->d_stoich.p, g, tv, pr, A->diag.p,
+        n, ld, ns, nr, K, K == g.ng ? 1 : 0, sp->states.p, sp->state_conn.p, sp->sink_mask.p, sp->d_stoich.p, g, tv, pr, A->diag.p,
         A->col.p, reinterpret_cast<VT*>(A->val.p), reinterpret_cast<VT*>(A->tvd.p), A->tvc.p,
         reinterpret_cast<VT*>(A->tvv.p), d_dmax.p);
   NC_CHECK_HIP(hipGetLastError());
+  // slots with one column offset in every nonzero row (read back with the final sync)
+  const int kd = std::min<int>(K, kMaxReactions);
+  std::vector<int32_t> dia_h(2 * std::max(kd, 1));
+  DevBuf<int32_t> d_dia(2 * std::max(kd, 1));
+  if (n && kd) {
+    for (int k = 0; k < kd; ++k) dia_h[2 * k] = INT32_MAX, dia_h[2 * k + 1] = INT32_MIN;
+    NC_CHECK_HIP(hipMemcpyAsync(d_dia.p, dia_h.data(), sizeof(int32_t) * 2 * kd, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < kd; ++k)
+      k_dia_detect<VT><<<std::min<unsigned>(grid_for(n, kBlock), 256), kBlock, 0, s>>>(
+          n, A->col.p + (int64_t)k * ld, reinterpret_cast<const VT*>(A->val.p) + (int64_t)k * ld, d_dia.p + 2 * k);
+    NC_CHECK_HIP(hipGetLastError());
+    NC_CHECK_HIP(hipMemcpyAsync(dia_h.data(), d_dia.p, sizeof(int32_t) * 2 * kd, hipMemcpyDeviceToHost, s));
+  }
   // sink entries, grouped by sink row (reaction), ascending source state within a row
   DevBuf<uint32_t> flags(std::max<int64_t>(n * nr, 1)), pos(std::max<int64_t>(n * nr, 1));
   DevBuf<unsigned char> scan_tmp;
@@ -513,6 +584,15 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   unsigned long long dmax_bits = 0;
   NC_CHECK_HIP(hipMemcpyAsync(&dmax_bits, d_dmax.p, sizeof(dmax_bits), hipMemcpyDeviceToHost, s));
   NC_CHECK_HIP(hipStreamSynchronize(s));
+  A->dia_mask = 0;
+  A->dia_off.assign(kd, 0);
+  for (int k = 0; n && k < kd; ++k) {
+    const int32_t lo = dia_h[2 * k], hi = dia_h[2 * k + 1];
+    if (lo > hi || lo == hi) {  // no nonzero entry, or one offset throughout
+      A->dia_mask |= 1u << k;
+      A->dia_off[k] = lo > hi ? 0 : lo;
+    }
+  }
   double dmax;
   std::memcpy(&dmax, &dmax_bits, sizeof(dmax));
   A->norm1 = 2.0 * dmax;
@@ -536,6 +616,7 @@ double matrix_bytes_per_matvec(const nc_matrix_s* A) {
   const double n = (double)A->n;
   double b = n * 8.0                                  // diag
              + n * A->K * (4.0 + vs)                  // ELL slots
+             - n * __builtin_popcount(A->dia_mask) * 4.0  // computed columns
              + n * A->tv_r.size() * (4.0 + 2.0 * vs)  // time-varying slots
              + (double)A->n_sink_entries * (4.0 + vs)  // sink lists
              + (double)(A->nr + 1) * 8.0;
@@ -712,6 +793,7 @@ int nc_matrix_info(nc_matrix A, nc_matrix_info_t* info) {
         (double)A->nnz_ref * (sv + 4.0) + (double)(A->n_rows + 1) * 4.0 + 2.0 * (double)A->n_rows * sv;
     info->norm1 = A->norm1;
     info->rows_per_thread = A->rows_per_thread;
+    info->computed_col_slots = __builtin_popcount(A->dia_mask);
   });
 }
 

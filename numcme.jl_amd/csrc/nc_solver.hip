@@ -209,6 +209,66 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   // visible across XCDs without a per-block L2 writeback fence
 }
 
+// VM_DUAL with the basis split across the workgroup's waves (wave w owns V_w, V_{w+4},
+// ...). Every wave streams the same rows of x and V_last, which the other waves' reads
+// keep in L2, and only its own basis vectors, so a thread holds 2 ceil(MAXV / 4)
+// accumulators instead of 2 MAXV and the pass keeps its occupancy at high column counts.
+// Same partial-sum layout as k_vec (each value is written by the wave that owns it).
+template <typename BT, int MAXV>
+__global__ __launch_bounds__(kBlock) void k_vec_dual(VecArgs<BT> a) {
+  constexpr int W = kBlock / 64, MVW = (MAXV + W - 1) / W, R = 4;
+  const int nv = a.nv;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double xs = a.xscale ? *a.xscale : 1.0;
+  double dacc[MVW], gacc[MVW], nacc = 0.0;
+#pragma unroll
+  for (int k = 0; k < MVW; ++k) dacc[k] = gacc[k] = 0.0;
+  const BT* Vl = a.V + (int64_t)(nv - 1) * a.N;
+  const int64_t groups = a.N / R;
+  const int64_t stride = (int64_t)gridDim.x * 64;
+  for (int64_t g = (int64_t)blockIdx.x * 64 + lane; g < groups; g += stride) {
+    const int64_t n0 = g * R;
+    const Pack<double, R> xi = *reinterpret_cast<const Pack<double, R>*>(a.xin + n0);
+    double x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = xs * xi.v[r];
+    const Pack<BT, R> vl = *reinterpret_cast<const Pack<BT, R>*>(Vl + n0);
+#pragma unroll
+    for (int k = 0; k < MVW; ++k) {
+      const int i = k * W + wave;
+      if (i < nv) {
+        const Pack<BT, R> vv = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)i * a.N + n0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          dacc[k] += (double)vv.v[r] * x[r];
+          gacc[k] += (double)vv.v[r] * (double)vl.v[r];
+        }
+      }
+    }
+    if (wave == 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) nacc += x[r] * x[r];
+    }
+  }
+  const int64_t nb = gridDim.x;
+#pragma unroll
+  for (int k = 0; k < MVW; ++k) {
+    const int i = k * W + wave;
+    if (i < nv) {
+      const double d = wave_sum(dacc[k]) * (a.sc ? a.sc[i] : 1.0);
+      const double gm = wave_sum(gacc[k]);
+      if (lane == 0) {
+        a.partials[(int64_t)i * nb + blockIdx.x] = d;
+        a.partials[(int64_t)(nv + i) * nb + blockIdx.x] = gm;
+      }
+    }
+  }
+  if (wave == 0) {
+    const double v = wave_sum(nacc);
+    if (lane == 0) a.partials[(int64_t)(2 * nv) * nb + blockIdx.x] = v;
+  }
+}
+
 // Finish of the first pass of Arnoldi column j (VM_DUAL over V_0..V_j with x = q = A V_j):
 // tot[i] = V_i.q, tot[nv+i] = V_i.V_j. Sets the norm of V_j (H(j, j-1), sc_j), the new
 // column of the Gram matrix G of the scaled basis, and solves G h = a with
@@ -340,7 +400,21 @@ void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
   int nvals = 1;
   switch (a.mode) {
     case VM_DOTS: launch_vec_mode<BT, VM_DOTS>(a, grid, s); nvals = a.nv + 1; break;
-    case VM_DUAL: launch_vec_mode<BT, VM_DUAL>(a, grid, s); nvals = 2 * a.nv + 1; break;
+    case VM_DUAL:
+      nvals = 2 * a.nv + 1;
+      if (a.nv > 4 && a.nv <= 32 && !a.xout && !a.vout && a.finish != FIN_NONE) {
+        if (a.nv <= 8)
+          k_vec_dual<BT, 8><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 16)
+          k_vec_dual<BT, 16><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 24)
+          k_vec_dual<BT, 24><<<grid, kBlock, 0, s>>>(a);
+        else
+          k_vec_dual<BT, 32><<<grid, kBlock, 0, s>>>(a);
+      } else {
+        launch_vec_mode<BT, VM_DUAL>(a, grid, s);
+      }
+      break;
     default: launch_vec_mode<BT, VM_UPDATE>(a, grid, s); break;
   }
   if (a.finish != FIN_NONE) k_vec_finish<BT><<<1, kFinBlock, 0, s>>>(a, grid, nvals);

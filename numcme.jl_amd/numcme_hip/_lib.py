@@ -69,7 +69,7 @@ class nc_matrix_info_t(C.Structure):
         ("csr_bytes_per_matvec", C.c_double),
         ("norm1", C.c_double),
         ("rows_per_thread", C.c_int32),
-        ("pad0", C.c_int32),
+        ("computed_col_slots", C.c_int32),
     ]
 
 

@@ -127,6 +127,25 @@ def test_nnz_toggle_c2(gpu):
     assert info["nnz_reference"] == 5240832
     assert info["ell_width"] == 4
     assert info["n_sink_entries"] == 2048
+    assert info["computed_col_slots"] == 4  # lattice order: every slot is row + constant offset
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_computed_columns_match_loaded_columns(gpu, precision):
+    # the same box in lattice order (columns computed) and in a shuffled order (columns
+    # loaded): the state rows of the two mat-vecs agree bit for bit after un-permuting
+    S, props, th = models.toggle()
+    states = models.box_states([57, 43])
+    perm = np.random.default_rng(3).permutation(len(states))
+    A1 = nh.FspMatrixSparse(nh.StateSpaceSparse(S, states), models.to_product(props), th, precision=precision)
+    A2 = nh.FspMatrixSparse(nh.StateSpaceSparse(S, states[perm]), models.to_product(props), th, precision=precision)
+    assert A1.info()["computed_col_slots"] == 4 and A2.info()["computed_col_slots"] == 0
+    n = len(states)
+    v1 = np.random.default_rng(4).random(A1.rowcount)
+    v2 = np.concatenate([v1[:n][perm], v1[n:]])
+    w1, w2 = nh.matvec(0.0, A1, v1), nh.matvec(0.0, A2, v2)
+    assert np.array_equal(w1[:n][perm], w2[:n])
+    assert np.allclose(w1[n:], w2[n:], rtol=1e-14, atol=0)
 
 
 @pytest.mark.parametrize("which", ["toggle", "gene4", "telegraph_sens"])
