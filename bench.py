@@ -226,7 +226,7 @@ def c4_instance(nh, spec, precision):
     return A, u0
 
 
-def c4_batch(nh, L, n_plain=32):
+def c4_batch(nh, L, n_plain=32, graphs=True):
     """Config C4: 512 uneven instances in one batched FP32 solve against the plain FP64
     single-solve path (timed on an evenly spaced sample by size, scaled to 512)."""
     table = json.loads(C4_TABLE.read_text())
@@ -237,9 +237,9 @@ def c4_batch(nh, L, n_plain=32):
         mats.append(A)
         u0s.append(u0)
     B = nh.BatchFsp(mats)
-    B.solve(u0s, ts, fetch=False)  # warm-up: captures the iteration graph
+    B.solve(u0s, ts, fetch=False, graphs=graphs)  # warm-up: captures the iteration graph
     t0 = time.perf_counter()
-    B.solve(u0s, ts, fetch=False)
+    B.solve(u0s, ts, fetch=False, graphs=graphs)
     batch_s = time.perf_counter() - t0
     stats = dict(B.stats)
     order = np.argsort([t["states"] for t in table])
@@ -253,6 +253,7 @@ def c4_batch(nh, L, n_plain=32):
             L.call("nc_solve_opts_default", C.byref(o))
             o.method = L.NC_METHOD_RK45 if method == "rk45" else L.NC_METHOD_KRYLOV
             o.rtol, o.atol = 1e-8, 1e-14
+            o.no_graphs = 0 if graphs else 1
             out = np.empty(len(ts) * len(u0))
             t1 = time.perf_counter()
             L.call("nc_solve_fixed", A64.handle, C.byref(o), L.ptr(u0, C.c_double), len(ts), L.ptr(ts, C.c_double),
@@ -538,7 +539,7 @@ def main():
     if not args.no_c3:
         line["c3_sensitivity"] = c3_sensitivity(nh, L)
     if not args.no_c4:
-        line["c4_batch"] = c4_batch(nh, L)
+        line["c4_batch"] = c4_batch(nh, L, graphs=not args.no_graphs)
     if not args.no_c5:
         plain = {"both": ("rk45", "krylov"), "none": ()}.get(args.c5_plain, (args.c5_plain,))
         line["c5_learned"] = c5_learned(nh, plain_methods=plain, log=lambda m: print(m, file=sys.stderr, flush=True))
