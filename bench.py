@@ -237,6 +237,7 @@ def c4_batch(nh, L, n_plain=32, graphs=True):
         mats.append(A)
         u0s.append(u0)
     B = nh.BatchFsp(mats)
+    rows_per_thread, tune_ms = B.autotune(reps=3)
     B.solve(u0s, ts, fetch=False, graphs=graphs)  # warm-up: captures the iteration graph
     t0 = time.perf_counter()
     B.solve(u0s, ts, fetch=False, graphs=graphs)
@@ -271,6 +272,8 @@ def c4_batch(nh, L, n_plain=32, graphs=True):
         "batched_fp32_solve_s": batch_s,
         "batched_solves_per_s": len(table) / batch_s,
         "batched_iterations": stats["host_syncs"],
+        "batched_rows_per_thread": rows_per_thread,
+        "autotune_iteration_ms": tune_ms,
         "plain_fp64_solves_per_s": len(table) / plain_est,
         "plain_fp64_rk45_solves_per_s": len(table) * n_plain / (plain["rk45"] * len(table)),
         "plain_fp64_krylov_solves_per_s": len(table) * n_plain / (plain["krylov"] * len(table)),

@@ -282,6 +282,14 @@ int nc_batch_info(nc_batch B, int64_t* total_rows, int64_t* instance_rows);
  * rtol, atol, h0 and no_graphs. Snapshots stay on the device. */
 int nc_batch_solve(nc_batch B, const nc_solve_opts* opts, const double* u0, int32_t n_out,
                    const double* t_out, nc_solve_stats* st);
+/* Autotuner of the batched kernels' launch shape: times reps batched DP5(4) iterations at
+ * 1, 2 and 4 rows per thread (256, 512 and 1024 rows per workgroup) and keeps the fastest
+ * for later solves. Instances are padded to 256-row granules, so every shape runs on the
+ * same layout. Outputs (either may be null): the rows per thread chosen and the time of one
+ * iteration in ms. No reference counterpart. */
+int nc_batch_autotune(nc_batch B, int32_t reps, int32_t* rows_per_thread, double* best_ms);
+/* Set the launch shape by hand (1, 2 or 4 rows per thread). */
+int nc_batch_set_rows_per_thread(nc_batch B, int32_t rows);
 /* Copy snapshot k of instance inst (its n_rows doubles). */
 int nc_batch_get_snapshot(nc_batch B, int32_t inst, int32_t k, double* u);
 

@@ -2,10 +2,11 @@
 //
 // Layout: every instance's slot-major generator is copied into one concatenated ELL
 // (ld = total rows). Instance i owns rows [row_off[i], row_off[i] + rows_pad[i]), padded
-// to whole 1024-row tiles, so a 256-thread block of 4 rows per thread never straddles two
-// instances: tile_inst[tile] names the owner. Rows n_i .. n_i + nr_i - 1 of an instance are
-// its sinks (written by one workgroup per (instance, reaction) segment); padding rows stay
-// zero. Vectors use the same row space.
+// to whole 256-row granules; gran_inst[granule] names the owner. A thread works on R
+// consecutive rows (R = 1, 2 or 4, chosen by nc_batch_autotune), which never straddle a
+// granule, so every thread has one owner instance. Rows n_i .. n_i + nr_i - 1 of an
+// instance are its sinks (written by one workgroup per (instance, reaction) segment);
+// padding rows stay zero. Vectors use the same row space.
 //
 // Integrator: Dormand-Prince 5(4) with per-instance t, h and error control. One iteration
 // (accept/snapshot of the previous step, six stages with batched mat-vecs, the error
@@ -23,8 +24,7 @@ namespace nc {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTile = 1024;  // rows per block (4 per thread)
-constexpr int R4 = 4;
+constexpr int kGran = 256;  // rows of one ownership granule (instances are padded to it)
 
 template <typename T, int R>
 struct alignas(sizeof(T) * R) Pack {
@@ -47,7 +47,7 @@ __constant__ double c_a[7][6] = {
 __constant__ double c_e[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525, -1.0 / 40};
 
 struct Dev {
-  const int32_t* tile_inst;  // owner instance of each tile
+  const int32_t* gran_inst;  // owner instance of each granule
   const int64_t* row_off;    // first row of each instance
   const int64_t* nstates;    // states of each instance
   const double* h;           // step size of the current iteration (0: inactive)
@@ -55,54 +55,62 @@ struct Dev {
   const int32_t* snap;       // snapshot slot to fill after accepting, or -1
 };
 
-// y_s = u + h_i sum_{j<s} a_sj k_j over the instance's rows, and its VT copy
-template <typename VT>
+// Rows of thread t of block b: [(b * kBlock + t) * R, ... + R), inside one granule.
+template <int R>
+__device__ inline int64_t first_row() {
+  return ((int64_t)blockIdx.x * kBlock + threadIdx.x) * R;
+}
+
+// y_s = u + h_i sum_{j<s} a_sj k_j over the instance's rows: its VT copy, and at s = 6
+// (y_{n+1}, read by the error partials and the accept) its FP64 value
+template <typename VT, int R>
 __global__ __launch_bounds__(kBlock) void k_bstage(Dev d, int64_t ld, int stage, const double* u, const double* k,
                                                    double* ytmp, VT* vin) {
-  const int inst = d.tile_inst[blockIdx.x];
-  const double h = d.h[inst];
+  const int64_t n0 = first_row<R>();
+  if (n0 >= ld) return;
+  const double h = d.h[d.gran_inst[n0 / kGran]];
   if (h == 0.0) return;
-  const int64_t n0 = (int64_t)blockIdx.x * kTile + threadIdx.x * R4;
-  Pack<double, R4> y = *reinterpret_cast<const Pack<double, R4>*>(u + n0);
-  double acc[R4] = {0, 0, 0, 0};
+  Pack<double, R> y = *reinterpret_cast<const Pack<double, R>*>(u + n0);
+  double acc[R] = {};
   for (int j = 0; j < stage; ++j) {
     const double a = c_a[stage][j];
-    const Pack<double, R4> kj = *reinterpret_cast<const Pack<double, R4>*>(k + (int64_t)j * ld + n0);
+    const Pack<double, R> kj = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
 #pragma unroll
-    for (int r = 0; r < R4; ++r) acc[r] += a * kj.v[r];
+    for (int r = 0; r < R; ++r) acc[r] += a * kj.v[r];
   }
-  Pack<VT, R4> o;
+  Pack<VT, R> o;
 #pragma unroll
-  for (int r = 0; r < R4; ++r) {
+  for (int r = 0; r < R; ++r) {
     y.v[r] += h * acc[r];
     o.v[r] = (VT)y.v[r];
   }
-  *reinterpret_cast<Pack<double, R4>*>(ytmp + n0) = y;
-  *reinterpret_cast<Pack<VT, R4>*>(vin + n0) = o;
+  if (stage == 6) *reinterpret_cast<Pack<double, R>*>(ytmp + n0) = y;
+  *reinterpret_cast<Pack<VT, R>*>(vin + n0) = o;
 }
 
-// out = A v over the state rows of active instances (4 rows per thread)
-template <typename VT>
+// out = A v over the state rows of active instances (R rows per thread)
+template <typename VT, int R>
 __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, const double* diag, const int32_t* col,
                                                     const VT* val, const VT* v, double* out) {
-  const int inst = d.tile_inst[blockIdx.x];
+  const int64_t n0 = first_row<R>();
+  if (n0 >= ld) return;
+  const int inst = d.gran_inst[n0 / kGran];
   if (d.h[inst] == 0.0) return;
-  const int64_t n0 = (int64_t)blockIdx.x * kTile + threadIdx.x * R4;
   const int64_t nend = d.row_off[inst] + d.nstates[inst];
   if (n0 >= nend) return;
-  const Pack<double, R4> dg = *reinterpret_cast<const Pack<double, R4>*>(diag + n0);
-  const Pack<VT, R4> vi = *reinterpret_cast<const Pack<VT, R4>*>(v + n0);
-  double acc[R4];
+  const Pack<double, R> dg = *reinterpret_cast<const Pack<double, R>*>(diag + n0);
+  const Pack<VT, R> vi = *reinterpret_cast<const Pack<VT, R>*>(v + n0);
+  double acc[R];
 #pragma unroll
-  for (int r = 0; r < R4; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
+  for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
   for (int s = 0; s < K; ++s) {
-    const Pack<int32_t, R4> c = *reinterpret_cast<const Pack<int32_t, R4>*>(col + (int64_t)s * ld + n0);
-    const Pack<VT, R4> w = *reinterpret_cast<const Pack<VT, R4>*>(val + (int64_t)s * ld + n0);
+    const Pack<int32_t, R> c = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
+    const Pack<VT, R> w = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
 #pragma unroll
-    for (int r = 0; r < R4; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
+    for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
   }
 #pragma unroll
-  for (int r = 0; r < R4; ++r)
+  for (int r = 0; r < R; ++r)
     if (n0 + r < nend) out[n0 + r] = acc[r];
 }
 
@@ -122,58 +130,69 @@ __global__ __launch_bounds__(kBlock) void k_bsinks(Dev d, const int32_t* seg_ins
   if (threadIdx.x == 0) out[seg_row[seg]] = part[0] + part[1] + part[2] + part[3];
 }
 
-// Error partials per tile: sum ((h sum_j e_j k_j) / (atol + rtol max(|u|, |y|)))^2
-__global__ __launch_bounds__(kBlock) void k_berr(Dev d, int64_t ld, double atol, double rtol, const double* u,
-                                                 const double* y, const double* k, double* partials) {
-  const int inst = d.tile_inst[blockIdx.x];
-  const double h = d.h[inst];
+// Error partials per granule: sum ((h sum_j e_j k_j) / (atol + rtol max(|u|, |y|)))^2.
+// A block covers R granules; granule j is waves [j W, (j + 1) W) with W = 4 / R.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_berr(Dev d, int64_t ld, int64_t grans, double atol, double rtol,
+                                                 const double* u, const double* y, const double* k, double* partials) {
+  const int64_t n0 = first_row<R>();
   double acc = 0.0;
+  const double h = n0 < ld ? d.h[d.gran_inst[n0 / kGran]] : 0.0;
   if (h != 0.0) {
-    const int64_t n0 = (int64_t)blockIdx.x * kTile + threadIdx.x * R4;
-    const Pack<double, R4> uu = *reinterpret_cast<const Pack<double, R4>*>(u + n0);
-    const Pack<double, R4> yy = *reinterpret_cast<const Pack<double, R4>*>(y + n0);
-    double e[R4] = {0, 0, 0, 0};
+    const Pack<double, R> uu = *reinterpret_cast<const Pack<double, R>*>(u + n0);
+    const Pack<double, R> yy = *reinterpret_cast<const Pack<double, R>*>(y + n0);
+    double e[R] = {};
     for (int j = 0; j < 7; ++j) {
       if (c_e[j] == 0.0) continue;
-      const Pack<double, R4> kj = *reinterpret_cast<const Pack<double, R4>*>(k + (int64_t)j * ld + n0);
+      const Pack<double, R> kj = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
 #pragma unroll
-      for (int r = 0; r < R4; ++r) e[r] += c_e[j] * kj.v[r];
+      for (int r = 0; r < R; ++r) e[r] += c_e[j] * kj.v[r];
     }
 #pragma unroll
-    for (int r = 0; r < R4; ++r) {
+    for (int r = 0; r < R; ++r) {
       const double sc = atol + rtol * fmax(fabs(uu.v[r]), fabs(yy.v[r]));
       const double q = h * e[r] / sc;
       acc += q * q;
     }
   }
+  constexpr int W = (kBlock / 64) / R;
   __shared__ double part[kBlock / 64];
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) partials[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+  if ((int)threadIdx.x < R) {
+    const int64_t g = (int64_t)blockIdx.x * R + threadIdx.x;
+    if (g < grans) {
+      double sum = 0.0;
+      for (int w = 0; w < W; ++w) sum += part[threadIdx.x * W + w];
+      partials[g] = sum;
+    }
+  }
 }
 
-// Per-instance fold of the tile partials (tiles of an instance are contiguous)
-__global__ void k_bfold(int n_inst, const int64_t* tile_start, const double* partials, double* err) {
+// Per-instance fold of the granule partials (granules of an instance are contiguous)
+__global__ void k_bfold(int n_inst, const int64_t* gran_start, const double* partials, double* err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_inst) return;
   double s = 0.0;
-  for (int64_t t = tile_start[i]; t < tile_start[i + 1]; ++t) s += partials[t];
+  for (int64_t g = gran_start[i]; g < gran_start[i + 1]; ++g) s += partials[g];
   err[i] = s;
 }
 
 // Accept the last step where asked (u <- y, k0 <- k6) and fill requested snapshots.
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_baccept(Dev d, int64_t ld, double* u, const double* y, double* k,
                                                     double* snaps) {
-  const int inst = d.tile_inst[blockIdx.x];
-  const int64_t n0 = (int64_t)blockIdx.x * kTile + threadIdx.x * R4;
+  const int64_t n0 = first_row<R>();
+  if (n0 >= ld) return;
+  const int inst = d.gran_inst[n0 / kGran];
   if (d.accept[inst]) {
-    *reinterpret_cast<Pack<double, R4>*>(u + n0) = *reinterpret_cast<const Pack<double, R4>*>(y + n0);
-    *reinterpret_cast<Pack<double, R4>*>(k + n0) = *reinterpret_cast<const Pack<double, R4>*>(k + 6 * ld + n0);
+    *reinterpret_cast<Pack<double, R>*>(u + n0) = *reinterpret_cast<const Pack<double, R>*>(y + n0);
+    *reinterpret_cast<Pack<double, R>*>(k + n0) = *reinterpret_cast<const Pack<double, R>*>(k + 6 * ld + n0);
   }
   const int sl = d.snap[inst];
   if (sl >= 0)
-    *reinterpret_cast<Pack<double, R4>*>(snaps + (int64_t)sl * ld + n0) = *reinterpret_cast<const Pack<double, R4>*>(u + n0);
+    *reinterpret_cast<Pack<double, R>*>(snaps + (int64_t)sl * ld + n0) = *reinterpret_cast<const Pack<double, R>*>(u + n0);
 }
 
 // Copy one instance's slot-major ELL into the concatenated arrays.
@@ -206,8 +225,9 @@ struct nc_batch_s {
   int n = 0, precision = NC_FP64;
   size_t vs = 8;
   int K = 0;
-  int64_t ld = 0, tiles = 0;
-  std::vector<int64_t> row_off, nstates, nrows, tile_start;
+  int64_t ld = 0, grans = 0;
+  int rows_per_thread = 4;  // R of the kernels (1, 2 or 4)
+  std::vector<int64_t> row_off, nstates, nrows, gran_start;
   nc::Stream stream;
   nc::DevBuf<double> diag;
   nc::DevBuf<int32_t> col;
@@ -216,8 +236,8 @@ struct nc_batch_s {
   nc::DevBuf<int32_t> seg_inst, sink_src;
   nc::DevBuf<int64_t> seg_row, seg_off;
   nc::DevBuf<unsigned char> sink_val;
-  nc::DevBuf<int32_t> tile_inst;
-  nc::DevBuf<int64_t> d_row_off, d_nstates, d_tile_start;
+  nc::DevBuf<int32_t> gran_inst;
+  nc::DevBuf<int64_t> d_row_off, d_nstates, d_gran_start;
   // integrator state
   nc::DevBuf<double> u, ytmp, k, partials, err, snaps;
   nc::DevBuf<unsigned char> vin;
@@ -303,23 +323,23 @@ nc_batch_s* batch_create(int n, const nc_matrix* mats) {
     NC_REQUIRE(A->precision == B->precision, NC_ERR_ARG, "all matrices of a batch must share one precision");
     NC_REQUIRE(A->tv_r.empty(), NC_ERR_UNSUPPORTED, "batched solves take time-invariant generators");
     B->K = std::max(B->K, A->K);
-    const int64_t rows = (A->n_rows + kTile - 1) / kTile * kTile;
+    const int64_t rows = (A->n_rows + kGran - 1) / kGran * kGran;
     B->row_off.push_back(off);
     B->nstates.push_back(A->n);
     B->nrows.push_back(A->n_rows);
-    off += std::max<int64_t>(rows, kTile);
+    off += std::max<int64_t>(rows, kGran);
   }
   NC_REQUIRE(off < (int64_t)0x7FFFFFF0, NC_ERR_OVERFLOW, "batch exceeds 2^31 rows");
   B->ld = off;
-  B->tiles = off / kTile;
-  std::vector<int32_t> tile_inst(B->tiles);
-  B->tile_start.assign(n + 1, 0);
+  B->grans = off / kGran;
+  std::vector<int32_t> gran_inst(B->grans);
+  B->gran_start.assign(n + 1, 0);
   for (int i = 0; i < n; ++i) {
-    const int64_t t0 = B->row_off[i] / kTile, t1 = (i + 1 < n ? B->row_off[i + 1] : off) / kTile;
-    B->tile_start[i] = t0;
-    for (int64_t t = t0; t < t1; ++t) tile_inst[t] = i;
+    const int64_t g0 = B->row_off[i] / kGran, g1 = (i + 1 < n ? B->row_off[i + 1] : off) / kGran;
+    B->gran_start[i] = g0;
+    for (int64_t g = g0; g < g1; ++g) gran_inst[g] = i;
   }
-  B->tile_start[n] = B->tiles;
+  B->gran_start[n] = B->grans;
   hipStream_t s = B->stream.s;
   const int64_t ld = B->ld;
   B->diag.alloc(ld);
@@ -328,14 +348,14 @@ nc_batch_s* batch_create(int n, const nc_matrix* mats) {
   NC_CHECK_HIP(hipMemsetAsync(B->diag.p, 0, sizeof(double) * ld, s));
   NC_CHECK_HIP(hipMemsetAsync(B->col.p, 0, sizeof(int32_t) * B->K * ld, s));
   NC_CHECK_HIP(hipMemsetAsync(B->val.p, 0, B->vs * B->K * ld, s));
-  B->tile_inst.alloc(B->tiles);
+  B->gran_inst.alloc(B->grans);
   B->d_row_off.alloc(n);
   B->d_nstates.alloc(n);
-  B->d_tile_start.alloc(n + 1);
-  NC_CHECK_HIP(hipMemcpyAsync(B->tile_inst.p, tile_inst.data(), sizeof(int32_t) * B->tiles, hipMemcpyHostToDevice, s));
+  B->d_gran_start.alloc(n + 1);
+  NC_CHECK_HIP(hipMemcpyAsync(B->gran_inst.p, gran_inst.data(), sizeof(int32_t) * B->grans, hipMemcpyHostToDevice, s));
   NC_CHECK_HIP(hipMemcpyAsync(B->d_row_off.p, B->row_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
   NC_CHECK_HIP(hipMemcpyAsync(B->d_nstates.p, B->nstates.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
-  NC_CHECK_HIP(hipMemcpyAsync(B->d_tile_start.p, B->tile_start.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->d_gran_start.p, B->gran_start.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
   if (B->precision == NC_FP32)
     batch_build<float>(B.get(), mats);
   else
@@ -345,7 +365,7 @@ nc_batch_s* batch_create(int n, const nc_matrix* mats) {
   B->ytmp.alloc(ld);
   B->k.alloc((size_t)7 * ld);
   B->vin.alloc(ld * B->vs);
-  B->partials.alloc(B->tiles);
+  B->partials.alloc(B->grans);
   B->err.alloc(n);
   B->d_h.alloc(n);
   B->d_accept.alloc(n);
@@ -363,15 +383,18 @@ nc_batch_s* batch_create(int n, const nc_matrix* mats) {
 }
 
 Dev dev_view(nc_batch_s* B) {
-  return Dev{B->tile_inst.p, B->d_row_off.p, B->d_nstates.p, B->d_h.p, B->d_accept.p, B->d_snap.p};
+  return Dev{B->gran_inst.p, B->d_row_off.p, B->d_nstates.p, B->d_h.p, B->d_accept.p, B->d_snap.p};
 }
 
-template <typename VT>
+// blocks of a launch covering every row with R rows per thread
+unsigned row_grid(const nc_batch_s* B, int R) { return (unsigned)((B->grans + R - 1) / R); }
+
+template <typename VT, int R>
 void launch_matvec(nc_batch_s* B, const VT* v, double* out) {
   const Dev d = dev_view(B);
   hipStream_t s = B->stream.s;
-  k_bmatvec<VT><<<(unsigned)B->tiles, kBlock, 0, s>>>(d, B->ld, B->K, B->diag.p, B->col.p,
-                                                      reinterpret_cast<const VT*>(B->val.p), v, out);
+  k_bmatvec<VT, R><<<row_grid(B, R), kBlock, 0, s>>>(d, B->ld, B->K, B->diag.p, B->col.p,
+                                                     reinterpret_cast<const VT*>(B->val.p), v, out);
   if (B->nseg)
     k_bsinks<VT><<<(unsigned)B->nseg, kBlock, 0, s>>>(d, B->seg_inst.p, B->seg_row.p, B->seg_off.p, B->sink_src.p,
                                                       reinterpret_cast<const VT*>(B->sink_val.p), v, out);
@@ -379,18 +402,58 @@ void launch_matvec(nc_batch_s* B, const VT* v, double* out) {
 }
 
 // accept/snapshot of the previous step, then one full DP5 attempt and the error fold
-template <typename VT>
-void iteration(nc_batch_s* B, double atol, double rtol) {
+template <typename VT, int R>
+void iteration_r(nc_batch_s* B, double atol, double rtol) {
   const Dev d = dev_view(B);
   hipStream_t s = B->stream.s;
-  const unsigned g = (unsigned)B->tiles;
-  k_baccept<<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+  const unsigned g = row_grid(B, R);
+  k_baccept<R><<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
   for (int st = 1; st < 7; ++st) {
-    k_bstage<VT><<<g, kBlock, 0, s>>>(d, B->ld, st, B->u.p, B->k.p, B->ytmp.p, reinterpret_cast<VT*>(B->vin.p));
-    launch_matvec<VT>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p + (int64_t)st * B->ld);
+    k_bstage<VT, R><<<g, kBlock, 0, s>>>(d, B->ld, st, B->u.p, B->k.p, B->ytmp.p, reinterpret_cast<VT*>(B->vin.p));
+    launch_matvec<VT, R>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p + (int64_t)st * B->ld);
   }
-  k_berr<<<g, kBlock, 0, s>>>(d, B->ld, atol, rtol, B->u.p, B->ytmp.p, B->k.p, B->partials.p);
-  k_bfold<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(B->n, B->d_tile_start.p, B->partials.p, B->err.p);
+  k_berr<R><<<g, kBlock, 0, s>>>(d, B->ld, B->grans, atol, rtol, B->u.p, B->ytmp.p, B->k.p, B->partials.p);
+  k_bfold<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(B->n, B->d_gran_start.p, B->partials.p, B->err.p);
+  NC_CHECK_HIP(hipGetLastError());
+}
+
+template <typename VT>
+void iteration(nc_batch_s* B, double atol, double rtol) {
+  switch (B->rows_per_thread) {
+    case 1: iteration_r<VT, 1>(B, atol, rtol); break;
+    case 2: iteration_r<VT, 2>(B, atol, rtol); break;
+    default: iteration_r<VT, 4>(B, atol, rtol); break;
+  }
+}
+
+// k0 = A u for every instance (stage 0 copies u into the mat-vec input)
+template <typename VT, int R>
+void first_stage_r(nc_batch_s* B) {
+  hipStream_t s = B->stream.s;
+  k_bstage<VT, R><<<row_grid(B, R), kBlock, 0, s>>>(dev_view(B), B->ld, 0, B->u.p, B->k.p, B->ytmp.p,
+                                                    reinterpret_cast<VT*>(B->vin.p));
+  launch_matvec<VT, R>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p);
+}
+
+template <typename VT>
+void first_stage(nc_batch_s* B) {
+  switch (B->rows_per_thread) {
+    case 1: first_stage_r<VT, 1>(B); break;
+    case 2: first_stage_r<VT, 2>(B); break;
+    default: first_stage_r<VT, 4>(B); break;
+  }
+}
+
+void flush_accept(nc_batch_s* B) {
+  hipStream_t s = B->stream.s;
+  const int R = B->rows_per_thread;
+  const Dev d = dev_view(B);
+  if (R == 1)
+    k_baccept<1><<<row_grid(B, 1), kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+  else if (R == 2)
+    k_baccept<2><<<row_grid(B, 2), kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+  else
+    k_baccept<4><<<row_grid(B, 4), kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
   NC_CHECK_HIP(hipGetLastError());
 }
 
@@ -421,12 +484,7 @@ void batch_solve_t(nc_batch_s* B, const nc_solve_opts& o, const double* u0, int 
   NC_CHECK_HIP(hipMemcpyAsync(B->d_h.p, B->h_pin, sizeof(double) * n, hipMemcpyHostToDevice, s));
   NC_CHECK_HIP(hipMemcpyAsync(B->d_accept.p, B->acc_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
   NC_CHECK_HIP(hipMemcpyAsync(B->d_snap.p, B->snap_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-  {
-    const unsigned g = (unsigned)B->tiles;
-    k_bstage<VT><<<g, kBlock, 0, s>>>(dev_view(B), B->ld, 0, B->u.p, B->k.p, B->ytmp.p,
-                                      reinterpret_cast<VT*>(B->vin.p));
-    launch_matvec<VT>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p);
-  }
+  first_stage<VT>(B);
   // per-instance controller state (the first output time is the start)
   std::vector<double> t(n, n_out ? t_out[0] : 0.0), h(n, 0.0), h_try(n, 0.0);
   std::vector<int> next(n, 1), done(n, 0), clamped(n, 0);
@@ -458,8 +516,7 @@ void batch_solve_t(nc_batch_s* B, const nc_solve_opts& o, const double* u0, int 
     NC_CHECK_HIP(hipMemcpyAsync(B->d_snap.p, B->snap_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
     if (!active) {
       // flush the last accepts and snapshots
-      k_baccept<<<(unsigned)B->tiles, kBlock, 0, s>>>(dev_view(B), B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
-      NC_CHECK_HIP(hipGetLastError());
+      flush_accept(B);
       NC_CHECK_HIP(hipStreamSynchronize(s));
       break;
     }
@@ -533,6 +590,64 @@ int nc_batch_create(int32_t n, const nc_matrix* mats, nc_batch* out) {
 
 int nc_batch_destroy(nc_batch B) {
   return guard([&] { delete B; });
+}
+
+int nc_batch_autotune(nc_batch B, int32_t reps, int32_t* rows_per_thread, double* best_ms) {
+  return guard([&] {
+    NC_REQUIRE(B && reps >= 1, NC_ERR_ARG, "bad arguments");
+    hipStream_t s = B->stream.s;
+    // every instance active with a tiny step; nothing accepted, nothing saved
+    for (int i = 0; i < B->n; ++i) B->h_pin[i] = 1e-9, B->acc_pin[i] = 0, B->snap_pin[i] = -1;
+    NC_CHECK_HIP(hipMemcpyAsync(B->d_h.p, B->h_pin, sizeof(double) * B->n, hipMemcpyHostToDevice, s));
+    NC_CHECK_HIP(hipMemcpyAsync(B->d_accept.p, B->acc_pin, sizeof(int32_t) * B->n, hipMemcpyHostToDevice, s));
+    NC_CHECK_HIP(hipMemcpyAsync(B->d_snap.p, B->snap_pin, sizeof(int32_t) * B->n, hipMemcpyHostToDevice, s));
+    hipEvent_t e0, e1;
+    NC_CHECK_HIP(hipEventCreate(&e0));
+    NC_CHECK_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int best_r = B->rows_per_thread;
+    for (int r : {1, 2, 4}) {
+      B->rows_per_thread = r;
+      auto it = [&] {
+        if (B->precision == NC_FP32)
+          iteration<float>(B, 1e-14, 1e-8);
+        else
+          iteration<double>(B, 1e-14, 1e-8);
+      };
+      it();  // warm-up
+      NC_CHECK_HIP(hipEventRecord(e0, s));
+      for (int k = 0; k < reps; ++k) it();
+      NC_CHECK_HIP(hipEventRecord(e1, s));
+      NC_CHECK_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      NC_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / reps < best) best = ms / reps, best_r = r;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    B->rows_per_thread = best_r;
+    // a captured iteration was recorded with the previous launch shape
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->exec = nullptr;
+    B->graph = nullptr;
+    if (rows_per_thread) *rows_per_thread = best_r;
+    if (best_ms) *best_ms = best;
+  });
+}
+
+int nc_batch_set_rows_per_thread(nc_batch B, int32_t rows) {
+  return guard([&] {
+    NC_REQUIRE(B, NC_ERR_ARG, "null batch");
+    NC_REQUIRE(rows == 1 || rows == 2 || rows == 4, NC_ERR_ARG, "rows per thread must be 1, 2 or 4");
+    if (rows != B->rows_per_thread) {
+      if (B->exec) (void)hipGraphExecDestroy(B->exec);
+      if (B->graph) (void)hipGraphDestroy(B->graph);
+      B->exec = nullptr;
+      B->graph = nullptr;
+    }
+    B->rows_per_thread = rows;
+  });
 }
 
 int nc_batch_info(nc_batch B, int64_t* total_rows, int64_t* instance_rows) {

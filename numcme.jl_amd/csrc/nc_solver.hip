@@ -439,7 +439,9 @@ __constant__ double c_dp_e[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -1
 // accepted step hands k6 = f(y_{n+1}) on as the next k0 by flipping sw, not by copying).
 __device__ __host__ inline int rk_slot(int j, int sw) { return (sw && (j == 0 || j == 6)) ? 6 - j : j; }
 
-// Stage input y_i = y + h * sum_{j<i} a_ij k_j ; writes FP64 (ytmp) and the mat-vec copy.
+// Stage input y_i = y + h * sum_{j<i} a_ij k_j ; writes the mat-vec copy, and at the last
+// stage (i = 6, whose input is y_{n+1}) also the FP64 value (ytmp), the only stage input
+// the error estimate and the accept read.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, int stage, double h, const double* y, const double* k,
                                                      int sw, double* ytmp, VT* vin) {
@@ -448,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, int stage, doubl
     double acc = 0.0;
     for (int j = 0; j < stage; ++j) acc += c_dp_a[stage][j] * k[(int64_t)rk_slot(j, sw) * N + n];
     const double v = y[n] + h * acc;
-    ytmp[n] = v;
+    if (stage == 6) ytmp[n] = v;
     vin[n] = (VT)v;
   }
 }

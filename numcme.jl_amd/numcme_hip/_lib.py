@@ -191,6 +191,8 @@ SIGNATURES = {
     "nc_mlp_forward": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float)],
     "nc_mlp_grad": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), PD],
     "nc_mlp_train": [P, I64, C.POINTER(C.c_float), C.POINTER(C.c_float), I32, I32, C.c_float, C.c_uint64, PD],
+    "nc_batch_autotune": [P, I32, C.POINTER(I32), PD],
+    "nc_batch_set_rows_per_thread": [P, I32],
     "nc_mlp_bench": [P, I64, I32, I32, PD],
     "nc_adapter_features": [P, P, I64, PD, PD, C.c_double, C.c_double, C.POINTER(I64), C.POINTER(I64),
                             C.POINTER(C.c_float)],

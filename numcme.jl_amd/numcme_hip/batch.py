@@ -34,6 +34,16 @@ class BatchFsp(L.Handle):
     def __len__(self):
         return len(self.matrices)
 
+    def autotune(self, reps=3):
+        """Pick the kernels' rows per thread (1, 2 or 4) by timing batched iterations;
+        returns (rows_per_thread, ms per iteration)."""
+        r, ms = C.c_int32(), C.c_double()
+        L.call("nc_batch_autotune", self.handle, int(reps), C.byref(r), C.byref(ms))
+        return r.value, ms.value
+
+    def set_rows_per_thread(self, rows):
+        L.call("nc_batch_set_rows_per_thread", self.handle, int(rows))
+
     def solve(self, u0s, ts, rtol=1e-8, atol=1e-14, h0=0.0, graphs=True, fetch=True):
         """Integrate instance i from ``u0s[i]`` through the shared output times ``ts``."""
         if len(u0s) != len(self):

@@ -35,10 +35,16 @@ def _instances(precision):
     return out, mats, u0s
 
 
-@pytest.mark.parametrize("precision,bar", [("fp64", 1e-8), ("fp32", 1e-6)])
-def test_batch_matches_oracle(gpu, precision, bar):
+@pytest.mark.parametrize("precision,bar,rows", [("fp64", 1e-8, 1), ("fp64", 1e-8, 2), ("fp64", 1e-8, 4),
+                                               ("fp32", 1e-6, 4), ("fp32", 1e-6, "tuned")])
+def test_batch_matches_oracle(gpu, precision, bar, rows):
     inst, mats, u0s = _instances(precision)
     B = nh.BatchFsp(mats)
+    if rows == "tuned":
+        r, ms = B.autotune(reps=2)
+        assert r in (1, 2, 4) and ms > 0
+    else:
+        B.set_rows_per_thread(rows)
     ts = np.array([0.0, 0.5, 2.0, 5.0])
     snaps = B.solve(u0s, ts, rtol=1e-10, atol=1e-16)
     for (S, props, th, states), snap in zip(inst, snaps):
@@ -69,3 +75,10 @@ def test_batch_rejects_time_varying(gpu):
     A = nh.FspMatrixSparse(sp, models.to_product(props), th)
     with pytest.raises(nh.NumCmeHipError):
         nh.BatchFsp([A])
+
+
+def test_batch_rows_per_thread_rejected(gpu):
+    _, mats, _ = _instances("fp64")
+    B = nh.BatchFsp(mats)
+    with pytest.raises(nh.ArgumentError):
+        B.set_rows_per_thread(3)
