@@ -304,9 +304,11 @@ def _l1_between(a, b):
     return float(sum(abs(da.get(x, 0.0) - db.get(x, 0.0)) for x in set(da) | set(db)))
 
 
-def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_methods=("rk45", "krylov"), log=None):
+def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=(2, 4, 6), learned_levels=6, log=None):
     """Config C5: the cascade from (0,0,0) with fsptol 1e-4 over [0, T], adaptive FSP with
-    the learned adapter against plain SelectiveRStepAdapter(2, 2). The network is trained
+    the learned adapter against plain SelectiveRStepAdapter(2, R), each at its best
+    expansion depth (the learned adapter expands learned_levels levels from the states it
+    keeps; the plain FP64 baseline is the fastest of plain_levels). The network is trained
     on the device on records from plain runs over n_scalings seeded parameter scalings."""
     def solve(model, adapter, ode, tend, records=None):
         alg = nh.AdaptiveFspSparse(ode_method=ode, space_adapter=adapter)
@@ -316,7 +318,6 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_methods=("rk45", "kryl
         return out, time.perf_counter() - t0
 
     rk = lambda prec: nh.RK45(rtol=1e-6, atol=1e-10, precision=prec)
-    kr = lambda prec: nh.KrylovExpv(m=30, tol=1e-8, precision=prec)
     say = log or (lambda m: None)
     rng = np.random.default_rng(0)
     rec = nh.TrainingRecords(capacity=1_000_000, per_adaptation=1024, label_mass=1e-10)
@@ -334,17 +335,16 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_methods=("rk45", "kryl
     say(f"c5: trained in {train_s:.1f}s, loss {loss:.4f}, accuracy {acc:.4f}")
 
     model = cascade_model(nh, C5_THETA)
-    learned = nh.LearnedAdapter(net, 2, 2, False, threshold=0.3)
-    plain = nh.SelectiveRStepAdapter(2, 2, False)
+    learned = nh.LearnedAdapter(net, 2, learned_levels, False, threshold=0.3)
     solve(model, learned, rk("fp32"), T)  # warm-up
     out_l, t_l = solve(model, learned, rk("fp32"), T)
     out_l64, t_l64 = solve(model, learned, rk("fp64"), T)
     say(f"c5: learned {t_l:.2f}s (fp64 {t_l64:.2f}s)")
     plain_runs = {}
-    for m in plain_methods:
-        out_p, t_p = solve(model, plain, kr("fp64") if m == "krylov" else rk("fp64"), T)
-        plain_runs[m] = (out_p, t_p)
-        say(f"c5: plain fp64 {m} {t_p:.2f}s")
+    for lv in plain_levels:
+        out_p, t_p = solve(model, nh.SelectiveRStepAdapter(2, lv, False), rk("fp64"), T)
+        plain_runs[f"rk45 R={lv}"] = (out_p, t_p)
+        say(f"c5: plain fp64 rk45 R={lv} {t_p:.2f}s")
     l1_prec = max(_l1_between(a, b) for a, b in zip(out_l.p, out_l64.p))
     if plain_runs:
         best = min(plain_runs, key=lambda m: plain_runs[m][1])
@@ -360,8 +360,9 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_methods=("rk45", "kryl
     st_p = out_p.stats if out_p is not None else {"max_states": None, "adaptations": None}
     return {
         "workload": f"cascade (0->A 10, A->A+B 2A, B->B+C B, decay 0.5A 0.2B 0.1C) from (0,0,0), fsptol 1e-4, "
-                    f"t in [0, {T:g}] with 6 outputs; learned adapter (threshold 0.3) vs plain "
-                    f"SelectiveRStepAdapter(2, 2, false)",
+                    f"t in [0, {T:g}] with 6 outputs; LearnedAdapter(2, {learned_levels}, threshold 0.3) in FP32 vs "
+                    f"plain SelectiveRStepAdapter(2, R, false) in FP64, R in {list(plain_levels)}, each with device "
+                    f"DP5(4) rtol 1e-6",
         "training": {"records": int(len(X)), "positive_fraction": float(Y.mean()), "scalings": n_scalings,
                      "record_horizon": T_rec, "label_mass": 1e-10, "record_s": gen_s, "train_s": train_s,
                      "epochs": 20, "final_loss": loss, "train_accuracy": acc},
@@ -369,7 +370,7 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_methods=("rk45", "kryl
         "learned_fp64_rk45_s": t_l64,
         "learned_solves_per_s": 1.0 / t_l,
         "plain_fp64_solves_per_s": 1.0 / t_p,
-        "plain_fp64_method": best,
+        "plain_fp64_best": best,
         "plain_fp64_times_s": {m: v[1] for m, v in plain_runs.items()},
         "speedup_learned_vs_plain_fp64": t_p / t_l,
         "learned_max_states": st_l["max_states"],
@@ -423,8 +424,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the forward-sensitivity (C3) measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched (C4) measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the learned-adapter (C5) measurement")
-    ap.add_argument("--c5-plain", default="both", choices=["both", "rk45", "krylov", "none"],
-                    help="plain FP64 integrators timed for C5 (the faster one is the comparison)")
+    ap.add_argument("--c5-plain", default="all", choices=["all", "none"],
+                    help="time the plain FP64 adapters for C5 (none: learned only, for profiler runs)")
     ap.add_argument("--spmv-probe", type=int, default=0, metavar="REPS",
                     help="only build the matrix, launch the mat-vec REPS times back to back and run a plain "
                          "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
@@ -544,8 +545,8 @@ def main():
     if not args.no_c4:
         line["c4_batch"] = c4_batch(nh, L, graphs=not args.no_graphs)
     if not args.no_c5:
-        plain = {"both": ("rk45", "krylov"), "none": ()}.get(args.c5_plain, (args.c5_plain,))
-        line["c5_learned"] = c5_learned(nh, plain_methods=plain, log=lambda m: print(m, file=sys.stderr, flush=True))
+        plain = (2, 4, 6) if args.c5_plain == "all" else ()
+        line["c5_learned"] = c5_learned(nh, plain_levels=plain, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(model.parameters, fp64_steps)
     print(json.dumps(line), flush=True)
