@@ -304,7 +304,7 @@ def _l1_between(a, b):
     return float(sum(abs(da.get(x, 0.0) - db.get(x, 0.0)) for x in set(da) | set(db)))
 
 
-def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=(2, 4, 6), learned_levels=6, log=None):
+def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=(2, 6, 12), learned_levels=8, log=None):
     """Config C5: the cascade from (0,0,0) with fsptol 1e-4 over [0, T], adaptive FSP with
     the learned adapter against plain SelectiveRStepAdapter(2, R), each at its best
     expansion depth (the learned adapter expands learned_levels levels from the states it
@@ -373,6 +373,7 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=(2, 4, 6), lear
         "plain_fp64_best": best,
         "plain_fp64_times_s": {m: v[1] for m, v in plain_runs.items()},
         "speedup_learned_vs_plain_fp64": t_p / t_l,
+        "speedup_learned_vs_plain_fp64_R2": (plain_runs["rk45 R=2"][1] / t_l) if "rk45 R=2" in plain_runs else None,
         "learned_max_states": st_l["max_states"],
         "plain_max_states": st_p["max_states"],
         "learned_adaptations": st_l["adaptations"],
@@ -545,7 +546,7 @@ def main():
     if not args.no_c4:
         line["c4_batch"] = c4_batch(nh, L, graphs=not args.no_graphs)
     if not args.no_c5:
-        plain = (2, 4, 6) if args.c5_plain == "all" else ()
+        plain = (2, 6, 12) if args.c5_plain == "all" else ()
         line["c5_learned"] = c5_learned(nh, plain_levels=plain, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(model.parameters, fp64_steps)

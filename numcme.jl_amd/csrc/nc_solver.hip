@@ -28,7 +28,7 @@ namespace nc {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxRedBlocks = 1024;
+constexpr int kMaxRedBlocks = 2048;
 constexpr int kPartStrideMax = 65;  // partial sums per block: up to 2 x 32 dots + a norm
 
 enum Finish : int { FIN_NONE = 0, FIN_ARN, FIN_LAST, FIN_BETA, FIN_SUMS };
@@ -324,11 +324,11 @@ __device__ void arnoldi_finish(const KLayout& L, double* kb, double* gram, int j
 // One block: fold the per-block partials (one wave per value, fixed lane->block
 // assignment and a fixed shuffle tree, so the result is deterministic), then apply the
 // finishing step.
-template <typename BT>
+template <typename BT, int kPer>
 __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigned nblocks, int nvals) {
+  // kPer partials per lane (nblocks <= 64 kPer), all loads issued together
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __shared__ double tot[kPartStrideMax];
-  constexpr int kPer = kMaxRedBlocks / 64;  // partials per lane, all loads issued together
   for (int k = wave; k < nvals; k += kFinBlock / 64) {
     double v[kPer];
 #pragma unroll
@@ -417,7 +417,12 @@ void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
       break;
     default: launch_vec_mode<BT, VM_UPDATE>(a, grid, s); break;
   }
-  if (a.finish != FIN_NONE) k_vec_finish<BT><<<1, kFinBlock, 0, s>>>(a, grid, nvals);
+  if (a.finish != FIN_NONE) {
+    if (grid <= 512)
+      k_vec_finish<BT, 8><<<1, kFinBlock, 0, s>>>(a, grid, nvals);
+    else
+      k_vec_finish<BT, kMaxRedBlocks / 64><<<1, kFinBlock, 0, s>>>(a, grid, nvals);
+  }
   NC_CHECK_HIP(hipGetLastError());
 }
 
@@ -536,12 +541,17 @@ double round_step(double x) {
 
 void solver_reduce_setup(nc_solver_s* S, int nvals) {
   (void)nvals;
-  // 512 blocks of 256 threads (two per CU) measured best for the vector passes
-  S->red_blocks = std::min<unsigned>(kVecBlocks, grid_for(S->Np / 4, kBlock));
+  // Grid of the vector passes: 2048 blocks of 256 threads for a Krylov basis stored in
+  // FP32 and 512 otherwise measured best on the C2 solve (more blocks hide the latency
+  // of the basis streams; beyond that the one-block fold of the partials dominates)
+  const unsigned vb = (S->method == NC_METHOD_KRYLOV && S->vs == 4) ? (unsigned)kMaxRedBlocks : kVecBlocks;
+  S->red_blocks = std::min<unsigned>(vb, grid_for(S->Np / 4, kBlock));
   // the RK error pass puts 3 R gathered sink values behind its partials
   S->partials.reserve(std::max<size_t>((size_t)S->red_blocks * kPartStrideMax, S->red_blocks + 3 * kMaxReactions));
   if (!S->hpart)
-    S->hpart = static_cast<double*>(pinned_alloc(sizeof(double) * (kMaxRedBlocks + 4 * kMaxReactions)));
+    // partials (up to kMaxRedBlocks) and the RK step's 3 R gathered sink values, then the
+    // carried sinks of u and the step's three sink sets (4 R) behind them
+    S->hpart = static_cast<double*>(pinned_alloc(sizeof(double) * (kMaxRedBlocks + 7 * kMaxReactions)));
 }
 
 // The right-hand-side operator: the FSP generator, or the block sensitivity operator.
@@ -866,7 +876,7 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
   hipStream_t s = S->s;
   const bool fp32 = A->precision == NC_FP32;
   const double rtol = S->opts.rtol, atol = S->opts.atol;
-  const unsigned g = std::min<unsigned>(4 * kMaxRedBlocks, grid_for(N, kBlock));
+  const unsigned g = std::min<unsigned>(4096, grid_for(N, kBlock));  // elementwise stage kernels
   double* k = S->k.p;
   auto kp = [&](int j) { return k + (int64_t)rk_slot(j, S->ksw) * N; };
   auto to_vin = [&](const double* src) {
@@ -912,7 +922,8 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
     S->h = S->opts.h0 > 0 ? S->opts.h0 : (fnorm > 0 ? 0.01 / fnorm : 1e-3);
     S->h = std::max(S->h, 1e-12);
   }
-  double* hs = S->hpart + kMaxRedBlocks;  // pinned: sinks of u, y_{n+1}, k0, k6 (R each)
+  // pinned: sinks of u, y_{n+1}, k0, k6 (R each), behind the readback region
+  double* hs = S->hpart + kMaxRedBlocks + 3 * kMaxReactions;
   if (ev && R > 0) {
     NC_CHECK_HIP(hipMemcpyAsync(hs, S->u.p + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
