@@ -170,13 +170,18 @@ __global__ __launch_bounds__(kBlock) void k_berr(Dev d, int64_t ld, int64_t gran
   }
 }
 
-// Per-instance fold of the granule partials (granules of an instance are contiguous)
-__global__ void k_bfold(int n_inst, const int64_t* gran_start, const double* partials, double* err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_inst) return;
-  double s = 0.0;
-  for (int64_t g = gran_start[i]; g < gran_start[i + 1]; ++g) s += partials[g];
-  err[i] = s;
+// Per-instance fold of the granule partials (granules of an instance are contiguous): one
+// wavefront per instance, lane l adding granules l, l + 64, ... in order, then a fixed
+// shuffle tree (deterministic)
+__global__ __launch_bounds__(kBlock) void k_bfold(int n_inst, const int64_t* gran_start, const double* partials,
+                                                  double* err) {
+  const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n_inst) return;  // whole wavefronts leave together
+  double sum = 0.0;
+  for (int64_t g = gran_start[i] + lane; g < gran_start[i + 1]; g += 64) sum += partials[g];
+  sum = wave_sum(sum);
+  if (lane == 0) err[i] = sum;
 }
 
 // Accept the last step where asked (u <- y, k0 <- k6) and fill requested snapshots.
@@ -413,7 +418,8 @@ void iteration_r(nc_batch_s* B, double atol, double rtol) {
     launch_matvec<VT, R>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p + (int64_t)st * B->ld);
   }
   k_berr<R><<<g, kBlock, 0, s>>>(d, B->ld, B->grans, atol, rtol, B->u.p, B->ytmp.p, B->k.p, B->partials.p);
-  k_bfold<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(B->n, B->d_gran_start.p, B->partials.p, B->err.p);
+  k_bfold<<<(unsigned)((B->n + kBlock / 64 - 1) / (kBlock / 64)), kBlock, 0, s>>>(B->n, B->d_gran_start.p,
+                                                                                B->partials.p, B->err.p);
   NC_CHECK_HIP(hipGetLastError());
 }
 
