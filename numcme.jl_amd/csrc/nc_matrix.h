@@ -39,6 +39,13 @@ struct nc_matrix_s {
   nc::DevBuf<int64_t> sink_off;
   nc::DevBuf<int32_t> sink_src;
   nc::DevBuf<unsigned char> sink_val;
+  // long sink rows (adaptive spaces have large boundaries) are reduced in fixed chunks by
+  // many workgroups and folded in order by a second kernel; 0 chunks: one workgroup per row
+  int32_t n_chunks = 0;
+  nc::DevBuf<int64_t> chunk_start;  // n_chunks + 1 entry offsets
+  nc::DevBuf<int32_t> chunk_row;    // sink row of each chunk
+  nc::DevBuf<int32_t> chunk_first;  // nr + 1: first chunk of each sink row
+  nc::DevBuf<double> chunk_part;    // n_chunks partial sums
   // slots whose stored column is i + dia_off[k] in every row with a nonzero value (box
   // spaces in lattice order): the mat-vec computes those columns instead of loading them
   uint32_t dia_mask = 0;

@@ -249,6 +249,10 @@ struct MatArgs {
   double sink_scale[kMaxReactions];
   uint32_t dia_mask;              // bit k: slot k's column is i + dia_off[k]
   int32_t dia_off[kMaxReactions];
+  int32_t n_chunks;               // > 0: the extra workgroups reduce sink-entry chunks
+  const int64_t* chunk_start;
+  const int32_t* chunk_row;
+  double* chunk_part;
 };
 
 template <typename T, int R>
@@ -335,8 +339,10 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
       if (i < a.n) out[i] = (OT)(accumulate ? (double)out[i] + acc[r] : acc[r]);
     }
   } else {
-    const int r = (int)((int64_t)blockIdx.x - row_blocks);
-    const int64_t e0 = a.sink_off[r], e1 = a.sink_off[r + 1];
+    const int c = (int)((int64_t)blockIdx.x - row_blocks);
+    const int r = a.n_chunks ? a.chunk_row[c] : c;
+    const int64_t e0 = a.n_chunks ? a.chunk_start[c] : a.sink_off[r];
+    const int64_t e1 = a.n_chunks ? a.chunk_start[c + 1] : a.sink_off[r + 1];
     double acc = 0.0;
     for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) acc += (double)a.sink_val[e] * (double)v[a.sink_src[e]];
     __shared__ double part[kBlock / 64];
@@ -346,11 +352,31 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
     if (threadIdx.x == 0) {
       double s = 0.0;
       for (int w = 0; w < kBlock / 64; ++w) s += part[w];
-      s *= a.sink_scale[r];
-      const int64_t o = a.n + r;
-      out[o] = (OT)(accumulate ? (double)out[o] + s : s);
+      if (a.n_chunks) {
+        a.chunk_part[c] = s;  // scaled and added in order by k_sink_fold
+      } else {
+        s *= a.sink_scale[r];
+        const int64_t o = a.n + r;
+        out[o] = (OT)(accumulate ? (double)out[o] + s : s);
+      }
     }
   }
+}
+
+struct SinkScale {
+  double v[kMaxReactions];
+};
+
+// Sink row r = sum of its chunk partials in chunk order, times the row's time factor
+template <typename OT>
+__global__ void k_sink_fold(int nr, const int32_t* chunk_first, const double* chunk_part, SinkScale sc, int64_t n,
+                            OT* out, int accumulate) {
+  const int r = threadIdx.x;
+  if (r >= nr) return;
+  double s = 0.0;
+  for (int c = chunk_first[r]; c < chunk_first[r + 1]; ++c) s += chunk_part[c];
+  s *= sc.v[r];
+  out[n + r] = (OT)(accumulate ? (double)out[n + r] + s : s);
 }
 
 // Launch timing: when e0/e1 are given the kernel is launched with hipExtLaunchKernelGGL,
@@ -360,14 +386,30 @@ struct LaunchCfg {
   unsigned grid;
   hipStream_t s;
   hipEvent_t e0, e1;
+  const int32_t* chunk_first;
 };
 
 template <typename VT, typename OT, int K, int R>
 void launch_one(const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
-  if (c.e0)
-    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo, acc);
-  else
+  if (!a.n_chunks) {
+    if (c.e0)
+      hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo, acc);
+    else
+      k_matvec<VT, OT, K, R><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+    return;
+  }
+  // chunked sinks: the events bracket both kernels (start of the first, end of the fold)
+  SinkScale sc;
+  for (int r = 0; r < a.nr; ++r) sc.v[r] = a.sink_scale[r];
+  if (c.e0) {
+    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, nullptr, 0, a, vv, oo,
+                          acc);
+    hipExtLaunchKernelGGL(k_sink_fold<OT>, dim3(1), dim3(64), 0, c.s, nullptr, c.e1, 0, a.nr, c.chunk_first,
+                          (const double*)a.chunk_part, sc, a.n, oo, acc);
+  } else {
     k_matvec<VT, OT, K, R><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+    k_sink_fold<OT><<<1, 64, 0, c.s>>>(a.nr, c.chunk_first, a.chunk_part, sc, a.n, oo, acc);
+  }
 }
 
 template <typename VT, typename OT, int R>
@@ -414,6 +456,10 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   a.sink_src = A->sink_src.p;
   a.sink_val = reinterpret_cast<const VT*>(A->sink_val.p);
   a.dia_mask = A->dia_mask;
+  a.n_chunks = A->n_chunks;
+  a.chunk_start = A->chunk_start.p;
+  a.chunk_row = A->chunk_row.p;
+  a.chunk_part = A->chunk_part.p;
   for (size_t k = 0; k < A->dia_off.size() && k < (size_t)kMaxReactions; ++k) a.dia_off[k] = A->dia_off[k];
   const nc_model_s* m = A->model;
   for (int r = 0; r < A->nr; ++r) a.sink_scale[r] = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
@@ -422,7 +468,7 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   const int R = A->rows_per_thread;
   const int64_t row_threads = (A->n + R - 1) / R;
   const int64_t row_blocks = (row_threads + kBlock - 1) / kBlock;
-  const LaunchCfg c{(unsigned)(row_blocks + A->nr), s, e0, e1};
+  const LaunchCfg c{(unsigned)(row_blocks + (A->n_chunks ? A->n_chunks : A->nr)), s, e0, e1, A->chunk_first.p};
   const VT* vv = reinterpret_cast<const VT*>(v);
   const int acc = accumulate ? 1 : 0;
   if (out_fp64)
@@ -562,6 +608,39 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   }
   A->sink_off_h[nr] = total;
   A->n_sink_entries = total;
+  // long sink rows: 4096-entry chunks, each reduced by its own workgroup (the host
+  // vectors stay alive until the build's final synchronisation)
+  constexpr int64_t kSinkChunk = 4096, kChunkFrom = 8192;
+  std::vector<int64_t> chunk_start_h;
+  std::vector<int32_t> chunk_row_h, chunk_first_h;
+  {
+    int64_t longest = 0;
+    for (int r = 0; r < nr; ++r) longest = std::max<int64_t>(longest, A->sink_off_h[r + 1] - A->sink_off_h[r]);
+    A->n_chunks = 0;
+    if (longest > kChunkFrom) {
+      chunk_first_h.assign(nr + 1, 0);
+      for (int r = 0; r < nr; ++r) {
+        chunk_first_h[r] = (int32_t)chunk_row_h.size();
+        for (int64_t e = A->sink_off_h[r]; e < A->sink_off_h[r + 1]; e += kSinkChunk) {
+          chunk_start_h.push_back(e);
+          chunk_row_h.push_back(r);
+        }
+      }
+      chunk_first_h[nr] = (int32_t)chunk_row_h.size();
+      chunk_start_h.push_back(total);
+      A->n_chunks = (int32_t)chunk_row_h.size();
+      A->chunk_start.alloc(chunk_start_h.size());
+      A->chunk_row.alloc(chunk_row_h.size());
+      A->chunk_first.alloc(chunk_first_h.size());
+      A->chunk_part.alloc(chunk_row_h.size());
+      NC_CHECK_HIP(hipMemcpyAsync(A->chunk_start.p, chunk_start_h.data(), sizeof(int64_t) * chunk_start_h.size(),
+                                  hipMemcpyHostToDevice, s));
+      NC_CHECK_HIP(hipMemcpyAsync(A->chunk_row.p, chunk_row_h.data(), sizeof(int32_t) * chunk_row_h.size(),
+                                  hipMemcpyHostToDevice, s));
+      NC_CHECK_HIP(hipMemcpyAsync(A->chunk_first.p, chunk_first_h.data(), sizeof(int32_t) * chunk_first_h.size(),
+                                  hipMemcpyHostToDevice, s));
+    }
+  }
   A->sink_off.alloc(nr + 1);
   NC_CHECK_HIP(hipMemcpyAsync(A->sink_off.p, A->sink_off_h.data(), sizeof(int64_t) * (nr + 1), hipMemcpyHostToDevice, s));
   A->sink_src.alloc(std::max<uint32_t>(total, 1));

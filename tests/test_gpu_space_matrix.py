@@ -238,3 +238,32 @@ def test_rows_per_thread_variants_identical(gpu, precision):
     assert np.abs(outs[0] - ref).max() <= tol * np.abs(ref).max()
     rows, ms = A.autotune(reps=5)
     assert rows in (1, 2, 4) and ms > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_long_sink_rows_vs_oracle(gpu, precision):
+    # a jump reaction (+100 in species 0) on a 128 x 128 box: 12,800 states share its sink
+    # row, so the row is reduced in chunks by several workgroups and folded in order
+    S = np.array([[100, -1, 0, 0], [0, 0, 1, -1]])
+    props = [("ti", lambda x, p: p[0] + 0.0 * x[0]), ("ti", lambda x, p: p[1] * x[0]),
+             ("ti", lambda x, p: p[2] + 0.0 * x[1]), ("ti", lambda x, p: p[3] * x[1])]
+    th = np.array([0.7, 0.05, 3.0, 0.1])
+    states = models.box_states([127, 127])
+    _, _, Ad, Ao = build_pair(S, props, th, states, precision=precision)
+    info = Ad.info()
+    assert info["n_sink_entries"] > 8192
+    rng = np.random.default_rng(2)
+    v = rng.random(Ao.rowcount)
+    vin = v if precision == "fp64" else v.astype(np.float32).astype(float)
+    wo = Ao.matvec(0.0, vin)
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    for acc in (False, True):
+        if acc:
+            w = np.full(Ao.rowcount, 0.5)
+            Ad.matvecadd_(w, 0.0, v)
+            w -= 0.5
+        else:
+            w = nh.matvec(0.0, Ad, v)
+        assert np.abs(w - wo).max() <= tol * np.abs(wo).max()
+        # the sink row of the jump: every state with x0 + 100 > 127
+        assert abs(w[len(states)] - wo[len(states)]) <= tol * abs(wo[len(states)])
