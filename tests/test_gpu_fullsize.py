@@ -1,0 +1,134 @@
+"""Checks at the full sizes of the measured configurations, through properties that do not
+depend on an oracle solve of that size:
+
+* C3 (262,144 states, 12 parameters): probability conservation, zero total sensitivity
+  (d/dtheta of the total mass), FP32 storage against FP64, and each sensitivity against
+  central differences of two plain FP64 probability solves (a time-invariant parameter and
+  the separable k21).
+* C4 (all 512 instances of the committed table): every instance conserves probability at
+  every output, and instances across the size range agree with their FP64 single solves.
+* C5 (cascade to t = 5, fsptol 1e-4): the learned adapter keeps the sink bound at every
+  output and stays within 2 fsptol of the plain adapter's solution.
+"""
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import models
+import numcme_hip as nh
+from numcme_hip import _lib as L
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _c3_solve(precision, theta=None):
+    S, props, th = models.gene4() if theta is None else models.gene4(theta)
+    states = models.gene4_states(255)
+    model = nh.CmeModelWithSensitivity(nh.CmeModel(S, models.to_product(props), th))
+    p0 = np.zeros(len(states))
+    p0[0] = 1.0
+    ic = nh.forwardsens_initial_condition(states, p0, [np.zeros(len(states)) for _ in th])
+    alg = nh.AdaptiveForwardSensFspSparse(space_adapter=nh.ForwardSensRStepAdapter(0, 1, False),
+                                          ode_method=nh.RK45(rtol=1e-8, atol=1e-14, precision=precision))
+    ts = np.arange(0.0, 901.0, 300.0)
+    out = nh.solve(model, ic, (0.0, 900.0), alg, saveat=ts, fsptol=1.0)
+    assert out.stats["adaptations"] == 0
+    return out, states, S, props, th
+
+
+def _prob_final(S, props, th, states):
+    model = nh.CmeModel(S, models.to_product(props), th)
+    p0 = np.zeros(len(states))
+    p0[0] = 1.0
+    out = nh.solve(model, nh.FspVectorSparse(states, p0), (0.0, 900.0), nh.RK45(rtol=1e-10, atol=1e-16),
+                   saveat=[900.0])
+    return out.p[-1].values
+
+
+def test_c3_full_size_invariants_and_central_differences(gpu):
+    o64, states, S, props, th = _c3_solve("fp64")
+    o32, *_ = _c3_solve("fp32")
+    P = len(th)
+    for k in range(len(o64)):
+        assert abs(o64.p[k].values.sum() + o64.sinks[k].sum() - 1.0) <= 1e-9
+        assert abs(o32.p[k].values.sum() + o32.sinks[k].sum() - 1.0) <= 1e-6
+        for ip in range(P):
+            assert abs(o64.S[k][ip].values.sum() + np.sum(o64.dsinks[k][ip])) <= 1e-10
+        assert np.abs(o64.p[k].values - o32.p[k].values).sum() <= 1e-6
+        for ip in range(P):
+            a, b = o64.S[k][ip].values, o32.S[k][ip].values
+            assert np.abs(a - b).max() <= 1e-5 * max(np.abs(a).max(), 1e-300), (k, ip)
+    # central differences of the probability solve (k_export: time-invariant; k21: separable)
+    for ip in (10, 3):
+        d = 1e-4 * th[ip]
+        tp, tm = th.copy(), th.copy()
+        tp[ip] += d
+        tm[ip] -= d
+        fd = (_prob_final(S, props, tp, states) - _prob_final(S, props, tm, states)) / (2 * d)
+        sens = o64.S[-1][ip].values
+        assert np.abs(fd - sens).max() <= 1e-5 * np.abs(sens).max(), ip
+
+
+def test_c4_full_batch_invariants(gpu):
+    from bench import c4_instance  # the bench's instance builder (model, space, generator)
+
+    table = json.loads((ROOT / "tests" / "golden" / "c4_instances.json").read_text())
+    mats, u0s = [], []
+    for spec in table:
+        A, u0 = c4_instance(nh, spec, "fp32")
+        mats.append(A)
+        u0s.append(u0)
+    B = nh.BatchFsp(mats)
+    ts = np.arange(0.0, 11.0)
+    snaps = B.solve(u0s, ts)
+    for i, sn in enumerate(snaps):
+        for k in range(len(ts)):
+            assert abs(sn[k].sum() - 1.0) <= 1e-6, (i, k)
+            assert sn[k].min() >= -1e-9, (i, k)
+    order = np.argsort([t["states"] for t in table])
+    for i in order[[0, 128, 384, 511]]:
+        A64, u0 = c4_instance(nh, table[i], "fp64")
+        o = L.nc_solve_opts()
+        L.call("nc_solve_opts_default", C.byref(o))
+        o.method, o.rtol, o.atol = L.NC_METHOD_KRYLOV, 1e-8, 1e-14
+        out = np.empty(len(ts) * len(u0))
+        L.call("nc_solve_fixed", A64.handle, C.byref(o), L.ptr(np.ascontiguousarray(u0), C.c_double), len(ts),
+               L.ptr(ts, C.c_double), L.ptr(out, C.c_double), None)
+        ref = out.reshape(len(ts), len(u0))
+        for k in range(len(ts)):
+            assert np.abs(snaps[int(i)][k] - ref[k]).sum() <= 1e-6, (int(i), k)
+
+
+def test_c5_learned_full_horizon(gpu):
+    import bench
+
+    rng = np.random.default_rng(0)
+    rec = nh.TrainingRecords(capacity=400_000, per_adaptation=1024, label_mass=1e-10)
+    rk = lambda prec: nh.RK45(rtol=1e-6, atol=1e-10, precision=prec)
+    for _ in range(16):
+        th = np.asarray(bench.C5_THETA) * np.exp(rng.uniform(np.log(0.5), np.log(2.0), size=6))
+        nh.solve(bench.cascade_model(nh, th), nh.FspVectorSparse([[0, 0, 0]], [1.0]), (0.0, 1.5),
+                 nh.AdaptiveFspSparse(ode_method=rk("fp32"), space_adapter=nh.SelectiveRStepAdapter(2, 2, False)),
+                 saveat=np.linspace(0.0, 1.5, 6), fsptol=1e-4, records=rec)
+    net, _ = nh.train_adapter(*rec.arrays(), epochs=20, batch=4096, lr=2e-3, seed=0)
+    model = bench.cascade_model(nh, bench.C5_THETA)
+    ts = np.linspace(0.0, 5.0, 6)
+    fsptol = 1e-4
+    run = lambda ad, prec: nh.solve(model, nh.FspVectorSparse([[0, 0, 0]], [1.0]), (0.0, 5.0),
+                                    nh.AdaptiveFspSparse(ode_method=rk(prec), space_adapter=ad), saveat=ts,
+                                    fsptol=fsptol)
+    learned = run(nh.LearnedAdapter(net, 2, 8, False, threshold=0.3), "fp32")
+    plain = run(nh.SelectiveRStepAdapter(2, 12, False), "fp64")
+    assert learned.stats["learned_adaptations"] > 0
+    assert learned.stats["max_states"] < plain.stats["max_states"] / 4
+    for t, p, s in zip(learned.t, learned.p, learned.sinks):
+        assert s.sum() <= fsptol * t / 5.0 + 1e-12
+        assert 1.0 - fsptol <= p.sum() + s.sum() <= 1.0 + 1e-6
+    for a, b in zip(learned.p, plain.p):
+        da = {tuple(x): v for x, v in zip(a.states.tolist(), a.values)}
+        db = {tuple(x): v for x, v in zip(b.states.tolist(), b.values)}
+        assert sum(abs(da.get(x, 0.0) - db.get(x, 0.0)) for x in set(da) | set(db)) <= 2 * fsptol
