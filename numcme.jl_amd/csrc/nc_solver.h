@@ -23,6 +23,7 @@ struct nc_solver_s {
   int32_t n_snaps = 0;
   // reduction scratch
   nc::DevBuf<double> partials;
+  nc::DevBuf<double> fold;  // folded totals of a vector pass and the fold's arrival counter
   unsigned red_blocks = 0;
   // Krylov
   int m = 30;

@@ -69,6 +69,8 @@ struct VecArgs {
   int finish;
   double* kb;
   double* gram;           // (m+1) x (m+1) Gram matrix of the scaled basis (FIN_ARN)
+  double* tot;            // kPartStrideMax folded totals (finish)
+  unsigned* counter;      // arrival counter of the fold's workgroups, zero between passes
   KLayout L;
   int j;
   double btol;
@@ -321,26 +323,43 @@ __device__ void arnoldi_finish(const KLayout& L, double* kb, double* gram, int j
   }
 }
 
-// One block: fold the per-block partials (one wave per value, fixed lane->block
-// assignment and a fixed shuffle tree, so the result is deterministic), then apply the
-// finishing step.
-template <typename BT, int kPer>
+// Fold of the per-block partials, then the finishing step. One workgroup per value:
+// workgroup k sums value k over the pass's blocks in a fixed order (thread t takes blocks
+// t, t + 1024, ..., then a fixed shuffle tree), so the totals are deterministic, and
+// stores it. The workgroup that arrives last at a device-scope counter (release fence
+// before the increment, acquire fence after it) reads every total, re-arms the counter
+// for the next pass and applies the finishing step. The values are folded by nvals
+// compute units at once instead of one.
+template <typename BT>
 __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigned nblocks, int nvals) {
-  // kPer partials per lane (nblocks <= 64 kPer), all loads issued together
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __shared__ double tot[kPartStrideMax];
-  for (int k = wave; k < nvals; k += kFinBlock / 64) {
-    double v[kPer];
-#pragma unroll
-    for (int t = 0; t < kPer; ++t) {
-      const unsigned b = lane + 64u * t;
-      v[t] = b < nblocks ? a.partials[(int64_t)k * nblocks + b] : 0.0;
+  __shared__ double wsum[kFinBlock / 64];
+  __shared__ int s_last;
+  {
+    const int k = blockIdx.x;
+    double v = 0.0;
+    for (unsigned b = threadIdx.x; b < nblocks; b += kFinBlock) v += a.partials[(int64_t)k * nblocks + b];
+    v = wave_sum(v);
+    if (lane == 0) wsum[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < kFinBlock / 64; ++w) sum += wsum[w];
+      __hip_atomic_store(reinterpret_cast<unsigned long long*>(a.tot) + k, (unsigned long long)__double_as_longlong(sum),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+      const unsigned prev = atomicAdd(a.counter, 1u);
+      const int last = prev + 1u == gridDim.x;
+      if (last) __threadfence();
+      s_last = last;
     }
-    double s = 0.0;
-#pragma unroll
-    for (int t = 0; t < kPer; ++t) s += v[t];
-    s = wave_sum(s);
-    if (lane == 0) tot[k] = s;
+    __syncthreads();
+    if (!s_last) return;
+    if (threadIdx.x < nvals)
+      tot[threadIdx.x] = __longlong_as_double((long long)__hip_atomic_load(
+          reinterpret_cast<unsigned long long*>(a.tot) + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (threadIdx.x == 0) __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
   const KLayout& L = a.L;
@@ -395,8 +414,11 @@ void launch_vec_mode(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
 }
 
 template <typename BT>
-void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
+void launch_vec(VecArgs<BT> a, const nc_solver_s* S, hipStream_t s) {
   if (a.N % 4) throw Error(NC_ERR_INTERNAL, "vector passes need a padded length");
+  const unsigned grid = S->red_blocks;
+  a.tot = S->fold.p;
+  a.counter = reinterpret_cast<unsigned*>(S->fold.p + kPartStrideMax);
   int nvals = 1;
   switch (a.mode) {
     case VM_DOTS: launch_vec_mode<BT, VM_DOTS>(a, grid, s); nvals = a.nv + 1; break;
@@ -418,10 +440,8 @@ void launch_vec(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
     default: launch_vec_mode<BT, VM_UPDATE>(a, grid, s); break;
   }
   if (a.finish != FIN_NONE) {
-    if (grid <= 512)
-      k_vec_finish<BT, 8><<<1, kFinBlock, 0, s>>>(a, grid, nvals);
-    else
-      k_vec_finish<BT, kMaxRedBlocks / 64><<<1, kFinBlock, 0, s>>>(a, grid, nvals);
+    if (nvals > kPartStrideMax || !a.tot || !a.counter) throw Error(NC_ERR_INTERNAL, "vector pass fold not set up");
+    k_vec_finish<BT><<<nvals, kFinBlock, 0, s>>>(a, grid, nvals);
   }
   NC_CHECK_HIP(hipGetLastError());
 }
@@ -548,6 +568,11 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
   S->red_blocks = std::min<unsigned>(vb, grid_for(S->Np / 4, kBlock));
   // the RK error pass puts 3 R gathered sink values behind its partials
   S->partials.reserve(std::max<size_t>((size_t)S->red_blocks * kPartStrideMax, S->red_blocks + 3 * kMaxReactions));
+  if (!S->fold.p) {
+    // folded totals, then the fold's arrival counter (zero between passes)
+    S->fold.alloc(kPartStrideMax + 1);
+    NC_CHECK_HIP(hipMemsetAsync(S->fold.p, 0, sizeof(double) * (kPartStrideMax + 1), S->s));
+  }
   if (!S->hpart)
     // partials (up to kMaxRedBlocks) and the RK step's 3 R gathered sink values, then the
     // carried sinks of u and the step's three sink sets (4 R) behind them
@@ -609,7 +634,7 @@ static void arnoldi_iteration(nc_solver_s* S, int j) {
   a.mode = VM_DUAL;
   a.xin = S->q.p;
   a.finish = FIN_ARN;
-  launch_vec(a, S->red_blocks, s);
+  launch_vec(a, S, s);
   a.mode = VM_UPDATE;
   a.sc = S->kbuf.p + L.sc(0);
   a.xscale = S->kbuf.p + L.sc(j);
@@ -617,7 +642,7 @@ static void arnoldi_iteration(nc_solver_s* S, int j) {
   a.csign = -1.0;
   a.vout = Vp(j + 1);
   a.finish = FIN_NONE;
-  launch_vec(a, S->red_blocks, s);
+  launch_vec(a, S, s);
 }
 
 template <typename BT>
@@ -637,7 +662,7 @@ static void arnoldi_last(nc_solver_s* S) {
   a.L = L;
   a.btol = kBreakdownTol;
   a.finish = FIN_LAST;
-  launch_vec(a, S->red_blocks, S->s);
+  launch_vec(a, S, S->s);
 }
 
 // The device work of one Arnoldi process (m + 1 mat-vecs, the projections and the
@@ -845,11 +870,11 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
       if (fp32) {
         VecArgs<float> a{};
         fill(a, reinterpret_cast<const float*>(Vp(1)));
-        launch_vec(a, S->red_blocks, s);
+        launch_vec(a, S, s);
       } else {
         VecArgs<double> a{};
         fill(a, reinterpret_cast<const double*>(Vp(1)));
-        launch_vec(a, S->red_blocks, s);
+        launch_vec(a, S, s);
       }
     }
     // the new beta (written on the device by the combination) arrives with the next H
@@ -915,7 +940,7 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
     a.kb = S->kbuf.p;
     a.L = KLayout{S->m};
     a.finish = FIN_SUMS;
-    launch_vec(a, S->red_blocks, s);
+    launch_vec(a, S, s);
     NC_CHECK_HIP(hipMemcpyAsync(sums.data(), S->kbuf.p, sizeof(double), hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
     const double fnorm = std::sqrt(sums[0]);
@@ -1287,7 +1312,7 @@ void solver_set_state_device(nc_solver_s* S, double t0, const double* u_dev) {
   a.kb = S->kbuf.p + L.misc(5);
   a.L = L;
   a.finish = FIN_SUMS;
-  launch_vec(a, S->red_blocks, S->s);
+  launch_vec(a, S, S->s);
   double b2 = 0.0;
   NC_CHECK_HIP(hipMemcpyAsync(&b2, S->kbuf.p + L.misc(5), sizeof(double), hipMemcpyDeviceToHost, S->s));
   NC_CHECK_HIP(hipStreamSynchronize(S->s));
