@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
+#include <vector>
 
 #include "nc_sens.h"
 
@@ -24,6 +26,18 @@ struct PairMeta {
   int32_t r, arr, cslot, q;
   int64_t sink_off;
 };
+
+// One derivative term of the row part: the pair's state-factor derivative (kind 0:
+// ddiag/dpred row idx, predecessor column slot cslot, scaled by dscale[pair]) or its
+// time-factor derivative (kind 1: time-varying slot idx, scaled by sfscale[pair]), added
+// to accumulator kk of the wave that owns the pair's block (kind -1: padding).
+struct Term {
+  int8_t kk, pad;
+  int16_t kind;
+  int16_t idx, cslot;
+  int16_t pair, pad2;
+};
+constexpr int kMaxTerms = 2 * kMaxPairs;
 
 template <typename VT>
 struct SensArgs {
@@ -43,6 +57,8 @@ struct SensArgs {
   const int32_t* dcol;
   const VT* dsink;
   const PairMeta* meta;  // pairs sorted by parameter
+  const Term* terms;     // derivative terms of the row part, grouped by wave
+  int32_t tstart[kBlock / 64 + 1];  // terms of wave w: tstart[w] .. tstart[w+1)
   int32_t pstart[kMaxSensBlocks + 1];  // pairs of block b (b >= 1): pstart[b] .. pstart[b+1)
   double tv_scale[kMaxReactions];
   double sink_scale[kMaxReactions];
@@ -55,63 +71,122 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-template <typename VT, int MB>
+// One workgroup per tile of 64 rows; its four waves share the tile's rows and split the
+// P + 1 vector blocks (wave w owns blocks w, w + 4, ...), so a thread holds MBW <= 8
+// block accumulators instead of P + 1. A's slot arrays are read by the four waves of one
+// compute unit (one HBM read, the rest from its caches). A row's loads are issued in
+// batches whose addresses do not depend on each other (all slot columns and values, then
+// every gather for them; the derivative terms TB at a time), so a thread waits for a few
+// memory round trips instead of one per slot and term. The derivative terms dA_p v_0 of a
+// wave's blocks come from a host-built list (Term). The nb x nr sink rows come first in
+// the grid, one workgroup each, so that their longer reductions overlap the row tiles.
+template <typename VT, int MBW, int K>
 __global__ __launch_bounds__(kBlock) void k_sens_matvec(SensArgs<VT> a, const VT* __restrict__ v,
                                                         double* __restrict__ out) {
-  const int64_t row_blocks = (a.n + kBlock - 1) / kBlock;
+  constexpr int W = kBlock / 64, KK = K > 0 ? K : 1, TB = 4;
   const int nb = a.nb;
-  if ((int64_t)blockIdx.x < row_blocks) {
-    // XCD-contiguous row blocks, as in the plain mat-vec
-    const int64_t b = blockIdx.x, q8 = row_blocks >> 3, r8 = row_blocks & 7, x = b & 7;
-    const int64_t i = (x * q8 + (x < r8 ? x : r8) + (b >> 3)) * kBlock + threadIdx.x;
+  const int64_t nsink = (int64_t)nb * a.nr, nsink8 = (nsink + 7) & ~int64_t(7);
+  if ((int64_t)blockIdx.x >= nsink8) {
+    // the derivative terms, staged in LDS (uniform reads; no load waits on them later)
+    __shared__ Term s_terms[kMaxTerms];
+    const int nterms = a.tstart[W];
+    for (int t = threadIdx.x; t < nterms; t += kBlock) s_terms[t] = a.terms[t];
+    __syncthreads();
+    const int64_t tiles = (a.n + 63) / 64;
+    // XCD-contiguous tiles, as in the plain mat-vec (the sink slots are padded to a
+    // multiple of 8 so that b and blockIdx.x share their XCD)
+    const int64_t b = (int64_t)blockIdx.x - nsink8, q8 = tiles >> 3, r8 = tiles & 7, x = b & 7;
+    const int64_t tile = x * q8 + (x < r8 ? x : r8) + (b >> 3);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t i = tile * 64 + (threadIdx.x & 63);
     if (i >= a.n) return;
-    double acc[MB], vi[MB];
+    const int64_t ld = a.ld, vs = a.vstride;
+    // Every load below is unconditional (block indices clamped to nb - 1, term and
+    // time-varying slot indices to their last entry, padding weighted by zero), so the
+    // loads of a batch are issued back to back; accumulators beyond nb are never stored.
+    const VT* vb[MBW];
+#pragma unroll
+    for (int k = 0; k < MBW; ++k) vb[k] = v + (int64_t)min(wave + W * k, nb - 1) * vs;
+    double acc[MBW], vi[MBW];
     const double dg = a.diag[i];
+    const double v0 = (double)v[i];
 #pragma unroll
-    for (int k = 0; k < MB; ++k) {
-      vi[k] = k < nb ? (double)v[k * a.vstride + i] : 0.0;
-      acc[k] = dg * vi[k];
-    }
-    for (int k = 0; k < a.K; ++k) {
-      const int32_t c = a.col[k * a.ld + i];
-      const double w = (double)a.val[k * a.ld + i];
+    for (int k = 0; k < MBW; ++k) vi[k] = (double)vb[k][i];
 #pragma unroll
-      for (int bb = 0; bb < MB; ++bb)
-        if (bb < nb) acc[bb] += w * (double)v[bb * a.vstride + c];
-    }
-    for (int q = 0; q < a.ntv; ++q) {
-      const double s = a.tv_scale[q];
-      const double d = (double)a.tvd[q * a.ld + i], w = (double)a.tvv[q * a.ld + i];
-      const int32_t c = a.tvc[q * a.ld + i];
+    for (int k = 0; k < MBW; ++k) acc[k] = dg * vi[k];
+    if constexpr (K > 0) {
+      int32_t c[KK];
+      double w[KK];
 #pragma unroll
-      for (int bb = 0; bb < MB; ++bb)
-        if (bb < nb) acc[bb] += s * (d * vi[bb] + w * (double)v[bb * a.vstride + c]);
-    }
-    // dA_p v_0 (block p + 1)
-#pragma unroll
-    for (int bb = 1; bb < MB; ++bb) {
-      if (bb >= nb) break;
-      double add = 0.0;
-      for (int e = a.pstart[bb]; e < a.pstart[bb + 1]; ++e) {
-        const PairMeta m = a.meta[e];
-        if (m.arr >= 0) {
-          const int32_t c = a.dcol[m.cslot * a.ld + i];
-          add += a.dscale[e] * ((double)a.ddiag[m.arr * a.ld + i] * vi[0] + (double)a.dpred[m.arr * a.ld + i] * (double)v[c]);
-        }
-        if (m.q >= 0 && a.sfscale[e] != 0.0) {
-          const int q = m.q;
-          const int32_t c = a.tvc[q * a.ld + i];
-          add += a.sfscale[e] * ((double)a.tvd[q * a.ld + i] * vi[0] + (double)a.tvv[q * a.ld + i] * (double)v[c]);
-        }
+      for (int k = 0; k < K; ++k) {
+        c[k] = a.col[k * ld + i];
+        w[k] = (double)a.val[k * ld + i];
       }
-      acc[bb] += add;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int kk = 0; kk < MBW; ++kk) acc[kk] += w[k] * (double)vb[kk][c[k]];
+      }
+    } else {
+      for (int k = 0; k < a.K; ++k) {
+        const int32_t c = a.col[k * ld + i];
+        const double w = (double)a.val[k * ld + i];
+#pragma unroll
+        for (int kk = 0; kk < MBW; ++kk) acc[kk] += w * (double)vb[kk][c];
+      }
+    }
+    for (int q0 = 0; q0 < a.ntv; q0 += TB) {
+      int32_t c[TB];
+      double d[TB], w[TB], sq[TB];
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+        const int q = min(q0 + u, a.ntv - 1);
+        sq[u] = q0 + u < a.ntv ? a.tv_scale[q] : 0.0;
+        c[u] = a.tvc[q * ld + i];
+        d[u] = (double)a.tvd[q * ld + i];
+        w[u] = (double)a.tvv[q * ld + i];
+      }
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+#pragma unroll
+        for (int kk = 0; kk < MBW; ++kk) acc[kk] += sq[u] * (d[u] * vi[kk] + w[u] * (double)vb[kk][c[u]]);
+      }
+    }
+    // dA_p v_0 into the blocks p + 1 this wave owns
+    const int t1 = a.tstart[wave + 1];
+    for (int t0 = a.tstart[wave]; t0 < t1; t0 += TB) {
+      int32_t c[TB], kk_of[TB];
+      double d[TB], w[TB], g[TB], sc[TB];
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+        const Term T = s_terms[min(t0 + u, t1 - 1)];
+        const bool st = T.kind == 0;
+        sc[u] = t0 + u < t1 ? (st ? a.dscale[T.pair] : a.sfscale[T.pair]) : 0.0;
+        kk_of[u] = T.kk;
+        const int32_t* cp = st ? a.dcol + (int64_t)T.cslot * ld : a.tvc + (int64_t)T.idx * ld;
+        const VT* dp = st ? a.ddiag + (int64_t)T.idx * ld : a.tvd + (int64_t)T.idx * ld;
+        const VT* wp = st ? a.dpred + (int64_t)T.idx * ld : a.tvv + (int64_t)T.idx * ld;
+        c[u] = cp[i];
+        d[u] = (double)dp[i];
+        w[u] = (double)wp[i];
+      }
+#pragma unroll
+      for (int u = 0; u < TB; ++u) g[u] = (double)v[c[u]];
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+        const double add = sc[u] * (d[u] * v0 + w[u] * g[u]);
+#pragma unroll
+        for (int kk = 0; kk < MBW; ++kk) acc[kk] += kk == kk_of[u] ? add : 0.0;
+      }
     }
 #pragma unroll
-    for (int bb = 0; bb < MB; ++bb)
-      if (bb < nb) out[bb * a.ostride + i] = acc[bb];
-  } else {
+    for (int kk = 0; kk < MBW; ++kk) {
+      const int bb = wave + W * kk;
+      if (bb < nb) out[bb * a.ostride + i] = acc[kk];
+    }
+  } else if ((int64_t)blockIdx.x < nsink) {
     // sink row r of block bb: one workgroup, deterministic tree
-    const int sb = (int)((int64_t)blockIdx.x - row_blocks);
+    const int sb = (int)blockIdx.x;
     const int bb = sb / a.nr, r = sb % a.nr;
     const int64_t e0 = a.sink_off[r], e1 = a.sink_off[r + 1];
     const VT* vb = v + bb * a.vstride;
@@ -136,9 +211,9 @@ __global__ __launch_bounds__(kBlock) void k_sens_matvec(SensArgs<VT> a, const VT
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < kBlock / 64; ++w) s += part[w];
-      out[bb * a.ostride + a.n + r] = s;
+      double sum = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) sum += part[w];
+      out[bb * a.ostride + a.n + r] = sum;
     }
   }
 }
@@ -246,6 +321,8 @@ void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, 
   a.dcol = SA->dcol.p;
   a.dsink = reinterpret_cast<const VT*>(SA->dsink.p);
   a.meta = reinterpret_cast<const PairMeta*>(SA->meta_dev.p);
+  a.terms = reinterpret_cast<const Term*>(SA->terms_dev.p);
+  for (int w = 0; w <= kBlock / 64; ++w) a.tstart[w] = SA->tstart[w];
   matrix_scales(A, t, a.sink_scale, a.tv_scale);
   for (int b = 0; b <= SA->P + 1; ++b) a.pstart[b] = SA->pstart[b];
   for (size_t k = 0; k < SA->pairs.size(); ++k) {
@@ -255,8 +332,8 @@ void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, 
                        ? prog_eval(m->dtprog(p.r, p.p), m->dtlen(p.r, p.p), t, nullptr, m->params.data())
                        : 0.0;
   }
-  const int64_t row_blocks = (A->n + kBlock - 1) / kBlock;
-  const unsigned grid = (unsigned)(row_blocks + (int64_t)a.nb * A->nr);
+  const int64_t row_tiles = (A->n + 63) / 64;
+  const unsigned grid = (unsigned)(row_tiles + (((int64_t)a.nb * A->nr + 7) & ~int64_t(7)));
   const VT* vv = reinterpret_cast<const VT*>(v);
   auto go = [&](auto kern) {
     if (e0)
@@ -264,16 +341,29 @@ void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, 
     else
       kern<<<grid, kBlock, 0, s>>>(a, vv, out);
   };
-  if (a.nb <= 2)
-    go(k_sens_matvec<VT, 2>);
-  else if (a.nb <= 4)
-    go(k_sens_matvec<VT, 4>);
+  static_assert(kMaxSensBlocks <= 8 * (kBlock / 64), "blocks per wave");
+  auto by_k = [&](auto mbw) {
+    constexpr int M = decltype(mbw)::value;
+    switch (a.K) {
+      case 1: go(k_sens_matvec<VT, M, 1>); break;
+      case 2: go(k_sens_matvec<VT, M, 2>); break;
+      case 3: go(k_sens_matvec<VT, M, 3>); break;
+      case 4: go(k_sens_matvec<VT, M, 4>); break;
+      case 5: go(k_sens_matvec<VT, M, 5>); break;
+      case 6: go(k_sens_matvec<VT, M, 6>); break;
+      case 7: go(k_sens_matvec<VT, M, 7>); break;
+      case 8: go(k_sens_matvec<VT, M, 8>); break;
+      default: go(k_sens_matvec<VT, M, 0>); break;
+    }
+  };
+  if (a.nb <= 4)
+    by_k(std::integral_constant<int, 1>{});
   else if (a.nb <= 8)
-    go(k_sens_matvec<VT, 8>);
+    by_k(std::integral_constant<int, 2>{});
   else if (a.nb <= 16)
-    go(k_sens_matvec<VT, 16>);
+    by_k(std::integral_constant<int, 4>{});
   else
-    go(k_sens_matvec<VT, 32>);
+    by_k(std::integral_constant<int, 8>{});
   NC_CHECK_HIP(hipGetLastError());
 }
 
@@ -361,6 +451,40 @@ nc_sensmatrix_s* sensmatrix_create(nc_space_s* sp, nc_model_s* m, int precision)
   hipStream_t s = A->stream.s;
   SA->meta_dev.alloc(meta.size() * sizeof(PairMeta));
   NC_CHECK_HIP(hipMemcpyAsync(SA->meta_dev.p, meta.data(), meta.size() * sizeof(PairMeta), hipMemcpyHostToDevice, s));
+  // the row part's derivative terms, grouped by the wave that owns the pair's block
+  // (block b = p + 1 belongs to wave b % 4, accumulator b / 4)
+  {
+    constexpr int W = kBlock / 64;
+    std::vector<Term> terms;
+    for (int w = 0; w < W; ++w) {
+      SA->tstart[w] = (int32_t)terms.size();
+      for (size_t k = 0; k < SA->pairs.size(); ++k) {
+        const SensPair& p = SA->pairs[k];
+        const int bb = p.p + 1;
+        if (bb % W != w) continue;
+        Term T{};
+        T.kk = (int8_t)(bb / W);
+        T.pair = (int16_t)k;
+        if (p.arr >= 0) {
+          T.kind = 0;
+          T.idx = (int16_t)p.arr;
+          T.cslot = (int16_t)p.cslot;
+          terms.push_back(T);
+        }
+        if (p.q >= 0) {
+          T.kind = 1;
+          T.idx = (int16_t)p.q;
+          T.cslot = 0;
+          terms.push_back(T);
+        }
+      }
+    }
+    SA->tstart[W] = (int32_t)terms.size();
+    NC_REQUIRE((int)terms.size() <= kMaxTerms, NC_ERR_UNSUPPORTED, "too many derivative terms");
+    if (terms.empty()) terms.push_back(Term{0, 0, -1, 0, 0});
+    SA->terms_dev.alloc(terms.size() * sizeof(Term));
+    NC_CHECK_HIP(hipMemcpyAsync(SA->terms_dev.p, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice, s));
+  }
   if (SA->has_joint && sp->n) {
     SA->states.alloc((size_t)sp->n * sp->ns);
     SA->state_conn.alloc((size_t)sp->n * sp->nr);
