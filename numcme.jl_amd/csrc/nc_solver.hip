@@ -35,6 +35,7 @@ enum Finish : int { FIN_NONE = 0, FIN_ARN, FIN_LAST, FIN_BETA, FIN_SUMS };
 constexpr double kBreakdownTol = 1e-7;   // Expokit's happy-breakdown threshold
 constexpr int kFinBlock = 1024;  // threads of the one-block partial-sum fold
 constexpr unsigned kVecBlocks = 512;  // grid of the vector passes
+constexpr int kLoadChunk = 8;  // basis vectors whose loads a vector pass issues together
 constexpr int kJacobiSweeps = 3;  // G h = a with G = I + O(1e-7): the error falls 1e-6 per sweep
 
 // kbuf layout (doubles)
@@ -59,7 +60,7 @@ struct VecArgs {
   const double* sc;       // per-vector scale (device, may be null = 1)
   const double* coef;     // per-vector coefficient (device, may be null)
   double csign;           // multiplies coef
-  const double* xin;      // FP64 input (may be null = 0)
+  const double* xin;      // FP64 input
   const double* xscale;   // device scalar multiplying xin (may be null = 1)
   double* xout;           // FP64 outputs (nullable)
   void* vout;             // VT output (nullable)
@@ -130,30 +131,45 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += stride) {
     const int64_t n0 = g * R;
     double x[R];
-    if (a.xin) {
-      const Pack<double, R> xi = *reinterpret_cast<const Pack<double, R>*>(a.xin + n0);
-#pragma unroll
-      for (int r = 0; r < R; ++r) x[r] = xs * xi.v[r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < R; ++r) x[r] = 0.0;
-    }
+    const Pack<double, R> xi = *reinterpret_cast<const Pack<double, R>*>(a.xin + n0);
     Pack<BT, R> vl;
     if constexpr (DUAL) vl = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)(nv - 1) * a.N + n0);
+    if constexpr (MAXV == 0) {
 #pragma unroll
-    for (int i = 0; i < MAXV; ++i) {
-      if (i < nv) {
-        const Pack<BT, R> vv = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)i * a.N + n0);
+      for (int r = 0; r < R; ++r) x[r] = xs * xi.v[r];
+    }
+    // The basis is read in chunks of kLoadChunk vectors whose loads are issued together
+    // (unconditional, index clamped to nv - 1: a guarded load becomes a branch that
+    // waits for it) and kept ahead of their arithmetic by a scheduling barrier; left to
+    // itself the scheduler reuses one register set and waits after every load, which
+    // streams the basis one vector at a time. Vectors beyond nv carry a zero coefficient
+    // (update) or land in dots that are never stored.
+#pragma unroll
+    for (int i0 = 0; i0 < MAXV; i0 += kLoadChunk) {
+      Pack<BT, R> vv[kLoadChunk];
+#pragma unroll
+      for (int u = 0; u < kLoadChunk; ++u)
+        if (i0 + u < MAXV)
+          vv[u] = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)min(i0 + u, nv - 1) * a.N + n0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (i0 == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[r] = xs * xi.v[r];
+      }
+#pragma unroll
+      for (int u = 0; u < kLoadChunk; ++u) {
+        const int i = i0 + u;
+        if (i >= MAXV) break;
         if constexpr (UPD) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) x[r] += s_cs[i] * (double)vv.v[r];
+          for (int r = 0; r < R; ++r) x[r] += s_cs[i] * (double)vv[u].v[r];
         } else {
 #pragma unroll
-          for (int r = 0; r < R; ++r) dacc[i] += (double)vv.v[r] * x[r];
+          for (int r = 0; r < R; ++r) dacc[i] += (double)vv[u].v[r] * x[r];
         }
         if constexpr (DUAL) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) gacc[i] += (double)vv.v[r] * (double)vl.v[r];
+          for (int r = 0; r < R; ++r) gacc[i] += (double)vv[u].v[r] * (double)vl.v[r];
         }
       }
     }
@@ -231,20 +247,22 @@ __global__ __launch_bounds__(kBlock) void k_vec_dual(VecArgs<BT> a) {
   for (int64_t g = (int64_t)blockIdx.x * 64 + lane; g < groups; g += stride) {
     const int64_t n0 = g * R;
     const Pack<double, R> xi = *reinterpret_cast<const Pack<double, R>*>(a.xin + n0);
+    const Pack<BT, R> vl = *reinterpret_cast<const Pack<BT, R>*>(Vl + n0);
+    // all of this wave's basis loads issued together (unconditional, clamped; see k_vec)
+    Pack<BT, R> vv[MVW];
+#pragma unroll
+    for (int k = 0; k < MVW; ++k)
+      vv[k] = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)min(k * W + wave, nv - 1) * a.N + n0);
+    __builtin_amdgcn_sched_barrier(0);
     double x[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = xs * xi.v[r];
-    const Pack<BT, R> vl = *reinterpret_cast<const Pack<BT, R>*>(Vl + n0);
 #pragma unroll
     for (int k = 0; k < MVW; ++k) {
-      const int i = k * W + wave;
-      if (i < nv) {
-        const Pack<BT, R> vv = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)i * a.N + n0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          dacc[k] += (double)vv.v[r] * x[r];
-          gacc[k] += (double)vv.v[r] * (double)vl.v[r];
-        }
+      for (int r = 0; r < R; ++r) {
+        dacc[k] += (double)vv[k].v[r] * x[r];
+        gacc[k] += (double)vv[k].v[r] * (double)vl.v[r];
       }
     }
     if (wave == 0) {
@@ -403,10 +421,16 @@ void launch_vec_mode(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
     k_vec<BT, 4, MODE><<<grid, kBlock, 0, s>>>(a);
   else if (nv <= 8)
     k_vec<BT, 8, MODE><<<grid, kBlock, 0, s>>>(a);
+  else if (nv <= 12)
+    k_vec<BT, 12, MODE><<<grid, kBlock, 0, s>>>(a);
   else if (nv <= 16)
     k_vec<BT, 16, MODE><<<grid, kBlock, 0, s>>>(a);
+  else if (nv <= 20)
+    k_vec<BT, 20, MODE><<<grid, kBlock, 0, s>>>(a);
   else if (nv <= 24)
     k_vec<BT, 24, MODE><<<grid, kBlock, 0, s>>>(a);
+  else if (nv <= 28)
+    k_vec<BT, 28, MODE><<<grid, kBlock, 0, s>>>(a);
   else if (nv <= 32)
     k_vec<BT, 32, MODE><<<grid, kBlock, 0, s>>>(a);
   else
@@ -416,6 +440,7 @@ void launch_vec_mode(const VecArgs<BT>& a, unsigned grid, hipStream_t s) {
 template <typename BT>
 void launch_vec(VecArgs<BT> a, const nc_solver_s* S, hipStream_t s) {
   if (a.N % 4) throw Error(NC_ERR_INTERNAL, "vector passes need a padded length");
+  if (!a.xin) throw Error(NC_ERR_INTERNAL, "vector passes need an input vector");
   const unsigned grid = S->red_blocks;
   a.tot = S->fold.p;
   a.counter = reinterpret_cast<unsigned*>(S->fold.p + kPartStrideMax);
@@ -427,10 +452,16 @@ void launch_vec(VecArgs<BT> a, const nc_solver_s* S, hipStream_t s) {
       if (a.nv > 4 && a.nv <= 32 && !a.xout && !a.vout && a.finish != FIN_NONE) {
         if (a.nv <= 8)
           k_vec_dual<BT, 8><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 12)
+          k_vec_dual<BT, 12><<<grid, kBlock, 0, s>>>(a);
         else if (a.nv <= 16)
           k_vec_dual<BT, 16><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 20)
+          k_vec_dual<BT, 20><<<grid, kBlock, 0, s>>>(a);
         else if (a.nv <= 24)
           k_vec_dual<BT, 24><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 28)
+          k_vec_dual<BT, 28><<<grid, kBlock, 0, s>>>(a);
         else
           k_vec_dual<BT, 32><<<grid, kBlock, 0, s>>>(a);
       } else {
