@@ -184,8 +184,9 @@ int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms)
 /* Mat-vec launch shape: consecutive state rows per thread (1, 2 or 4). No reference
  * counterpart (the reference's CSC mul! has no launch shape). */
 int nc_matrix_set_rows_per_thread(nc_matrix A, int32_t rows);
-/* Time every launch shape on this matrix (`reps` launches each) and keep the fastest;
- * *best_ms receives its mean duration. No reference counterpart. */
+/* Time every launch shape on this matrix (`reps` launches each, every one after a read of
+ * 512 MB that evicts the matrix from the caches, as a solve's vector passes do) and keep
+ * the fastest; *best_ms receives its mean kernel duration. No reference counterpart. */
 int nc_matrix_autotune(nc_matrix A, int32_t reps, double* best_ms);
 
 /* ======================================================================================

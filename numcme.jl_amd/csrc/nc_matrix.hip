@@ -272,7 +272,7 @@ __device__ inline double wave_sum(double v) {
 
 // Thread t owns state rows [R*t, R*t+R). Matrix streams are read with one R-wide vector
 // load per slot; v[i] and the predecessor gathers are scalar.
-template <typename VT, typename OT, int KT, int R>
+template <typename VT, typename OT, int KT, int R, int DM>
 __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __restrict__ v, OT* __restrict__ out,
                                                    int accumulate) {
   const int64_t row_threads = (a.n + R - 1) / R;
@@ -285,19 +285,26 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
     const int64_t t = lb * kBlock + threadIdx.x;
     const int64_t i0 = t * R;
     if (i0 >= a.n) return;
+    // Loads are unconditional (clamped indices, selects) and issued in two batches: the
+    // row's own entries and every slot's column and value, then every gather. A load
+    // under a branch is waited for at the end of its block, which would make the slots
+    // one memory round trip each.
     const Pack<double, R> dg = ldpack<double, R>(a.diag + i0);
     double vi[R], acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      vi[r] = (i0 + r < a.n) ? (double)v[i0 + r] : 0.0;
-      acc[r] = dg.v[r] * vi[r];
+      const double vr = (double)v[min(i0 + r, a.n - 1)];
+      vi[r] = i0 + r < a.n ? vr : 0.0;
     }
     const int K = KT > 0 ? KT : a.K;
     // column of slot k for rows i0..i0+R-1: loaded, or i + dia_off[k] clamped to the
-    // states (rows whose value is 0 may point anywhere; they add nothing)
+    // states (rows whose value is 0 may point anywhere; they add nothing). DM fixes the
+    // kind for every slot (0 loaded, 1 computed) so the fixed-K kernels carry no branch;
+    // a matrix with both kinds uses the generic kernel (DM 2), which tests each slot.
     auto slot_cols = [&](int k) {
       Pack<int32_t, R> c;
-      if ((a.dia_mask >> k) & 1u) {
+      const bool dia = DM == 1 || (DM == 2 && ((a.dia_mask >> k) & 1u));
+      if (dia) {
         const int64_t hi = a.n - 1;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -309,16 +316,34 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
       }
       return c;
     };
+    if constexpr (KT > 0) {
+      // values and columns (computed columns need no load), then every gather; with
+      // computed columns the gathers go out with the values, one round trip in all
+      Pack<int32_t, R> c[KT];
+      Pack<VT, R> w[KT];
 #pragma unroll
-    for (int k = 0; k < (KT > 0 ? KT : 8); ++k) {
-      if (KT == 0 && k >= K) break;
-      const Pack<int32_t, R> c = slot_cols(k);
-      const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
+      for (int k = 0; k < KT; ++k) {
+        w[k] = ldpack<VT, R>(a.val + k * a.ld + i0);
+        c[k] = slot_cols(k);
+      }
+      double gv[KT][R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
-    }
-    if (KT == 0) {
-      for (int k = 8; k < K; ++k) {
+      for (int k = 0; k < KT; ++k) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) gv[k][r] = (double)v[c[k].v[r]];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * vi[r];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += (double)w[k].v[r] * gv[k][r];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * vi[r];
+      for (int k = 0; k < K; ++k) {
         const Pack<int32_t, R> c = slot_cols(k);
         const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
 #pragma unroll
@@ -389,42 +414,55 @@ struct LaunchCfg {
   const int32_t* chunk_first;
 };
 
-template <typename VT, typename OT, int K, int R>
+template <typename VT, typename OT, int K, int R, int DM>
 void launch_one(const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
   if (!a.n_chunks) {
     if (c.e0)
-      hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo, acc);
+      hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo, acc);
     else
-      k_matvec<VT, OT, K, R><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+      k_matvec<VT, OT, K, R, DM><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
     return;
   }
   // chunked sinks: the events bracket both kernels (start of the first, end of the fold)
   SinkScale sc;
   for (int r = 0; r < a.nr; ++r) sc.v[r] = a.sink_scale[r];
   if (c.e0) {
-    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, nullptr, 0, a, vv, oo,
+    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, nullptr, 0, a, vv, oo,
                           acc);
     hipExtLaunchKernelGGL(k_sink_fold<OT>, dim3(1), dim3(64), 0, c.s, nullptr, c.e1, 0, a.nr, c.chunk_first,
                           (const double*)a.chunk_part, sc, a.n, oo, acc);
   } else {
-    k_matvec<VT, OT, K, R><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+    k_matvec<VT, OT, K, R, DM><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
     k_sink_fold<OT><<<1, 64, 0, c.s>>>(a.nr, c.chunk_first, a.chunk_part, sc, a.n, oo, acc);
   }
 }
 
+template <typename VT, typename OT, int R, int DM>
+void dispatch_kd(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
+  switch (K) {
+    case 1: launch_one<VT, OT, 1, R, DM>(c, a, vv, oo, acc); break;
+    case 2: launch_one<VT, OT, 2, R, DM>(c, a, vv, oo, acc); break;
+    case 3: launch_one<VT, OT, 3, R, DM>(c, a, vv, oo, acc); break;
+    case 4: launch_one<VT, OT, 4, R, DM>(c, a, vv, oo, acc); break;
+    case 5: launch_one<VT, OT, 5, R, DM>(c, a, vv, oo, acc); break;
+    case 6: launch_one<VT, OT, 6, R, DM>(c, a, vv, oo, acc); break;
+    case 7: launch_one<VT, OT, 7, R, DM>(c, a, vv, oo, acc); break;
+    case 8: launch_one<VT, OT, 8, R, DM>(c, a, vv, oo, acc); break;
+    default: launch_one<VT, OT, 0, R, 2>(c, a, vv, oo, acc); break;
+  }
+}
+
+// Column mode: every slot loaded (0) or computed (1) takes the fixed-K kernels; a mix of
+// the two takes the generic kernel, which decides per slot.
 template <typename VT, typename OT, int R>
 void dispatch_k(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
-  switch (K) {
-    case 1: launch_one<VT, OT, 1, R>(c, a, vv, oo, acc); break;
-    case 2: launch_one<VT, OT, 2, R>(c, a, vv, oo, acc); break;
-    case 3: launch_one<VT, OT, 3, R>(c, a, vv, oo, acc); break;
-    case 4: launch_one<VT, OT, 4, R>(c, a, vv, oo, acc); break;
-    case 5: launch_one<VT, OT, 5, R>(c, a, vv, oo, acc); break;
-    case 6: launch_one<VT, OT, 6, R>(c, a, vv, oo, acc); break;
-    case 7: launch_one<VT, OT, 7, R>(c, a, vv, oo, acc); break;
-    case 8: launch_one<VT, OT, 8, R>(c, a, vv, oo, acc); break;
-    default: launch_one<VT, OT, 0, R>(c, a, vv, oo, acc); break;
-  }
+  const uint32_t all = K >= 32 ? 0xffffffffu : ((1u << K) - 1u);
+  if (a.dia_mask == 0u)
+    dispatch_kd<VT, OT, R, 0>(K, c, a, vv, oo, acc);
+  else if ((a.dia_mask & all) == all)
+    dispatch_kd<VT, OT, R, 1>(K, c, a, vv, oo, acc);
+  else
+    launch_one<VT, OT, 0, R, 2>(c, a, vv, oo, acc);
 }
 
 template <typename VT, typename OT>
@@ -798,6 +836,56 @@ double bench_matvec(nc_matrix_s* A, double t, int reps) {
   return ms / reps;
 }
 
+// Streams n 16-byte words (reads only, so that no dirty lines are left behind) to evict
+// whatever the caches hold; the xor is stored only if it hits an arbitrary constant, which
+// keeps the loads alive.
+__global__ void k_cache_flush(const uint4* p, int64_t n, unsigned* sink) {
+  unsigned acc = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint4 w = p[i];
+    acc ^= w.x ^ w.y ^ w.z ^ w.w;
+  }
+  if (acc == 0x9e3779b9u) sink[0] = acc;
+}
+
+// Mean kernel time of the mat-vec with cold caches: before every timed launch a read of
+// twice the 256 MB Infinity Cache evicts the matrix and its vectors, as the basis streams
+// of a Krylov step do between its mat-vecs. Durations come from events stamped by the
+// dispatch itself, so the flushes are not counted.
+double bench_matvec_cold(nc_matrix_s* A, double t, int reps) {
+  constexpr int64_t kFlushWords = (int64_t(512) << 20) / 16;
+  hipStream_t s = A->stream.s;
+  const int64_t N = A->n_rows;
+  DevBuf<uint4> fl(kFlushWords);
+  DevBuf<unsigned> sink(1);
+  DevBuf<unsigned char> v(N * A->vsize), o(N * A->vsize);
+  NC_CHECK_HIP(hipMemsetAsync(fl.p, 0, kFlushWords * 16, s));
+  NC_CHECK_HIP(hipMemsetAsync(v.p, 0, N * A->vsize, s));
+  NC_CHECK_HIP(hipMemsetAsync(o.p, 0, N * A->vsize, s));
+  matvec_harvest_timing(A);
+  const bool was = A->timing;
+  const double ms0 = A->timed_ms;
+  const int64_t n0 = A->timed_count;
+  A->timing = false;
+  matvec_launch(A, t, v.p, o.p, false, s);  // warm-up (and joint refresh)
+  A->timing = true;
+  A->timed_ms = 0.0;
+  A->timed_count = 0;
+  for (int k = 0; k < reps; ++k) {
+    k_cache_flush<<<4096, 256, 0, s>>>(fl.p, kFlushWords, sink.p);
+    matvec_launch(A, t, (k & 1) ? o.p : v.p, (k & 1) ? v.p : o.p, false, s);
+  }
+  NC_CHECK_HIP(hipGetLastError());
+  matvec_harvest_timing(A);
+  const double ms = A->timed_count ? A->timed_ms / (double)A->timed_count : 0.0;
+  A->timing = was;
+  A->timed_ms = ms0;
+  A->timed_count = n0;
+  NC_CHECK_HIP(hipStreamSynchronize(s));
+  return ms;
+}
+
 }  // namespace
 
 namespace nc {
@@ -891,8 +979,7 @@ int nc_matrix_autotune(nc_matrix A, int32_t reps, double* best_ms) {
     int best_r = A->rows_per_thread;
     for (int r : {1, 2, 4}) {
       A->rows_per_thread = r;
-      bench_matvec(A, 0.0, 2);
-      const double ms = bench_matvec(A, 0.0, reps);
+      const double ms = bench_matvec_cold(A, 0.0, reps);
       if (ms < best) best = ms, best_r = r;
     }
     A->rows_per_thread = best_r;

@@ -497,17 +497,45 @@ __device__ __host__ inline int rk_slot(int j, int sw) { return (sw && (j == 0 ||
 
 // Stage input y_i = y + h * sum_{j<i} a_ij k_j ; writes the mat-vec copy, and at the last
 // stage (i = 6, whose input is y_{n+1}) also the FP64 value (ytmp), the only stage input
-// the error estimate and the accept read.
+// the error estimate and the accept read. The stage is a template argument so that its
+// STAGE loads of k are issued together (a runtime trip count would wait for each before
+// the next); two elements per thread.
+template <typename VT, int STAGE>
+__global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, double h, const double* y, const double* k, int sw,
+                                                     double* ytmp, VT* vin) {
+  constexpr int R = 2;
+  const int64_t stride = (int64_t)gridDim.x * kBlock * R;
+  for (int64_t n = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * R; n < N; n += stride) {
+    Pack<double, R> kv[STAGE];
+#pragma unroll
+    for (int j = 0; j < STAGE; ++j) kv[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)rk_slot(j, sw) * N + n);
+    const Pack<double, R> yv = *reinterpret_cast<const Pack<double, R>*>(y + n);
+    Pack<double, R> o;
+    Pack<VT, R> ov;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < STAGE; ++j) acc += c_dp_a[STAGE][j] * kv[j].v[r];
+      o.v[r] = yv.v[r] + h * acc;
+      ov.v[r] = (VT)o.v[r];
+    }
+    if constexpr (STAGE == 6) *reinterpret_cast<Pack<double, R>*>(ytmp + n) = o;
+    *reinterpret_cast<Pack<VT, R>*>(vin + n) = ov;
+  }
+}
+
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, int stage, double h, const double* y, const double* k,
-                                                     int sw, double* ytmp, VT* vin) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) {
-    double acc = 0.0;
-    for (int j = 0; j < stage; ++j) acc += c_dp_a[stage][j] * k[(int64_t)rk_slot(j, sw) * N + n];
-    const double v = y[n] + h * acc;
-    if (stage == 6) ytmp[n] = v;
-    vin[n] = (VT)v;
+void launch_rk_stage(int stage, unsigned grid, hipStream_t s, int64_t N, double h, const double* y, const double* k,
+                     int sw, double* ytmp, VT* vin) {
+  switch (stage) {
+    case 1: k_rk_stage<VT, 1><<<grid, kBlock, 0, s>>>(N, h, y, k, sw, ytmp, vin); break;
+    case 2: k_rk_stage<VT, 2><<<grid, kBlock, 0, s>>>(N, h, y, k, sw, ytmp, vin); break;
+    case 3: k_rk_stage<VT, 3><<<grid, kBlock, 0, s>>>(N, h, y, k, sw, ytmp, vin); break;
+    case 4: k_rk_stage<VT, 4><<<grid, kBlock, 0, s>>>(N, h, y, k, sw, ytmp, vin); break;
+    case 5: k_rk_stage<VT, 5><<<grid, kBlock, 0, s>>>(N, h, y, k, sw, ytmp, vin); break;
+    case 6: k_rk_stage<VT, 6><<<grid, kBlock, 0, s>>>(N, h, y, k, sw, ytmp, vin); break;
+    default: throw Error(NC_ERR_INTERNAL, "bad RK stage");
   }
 }
 
@@ -592,10 +620,10 @@ double round_step(double x) {
 
 void solver_reduce_setup(nc_solver_s* S, int nvals) {
   (void)nvals;
-  // Grid of the vector passes: 2048 blocks of 256 threads for a Krylov basis stored in
-  // FP32 and 512 otherwise measured best on the C2 solve (more blocks hide the latency
-  // of the basis streams; beyond that the one-block fold of the partials dominates)
-  const unsigned vb = (S->method == NC_METHOD_KRYLOV && S->vs == 4) ? (unsigned)kMaxRedBlocks : kVecBlocks;
+  // Grid of the vector passes: 1024 blocks of 256 threads for a Krylov basis stored in
+  // FP32 and 512 otherwise (on the 1024^2 toggle-switch solve, 256 to 4096 blocks are
+  // within 1% of each other since the passes issue their basis loads in batches)
+  const unsigned vb = (S->method == NC_METHOD_KRYLOV && S->vs == 4) ? 1024u : kVecBlocks;
   S->red_blocks = std::min<unsigned>(vb, grid_for(S->Np / 4, kBlock));
   // the RK error pass puts 3 R gathered sink values behind its partials
   S->partials.reserve(std::max<size_t>((size_t)S->red_blocks * kPartStrideMax, S->red_blocks + 3 * kMaxReactions));
@@ -933,6 +961,7 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
   const bool fp32 = A->precision == NC_FP32;
   const double rtol = S->opts.rtol, atol = S->opts.atol;
   const unsigned g = std::min<unsigned>(4096, grid_for(N, kBlock));  // elementwise stage kernels
+  const unsigned g2 = std::min<unsigned>(4096, grid_for(N / 2, kBlock));  // two elements per thread
   double* k = S->k.p;
   auto kp = [&](int j) { return k + (int64_t)rk_slot(j, S->ksw) * N; };
   auto to_vin = [&](const double* src) {
@@ -944,11 +973,9 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
   auto stages = [&](double h) {
     for (int stage = 1; stage < 7; ++stage) {
       if (fp32)
-        k_rk_stage<float><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ksw, S->ytmp.p,
-                                                reinterpret_cast<float*>(S->rkv.p));
+        launch_rk_stage<float>(stage, g2, s, N, h, S->u.p, k, S->ksw, S->ytmp.p, reinterpret_cast<float*>(S->rkv.p));
       else
-        k_rk_stage<double><<<g, kBlock, 0, s>>>(N, stage, h, S->u.p, k, S->ksw, S->ytmp.p,
-                                                 reinterpret_cast<double*>(S->rkv.p));
+        launch_rk_stage<double>(stage, g2, s, N, h, S->u.p, k, S->ksw, S->ytmp.p, reinterpret_cast<double*>(S->rkv.p));
       NC_CHECK_HIP(hipGetLastError());
       op_apply(S, S->t + h_dp_c[stage] * h, S->rkv.p, kp(stage));
       S->st.matvecs++;
