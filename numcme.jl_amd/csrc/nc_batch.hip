@@ -63,20 +63,23 @@ __device__ inline int64_t first_row() {
 
 // y_s = u + h_i sum_{j<s} a_sj k_j over the instance's rows: its VT copy, and at s = 6
 // (y_{n+1}, read by the error partials and the accept) its FP64 value
-template <typename VT, int R>
-__global__ __launch_bounds__(kBlock) void k_bstage(Dev d, int64_t ld, int stage, const double* u, const double* k,
+// (the stage is a template argument so that its STAGE loads of k go out together)
+template <typename VT, int R, int STAGE>
+__global__ __launch_bounds__(kBlock) void k_bstage(Dev d, int64_t ld, const double* u, const double* k,
                                                    double* ytmp, VT* vin) {
   const int64_t n0 = first_row<R>();
   if (n0 >= ld) return;
   const double h = d.h[d.gran_inst[n0 / kGran]];
   if (h == 0.0) return;
   Pack<double, R> y = *reinterpret_cast<const Pack<double, R>*>(u + n0);
-  double acc[R] = {};
-  for (int j = 0; j < stage; ++j) {
-    const double a = c_a[stage][j];
-    const Pack<double, R> kj = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
+  Pack<double, R> kj[STAGE > 0 ? STAGE : 1];
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] += a * kj.v[r];
+  for (int j = 0; j < STAGE; ++j) kj[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
+  double acc[R] = {};
+#pragma unroll
+  for (int j = 0; j < STAGE; ++j) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] += c_a[STAGE][j] * kj[j].v[r];
   }
   Pack<VT, R> o;
 #pragma unroll
@@ -84,12 +87,14 @@ __global__ __launch_bounds__(kBlock) void k_bstage(Dev d, int64_t ld, int stage,
     y.v[r] += h * acc[r];
     o.v[r] = (VT)y.v[r];
   }
-  if (stage == 6) *reinterpret_cast<Pack<double, R>*>(ytmp + n0) = y;
+  if constexpr (STAGE == 6) *reinterpret_cast<Pack<double, R>*>(ytmp + n0) = y;
   *reinterpret_cast<Pack<VT, R>*>(vin + n0) = o;
 }
 
-// out = A v over the state rows of active instances (R rows per thread)
-template <typename VT, int R>
+// out = A v over the state rows of active instances (R rows per thread). With the slot
+// count KT as a template argument every slot's column and value load goes out in one
+// batch and every gather in a second (KT = 0: runtime count K, one slot at a time).
+template <typename VT, int R, int KT>
 __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, const double* diag, const int32_t* col,
                                                     const VT* val, const VT* v, double* out) {
   const int64_t n0 = first_row<R>();
@@ -101,13 +106,37 @@ __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, co
   const Pack<double, R> dg = *reinterpret_cast<const Pack<double, R>*>(diag + n0);
   const Pack<VT, R> vi = *reinterpret_cast<const Pack<VT, R>*>(v + n0);
   double acc[R];
+  if constexpr (KT > 0) {
+    Pack<int32_t, R> c[KT];
+    Pack<VT, R> w[KT];
 #pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
-  for (int s = 0; s < K; ++s) {
-    const Pack<int32_t, R> c = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
-    const Pack<VT, R> w = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+    for (int s = 0; s < KT; ++s) {
+      c[s] = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
+      w[s] = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+    }
+    double gv[KT][R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
+    for (int s = 0; s < KT; ++s) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) gv[s][r] = (double)v[c[s].v[r]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
+#pragma unroll
+    for (int s = 0; s < KT; ++s) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] += (double)w[s].v[r] * gv[s][r];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
+    for (int s = 0; s < K; ++s) {
+      const Pack<int32_t, R> c = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
+      const Pack<VT, R> w = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
+    }
   }
 #pragma unroll
   for (int r = 0; r < R; ++r)
@@ -398,12 +427,43 @@ template <typename VT, int R>
 void launch_matvec(nc_batch_s* B, const VT* v, double* out) {
   const Dev d = dev_view(B);
   hipStream_t s = B->stream.s;
-  k_bmatvec<VT, R><<<row_grid(B, R), kBlock, 0, s>>>(d, B->ld, B->K, B->diag.p, B->col.p,
-                                                     reinterpret_cast<const VT*>(B->val.p), v, out);
+  auto go = [&](auto kern) {
+    kern<<<row_grid(B, R), kBlock, 0, s>>>(d, B->ld, B->K, B->diag.p, B->col.p, reinterpret_cast<const VT*>(B->val.p),
+                                           v, out);
+  };
+  switch (B->K) {
+    case 1: go(k_bmatvec<VT, R, 1>); break;
+    case 2: go(k_bmatvec<VT, R, 2>); break;
+    case 3: go(k_bmatvec<VT, R, 3>); break;
+    case 4: go(k_bmatvec<VT, R, 4>); break;
+    case 5: go(k_bmatvec<VT, R, 5>); break;
+    case 6: go(k_bmatvec<VT, R, 6>); break;
+    case 7: go(k_bmatvec<VT, R, 7>); break;
+    case 8: go(k_bmatvec<VT, R, 8>); break;
+    default: go(k_bmatvec<VT, R, 0>); break;
+  }
   if (B->nseg)
     k_bsinks<VT><<<(unsigned)B->nseg, kBlock, 0, s>>>(d, B->seg_inst.p, B->seg_row.p, B->seg_off.p, B->sink_src.p,
                                                       reinterpret_cast<const VT*>(B->sink_val.p), v, out);
   NC_CHECK_HIP(hipGetLastError());
+}
+
+template <typename VT, int R>
+void launch_stage(nc_batch_s* B, int stage, unsigned g) {
+  const Dev d = dev_view(B);
+  hipStream_t s = B->stream.s;
+  VT* vin = reinterpret_cast<VT*>(B->vin.p);
+  auto go = [&](auto kern) { kern<<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->k.p, B->ytmp.p, vin); };
+  switch (stage) {
+    case 0: go(k_bstage<VT, R, 0>); break;
+    case 1: go(k_bstage<VT, R, 1>); break;
+    case 2: go(k_bstage<VT, R, 2>); break;
+    case 3: go(k_bstage<VT, R, 3>); break;
+    case 4: go(k_bstage<VT, R, 4>); break;
+    case 5: go(k_bstage<VT, R, 5>); break;
+    case 6: go(k_bstage<VT, R, 6>); break;
+    default: throw Error(NC_ERR_INTERNAL, "bad RK stage");
+  }
 }
 
 // accept/snapshot of the previous step, then one full DP5 attempt and the error fold
@@ -414,7 +474,7 @@ void iteration_r(nc_batch_s* B, double atol, double rtol) {
   const unsigned g = row_grid(B, R);
   k_baccept<R><<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
   for (int st = 1; st < 7; ++st) {
-    k_bstage<VT, R><<<g, kBlock, 0, s>>>(d, B->ld, st, B->u.p, B->k.p, B->ytmp.p, reinterpret_cast<VT*>(B->vin.p));
+    launch_stage<VT, R>(B, st, g);
     launch_matvec<VT, R>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p + (int64_t)st * B->ld);
   }
   k_berr<R><<<g, kBlock, 0, s>>>(d, B->ld, B->grans, atol, rtol, B->u.p, B->ytmp.p, B->k.p, B->partials.p);
@@ -436,8 +496,8 @@ void iteration(nc_batch_s* B, double atol, double rtol) {
 template <typename VT, int R>
 void first_stage_r(nc_batch_s* B) {
   hipStream_t s = B->stream.s;
-  k_bstage<VT, R><<<row_grid(B, R), kBlock, 0, s>>>(dev_view(B), B->ld, 0, B->u.p, B->k.p, B->ytmp.p,
-                                                    reinterpret_cast<VT*>(B->vin.p));
+  (void)s;
+  launch_stage<VT, R>(B, 0, row_grid(B, R));
   launch_matvec<VT, R>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p);
 }
 
