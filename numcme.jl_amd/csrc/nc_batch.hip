@@ -72,12 +72,15 @@ __global__ __launch_bounds__(kBlock) void k_bstage(Dev d, int64_t ld, const doub
   const double h = d.h[d.gran_inst[n0 / kGran]];
   if (h == 0.0) return;
   Pack<double, R> y = *reinterpret_cast<const Pack<double, R>*>(u + n0);
+  // a_61 = 0: the last stage does not read k1
   Pack<double, R> kj[STAGE > 0 ? STAGE : 1];
 #pragma unroll
-  for (int j = 0; j < STAGE; ++j) kj[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
+  for (int j = 0; j < STAGE; ++j)
+    if (!(STAGE == 6 && j == 1)) kj[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
   double acc[R] = {};
 #pragma unroll
   for (int j = 0; j < STAGE; ++j) {
+    if (STAGE == 6 && j == 1) continue;
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] += c_a[STAGE][j] * kj[j].v[r];
   }
@@ -171,11 +174,17 @@ __global__ __launch_bounds__(kBlock) void k_berr(Dev d, int64_t ld, int64_t gran
     const Pack<double, R> uu = *reinterpret_cast<const Pack<double, R>*>(u + n0);
     const Pack<double, R> yy = *reinterpret_cast<const Pack<double, R>*>(y + n0);
     double e[R] = {};
-    for (int j = 0; j < 7; ++j) {
-      if (c_e[j] == 0.0) continue;
-      const Pack<double, R> kj = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
+    // e_1 = 0 in Dormand-Prince 5(4): k1 is not read (a compile-time skip, so that the six
+    // loads go out together)
+    Pack<double, R> kj[7];
 #pragma unroll
-      for (int r = 0; r < R; ++r) e[r] += c_e[j] * kj.v[r];
+    for (int j = 0; j < 7; ++j)
+      if (j != 1) kj[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      if (j == 1) continue;
+#pragma unroll
+      for (int r = 0; r < R; ++r) e[r] += c_e[j] * kj[j].v[r];
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {

@@ -506,9 +506,11 @@ __global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, double h, const 
   constexpr int R = 2;
   const int64_t stride = (int64_t)gridDim.x * kBlock * R;
   for (int64_t n = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * R; n < N; n += stride) {
+    // a_61 = 0: the last stage does not read k1
     Pack<double, R> kv[STAGE];
 #pragma unroll
-    for (int j = 0; j < STAGE; ++j) kv[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)rk_slot(j, sw) * N + n);
+    for (int j = 0; j < STAGE; ++j)
+      if (!(STAGE == 6 && j == 1)) kv[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)rk_slot(j, sw) * N + n);
     const Pack<double, R> yv = *reinterpret_cast<const Pack<double, R>*>(y + n);
     Pack<double, R> o;
     Pack<VT, R> ov;
@@ -516,7 +518,8 @@ __global__ __launch_bounds__(kBlock) void k_rk_stage(int64_t N, double h, const 
     for (int r = 0; r < R; ++r) {
       double acc = 0.0;
 #pragma unroll
-      for (int j = 0; j < STAGE; ++j) acc += c_dp_a[STAGE][j] * kv[j].v[r];
+      for (int j = 0; j < STAGE; ++j)
+        if (!(STAGE == 6 && j == 1)) acc += c_dp_a[STAGE][j] * kv[j].v[r];
       o.v[r] = yv.v[r] + h * acc;
       ov.v[r] = (VT)o.v[r];
     }
@@ -557,7 +560,10 @@ __global__ __launch_bounds__(kBlock) void k_rk_err(int64_t N, double h, double a
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += stride) {
     double e = 0.0;
-    for (int j = 0; j < 7; ++j) e += c_dp_e[j] * k[(int64_t)rk_slot(j, sw) * N + n];
+    // e_1 = 0 in Dormand-Prince 5(4): k1 is not read
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+      if (j != 1) e += c_dp_e[j] * k[(int64_t)rk_slot(j, sw) * N + n];
     e *= h;
     const double sc = atol + rtol * fmax(fabs(y[n]), fabs(ynew[n]));
     const double r = e / sc;
