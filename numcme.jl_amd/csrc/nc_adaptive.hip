@@ -163,11 +163,13 @@ int64_t select_drop(const double* p, int64_t n, double thr, bool strict, hipStre
   scan_fixed(w.sorted.p, w.cs.p, n, s, w.scan_tmp);
   k_count_drop<<<g, kBlock, 0, s>>>(w.cs.p, n, thr, strict ? 1 : 0, w.counter.p);
   NC_CHECK_HIP(hipGetLastError());
-  unsigned long long m = 0;
-  unsigned bad = 0;
-  NC_CHECK_HIP(hipMemcpyAsync(&m, w.counter.p, sizeof(m), hipMemcpyDeviceToHost, s));
-  NC_CHECK_HIP(hipMemcpyAsync(&bad, w.bad.p, sizeof(bad), hipMemcpyDeviceToHost, s));
+  unsigned long long* hm = static_cast<unsigned long long*>(host_stage(4));
+  unsigned* hbad = reinterpret_cast<unsigned*>(hm + 1);
+  NC_CHECK_HIP(hipMemcpyAsync(hm, w.counter.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipMemcpyAsync(hbad, w.bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   NC_CHECK_HIP(hipStreamSynchronize(s));
+  const unsigned long long m = *hm;
+  const unsigned bad = *hbad;
   NC_REQUIRE(!bad, NC_ERR_NUMERIC, "NaN in the probability vector at a space adaptation");
   if (m == 0) return 0;
   // dropids = sort(pids[1:dropcount])
@@ -285,9 +287,10 @@ std::vector<double> sink_derivative(nc_matrix_s* A, double t, const double* u, h
   } else {
     matvec_launch(A, t, u, du.p, false, s, true);
   }
-  std::vector<double> ds(A->nr);
-  NC_CHECK_HIP(hipMemcpyAsync(ds.data(), du.p + A->n, sizeof(double) * A->nr, hipMemcpyDeviceToHost, s));
+  double* hds = static_cast<double*>(host_stage(4));
+  NC_CHECK_HIP(hipMemcpyAsync(hds, du.p + A->n, sizeof(double) * A->nr, hipMemcpyDeviceToHost, s));
   NC_CHECK_HIP(hipStreamSynchronize(s));
+  std::vector<double> ds(hds, hds + A->nr);
   matvec_harvest_timing(A);
   return ds;
 }

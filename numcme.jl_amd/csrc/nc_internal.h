@@ -67,6 +67,12 @@ void pool_free(void* p);
 void pool_trim();  // return every cached device block to the driver
 void* pinned_alloc(size_t bytes);
 void pinned_free(void* p);
+// Pinned staging for small device-to-host reads, per host thread: slot k (0..15) is its own
+// 4 KiB region, so reads whose copy and use interleave take different slots. A copy into
+// pageable memory blocks the host until it completes; into these it is asynchronous, and
+// several reads share one synchronisation.
+constexpr size_t kHostStageSlot = 4096;
+void* host_stage(int slot);
 hipStream_t stream_acquire();        // non-blocking stream, default priority
 void stream_release(hipStream_t s);  // synchronises it, then keeps it for reuse
 

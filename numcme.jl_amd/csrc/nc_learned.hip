@@ -197,10 +197,10 @@ int64_t learned_select(nc_space_s* sp, nc_mlp_s* M, int64_t nE, double threshold
   mlp_forward_device(M, nE, w.feat.p, w.logits.p, s);
   k_select<<<grid_for(nE, kBlock), kBlock, 0, s>>>(nE, w.E.p, w.logits.p, thr, w.select.p, w.counter.p);
   NC_CHECK_HIP(hipGetLastError());
-  uint32_t kept = 0;
-  NC_CHECK_HIP(hipMemcpyAsync(&kept, w.counter.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  uint32_t* hk = static_cast<uint32_t*>(host_stage(5));
+  NC_CHECK_HIP(hipMemcpyAsync(hk, w.counter.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   NC_CHECK_HIP(hipStreamSynchronize(s));
-  return kept;
+  return *hk;
 }
 
 void records_label(nc_records_s* R, nc_space_s* sp, const double* p, hipStream_t s) {

@@ -174,6 +174,13 @@ __global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t n, const int32_t*
   }
 }
 
+// out[i] = in[i * stride] for i < n (n <= 64; one wave)
+__global__ void k_gather_strided(const uint32_t* in, int64_t stride, int n, uint32_t* out) {
+  const int i = threadIdx.x;
+  if (i < n) out[i] = in[(int64_t)i * stride];
+}
+
+
 __global__ void k_sink_flags(int64_t n, int nr, const uint32_t* sink_mask, uint32_t* flags) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n * nr) return;
@@ -593,12 +600,16 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
     k_build_count<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, nr, sp->state_conn.p, g, tv, d_kmax.p, d_counts.p,
                                                          d_counts.p + 2);
   NC_CHECK_HIP(hipGetLastError());
-  int32_t K = 0;
   unsigned long long counts[2 + kMaxReactions];
-  NC_CHECK_HIP(hipMemcpyAsync(&K, d_kmax.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NC_CHECK_HIP(hipMemcpyAsync(counts, d_counts.p, sizeof(counts), hipMemcpyDeviceToHost, s));
-  NC_CHECK_HIP(hipStreamSynchronize(s));
-  A->K = K;
+  {
+    unsigned char* h = static_cast<unsigned char*>(host_stage(1));
+    NC_CHECK_HIP(hipMemcpyAsync(h, d_counts.p, sizeof(counts), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipMemcpyAsync(h + sizeof(counts), d_kmax.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    std::memcpy(counts, h, sizeof(counts));
+    std::memcpy(&A->K, h + sizeof(counts), sizeof(int32_t));
+  }
+  const int32_t K = A->K;
   const size_t vs = sizeof(VT);
   A->diag.alloc(std::max<int64_t>(ld, 1));
   A->col.alloc(std::max<int64_t>((int64_t)K * ld, 1));
@@ -617,16 +628,19 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   NC_CHECK_HIP(hipGetLastError());
   // slots with one column offset in every nonzero row (read back with the final sync)
   const int kd = std::min<int>(K, kMaxReactions);
-  std::vector<int32_t> dia_h(2 * std::max(kd, 1));
+  // pinned: the DIA results and dmax land here with the final synchronisation
+  int32_t* dia_h = static_cast<int32_t*>(host_stage(3));
+  unsigned long long* dmax_h = reinterpret_cast<unsigned long long*>(dia_h + 2 * kMaxReactions);
   DevBuf<int32_t> d_dia(2 * std::max(kd, 1));
   if (n && kd) {
-    for (int k = 0; k < kd; ++k) dia_h[2 * k] = INT32_MAX, dia_h[2 * k + 1] = INT32_MIN;
-    NC_CHECK_HIP(hipMemcpyAsync(d_dia.p, dia_h.data(), sizeof(int32_t) * 2 * kd, hipMemcpyHostToDevice, s));
+    std::vector<int32_t> dia_init(2 * kd);
+    for (int k = 0; k < kd; ++k) dia_init[2 * k] = INT32_MAX, dia_init[2 * k + 1] = INT32_MIN;
+    NC_CHECK_HIP(hipMemcpyAsync(d_dia.p, dia_init.data(), sizeof(int32_t) * 2 * kd, hipMemcpyHostToDevice, s));
     for (int k = 0; k < kd; ++k)
       k_dia_detect<VT><<<std::min<unsigned>(grid_for(n, kBlock), 256), kBlock, 0, s>>>(
           n, A->col.p + (int64_t)k * ld, reinterpret_cast<const VT*>(A->val.p) + (int64_t)k * ld, d_dia.p + 2 * k);
     NC_CHECK_HIP(hipGetLastError());
-    NC_CHECK_HIP(hipMemcpyAsync(dia_h.data(), d_dia.p, sizeof(int32_t) * 2 * kd, hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipMemcpyAsync(dia_h, d_dia.p, sizeof(int32_t) * 2 * kd, hipMemcpyDeviceToHost, s));
   }
   // sink entries, grouped by sink row (reaction), ascending source state within a row
   DevBuf<uint32_t> flags(std::max<int64_t>(n * nr, 1)), pos(std::max<int64_t>(n * nr, 1));
@@ -638,9 +652,13 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   }
   A->sink_off_h.assign(nr + 1, 0);
   if (n) {
-    std::vector<uint32_t> starts(nr);
-    for (int r = 0; r < nr; ++r)
-      NC_CHECK_HIP(hipMemcpyAsync(&starts[r], pos.p + (int64_t)r * n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    // the first entry of each sink row (pos at r * n), gathered on the device and read back
+    // in one copy
+    DevBuf<uint32_t> d_starts(nr);
+    k_gather_strided<<<1, 64, 0, s>>>(pos.p, n, nr, d_starts.p);
+    NC_CHECK_HIP(hipGetLastError());
+    uint32_t* starts = static_cast<uint32_t*>(host_stage(2));
+    NC_CHECK_HIP(hipMemcpyAsync(starts, d_starts.p, sizeof(uint32_t) * nr, hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
     for (int r = 0; r < nr; ++r) A->sink_off_h[r] = starts[r];
   }
@@ -698,9 +716,9 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
     NC_CHECK_HIP(hipMemcpyAsync(A->states.p, sp->states.p, sizeof(int32_t) * n * ns, hipMemcpyDeviceToDevice, s));
     NC_CHECK_HIP(hipMemcpyAsync(A->state_conn.p, sp->state_conn.p, sizeof(int32_t) * n * nr, hipMemcpyDeviceToDevice, s));
   }
-  unsigned long long dmax_bits = 0;
-  NC_CHECK_HIP(hipMemcpyAsync(&dmax_bits, d_dmax.p, sizeof(dmax_bits), hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipMemcpyAsync(dmax_h, d_dmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   NC_CHECK_HIP(hipStreamSynchronize(s));
+  const unsigned long long dmax_bits = *dmax_h;
   A->dia_mask = 0;
   A->dia_off.assign(kd, 0);
   for (int k = 0; n && k < kd; ++k) {

@@ -145,6 +145,13 @@ void pinned_free(void* p) {
   c.live_host.erase(it);
 }
 
+void* host_stage(int slot) {
+  thread_local unsigned char* base = nullptr;  // kept until the process ends
+  if (!base) base = static_cast<unsigned char*>(pinned_alloc(16 * kHostStageSlot));
+  if (slot < 0 || slot >= 16) throw Error(NC_ERR_INTERNAL, "bad host staging slot");
+  return base + (size_t)slot * kHostStageSlot;
+}
+
 hipStream_t stream_acquire() {
   Cache& c = cache();
   const int dev = current_device();

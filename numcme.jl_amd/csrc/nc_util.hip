@@ -25,11 +25,11 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, DevBuf<uns
   tmp.reserve(bytes + 16);
   NC_CHECK_HIP(rocprim::exclusive_scan(tmp.p, bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s));
   if (total_host) {
-    uint32_t last_in = 0, last_out = 0;
-    NC_CHECK_HIP(hipMemcpyAsync(&last_in, in + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    NC_CHECK_HIP(hipMemcpyAsync(&last_out, out + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    uint32_t* h = static_cast<uint32_t*>(host_stage(0));
+    NC_CHECK_HIP(hipMemcpyAsync(h, in + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipMemcpyAsync(h + 1, out + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
-    *total_host = last_in + last_out;
+    *total_host = h[0] + h[1];
   }
 }
 
