@@ -46,6 +46,8 @@ struct nc_matrix_s {
   nc::DevBuf<int32_t> chunk_row;    // sink row of each chunk
   nc::DevBuf<int32_t> chunk_first;  // nr + 1: first chunk of each sink row
   nc::DevBuf<double> chunk_part;    // n_chunks partial sums
+  nc::DevBuf<unsigned> chunk_arrivals;  // chunk workgroups done in the running launch (a
+                                        // matrix runs one mat-vec at a time)
   // slots whose stored column is i + dia_off[k] in every row with a nonzero value (box
   // spaces in lattice order): the mat-vec computes those columns instead of loading them
   uint32_t dia_mask = 0;

@@ -259,7 +259,9 @@ struct MatArgs {
   int32_t n_chunks;               // > 0: the extra workgroups reduce sink-entry chunks
   const int64_t* chunk_start;
   const int32_t* chunk_row;
+  const int32_t* chunk_first;     // nr + 1: first chunk of each sink row
   double* chunk_part;
+  unsigned* chunk_arrivals;       // chunk workgroups done in this launch, zero between launches
 };
 
 template <typename T, int R>
@@ -378,6 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
     double acc = 0.0;
     for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) acc += (double)a.sink_val[e] * (double)v[a.sink_src[e]];
     __shared__ double part[kBlock / 64];
+    __shared__ int s_last;
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -385,30 +388,39 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
       double s = 0.0;
       for (int w = 0; w < kBlock / 64; ++w) s += part[w];
       if (a.n_chunks) {
-        a.chunk_part[c] = s;  // scaled and added in order by k_sink_fold
+        // chunk partial; the chunk workgroup that arrives last (device-scope counter,
+        // release fence before the increment, acquire after) folds them below
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(a.chunk_part) + c,
+                           (unsigned long long)__double_as_longlong(s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned prev = atomicAdd(a.chunk_arrivals, 1u);
+        const int last = prev + 1u == (unsigned)a.n_chunks;
+        if (last) __threadfence();
+        s_last = last;
       } else {
         s *= a.sink_scale[r];
         const int64_t o = a.n + r;
         out[o] = (OT)(accumulate ? (double)out[o] + s : s);
       }
     }
+    if (a.n_chunks) {
+      __syncthreads();
+      if (s_last) {
+        // sink row q = sum of its chunk partials in chunk order, times its time factor
+        const int q = threadIdx.x;
+        if (q < a.nr) {
+          double s = 0.0;
+          for (int k = a.chunk_first[q]; k < a.chunk_first[q + 1]; ++k)
+            s += __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<unsigned long long*>(a.chunk_part) + k,
+                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          s *= a.sink_scale[q];
+          const int64_t o = a.n + q;
+          out[o] = (OT)(accumulate ? (double)out[o] + s : s);
+        }
+        if (threadIdx.x == 0) __hip_atomic_store(a.chunk_arrivals, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
-}
-
-struct SinkScale {
-  double v[kMaxReactions];
-};
-
-// Sink row r = sum of its chunk partials in chunk order, times the row's time factor
-template <typename OT>
-__global__ void k_sink_fold(int nr, const int32_t* chunk_first, const double* chunk_part, SinkScale sc, int64_t n,
-                            OT* out, int accumulate) {
-  const int r = threadIdx.x;
-  if (r >= nr) return;
-  double s = 0.0;
-  for (int c = chunk_first[r]; c < chunk_first[r + 1]; ++c) s += chunk_part[c];
-  s *= sc.v[r];
-  out[n + r] = (OT)(accumulate ? (double)out[n + r] + s : s);
 }
 
 // Launch timing: when e0/e1 are given the kernel is launched with hipExtLaunchKernelGGL,
@@ -418,30 +430,14 @@ struct LaunchCfg {
   unsigned grid;
   hipStream_t s;
   hipEvent_t e0, e1;
-  const int32_t* chunk_first;
 };
 
 template <typename VT, typename OT, int K, int R, int DM>
 void launch_one(const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
-  if (!a.n_chunks) {
-    if (c.e0)
-      hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo, acc);
-    else
-      k_matvec<VT, OT, K, R, DM><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
-    return;
-  }
-  // chunked sinks: the events bracket both kernels (start of the first, end of the fold)
-  SinkScale sc;
-  for (int r = 0; r < a.nr; ++r) sc.v[r] = a.sink_scale[r];
-  if (c.e0) {
-    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, nullptr, 0, a, vv, oo,
-                          acc);
-    hipExtLaunchKernelGGL(k_sink_fold<OT>, dim3(1), dim3(64), 0, c.s, nullptr, c.e1, 0, a.nr, c.chunk_first,
-                          (const double*)a.chunk_part, sc, a.n, oo, acc);
-  } else {
+  if (c.e0)
+    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo, acc);
+  else
     k_matvec<VT, OT, K, R, DM><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
-    k_sink_fold<OT><<<1, 64, 0, c.s>>>(a.nr, c.chunk_first, a.chunk_part, sc, a.n, oo, acc);
-  }
 }
 
 template <typename VT, typename OT, int R, int DM>
@@ -505,6 +501,8 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   a.chunk_start = A->chunk_start.p;
   a.chunk_row = A->chunk_row.p;
   a.chunk_part = A->chunk_part.p;
+  a.chunk_first = A->chunk_first.p;
+  a.chunk_arrivals = A->chunk_arrivals.p;
   for (size_t k = 0; k < A->dia_off.size() && k < (size_t)kMaxReactions; ++k) a.dia_off[k] = A->dia_off[k];
   const nc_model_s* m = A->model;
   for (int r = 0; r < A->nr; ++r) a.sink_scale[r] = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
@@ -513,7 +511,7 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   const int R = A->rows_per_thread;
   const int64_t row_threads = (A->n + R - 1) / R;
   const int64_t row_blocks = (row_threads + kBlock - 1) / kBlock;
-  const LaunchCfg c{(unsigned)(row_blocks + (A->n_chunks ? A->n_chunks : A->nr)), s, e0, e1, A->chunk_first.p};
+  const LaunchCfg c{(unsigned)(row_blocks + (A->n_chunks ? A->n_chunks : A->nr)), s, e0, e1};
   const VT* vv = reinterpret_cast<const VT*>(v);
   const int acc = accumulate ? 1 : 0;
   if (out_fp64)
@@ -689,6 +687,8 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
       A->chunk_row.alloc(chunk_row_h.size());
       A->chunk_first.alloc(chunk_first_h.size());
       A->chunk_part.alloc(chunk_row_h.size());
+      A->chunk_arrivals.alloc(1);
+      NC_CHECK_HIP(hipMemsetAsync(A->chunk_arrivals.p, 0, sizeof(unsigned), s));
       NC_CHECK_HIP(hipMemcpyAsync(A->chunk_start.p, chunk_start_h.data(), sizeof(int64_t) * chunk_start_h.size(),
                                   hipMemcpyHostToDevice, s));
       NC_CHECK_HIP(hipMemcpyAsync(A->chunk_row.p, chunk_row_h.data(), sizeof(int32_t) * chunk_row_h.size(),
