@@ -97,10 +97,35 @@ __global__ __launch_bounds__(kBlock) void k_bstage(Dev d, int64_t ld, const doub
 // out = A v over the state rows of active instances (R rows per thread). With the slot
 // count KT as a template argument every slot's column and value load goes out in one
 // batch and every gather in a second (KT = 0: runtime count K, one slot at a time).
+// The nseg sink segments (one per instance and reaction with sink entries) come first in
+// the grid, one workgroup each, so that their reductions overlap the row blocks.
+struct SinkSegs {
+  int32_t nseg;
+  const int32_t* inst;  // owner instance
+  const int64_t* row;   // output row of the sink
+  const int64_t* off;   // nseg + 1: entry ranges
+  const int32_t* src;   // source state rows of the entries
+};
+
 template <typename VT, int R, int KT>
 __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, const double* diag, const int32_t* col,
-                                                    const VT* val, const VT* v, double* out) {
-  const int64_t n0 = first_row<R>();
+                                                    const VT* val, const VT* v, double* out, SinkSegs sg,
+                                                    const VT* sval) {
+  if ((int)blockIdx.x < sg.nseg) {
+    // sink row of one (instance, reaction) segment: one workgroup, deterministic tree
+    const int seg = blockIdx.x;
+    if (d.h[sg.inst[seg]] == 0.0) return;
+    double acc = 0.0;
+    for (int64_t e = sg.off[seg] + threadIdx.x; e < sg.off[seg + 1]; e += kBlock)
+      acc += (double)sval[e] * (double)v[sg.src[e]];
+    __shared__ double part[kBlock / 64];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[sg.row[seg]] = part[0] + part[1] + part[2] + part[3];
+    return;
+  }
+  const int64_t n0 = (((int64_t)blockIdx.x - sg.nseg) * kBlock + threadIdx.x) * R;
   if (n0 >= ld) return;
   const int inst = d.gran_inst[n0 / kGran];
   if (d.h[inst] == 0.0) return;
@@ -144,22 +169,6 @@ __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, co
 #pragma unroll
   for (int r = 0; r < R; ++r)
     if (n0 + r < nend) out[n0 + r] = acc[r];
-}
-
-// sink row of one (instance, reaction) segment
-template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_bsinks(Dev d, const int32_t* seg_inst, const int64_t* seg_row,
-                                                   const int64_t* seg_off, const int32_t* src, const VT* sval,
-                                                   const VT* v, double* out) {
-  const int seg = blockIdx.x;
-  if (d.h[seg_inst[seg]] == 0.0) return;
-  double acc = 0.0;
-  for (int64_t e = seg_off[seg] + threadIdx.x; e < seg_off[seg + 1]; e += kBlock) acc += (double)sval[e] * (double)v[src[e]];
-  __shared__ double part[kBlock / 64];
-  acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) out[seg_row[seg]] = part[0] + part[1] + part[2] + part[3];
 }
 
 // Error partials per granule: sum ((h sum_j e_j k_j) / (atol + rtol max(|u|, |y|)))^2.
@@ -436,9 +445,11 @@ template <typename VT, int R>
 void launch_matvec(nc_batch_s* B, const VT* v, double* out) {
   const Dev d = dev_view(B);
   hipStream_t s = B->stream.s;
+  const SinkSegs sg{(int32_t)B->nseg, B->seg_inst.p, B->seg_row.p, B->seg_off.p, B->sink_src.p};
+  const unsigned grid = row_grid(B, R) + (unsigned)B->nseg;
   auto go = [&](auto kern) {
-    kern<<<row_grid(B, R), kBlock, 0, s>>>(d, B->ld, B->K, B->diag.p, B->col.p, reinterpret_cast<const VT*>(B->val.p),
-                                           v, out);
+    kern<<<grid, kBlock, 0, s>>>(d, B->ld, B->K, B->diag.p, B->col.p, reinterpret_cast<const VT*>(B->val.p), v, out,
+                                 sg, reinterpret_cast<const VT*>(B->sink_val.p));
   };
   switch (B->K) {
     case 1: go(k_bmatvec<VT, R, 1>); break;
@@ -451,9 +462,6 @@ void launch_matvec(nc_batch_s* B, const VT* v, double* out) {
     case 8: go(k_bmatvec<VT, R, 8>); break;
     default: go(k_bmatvec<VT, R, 0>); break;
   }
-  if (B->nseg)
-    k_bsinks<VT><<<(unsigned)B->nseg, kBlock, 0, s>>>(d, B->seg_inst.p, B->seg_row.p, B->seg_off.p, B->sink_src.p,
-                                                      reinterpret_cast<const VT*>(B->sink_val.p), v, out);
   NC_CHECK_HIP(hipGetLastError());
 }
 
