@@ -247,6 +247,34 @@ __global__ __launch_bounds__(kBlock) void k_baccept(Dev d, int64_t ld, double* u
     *reinterpret_cast<Pack<double, R>*>(snaps + (int64_t)sl * ld + n0) = *reinterpret_cast<const Pack<double, R>*>(u + n0);
 }
 
+// k_baccept followed by stage 1 in one pass: the accepted (or kept) u and k0 stay in
+// registers for the stage, instead of being written by one kernel and read by the next.
+template <typename VT, int R>
+__global__ __launch_bounds__(kBlock) void k_baccept_stage1(Dev d, int64_t ld, double* u, const double* y, double* k,
+                                                           double* snaps, VT* vin) {
+  const int64_t n0 = first_row<R>();
+  if (n0 >= ld) return;
+  const int inst = d.gran_inst[n0 / kGran];
+  Pack<double, R> uu, k0;
+  if (d.accept[inst]) {
+    uu = *reinterpret_cast<const Pack<double, R>*>(y + n0);
+    k0 = *reinterpret_cast<const Pack<double, R>*>(k + 6 * ld + n0);
+    *reinterpret_cast<Pack<double, R>*>(u + n0) = uu;
+    *reinterpret_cast<Pack<double, R>*>(k + n0) = k0;
+  } else {
+    uu = *reinterpret_cast<const Pack<double, R>*>(u + n0);
+    k0 = *reinterpret_cast<const Pack<double, R>*>(k + n0);
+  }
+  const int sl = d.snap[inst];
+  if (sl >= 0) *reinterpret_cast<Pack<double, R>*>(snaps + (int64_t)sl * ld + n0) = uu;
+  const double h = d.h[inst];
+  if (h == 0.0) return;
+  Pack<VT, R> o;
+#pragma unroll
+  for (int r = 0; r < R; ++r) o.v[r] = (VT)(uu.v[r] + h * (c_a[1][0] * k0.v[r]));
+  *reinterpret_cast<Pack<VT, R>*>(vin + n0) = o;
+}
+
 // Copy one instance's slot-major ELL into the concatenated arrays.
 template <typename VT>
 __global__ void k_bcopy(int64_t n, int64_t ld_src, int K_src, int K, int64_t off, int64_t ld, const double* diag_s,
@@ -483,15 +511,17 @@ void launch_stage(nc_batch_s* B, int stage, unsigned g) {
   }
 }
 
-// accept/snapshot of the previous step, then one full DP5 attempt and the error fold
+// accept/snapshot of the previous step (fused with stage 1), then one full DP5 attempt and
+// the error fold
 template <typename VT, int R>
 void iteration_r(nc_batch_s* B, double atol, double rtol) {
   const Dev d = dev_view(B);
   hipStream_t s = B->stream.s;
   const unsigned g = row_grid(B, R);
-  k_baccept<R><<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+  k_baccept_stage1<VT, R><<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p,
+                                               reinterpret_cast<VT*>(B->vin.p));
   for (int st = 1; st < 7; ++st) {
-    launch_stage<VT, R>(B, st, g);
+    if (st > 1) launch_stage<VT, R>(B, st, g);
     launch_matvec<VT, R>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p + (int64_t)st * B->ld);
   }
   k_berr<R><<<g, kBlock, 0, s>>>(d, B->ld, B->grans, atol, rtol, B->u.p, B->ytmp.p, B->k.p, B->partials.p);
