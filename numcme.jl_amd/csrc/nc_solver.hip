@@ -31,7 +31,9 @@ constexpr int kBlock = 256;
 constexpr int kMaxRedBlocks = 2048;
 constexpr int kPartStrideMax = 65;  // partial sums per block: up to 2 x 32 dots + a norm
 
-enum Finish : int { FIN_NONE = 0, FIN_ARN, FIN_LAST, FIN_BETA, FIN_SUMS };
+// FIN_FOLD: the finish kernel only folds the partials into a.tot (the next pass, an
+// update with arn_pre set, applies the Arnoldi finishing step itself)
+enum Finish : int { FIN_NONE = 0, FIN_LAST, FIN_BETA, FIN_SUMS, FIN_FOLD };
 constexpr double kBreakdownTol = 1e-7;   // Expokit's happy-breakdown threshold
 constexpr int kFinBlock = 1024;  // threads of the one-block partial-sum fold
 constexpr unsigned kVecBlocks = 512;  // grid of the vector passes
@@ -69,8 +71,9 @@ struct VecArgs {
   double* partials;
   int finish;
   double* kb;
-  double* gram;           // (m+1) x (m+1) Gram matrix of the scaled basis (FIN_ARN)
+  double* gram;           // (m+1) x (m+1) Gram matrix of the scaled basis (arn_pre)
   double* tot;            // kPartStrideMax folded totals (finish)
+  int arn_pre;            // VM_UPDATE: apply column j's Arnoldi finishing step from tot first
   unsigned* counter;      // arrival counter of the fold's workgroups, zero between passes
   KLayout L;
   int j;
@@ -102,6 +105,74 @@ struct alignas(sizeof(T) * R) Pack {
   T v[R];
 };
 
+// Finishing step of Arnoldi column j from the folded totals tot (V_i.q at i, V_i.V_j at
+// nv + i, nv = j + 1), done by every workgroup of the update pass that follows instead of
+// by one workgroup in a launch of its own: |V_j| (H(j, j-1), sc_j), the Gram column of the
+// scaled basis, and h from G h = a, a_i = sc_i sc_j V_i.q, by Jacobi sweeps (G is the
+// identity up to the FP32 rounding of the stored basis). Leaves the update coefficients
+// -h_i sc_i in cs[0..nv) and sc_j in *xs; workgroup 0 also stores H(., j), h, sc_j and the
+// Gram column for the later columns (no other workgroup reads those entries). kBlock
+// threads: the sweeps give each row i eight lanes.
+__device__ void arnoldi_prologue(const KLayout& L, double* kb, double* gram, int j, double btol, const double* tot,
+                                 double* cs, double* xs) {
+  static_assert(kBlock == 256, "arnoldi_prologue assumes 256 threads");
+  const int nv = j + 1, ld = L.m + 1;
+  __shared__ double p_scj, p_nj, p_a[32], p_h[32], p_sc[32], p_G[32][33];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < j * j; e += kBlock) p_G[e / j][e % j] = gram[(e / j) * ld + e % j];
+  if (tid < j) p_sc[tid] = kb[L.sc(tid)];
+  if (tid == 0) {
+    double nj = 0.0, scj;
+    if (j >= 1) {
+      nj = sqrt(tot[nv + j]);
+      scj = nj > btol ? 1.0 / nj : 0.0;
+    } else {
+      scj = kb[L.sc(0)];
+    }
+    p_nj = nj;
+    p_scj = scj;
+  }
+  __syncthreads();
+  const double scj = p_scj;
+  if (tid < nv) {
+    const int r = tid;
+    const double scr = r == j ? scj : p_sc[r];
+    double grj = scr * scj * tot[nv + r];
+    if (r == j && scj == 0.0) grj = 1.0;  // a vanished vector: keep G invertible
+    p_G[r][j] = grj;
+    p_G[j][r] = grj;
+    const double ar = scr * scj * tot[r];
+    p_a[r] = ar;
+    p_h[r] = ar;
+  }
+  if (tid == 0) p_sc[j] = scj;
+  __syncthreads();
+  const int i = tid >> 3, kq = tid & 7;
+  for (int sweep = 0; sweep < kJacobiSweeps; ++sweep) {
+    double t = 0.0;
+    if (i < nv)
+      for (int k = kq; k < nv; k += 8) t += (k == i ? p_G[i][k] - 1.0 : p_G[i][k]) * p_h[k];
+    for (int off = 4; off > 0; off >>= 1) t += __shfl_down(t, off, 8);
+    __syncthreads();
+    if (kq == 0 && i < nv) p_h[i] = p_a[i] - t;
+    __syncthreads();
+  }
+  if (tid < nv) cs[tid] = -p_h[tid] * p_sc[tid];
+  if (tid == 0) *xs = scj;
+  if (blockIdx.x == 0) {
+    if (tid == 0 && j >= 1) {
+      kb[L.H(j, j - 1)] = p_nj;
+      kb[L.sc(j)] = scj;
+    }
+    if (tid < nv) {
+      kb[L.H(tid, j)] = p_h[tid];
+      kb[L.h(tid)] = p_h[tid];
+      gram[tid * ld + j] = p_G[tid][j];
+      gram[j * ld + tid] = p_G[j][tid];
+    }
+  }
+}
+
 template <typename BT, int MAXV, int MODE>
 __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   constexpr int MV = MAXV > 0 ? MAXV : 1;
@@ -111,15 +182,23 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   constexpr int R = 4;
   const int nv = a.nv;
   // per-vector coefficients live in LDS (uniform broadcast reads)
-  __shared__ double s_cs[MV], s_sc[MV];
-  if (threadIdx.x < MV) {
-    const int i = threadIdx.x;
-    const double sc = (i < nv) ? (a.sc ? a.sc[i] : 1.0) : 0.0;
-    s_sc[i] = sc;
-    s_cs[i] = (i < nv && a.coef) ? a.csign * a.coef[i] * sc : 0.0;
+  __shared__ double s_cs[MV], s_sc[MV], s_xs;
+  if (UPD && a.arn_pre) {
+    // the column finished here; a.nv counts the basis vectors V_0..V_j
+    if (threadIdx.x < MV) s_cs[threadIdx.x] = 0.0;
+    __syncthreads();
+    arnoldi_prologue(a.L, a.kb, a.gram, a.j, a.btol, a.tot, s_cs, &s_xs);
+  } else {
+    if (threadIdx.x < MV) {
+      const int i = threadIdx.x;
+      const double sc = (i < nv) ? (a.sc ? a.sc[i] : 1.0) : 0.0;
+      s_sc[i] = sc;
+      s_cs[i] = (i < nv && a.coef) ? a.csign * a.coef[i] * sc : 0.0;
+    }
+    if (threadIdx.x == 0) s_xs = a.xscale ? *a.xscale : 1.0;
   }
   __syncthreads();
-  const double xs = a.xscale ? *a.xscale : 1.0;
+  const double xs = s_xs;
   double dacc[DOTS ? MV : 1], gacc[DUAL ? MV : 1];
 #pragma unroll
   for (int i = 0; i < (DOTS ? MV : 1); ++i) dacc[i] = 0.0;
@@ -289,58 +368,6 @@ __global__ __launch_bounds__(kBlock) void k_vec_dual(VecArgs<BT> a) {
   }
 }
 
-// Finish of the first pass of Arnoldi column j (VM_DUAL over V_0..V_j with x = q = A V_j):
-// tot[i] = V_i.q, tot[nv+i] = V_i.V_j. Sets the norm of V_j (H(j, j-1), sc_j), the new
-// column of the Gram matrix G of the scaled basis, and solves G h = a with
-// a_i = sc_i sc_j V_i.q by Jacobi sweeps (G is the identity up to the FP32 rounding of
-// the stored basis). h is column j of H and the update coefficients of the second pass;
-// the new vector V_{j+1} = sc_j q - sum_i h_i sc_i V_i is then orthogonal to the stored
-// basis up to FP64 rounding, without a second projection pass.
-__device__ void arnoldi_finish(const KLayout& L, double* kb, double* gram, int j, double btol, const double* tot) {
-  // 1024 threads as a 32 x 32 grid (i, k) over the Gram block; the Jacobi sweeps reduce
-  // each row over k with shuffles inside its 32-lane group
-  const int nv = j + 1, ld = L.m + 1;
-  __shared__ double s_scj, s_a[32], s_h[32], s_G[32][33];
-  const int tid = threadIdx.x, i = tid >> 5, k = tid & 31;
-  if (i < j && k < j) s_G[i][k] = gram[i * ld + k];
-  if (tid == 0) {
-    if (j >= 1) {
-      const double nj = sqrt(tot[nv + j]);
-      kb[L.H(j, j - 1)] = nj;
-      kb[L.sc(j)] = nj > btol ? 1.0 / nj : 0.0;
-    }
-    s_scj = kb[L.sc(j)];
-  }
-  __syncthreads();
-  const double scj = s_scj;
-  if (tid < nv) {
-    const int r = tid;
-    const double scr = r == j ? scj : kb[L.sc(r)];
-    double grj = scr * scj * tot[nv + r];
-    if (r == j && scj == 0.0) grj = 1.0;  // a vanished vector: keep G invertible
-    gram[r * ld + j] = grj;
-    gram[j * ld + r] = grj;
-    s_G[r][j] = grj;
-    s_G[j][r] = grj;
-    const double ar = scr * scj * tot[r];
-    s_a[r] = ar;
-    s_h[r] = ar;
-  }
-  __syncthreads();
-  for (int sweep = 0; sweep < kJacobiSweeps; ++sweep) {
-    double t = 0.0;
-    if (i < nv && k < nv) t = (k == i ? s_G[i][k] - 1.0 : s_G[i][k]) * s_h[k];
-    for (int off = 16; off > 0; off >>= 1) t += __shfl_down(t, off, 32);
-    __syncthreads();
-    if (k == 0 && i < nv) s_h[i] = s_a[i] - t;
-    __syncthreads();
-  }
-  if (tid < nv) {
-    kb[L.H(tid, j)] = s_h[tid];
-    kb[L.h(tid)] = s_h[tid];
-  }
-}
-
 // Fold of the per-block partials, then the finishing step. One workgroup per value:
 // workgroup k sums value k over the pass's blocks in a fixed order (thread t takes blocks
 // t, t + 1024, ..., then a fixed shuffle tree), so the totals are deterministic, and
@@ -361,6 +388,15 @@ __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigne
     v = wave_sum(v);
     if (lane == 0) wsum[wave] = v;
     __syncthreads();
+    if (a.finish == FIN_FOLD) {
+      // the totals are read by the next launch: its kernel boundary makes them visible
+      if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kFinBlock / 64; ++w) sum += wsum[w];
+        a.tot[k] = sum;
+      }
+      return;
+    }
     if (threadIdx.x == 0) {
       double sum = 0.0;
       for (int w = 0; w < kFinBlock / 64; ++w) sum += wsum[w];
@@ -382,10 +418,6 @@ __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigne
   __syncthreads();
   const KLayout& L = a.L;
   double* kb = a.kb;
-  if (a.finish == FIN_ARN) {
-    arnoldi_finish(L, kb, a.gram, a.j, a.btol, tot);
-    return;
-  }
   if (threadIdx.x == 0) {
     const double nrm2 = tot[nvals - 1];
     switch (a.finish) {
@@ -672,9 +704,10 @@ static void copy_to_host(nc_solver_s* S, double* host, const double* dev) {
 // ---- Krylov expv ---------------------------------------------------------------------
 // Arnoldi column j in two passes over the stored basis V_0..V_j (FP32 or FP64, kept
 // unnormalised with scales sc_i):
-//   pass 1 (VM_DUAL):   V_i.q and V_i.V_j for q = A V_j; the finish sets |V_j|, the Gram
-//                       column of V_j and solves G h = a, a_i = sc_i sc_j V_i.q
-//   pass 2 (VM_UPDATE): V_{j+1} = sc_j q - sum_i h_i sc_i V_i
+//   pass 1 (VM_DUAL):   V_i.q and V_i.V_j for q = A V_j, folded into per-value totals
+//   pass 2 (VM_UPDATE): every workgroup first sets |V_j|, the Gram column of V_j and solves
+//                       G h = a, a_i = sc_i sc_j V_i.q, from the totals (arnoldi_prologue),
+//                       then V_{j+1} = sc_j q - sum_i h_i sc_i V_i
 // Projecting with the Gram matrix of the stored vectors makes V_{j+1} orthogonal to them
 // to FP64 rounding in one projection: the work a second classical Gram-Schmidt pass
 // would do, without a third pass over V.
@@ -698,13 +731,11 @@ static void arnoldi_iteration(nc_solver_s* S, int j) {
   a.vout_fp32 = std::is_same<BT, float>::value ? 1 : 0;
   a.mode = VM_DUAL;
   a.xin = S->q.p;
-  a.finish = FIN_ARN;
+  a.finish = FIN_FOLD;
   launch_vec(a, S, s);
+  // the update applies the finishing step (|V_j|, Gram column, h) from the folded totals
   a.mode = VM_UPDATE;
-  a.sc = S->kbuf.p + L.sc(0);
-  a.xscale = S->kbuf.p + L.sc(j);
-  a.coef = S->kbuf.p + L.h(0);
-  a.csign = -1.0;
+  a.arn_pre = 1;
   a.vout = Vp(j + 1);
   a.finish = FIN_NONE;
   launch_vec(a, S, s);
