@@ -3,15 +3,14 @@
 // Replaces the integrator loop of NumCME.jl's solve (src/transientcme/sparse/fspsolve.jl:
 // 10-41, a DifferentialEquations.jl/CVODE call around matvec!) with:
 //
-//  * Krylov expv for time-invariant generators: Arnoldi with classical Gram-Schmidt and a
-//    second Gram-Schmidt pass when the DGKS test asks for it (J. W. Daniel, W. B. Gragg,
-//    L. Kaufman & G. W. Stewart, Math. Comp. 30, 1976; decided on the device), FP64 dot
-//    products and coefficients, basis vectors stored in the matrix precision; the
-//    (m+2)x(m+2) Hessenberg exponential,
-//    the local error estimate and the step-size control run on the host, following
-//    R. B. Sidje, "Expokit: a software package for computing matrix exponentials",
-//    ACM TOMS 24(1), 1998 (algorithm of its EXPV routine). One host synchronisation
-//    per Krylov step.
+//  * Krylov expv for time-invariant generators: Arnoldi by classical Gram-Schmidt whose
+//    coefficients solve the Gram system of the stored basis (one projection pass and one
+//    update pass per column, orthogonal to FP64 rounding even with an FP32 basis), FP64
+//    dot products and coefficients, basis vectors stored in the matrix precision; the
+//    (m+2)x(m+2) Hessenberg exponential, the local error estimate and the step-size
+//    control run on the host, following R. B. Sidje, "Expokit: a software package for
+//    computing matrix exponentials", ACM TOMS 24(1), 1998 (algorithm of its EXPV
+//    routine). One host synchronisation per Krylov step.
 //  * Dormand-Prince 5(4) (J. R. Dormand & P. J. Prince, J. Comput. Appl. Math. 6, 1980)
 //    with FSAL, the Hairer-Norsett-Wanner RMS error norm computed on the device in FP64,
 //    and output times hit exactly by step clamping.
