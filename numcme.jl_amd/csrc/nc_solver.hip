@@ -116,17 +116,18 @@ __device__ void arnoldi_prologue(const KLayout& L, double* kb, double* gram, int
                                  double* cs, double* xs) {
   static_assert(kBlock == 256, "arnoldi_prologue assumes 256 threads");
   const int nv = j + 1, ld = L.m + 1;
-  __shared__ double p_scj, p_nj, p_a[32], p_h[32], p_sc[32], p_G[32][33];
+  __shared__ double p_scj, p_nj, p_a[32], p_h[32], p_sc[32], p_tot[kPartStrideMax], p_G[32][33];
   const int tid = threadIdx.x;
+  // every global read in one round trip: the previous Gram block, the scales, the totals
   for (int e = tid; e < j * j; e += kBlock) p_G[e / j][e % j] = gram[(e / j) * ld + e % j];
-  if (tid < j) p_sc[tid] = kb[L.sc(tid)];
+  if (tid < j || (j == 0 && tid == 0)) p_sc[tid] = kb[L.sc(tid)];  // sc(j), j >= 1, is set below
+  if (tid < 2 * nv) p_tot[tid] = tot[tid];
+  __syncthreads();
   if (tid == 0) {
-    double nj = 0.0, scj;
+    double nj = 0.0, scj = p_sc[0];
     if (j >= 1) {
-      nj = sqrt(tot[nv + j]);
+      nj = sqrt(p_tot[nv + j]);
       scj = nj > btol ? 1.0 / nj : 0.0;
-    } else {
-      scj = kb[L.sc(0)];
     }
     p_nj = nj;
     p_scj = scj;
@@ -136,11 +137,11 @@ __device__ void arnoldi_prologue(const KLayout& L, double* kb, double* gram, int
   if (tid < nv) {
     const int r = tid;
     const double scr = r == j ? scj : p_sc[r];
-    double grj = scr * scj * tot[nv + r];
+    double grj = scr * scj * p_tot[nv + r];
     if (r == j && scj == 0.0) grj = 1.0;  // a vanished vector: keep G invertible
     p_G[r][j] = grj;
     p_G[j][r] = grj;
-    const double ar = scr * scj * tot[r];
+    const double ar = scr * scj * p_tot[r];
     p_a[r] = ar;
     p_h[r] = ar;
   }
