@@ -11,6 +11,7 @@
 //    thread (slot-major ELL, 8/16-byte vector loads of the matrix streams, FP64
 //    accumulation) and one workgroup per sink row (segmented reduction). R is chosen by
 //    nc_matrix_autotune.
+#include <algorithm>
 #include <cfloat>
 
 #include <hip/hip_ext.h>
@@ -867,10 +868,11 @@ __global__ void k_cache_flush(const uint4* p, int64_t n, unsigned* sink) {
   if (acc == 0x9e3779b9u) sink[0] = acc;
 }
 
-// Mean kernel time of the mat-vec with cold caches: before every timed launch a read of
-// twice the 256 MB Infinity Cache evicts the matrix and its vectors, as the basis streams
-// of a Krylov step do between its mat-vecs. Durations come from events stamped by the
-// dispatch itself, so the flushes are not counted.
+// Median kernel time of the mat-vec with a cold matrix: before every timed launch a read
+// of twice the 256 MB Infinity Cache evicts the matrix, as the basis streams of a Krylov
+// step do between its mat-vecs, and the input vector is then rewritten, as the pass before
+// a mat-vec in a solve writes it. Durations come from events stamped by the dispatch
+// itself, so the flushes are not counted.
 double bench_matvec_cold(nc_matrix_s* A, double t, int reps) {
   constexpr int64_t kFlushWords = (int64_t(512) << 20) / 16;
   hipStream_t s = A->stream.s;
@@ -890,13 +892,29 @@ double bench_matvec_cold(nc_matrix_s* A, double t, int reps) {
   A->timing = true;
   A->timed_ms = 0.0;
   A->timed_count = 0;
+  std::vector<float> each;
   for (int k = 0; k < reps; ++k) {
+    // evict everything, then write the input vector: in a solve the mat-vec's input was
+    // just written by the previous pass while the matrix comes from HBM
+    unsigned char* in = (k & 1) ? o.p : v.p;
     k_cache_flush<<<4096, 256, 0, s>>>(fl.p, kFlushWords, sink.p);
-    matvec_launch(A, t, (k & 1) ? o.p : v.p, (k & 1) ? v.p : o.p, false, s);
+    NC_CHECK_HIP(hipMemsetAsync(in, 0, N * A->vsize, s));
+    matvec_launch(A, t, in, (k & 1) ? v.p : o.p, false, s);
+    if (A->ev_used * 2 >= A->ev.size() || k == reps - 1) {
+      // the launches' own durations, before the slots are reused
+      NC_CHECK_HIP(hipEventSynchronize(A->ev[2 * A->ev_used - 1]));
+      for (size_t q = 0; q < A->ev_used; ++q) {
+        float ms = 0.f;
+        NC_CHECK_HIP(hipEventElapsedTime(&ms, A->ev[2 * q], A->ev[2 * q + 1]));
+        each.push_back(ms);
+      }
+      A->ev_used = 0;
+    }
   }
   NC_CHECK_HIP(hipGetLastError());
-  matvec_harvest_timing(A);
-  const double ms = A->timed_count ? A->timed_ms / (double)A->timed_count : 0.0;
+  // median: single cold launches of ~10 us scatter by a few percent
+  std::sort(each.begin(), each.end());
+  const double ms = each.empty() ? 0.0 : (double)each[each.size() / 2];
   A->timing = was;
   A->timed_ms = ms0;
   A->timed_count = n0;

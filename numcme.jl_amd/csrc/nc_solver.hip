@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <type_traits>
 
 #include "nc_solver.h"
 
@@ -183,20 +184,13 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   const int nv = a.nv;
   // per-vector coefficients live in LDS (uniform broadcast reads)
   __shared__ double s_cs[MV], s_sc[MV], s_xs;
-  if (UPD && a.arn_pre) {
-    // the column finished here; a.nv counts the basis vectors V_0..V_j
-    if (threadIdx.x < MV) s_cs[threadIdx.x] = 0.0;
-    __syncthreads();
-    arnoldi_prologue(a.L, a.kb, a.gram, a.j, a.btol, a.tot, s_cs, &s_xs);
-  } else {
-    if (threadIdx.x < MV) {
-      const int i = threadIdx.x;
-      const double sc = (i < nv) ? (a.sc ? a.sc[i] : 1.0) : 0.0;
-      s_sc[i] = sc;
-      s_cs[i] = (i < nv && a.coef) ? a.csign * a.coef[i] * sc : 0.0;
-    }
-    if (threadIdx.x == 0) s_xs = a.xscale ? *a.xscale : 1.0;
+  if (threadIdx.x < MV) {
+    const int i = threadIdx.x;
+    const double sc = (i < nv) ? (a.sc ? a.sc[i] : 1.0) : 0.0;
+    s_sc[i] = sc;
+    s_cs[i] = (i < nv && a.coef) ? a.csign * a.coef[i] * sc : 0.0;
   }
+  if (threadIdx.x == 0) s_xs = a.xscale ? *a.xscale : 1.0;
   __syncthreads();
   const double xs = s_xs;
   double dacc[DOTS ? MV : 1], gacc[DUAL ? MV : 1];
@@ -304,6 +298,65 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
   }
   // k_vec_finish folds the partials in the next launch: the kernel boundary makes them
   // visible across XCDs without a per-block L2 writeback fence
+}
+
+// The Arnoldi update V_{j+1} = sc_j q - sum_i h_i sc_i V_i with the column's finishing step
+// (arnoldi_prologue) in every workgroup. Each thread issues its first q rows and its first
+// kLoadChunk basis rows before the prologue (its barriers wait for LDS only, so the loads
+// stay in flight while it runs), which hides the prologue behind the streams.
+template <typename BT, int MAXV>
+__global__ __launch_bounds__(kBlock) void k_arn_update(VecArgs<BT> a) {
+  constexpr int R = 4, C0 = MAXV < kLoadChunk ? MAXV : kLoadChunk;
+  const int nv = a.nv;
+  const int64_t groups = a.N / R, stride = (int64_t)gridDim.x * kBlock;
+  int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t gc = g < groups ? g : groups - 1;  // clamped: the loads are unconditional
+  Pack<double, R> xi = *reinterpret_cast<const Pack<double, R>*>(a.xin + gc * R);
+  Pack<BT, R> v0[C0];
+#pragma unroll
+  for (int u = 0; u < C0; ++u) v0[u] = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)min(u, nv - 1) * a.N + gc * R);
+  __shared__ double s_cs[MAXV], s_xs;
+  if (threadIdx.x < MAXV) s_cs[threadIdx.x] = 0.0;
+  __syncthreads();
+  arnoldi_prologue(a.L, a.kb, a.gram, a.j, a.btol, a.tot, s_cs, &s_xs);
+  __syncthreads();
+  const double xs = s_xs;
+  BT* vout = reinterpret_cast<BT*>(a.vout);
+  for (; g < groups; g += stride) {
+    const int64_t n0 = g * R;
+    if (g != gc) {  // a later grid-stride row group (not prefetched)
+      xi = *reinterpret_cast<const Pack<double, R>*>(a.xin + n0);
+#pragma unroll
+      for (int u = 0; u < C0; ++u) v0[u] = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)min(u, nv - 1) * a.N + n0);
+    }
+    double x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = xs * xi.v[r];
+#pragma unroll
+    for (int u = 0; u < C0; ++u) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] += s_cs[u] * (double)v0[u].v[r];
+    }
+#pragma unroll
+    for (int i0 = C0; i0 < MAXV; i0 += kLoadChunk) {
+      Pack<BT, R> vv[kLoadChunk];
+#pragma unroll
+      for (int u = 0; u < kLoadChunk; ++u)
+        if (i0 + u < MAXV)
+          vv[u] = *reinterpret_cast<const Pack<BT, R>*>(a.V + (int64_t)min(i0 + u, nv - 1) * a.N + n0);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < kLoadChunk; ++u) {
+        if (i0 + u >= MAXV) break;
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[r] += s_cs[i0 + u] * (double)vv[u].v[r];
+      }
+    }
+    Pack<BT, R> o;
+#pragma unroll
+    for (int r = 0; r < R; ++r) o.v[r] = (BT)x[r];
+    *reinterpret_cast<Pack<BT, R>*>(vout + n0) = o;
+  }
 }
 
 // VM_DUAL with the basis split across the workgroup's waves (wave w owns V_w, V_{w+4},
@@ -500,7 +553,30 @@ void launch_vec(VecArgs<BT> a, const nc_solver_s* S, hipStream_t s) {
         launch_vec_mode<BT, VM_DUAL>(a, grid, s);
       }
       break;
-    default: launch_vec_mode<BT, VM_UPDATE>(a, grid, s); break;
+    default:
+      if (a.arn_pre) {
+        if (a.vout_fp32 != (std::is_same<BT, float>::value ? 1 : 0) || a.xout || !a.vout || a.finish != FIN_NONE)
+          throw Error(NC_ERR_INTERNAL, "Arnoldi update pass misconfigured");
+        if (a.nv <= 8)
+          k_arn_update<BT, 8><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 12)
+          k_arn_update<BT, 12><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 16)
+          k_arn_update<BT, 16><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 20)
+          k_arn_update<BT, 20><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 24)
+          k_arn_update<BT, 24><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 28)
+          k_arn_update<BT, 28><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 32)
+          k_arn_update<BT, 32><<<grid, kBlock, 0, s>>>(a);
+        else
+          throw Error(NC_ERR_UNSUPPORTED, "Krylov dimension above 31 is not supported");
+      } else {
+        launch_vec_mode<BT, VM_UPDATE>(a, grid, s);
+      }
+      break;
   }
   if (a.finish != FIN_NONE) {
     if (nvals > kPartStrideMax || !a.tot || !a.counter) throw Error(NC_ERR_INTERNAL, "vector pass fold not set up");
