@@ -557,7 +557,9 @@ void launch_vec(VecArgs<BT> a, const nc_solver_s* S, hipStream_t s) {
       if (a.arn_pre) {
         if (a.vout_fp32 != (std::is_same<BT, float>::value ? 1 : 0) || a.xout || !a.vout || a.finish != FIN_NONE)
           throw Error(NC_ERR_INTERNAL, "Arnoldi update pass misconfigured");
-        if (a.nv <= 8)
+        if (a.nv <= 4)
+          k_arn_update<BT, 4><<<grid, kBlock, 0, s>>>(a);
+        else if (a.nv <= 8)
           k_arn_update<BT, 8><<<grid, kBlock, 0, s>>>(a);
         else if (a.nv <= 12)
           k_arn_update<BT, 12><<<grid, kBlock, 0, s>>>(a);
