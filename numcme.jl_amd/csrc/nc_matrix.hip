@@ -55,14 +55,28 @@ __device__ inline double xprog(const Progs& p, int r, double t, const int32_t* x
 
 __global__ void k_build_count(int64_t n, int nr, const int32_t* state_conn, Groups g, TvList tv, int32_t* kmax,
                               unsigned long long* offdiag_total, unsigned long long* tv_pred) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  // reduced over the wavefront first: one lane per wave issues the atomics (a grid of
+  // per-state atomics on three addresses serialised this kernel)
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool valid = i < n;
   int cnt = 0;
-  for (int k = 0; k < g.ng; ++k) cnt += state_conn[i * nr + g.leader[k]] >= 0;
-  atomicMax(kmax, cnt);
-  if (cnt) atomicAdd(offdiag_total, (unsigned long long)cnt);
-  for (int q = 0; q < tv.n; ++q)
-    if (state_conn[i * nr + tv.r[q]] >= 0) atomicAdd(&tv_pred[q], 1ull);
+  if (valid)
+    for (int k = 0; k < g.ng; ++k) cnt += state_conn[i * nr + g.leader[k]] >= 0;
+  int wmax = cnt;
+  unsigned long long wsum = (unsigned long long)cnt;
+  for (int off = 32; off > 0; off >>= 1) {
+    wmax = max(wmax, __shfl_down(wmax, off, 64));
+    wsum += __shfl_down(wsum, off, 64);
+  }
+  const bool lead = (threadIdx.x & 63) == 0;
+  if (lead) {
+    atomicMax(kmax, wmax);
+    if (wsum) atomicAdd(offdiag_total, wsum);
+  }
+  for (int q = 0; q < tv.n; ++q) {
+    const unsigned long long b = __ballot(valid && state_conn[i * nr + tv.r[q]] >= 0);
+    if (lead && b) atomicAdd(&tv_pred[q], (unsigned long long)__popcll(b));
+  }
 }
 
 // Rows i in [n, ld) are padding: zero values and column 0, so vectorised reads of a
@@ -72,18 +86,17 @@ __global__ void k_build_count(int64_t n, int nr, const int32_t* state_conn, Grou
 // in column i; the products then run in the same order with zeros added, so the row sums
 // are unchanged, and on lattice-ordered spaces each slot's column is row + constant.
 // Otherwise a row's entries are packed into its first slots.
+// One row of the build; returns |diag| (0 for a padding row).
 template <typename VT>
-__global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int positional, const int32_t* states,
-                             const int32_t* state_conn, const uint32_t* sink_mask, const int32_t* stoich, Groups g,
-                             TvList tv, Progs pr, double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc,
-                             VT* tvv, unsigned long long* diag_max_bits) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= ld) return;
+__device__ double build_fill_row(int64_t i, int64_t n, int64_t ld, int ns, int nr, int K, int positional,
+                                 const int32_t* states, const int32_t* state_conn, const uint32_t* sink_mask,
+                                 const int32_t* stoich, const Groups& g, const TvList& tv, const Progs& pr,
+                                 double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc, VT* tvv) {
   if (i >= n) {
     diag[i] = 0.0;
     for (int k = 0; k < K; ++k) col[k * ld + i] = 0, val[k * ld + i] = (VT)0;
     for (int q = 0; q < tv.n; ++q) tvd[q * ld + i] = (VT)0, tvv[q * ld + i] = (VT)0, tvc[q * ld + i] = 0;
-    return;
+    return 0.0;
   }
   int32_t x[kMaxSpecies], y[kMaxSpecies];
   for (int s = 0; s < ns; ++s) x[s] = states[i * ns + s];
@@ -126,7 +139,7 @@ __global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int p
     val[slot * ld + i] = (VT)0;
   }
   diag[i] = -d;
-  atomicMax(diag_max_bits, (unsigned long long)__double_as_longlong(d > 0 ? d : -d));
+  const double dabs = d > 0 ? d : -d;
   for (int q = 0; q < tv.n; ++q) {
     const int r = tv.r[q];
     const int32_t j = state_conn[i * nr + r];
@@ -144,6 +157,23 @@ __global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int p
     }
     tvc[q * ld + i] = j >= 0 ? j : (int32_t)i;
   }
+  return dabs;
+}
+
+// The build's rows, one per thread, and the largest |diag| (nonnegative doubles order as
+// their bit patterns), reduced over the wavefront before its one atomic.
+template <typename VT>
+__global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int positional, const int32_t* states,
+                             const int32_t* state_conn, const uint32_t* sink_mask, const int32_t* stoich, Groups g,
+                             TvList tv, Progs pr, double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc,
+                             VT* tvv, unsigned long long* diag_max_bits) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double dabs = 0.0;
+  if (i < ld)
+    dabs = build_fill_row<VT>(i, n, ld, ns, nr, K, positional, states, state_conn, sink_mask, stoich, g, tv, pr, diag,
+                              col, val, tvd, tvc, tvv);
+  for (int off = 32; off > 0; off >>= 1) dabs = fmax(dabs, __shfl_down(dabs, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(diag_max_bits, (unsigned long long)__double_as_longlong(dabs));
 }
 
 // Range of col[i] - i over the rows of one slot whose value is nonzero: mnmx[0] = min,
