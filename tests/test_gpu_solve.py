@@ -135,3 +135,49 @@ def test_graph_replays_match_eager(gpu):
     for k in range(len(ts)):
         assert np.array_equal(runs[0].p[k].values, runs[1].p[k].values)
         assert np.array_equal(runs[0].p[k].values, runs[2].p[k].values)
+
+
+@pytest.fixture(scope="module")
+def telegraph_short_ref():
+    S, props, th = models.telegraph()
+    states = models.telegraph_states(399)
+    p0 = np.zeros(800)
+    p0[0] = 1.0
+    ts = np.array([0.0, 1.0, 5.0])
+    return states, p0, ts, solve_fixed(S, models.to_oracle(props), th, states, p0, ts)
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", 1e-8), ("fp32", 1e-6)])
+@pytest.mark.parametrize("m", [3, 5, 17, 31])
+def test_krylov_dimensions(gpu, telegraph_short_ref, m, precision, bar):
+    """Krylov dimensions whose last Arnoldi column falls in different basis-size buckets of
+    the projection and update passes (m = 3, 5, 17, 31), against the exact solution of the
+    800-state telegraph model. (m = 2 is valid too, but its steps are so short that a solve
+    takes tens of seconds.)"""
+    S, props, th = models.telegraph()
+    states, p0, ts, ref = telegraph_short_ref
+    model = nh.CmeModel(S, models.to_product(props), th)
+    out = nh.solve(model, nh.FspVectorSparse(states, p0), (0.0, 5.0),
+                   nh.KrylovExpv(m=m, tol=1e-12, precision=precision), saveat=ts)
+    for k in range(len(ts)):
+        assert l1(out.p[k].values, ref.p[k]) <= bar
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", 1e-10), ("fp32", 1e-6)])
+def test_krylov_happy_breakdown(gpu, precision, bar):
+    """A 6-state space with m = 30: the Krylov space is exhausted after at most 6 vectors
+    (Expokit's happy breakdown), and the step is then exact. Mass is conserved up to the
+    Krylov truncation leak, O(tol) per unit time (1e-9 over t = 100, as for the
+    800-state model)."""
+    S, props, th = models.telegraph()
+    states = models.telegraph_states(2)
+    p0 = np.zeros(len(states))
+    p0[0] = 1.0
+    ts = np.array([0.0, 1.0, 10.0, 100.0])
+    ref = solve_fixed(S, models.to_oracle(props), th, states, p0, ts)
+    model = nh.CmeModel(S, models.to_product(props), th)
+    out = nh.solve(model, nh.FspVectorSparse(states, p0), (0.0, 100.0),
+                   nh.KrylovExpv(m=30, tol=1e-12, precision=precision), saveat=ts)
+    for k in range(len(ts)):
+        assert l1(out.p[k].values, ref.p[k]) <= bar
+        assert abs(out.p[k].sum() + out.sinks[k].sum() - 1.0) <= max(bar, 1e-9)
