@@ -82,3 +82,35 @@ def test_batch_rows_per_thread_rejected(gpu):
     B = nh.BatchFsp(mats)
     with pytest.raises(nh.ArgumentError):
         B.set_rows_per_thread(3)
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", 1e-8), ("fp32", 1e-6)])
+def test_batch_twelve_slots_vs_oracle(gpu, precision, bar):
+    # three-species instances with 12 distinct stoichiometries next to telegraph instances:
+    # the concatenated slot width is 12, above the fixed-slot-count batched kernels
+    from test_gpu_space_matrix import twelve_slot_model
+    inst = []
+    for k, bounds in enumerate(([5, 4, 3], [7, 3, 2], [4, 4, 4])):
+        S, props, th = twelve_slot_model()
+        inst.append((S, props, th * (1.0 + 0.3 * k), models.box_states(bounds)))
+    for nmax in (30, 55):
+        S, props, th = models.telegraph()
+        inst.append((S, props, th, models.telegraph_states(nmax)))
+    mats, u0s = [], []
+    for S, props, th, states in inst:
+        sp = nh.StateSpaceSparse(S, states)
+        mats.append(nh.FspMatrixSparse(sp, models.to_product(props), th, precision=precision))
+        u0 = np.zeros(len(states) + S.shape[1])
+        u0[0] = 1.0
+        u0s.append(u0)
+    B = nh.BatchFsp(mats)
+    ts = np.array([0.0, 0.5, 2.0])
+    snaps = B.solve(u0s, ts, rtol=1e-10, atol=1e-16)
+    for (S, props, th, states), snap in zip(inst, snaps):
+        p0 = np.zeros(len(states))
+        p0[0] = 1.0
+        ref = solve_fixed(S, models.to_oracle(props), th, states, p0, ts)
+        n = len(states)
+        for k in range(len(ts)):
+            assert np.abs(snap[k][:n] - ref.p[k]).sum() <= bar
+            assert np.abs(snap[k][n:] - ref.sinks[k]).sum() <= bar
