@@ -229,3 +229,43 @@ def toggle_grads():
     g[(1, 4)] = DTimeInvariant(lambda x, p: 1.0 * x[0])
     g[(3, 4)] = DTimeInvariant(lambda x, p: 1.0 * x[1])
     return g
+
+
+# ---- twelve distinct stoichiometries (slot counts above the fixed-count kernels) ---------
+def twelve_slot():
+    """Three species with births, deaths and all six conversions: 12 reactions with distinct
+    stoichiometries, so a generator on a box has 12 slots."""
+    cols, props = [], []
+    for s in range(3):
+        e = np.zeros(3, dtype=np.int64)
+        e[s] = 1
+        cols.append(e.copy())
+        props.append(("ti", (lambda s: lambda x, p: p[0] * (1.0 + 0.1 * s))(s)))
+        cols.append(-e)
+        props.append(("ti", (lambda s: lambda x, p: p[1] * x[s])(s)))
+    for a in range(3):
+        for b in range(3):
+            if a == b:
+                continue
+            c = np.zeros(3, dtype=np.int64)
+            c[a], c[b] = -1, 1
+            cols.append(c)
+            props.append(("ti", (lambda a, b: lambda x, p: p[2] * (1 + a + 2 * b) * x[a])(a, b)))
+    return np.stack(cols, axis=1), props, np.array([2.0, 0.3, 0.05])
+
+
+def twelve_slot_grads():
+    """Propensity partials of twelve_slot(), keyed (reaction, parameter): births
+    p0 (1 + 0.1 s), deaths p1 x_s, conversions p2 (1 + a + 2 b) x_a."""
+    from oracle.sensmatrix import DTimeInvariant
+    g, r = {}, 0
+    for s in range(3):
+        g[(r, 0)] = DTimeInvariant((lambda s: lambda x, p: (1.0 + 0.1 * s) + 0.0 * x[0])(s))
+        g[(r + 1, 1)] = DTimeInvariant((lambda s: lambda x, p: 1.0 * x[s])(s))
+        r += 2
+    for a in range(3):
+        for b in range(3):
+            if a != b:
+                g[(r, 2)] = DTimeInvariant((lambda a, b: lambda x, p: (1.0 + a + 2 * b) * x[a])(a, b))
+                r += 1
+    return g

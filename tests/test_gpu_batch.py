@@ -88,10 +88,9 @@ def test_batch_rows_per_thread_rejected(gpu):
 def test_batch_twelve_slots_vs_oracle(gpu, precision, bar):
     # three-species instances with 12 distinct stoichiometries next to telegraph instances:
     # the concatenated slot width is 12, above the fixed-slot-count batched kernels
-    from test_gpu_space_matrix import twelve_slot_model
     inst = []
     for k, bounds in enumerate(([5, 4, 3], [7, 3, 2], [4, 4, 4])):
-        S, props, th = twelve_slot_model()
+        S, props, th = models.twelve_slot()
         inst.append((S, props, th * (1.0 + 0.3 * k), models.box_states(bounds)))
     for nmax in (30, 55):
         S, props, th = models.telegraph()

@@ -154,37 +154,18 @@ def test_reference_adaptive_sens_solve(gpu):
         assert np.array_equal(out.sinks[first], OA.restart_sinks(out.sinks[last], t_ev, 400.0, 1e-6))
 
 
-def _twelve_slot_grads():
-    # propensity partials of tests/test_gpu_space_matrix.twelve_slot_model, keyed
-    # (reaction, parameter): births p0 (1 + 0.1 s), deaths p1 x_s, conversions
-    # p2 (1 + a + 2 b) x_a, in the model's reaction order
-    from oracle.sensmatrix import DTimeInvariant
-    g, r = {}, 0
-    for s in range(3):
-        g[(r, 0)] = DTimeInvariant((lambda s: lambda x, p: (1.0 + 0.1 * s) + 0.0 * x[0])(s))
-        g[(r + 1, 1)] = DTimeInvariant((lambda s: lambda x, p: 1.0 * x[s])(s))
-        r += 2
-    for a in range(3):
-        for b in range(3):
-            if a != b:
-                g[(r, 2)] = DTimeInvariant((lambda a, b: lambda x, p: (1.0 + a + 2 * b) * x[a])(a, b))
-                r += 1
-    return g
-
-
 @pytest.mark.parametrize("order", ["lattice", "shuffled"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_sens_matvec_twelve_slots(gpu, order, precision):
     # 12 slots: the generic (runtime slot count) sensitivity kernel, columns computed
     # (lattice order) or loaded (shuffled)
-    from test_gpu_space_matrix import twelve_slot_model
-    S, props, th = twelve_slot_model()
+    S, props, th = models.twelve_slot()
     model = nh.CmeModelWithSensitivity(nh.CmeModel(S, models.to_product(props), th))
     states = models.box_states([6, 5, 4])
     if order == "shuffled":
         states = states[np.random.default_rng(8).permutation(len(states))]
     SA = nh.ForwardSensFspMatrixSparse(model, nh.StateSpaceSparse(S, states), precision=precision)
-    ref = SensFspMatrix(StateSpace(S, states), models.to_oracle(props), th, _twelve_slot_grads())
+    ref = SensFspMatrix(StateSpace(S, states), models.to_oracle(props), th, models.twelve_slot_grads())
     v = np.random.default_rng(9).random(ref.n * (len(th) + 1))
     out = SA.matvec(0.0, v)
     vin = v if precision == "fp64" else v.astype(np.float32).astype(np.float64)

@@ -269,34 +269,12 @@ def test_long_sink_rows_vs_oracle(gpu, precision):
         assert abs(w[len(states)] - wo[len(states)]) <= tol * abs(wo[len(states)])
 
 
-def twelve_slot_model():
-    """Three species with births, deaths and all six conversions: 12 reactions with distinct
-    stoichiometries, so the mat-vec has 12 slots (above the fixed-slot-count kernels)."""
-    cols, props = [], []
-    for s in range(3):
-        e = np.zeros(3, dtype=np.int64)
-        e[s] = 1
-        cols.append(e.copy())
-        props.append(("ti", (lambda s: lambda x, p: p[0] * (1.0 + 0.1 * s))(s)))
-        cols.append(-e)
-        props.append(("ti", (lambda s: lambda x, p: p[1] * x[s])(s)))
-    for a in range(3):
-        for b in range(3):
-            if a == b:
-                continue
-            c = np.zeros(3, dtype=np.int64)
-            c[a], c[b] = -1, 1
-            cols.append(c)
-            props.append(("ti", (lambda a, b: lambda x, p: p[2] * (1 + a + 2 * b) * x[a])(a, b)))
-    return np.stack(cols, axis=1), props, np.array([2.0, 0.3, 0.05])
-
-
 @pytest.mark.parametrize("order", ["lattice", "shuffled"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_twelve_slot_matvec_vs_oracle(gpu, order, precision):
     # K = 12 > 8 takes the generic kernel (runtime slot count, per-slot column kind): in
     # lattice order every slot's column is computed, in a shuffled order every one loaded
-    S, props, th = twelve_slot_model()
+    S, props, th = models.twelve_slot()
     states = models.box_states([9, 8, 7])
     if order == "shuffled":
         states = states[np.random.default_rng(5).permutation(len(states))]

@@ -49,6 +49,7 @@ CASES = {
     "telegraph_sens": (models.telegraph_sens, models.telegraph_sens_grads, models.telegraph_states(30)),
     "gene4": (models.gene4, models.gene4_grads, models.gene4_states(5)),
     "toggle": (models.toggle, models.toggle_grads, models.box_states([40, 40])),
+    "twelve_slot": (models.twelve_slot, models.twelve_slot_grads, models.box_states([6, 5, 4])),
 }
 
 
