@@ -35,7 +35,9 @@ constexpr int kPartStrideMax = 65;  // partial sums per block: up to 2 x 32 dots
 // update with arn_pre set, applies the Arnoldi finishing step itself)
 enum Finish : int { FIN_NONE = 0, FIN_LAST, FIN_BETA, FIN_SUMS, FIN_FOLD };
 constexpr double kBreakdownTol = 1e-7;   // Expokit's happy-breakdown threshold
-constexpr int kFinBlock = 1024;  // threads of the one-block partial-sum fold
+// threads of each fold workgroup: one partial per thread at the 1,024-workgroup grid (a
+// thread looping over several would wait for each load in turn)
+constexpr int kFinBlock = 1024;
 constexpr unsigned kVecBlocks = 512;  // grid of the vector passes
 constexpr int kLoadChunk = 8;  // basis vectors whose loads a vector pass issues together
 constexpr int kJacobiSweeps = 3;  // G h = a with G = I + O(1e-7): the error falls 1e-6 per sweep
