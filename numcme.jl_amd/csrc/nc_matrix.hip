@@ -408,8 +408,26 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
     const int r = a.n_chunks ? a.chunk_row[c] : c;
     const int64_t e0 = a.n_chunks ? a.chunk_start[c] : a.sink_off[r];
     const int64_t e1 = a.n_chunks ? a.chunk_start[c + 1] : a.sink_off[r + 1];
+    // four entries per iteration (the thread's entries e, e + 256, ... in the same order as
+    // a plain loop), their loads clamped and issued together, then their gathers
+    constexpr int U = 4;
     double acc = 0.0;
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) acc += (double)a.sink_val[e] * (double)v[a.sink_src[e]];
+    for (int64_t eb = e0 + threadIdx.x; eb < e1; eb += (int64_t)kBlock * U) {
+      int32_t src[U];
+      double w[U], g[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t e = eb + (int64_t)u * kBlock;
+        const int64_t ec = e < e1 ? e : e1 - 1;
+        src[u] = a.sink_src[ec];
+        w[u] = (double)a.sink_val[ec];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) g[u] = (double)v[src[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (eb + (int64_t)u * kBlock < e1) acc += w[u] * g[u];
+    }
     __shared__ double part[kBlock / 64];
     __shared__ int s_last;
     acc = wave_sum(acc);
