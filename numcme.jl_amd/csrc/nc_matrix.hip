@@ -455,16 +455,20 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
     if (a.n_chunks) {
       __syncthreads();
       if (s_last) {
-        // sink row q = sum of its chunk partials in chunk order, times its time factor
-        const int q = threadIdx.x;
-        if (q < a.nr) {
+        // sink row q = sum of its chunk partials (lane l takes chunks l, l + 64, ..., then a
+        // fixed shuffle tree: deterministic), times its time factor; one wave per row
+        const int lane = threadIdx.x & 63;
+        for (int q = threadIdx.x >> 6; q < a.nr; q += kBlock / 64) {
           double s = 0.0;
-          for (int k = a.chunk_first[q]; k < a.chunk_first[q + 1]; ++k)
+          for (int k = a.chunk_first[q] + lane; k < a.chunk_first[q + 1]; k += 64)
             s += __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<unsigned long long*>(a.chunk_part) + k,
                                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-          s *= a.sink_scale[q];
-          const int64_t o = a.n + q;
-          out[o] = (OT)(accumulate ? (double)out[o] + s : s);
+          s = wave_sum(s);
+          if (lane == 0) {
+            s *= a.sink_scale[q];
+            const int64_t o = a.n + q;
+            out[o] = (OT)(accumulate ? (double)out[o] + s : s);
+          }
         }
         if (threadIdx.x == 0) __hip_atomic_store(a.chunk_arrivals, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
@@ -711,8 +715,8 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   }
   A->sink_off_h[nr] = total;
   A->n_sink_entries = total;
-  // long sink rows: 4096-entry chunks, each reduced by its own workgroup (the host
-  // vectors stay alive until the build's final synchronisation)
+  // long sink rows (above 8192 entries): 4096-entry chunks, each reduced by its own
+  // workgroup (the host vectors stay alive until the build's final synchronisation)
   constexpr int64_t kSinkChunk = 4096, kChunkFrom = 8192;
   std::vector<int64_t> chunk_start_h;
   std::vector<int32_t> chunk_row_h, chunk_first_h;
