@@ -190,18 +190,50 @@ __global__ __launch_bounds__(kBlock) void k_sens_matvec(SensArgs<VT> a, const VT
     const int bb = sb / a.nr, r = sb % a.nr;
     const int64_t e0 = a.sink_off[r], e1 = a.sink_off[r + 1];
     const VT* vb = v + bb * a.vstride;
+    // four entries per iteration, their loads clamped and issued together, then their
+    // gathers (the thread's entries keep the order of a plain loop)
+    constexpr int U = 4;
     double acc = 0.0;
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) acc += (double)a.sink_val[e] * (double)vb[a.sink_src[e]];
+    for (int64_t eb = e0 + threadIdx.x; eb < e1; eb += (int64_t)kBlock * U) {
+      int32_t src[U];
+      double w[U], g[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t e = eb + (int64_t)u * kBlock, ec = e < e1 ? e : e1 - 1;
+        src[u] = a.sink_src[ec];
+        w[u] = (double)a.sink_val[ec];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) g[u] = (double)vb[src[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (eb + (int64_t)u * kBlock < e1) acc += w[u] * g[u];
+    }
     acc *= a.sink_scale[r];
     if (bb >= 1) {
       for (int p = a.pstart[bb]; p < a.pstart[bb + 1]; ++p) {
         const PairMeta m = a.meta[p];
         if (m.r != r) continue;
         double s1 = 0.0, s2 = 0.0;
-        for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
-          const double v0 = (double)v[a.sink_src[e]];
-          if (m.arr >= 0) s1 += (double)a.dsink[m.sink_off + (e - e0)] * v0;
-          if (m.q >= 0) s2 += (double)a.sink_val[e] * v0;
+        const VT* ds = m.arr >= 0 ? a.dsink + m.sink_off : a.sink_val + e0;  // (dsink read only if arr >= 0)
+        for (int64_t eb = e0 + threadIdx.x; eb < e1; eb += (int64_t)kBlock * U) {
+          int32_t src[U];
+          double d[U], w[U], g[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int64_t e = eb + (int64_t)u * kBlock, ec = e < e1 ? e : e1 - 1;
+            src[u] = a.sink_src[ec];
+            d[u] = (double)ds[ec - e0];
+            w[u] = (double)a.sink_val[ec];
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) g[u] = (double)v[src[u]];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if (eb + (int64_t)u * kBlock >= e1) continue;
+            if (m.arr >= 0) s1 += d[u] * g[u];
+            if (m.q >= 0) s2 += w[u] * g[u];
+          }
         }
         acc += a.dscale[p] * s1 + a.sfscale[p] * s2;
       }
