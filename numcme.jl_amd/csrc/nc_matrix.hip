@@ -55,27 +55,62 @@ __device__ inline double xprog(const Progs& p, int r, double t, const int32_t* x
 
 __global__ void k_build_count(int64_t n, int nr, const int32_t* state_conn, Groups g, TvList tv, int32_t* kmax,
                               unsigned long long* offdiag_total, unsigned long long* tv_pred) {
-  // reduced over the wavefront first: one lane per wave issues the atomics (a grid of
-  // per-state atomics on three addresses serialised this kernel)
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const bool valid = i < n;
-  int cnt = 0;
-  if (valid)
-    for (int k = 0; k < g.ng; ++k) cnt += state_conn[i * nr + g.leader[k]] >= 0;
-  int wmax = cnt;
-  unsigned long long wsum = (unsigned long long)cnt;
+  // Grid-stride over the states with a bounded grid, reduced over the workgroup: one atomic
+  // per workgroup and counter (same-address atomics from every wave serialised this
+  // kernel). The first eight groups' connectivity loads of a row go out together.
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int wmax = 0;
+  unsigned long long wsum = 0;
+  unsigned long long tvn[4] = {0, 0, 0, 0};  // time-varying slots 0..3 (the rest per row below)
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    int cnt = 0;
+    if (g.ng > 0) {
+      const int64_t row = i * nr;
+      int32_t c[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[k] = state_conn[row + g.leader[min(k, g.ng - 1)]];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) cnt += (k < g.ng && c[k] >= 0) ? 1 : 0;
+      for (int k = 8; k < g.ng; ++k) cnt += state_conn[row + g.leader[k]] >= 0;
+    }
+    wmax = max(wmax, cnt);
+    wsum += (unsigned long long)cnt;
+    for (int q = 0; q < tv.n; ++q) {
+      const bool has = state_conn[i * nr + tv.r[q]] >= 0;
+      if (q < 4)
+        tvn[q] += has ? 1ull : 0ull;
+      else if (has)
+        atomicAdd(&tv_pred[q], 1ull);
+    }
+  }
   for (int off = 32; off > 0; off >>= 1) {
     wmax = max(wmax, __shfl_down(wmax, off, 64));
     wsum += __shfl_down(wsum, off, 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tvn[q] += __shfl_down(tvn[q], off, 64);
   }
-  const bool lead = (threadIdx.x & 63) == 0;
-  if (lead) {
-    atomicMax(kmax, wmax);
-    if (wsum) atomicAdd(offdiag_total, wsum);
+  __shared__ int s_max[kBlock / 64];
+  __shared__ unsigned long long s_sum[kBlock / 64], s_tv[4][kBlock / 64];
+  if (lane == 0) {
+    s_max[wave] = wmax;
+    s_sum[wave] = wsum;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s_tv[q][wave] = tvn[q];
   }
-  for (int q = 0; q < tv.n; ++q) {
-    const unsigned long long b = __ballot(valid && state_conn[i * nr + tv.r[q]] >= 0);
-    if (lead && b) atomicAdd(&tv_pred[q], (unsigned long long)__popcll(b));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int bmax = 0;
+    unsigned long long bsum = 0, btv[4] = {0, 0, 0, 0};
+    for (int w = 0; w < kBlock / 64; ++w) {
+      bmax = max(bmax, s_max[w]);
+      bsum += s_sum[w];
+      for (int q = 0; q < 4; ++q) btv[q] += s_tv[q][w];
+    }
+    atomicMax(kmax, bmax);
+    if (bsum) atomicAdd(offdiag_total, bsum);
+    for (int q = 0; q < 4 && q < tv.n; ++q)
+      if (btv[q]) atomicAdd(&tv_pred[q], btv[q]);
   }
 }
 
@@ -167,13 +202,21 @@ __global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int p
                              const int32_t* state_conn, const uint32_t* sink_mask, const int32_t* stoich, Groups g,
                              TvList tv, Progs pr, double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc,
                              VT* tvv, unsigned long long* diag_max_bits) {
+  // one row per thread (the rows' propensity programs keep every wave busy); |diag|
+  // reduced over the workgroup: one atomic per workgroup
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double dabs = 0.0;
   if (i < ld)
     dabs = build_fill_row<VT>(i, n, ld, ns, nr, K, positional, states, state_conn, sink_mask, stoich, g, tv, pr, diag,
                               col, val, tvd, tvc, tvv);
   for (int off = 32; off > 0; off >>= 1) dabs = fmax(dabs, __shfl_down(dabs, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(diag_max_bits, (unsigned long long)__double_as_longlong(dabs));
+  __shared__ double s_d[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) s_d[threadIdx.x >> 6] = dabs;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) dabs = fmax(dabs, s_d[w]);
+    atomicMax(diag_max_bits, (unsigned long long)__double_as_longlong(dabs));
+  }
 }
 
 // Range of col[i] - i over the rows of one slot whose value is nonzero: mnmx[0] = min,
@@ -648,7 +691,7 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   NC_CHECK_HIP(hipMemsetAsync(d_kmax.p, 0, sizeof(int32_t), s));
   NC_CHECK_HIP(hipMemsetAsync(d_counts.p, 0, sizeof(unsigned long long) * (2 + kMaxReactions), s));
   if (n)
-    k_build_count<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, nr, sp->state_conn.p, g, tv, d_kmax.p, d_counts.p,
+    k_build_count<<<std::min<unsigned>(grid_for(n, kBlock), 1024), kBlock, 0, s>>>(n, nr, sp->state_conn.p, g, tv, d_kmax.p, d_counts.p,
                                                          d_counts.p + 2);
   NC_CHECK_HIP(hipGetLastError());
   unsigned long long counts[2 + kMaxReactions];
