@@ -219,19 +219,39 @@ __global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int p
   }
 }
 
-// Range of col[i] - i over the rows of one slot whose value is nonzero: mnmx[0] = min,
-// mnmx[1] = max (a grid-stride pass, wave and workgroup reductions, one atomic pair per
-// workgroup).
+// Range of col[i] - i over the rows of slot k = blockIdx.y whose value is nonzero:
+// mnmx[2k] = min, mnmx[2k + 1] = max (a grid-stride pass taking four rows per step with
+// their loads issued together, wave and workgroup reductions, one atomic pair per
+// workgroup). Every slot in one launch.
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t n, const int32_t* col, const VT* val, int32_t* mnmx) {
+__global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t n, int64_t ld, const int32_t* col_all,
+                                                        const VT* val_all, int32_t* mnmx_all) {
+  constexpr int U = 4;
+  const int32_t* col = col_all + (int64_t)blockIdx.y * ld;
+  const VT* val = val_all + (int64_t)blockIdx.y * ld;
+  int32_t* mnmx = mnmx_all + 2 * blockIdx.y;
   __shared__ int32_t slo[kBlock / 64], shi[kBlock / 64];
   int32_t lo = INT32_MAX, hi = INT32_MIN;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    if (val[i] != (VT)0) {
-      const int32_t d = col[i] - (int32_t)i;
-      lo = min(lo, d);
-      hi = max(hi, d);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t ib = blockIdx.x * (int64_t)kBlock + threadIdx.x; ib < n; ib += stride * U) {
+    VT w[U];
+    int32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = ib + u * stride, ic = i < n ? i : n - 1;
+      w[u] = val[ic];
+      c[u] = col[ic];
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = ib + u * stride;
+      if (i < n && w[u] != (VT)0) {
+        const int32_t d = c[u] - (int32_t)i;
+        lo = min(lo, d);
+        hi = max(hi, d);
+      }
+    }
+  }
   for (int off = 32; off > 0; off >>= 1) {
     lo = min(lo, __shfl_xor(lo, off, 64));
     hi = max(hi, __shfl_xor(hi, off, 64));
@@ -730,9 +750,8 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
     std::vector<int32_t> dia_init(2 * kd);
     for (int k = 0; k < kd; ++k) dia_init[2 * k] = INT32_MAX, dia_init[2 * k + 1] = INT32_MIN;
     NC_CHECK_HIP(hipMemcpyAsync(d_dia.p, dia_init.data(), sizeof(int32_t) * 2 * kd, hipMemcpyHostToDevice, s));
-    for (int k = 0; k < kd; ++k)
-      k_dia_detect<VT><<<std::min<unsigned>(grid_for(n, kBlock), 256), kBlock, 0, s>>>(
-          n, A->col.p + (int64_t)k * ld, reinterpret_cast<const VT*>(A->val.p) + (int64_t)k * ld, d_dia.p + 2 * k);
+    k_dia_detect<VT><<<dim3(std::min<unsigned>(grid_for(n, kBlock), 256), (unsigned)kd), kBlock, 0, s>>>(
+        n, ld, A->col.p, reinterpret_cast<const VT*>(A->val.p), d_dia.p);
     NC_CHECK_HIP(hipGetLastError());
     NC_CHECK_HIP(hipMemcpyAsync(dia_h, d_dia.p, sizeof(int32_t) * 2 * kd, hipMemcpyDeviceToHost, s));
   }
