@@ -432,8 +432,9 @@ def main():
                          "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs one FSP solve per GPU; multi-GPU runs are not part of this "
-                                   "build (see DESIGN.md for how it would shard)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this engine runs one FSP solve per GPU on a single device; multi-GPU runs "
+                                   "are not supported (the state-index partition that would shard one solve "
+                                   "is documented with the library)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numcme_hip as nh

@@ -37,7 +37,7 @@ enum {
   NC_ERR_HIP = 2,         /* HIP runtime error                                            */
   NC_ERR_ALLOC = 3,       /* device or host allocation failed                             */
   NC_ERR_OVERFLOW = 4,    /* a size or index exceeds the library's 32-bit limits          */
-  NC_ERR_UNSUPPORTED = 5, /* feature outside what this build supports                    */
+  NC_ERR_UNSUPPORTED = 5, /* feature outside what the library supports                  */
   NC_ERR_NUMERIC = 6,     /* integrator failure (step size underflow, too many steps)     */
   NC_ERR_INTERNAL = 7,
   NC_ERR_NO_DEVICE = 8

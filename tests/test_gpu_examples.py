@@ -1,5 +1,6 @@
-"""The Python usage example of INTEGRATION.md, run as written (telegraph model, adaptive
-solve, then training records, device training and a learned-adapter solve)."""
+"""An end-to-end usage example written as a user of the Python host package would write it
+(telegraph model, adaptive solve, then training records, device training and a
+learned-adapter solve), with its results checked."""
 import numpy as np
 import pytest
 
