@@ -513,7 +513,7 @@ nc_sensmatrix_s* sensmatrix_create(nc_space_s* sp, nc_model_s* m, int precision)
     }
     SA->tstart[W] = (int32_t)terms.size();
     NC_REQUIRE((int)terms.size() <= kMaxTerms, NC_ERR_UNSUPPORTED, "too many derivative terms");
-    if (terms.empty()) terms.push_back(Term{0, 0, -1, 0, 0});
+    if (terms.empty()) terms.push_back(Term{0, 0, -1, 0, 0, 0, 0});
     SA->terms_dev.alloc(terms.size() * sizeof(Term));
     NC_CHECK_HIP(hipMemcpyAsync(SA->terms_dev.p, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice, s));
   }

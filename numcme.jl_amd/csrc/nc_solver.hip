@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_vec(VecArgs<BT> a) {
     if (lane == 0) red[ndots][wave] = v;
   }
   __syncthreads();
-  if (threadIdx.x <= ndots) {
+  if ((int)threadIdx.x <= ndots) {
     double s = 0.0;
     for (int w = 0; w < kBlock / 64; ++w) s += red[threadIdx.x][w];
     a.partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigne
     }
     __syncthreads();
     if (!s_last) return;
-    if (threadIdx.x < nvals)
+    if ((int)threadIdx.x < nvals)
       tot[threadIdx.x] = __longlong_as_double((long long)__hip_atomic_load(
           reinterpret_cast<unsigned long long*>(a.tot) + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     if (threadIdx.x == 0) __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
