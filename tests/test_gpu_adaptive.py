@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import models
+from fspcompare import l1_distance
 import numcme_hip as nh
 from numcme_hip import _lib as L
 from oracle import adaptive as OA
@@ -217,10 +218,7 @@ def test_adaptive_fp32_close_to_fp64(gpu):
     a, b = outs["fp64"], outs["fp32"]
     assert len(a) == len(b)
     for k in range(len(a)):
-        da = {tuple(x): v for x, v in zip(a.p[k].states.tolist(), a.p[k].values)}
-        db = {tuple(x): v for x, v in zip(b.p[k].states.tolist(), b.p[k].values)}
-        keys = set(da) | set(db)
-        assert sum(abs(da.get(x, 0.0) - db.get(x, 0.0)) for x in keys) <= 1e-6
+        assert l1_distance(a.p[k].states, a.p[k].values, b.p[k].states, b.p[k].values) <= 1e-6
 
 
 def test_adaptive_cascade_invariants(gpu):

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import models
+from fspcompare import l1_distance
 import numcme_hip as nh
 from oracle import learned as ol
 from oracle.statespace import StateSpace
@@ -141,9 +142,7 @@ def test_trained_adapter_keeps_the_sink_bound(gpu):
         assert s.sum() <= fsptol * t / tend + 1e-13
         assert 1.0 - fsptol <= p.sum() + s.sum() <= 1.0 + 1e-9
     # both solutions are within fsptol of the exact one in l1 (plus integration error)
-    a = {tuple(x): v for x, v in zip(plain.p[-1].states.tolist(), plain.p[-1].values)}
-    b = {tuple(x): v for x, v in zip(learned.p[-1].states.tolist(), learned.p[-1].values)}
-    d = sum(abs(a.get(x, 0.0) - b.get(x, 0.0)) for x in set(a) | set(b))
+    d = l1_distance(plain.p[-1].states, plain.p[-1].values, learned.p[-1].states, learned.p[-1].values)
     assert d <= 2 * fsptol + 1e-8
 
 
