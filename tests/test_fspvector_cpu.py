@@ -67,3 +67,20 @@ def test_dense_array():
     expect[2, 0], expect[0, 1], expect[1, 1] = 0.5, 0.25, 0.25
     assert np.array_equal(a, expect)
     assert p.state2idx[(0, 1)] == 2  # 1-based, as the reference's state dictionary
+
+
+def test_output_indexing():
+    """FspOutputSparse indexing (fspoutput.jl:195-209, 0-based here): slices, index lists,
+    the last entry, and the reference's error past the end."""
+    out = nh.FspOutputSparse()
+    for k in range(3):
+        out.t.append(float(k))
+        out.p.append(nh.FspVectorSparse([[k]], [1.0]))
+        out.sinks.append(np.zeros(2))
+    assert len(out) == 3
+    s = out[1]
+    assert s.t == 1.0 and s.p.states.tolist() == [[1]] and s.sinks.tolist() == [0.0, 0.0]
+    assert [x.t for x in out[[2, 0]]] == [2.0, 0.0]
+    assert out[-1].t == 2.0
+    with pytest.raises(ArgumentError, match="Requested index exceeds array limit."):
+        out[3]
