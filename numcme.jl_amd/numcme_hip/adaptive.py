@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+from fractions import Fraction
 from dataclasses import dataclass
 
 import numpy as np
@@ -120,10 +121,17 @@ class AdaptiveFspSparse:
 
 def _saveat_points(saveat, tspan):
     if isinstance(saveat, (int, float)) and not isinstance(saveat, bool):
-        # Julia's tspan[1]:saveat:tspan[2]
+        # Julia's tspan[1]:saveat:tspan[2]. Julia evaluates a float range through exact
+        # rationals when start, step and stop have short ones (0:0.1:1 holds 0.3, not
+        # 3 * 0.1): here the shortest decimal of each float, used when it converts back
+        # exactly, gives elements start + i*step correctly rounded from the rationals.
         a, b, h = float(tspan[0]), float(tspan[1]), float(saveat)
         if h <= 0:
             raise ArgumentError(L.NC_ERR_ARG, "saveat step must be positive")
+        fa, fb, fh = (Fraction(repr(x)) for x in (a, b, h))
+        if float(fa) == a and float(fb) == b and float(fh) == h and max(fa.denominator, fh.denominator) <= 10**9:
+            k = math.floor((fb - fa) / fh)
+            return np.array([float(fa + i * fh) for i in range(k + 1)], dtype=np.float64)
         k = math.floor((b - a) / h * (1 + 1e-12) + 1e-12)
         return np.array([a + i * h for i in range(k + 1)], dtype=np.float64)
     return np.asarray(list(saveat), dtype=np.float64)

@@ -84,3 +84,19 @@ def test_output_indexing():
     assert out[-1].t == 2.0
     with pytest.raises(ArgumentError, match="Requested index exceeds array limit."):
         out[3]
+
+
+def test_saveat_step_range():
+    """A numeric saveat is Julia's range tspan[1]:saveat:tspan[2], whose float elements come
+    from exact rationals: 0:0.1:1 holds 0.3 and 0.7, not 3 * 0.1 and 7 * 0.1."""
+    from numcme_hip.adaptive import _saveat_points
+
+    r = _saveat_points(0.1, (0.0, 1.0))
+    assert r.tolist() == [0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0]
+    assert 3 * 0.1 != 0.3  # the naive product is off by one ulp
+    assert _saveat_points(20, (0.0, 120.0)).tolist() == [0.0, 20.0, 40.0, 60.0, 80.0, 100.0, 120.0]
+    assert _saveat_points(0.25, (0.5, 1.6)).tolist() == [0.5, 0.75, 1.0, 1.25, 1.5]  # stop not on the grid
+    assert _saveat_points(1.0, (2.0, 1.0)).tolist() == []  # empty range
+    assert _saveat_points([0.0, 5.0], (0.0, 10.0)).tolist() == [0.0, 5.0]
+    with pytest.raises(ArgumentError):
+        _saveat_points(0.0, (0.0, 1.0))
