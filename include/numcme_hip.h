@@ -430,6 +430,42 @@ int nc_records_count(nc_records R, int64_t* n);
 int nc_records_get(nc_records R, float* features, float* labels);
 int nc_records_clear(nc_records R);
 
+/* ======================================================================================
+ * Sparse FSP vectors on the device — replace FspVectorSparse (src/fspvector/fspvector.jl).
+ * A vector holds n states (n x ns int32, row-major) and their n double values. Snapshots
+ * of a solve are handed out as vectors without leaving the device; marginals and dense
+ * conversion run on the device.
+ * ==================================================================================== */
+typedef struct nc_fspvec_s* nc_fspvec;
+/* FspVectorSparse(states, values) — fspvector.jl:28-35 (the caller checks equal lengths,
+ * :29). The states are kept in the given order, duplicates included. */
+int nc_fspvec_create(int32_t ns, int64_t n, const int32_t* states, const double* values, nc_fspvec* out);
+int nc_fspvec_destroy(nc_fspvec v);
+/* nnz(p) — fspvector.jl:24 — and the species count. */
+int nc_fspvec_size(nc_fspvec v, int64_t* n, int32_t* ns);
+/* get_states / get_values — fspvector.jl:20-21. states: n x ns; either may be null. */
+int nc_fspvec_get(nc_fspvec v, int32_t* states, double* values);
+/* sum(p) — fspvector.jl:57-59 (a fixed-order tree sum). */
+int nc_fspvec_sum(nc_fspvec v, double* total);
+/* sum(p, dims) — fspvector.jl:66-99. dims: n_dims 0-based species (repeats allowed) summed
+ * out; the result lists the reduced states in order of first appearance, each value the
+ * left-to-right sum of its states' values in state order (the reference's loop, bit for
+ * bit). NC_ERR_ARG for a dimension outside 0..ns-1 (the reference's ArgumentError). */
+int nc_fspvec_marginal(nc_fspvec v, int32_t n_dims, const int32_t* dims, nc_fspvec* out);
+/* Array(p) — fspvector.jl:107-129. bounds (ns entries) receives max count + 1 per species;
+ * with out non-null (capacity >= prod(bounds) doubles) the dense array is written
+ * column-major (species 0 fastest, Julia's layout), zeros elsewhere; for a repeated state
+ * the last one wins. NC_ERR_ARG for an empty vector (the reference's ArgumentError). */
+int nc_fspvec_dense(nc_fspvec v, int64_t* bounds, int64_t capacity, double* out);
+/* Snapshot k of a fixed-space solve as a device vector over the space's states: block 0 is
+ * p, block b >= 1 the sensitivity to parameter b-1 of a sensitivity solver (the p /
+ * S_b fields of FspOutputSparse, fspoutput.jl:19-23, and sensoutputsparse.jl:20-68).
+ * space: the space the solver's generator was built from. sinks (nr doubles) may be null. */
+int nc_solver_get_fspvec(nc_solver s, int32_t k, int32_t block, nc_space space, nc_fspvec* p, double* sinks);
+/* Snapshot k of an adaptive solve as a device vector: param == -1 gives p, param >= 0 the
+ * sensitivity to that parameter. sinks (nr doubles) may be null. */
+int nc_output_get_fspvec(nc_output o, int32_t k, int32_t param, nc_fspvec* p, double* sinks);
+
 /* ---- helpers exposed for testing --------------------------------------------------- */
 /* Dense matrix exponential of an n x n row-major matrix (Pade 13 scaling and squaring,
  * Higham 2005) — the host-side Hessenberg exponential of the Krylov integrator. */

@@ -22,6 +22,7 @@
 #include <memory>
 #include <string>
 
+#include "nc_fspvec.h"
 #include "nc_learned.h"
 #include "nc_solver.h"
 
@@ -226,7 +227,7 @@ struct nc_output_s {
   int P = 0;  // sensitivity parameters (each snapshot holds P+1 blocks of n + nr)
   struct Space {
     int64_t n = 0;
-    nc::DevBuf<int32_t> states;
+    std::shared_ptr<nc::DevBuf<int32_t>> states;  // shared with the snapshot vectors
   };
   struct Snap {
     double t = 0.0;
@@ -244,12 +245,8 @@ namespace {
 void output_add_space(nc_output_s* o, nc_space_s* sp) {
   auto S = std::make_unique<nc_output_s::Space>();
   S->n = sp->n;
-  if (sp->n) {
-    S->states.alloc((size_t)sp->n * sp->ns);
-    NC_CHECK_HIP(hipMemcpyAsync(S->states.p, sp->states.p, sizeof(int32_t) * sp->n * sp->ns, hipMemcpyDeviceToDevice,
-                                sp->stream.s));
-    NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
-  }
+  S->states = share_states(sp->states.p, sp->n, sp->ns, sp->stream.s);
+  NC_CHECK_HIP(hipStreamSynchronize(sp->stream.s));
   o->spaces.push_back(std::move(S));
 }
 
@@ -270,8 +267,8 @@ void accumulate_stats(nc_solve_stats& tot, const nc_solver_s* S) {
   tot.matvecs += S->st.matvecs;
   tot.host_syncs += S->st.host_syncs;
   tot.error_estimate += S->st.error_estimate;
-  tot.matvecs_timed += S->A->timed_count;
-  tot.matvec_ms += S->A->timed_ms;
+  tot.matvecs_timed += S->st.matvecs_timed;
+  tot.matvec_ms += S->st.matvec_ms;
 }
 
 // Sink rows of du = A(t) u (get_du! of the integrator, rstepadapters.jl:101-104).
@@ -645,6 +642,20 @@ int nc_output_get_sens(nc_output o, int32_t k, int32_t param, double* S, double*
   });
 }
 
+int nc_output_get_fspvec(nc_output o, int32_t k, int32_t param, nc_fspvec* p, double* sinks) {
+  return guard([&] {
+    NC_REQUIRE(o && p && k >= 0 && k < (int32_t)o->snaps.size(), NC_ERR_ARG, "snapshot index out of range");
+    NC_REQUIRE(param >= -1 && param < o->P, NC_ERR_ARG, "parameter index out of range");
+    const auto& P = *o->snaps[k];
+    const auto& Sp = *o->spaces[P.space];
+    const int64_t Nb = Sp.n + o->nr;
+    const double* base = P.u.p + (size_t)(param + 1) * Nb;
+    Stream st;
+    *p = fspvec_from_device(o->ns, Sp.n, Sp.states, base, st.s);
+    if (sinks) NC_CHECK_HIP(hipMemcpy(sinks, base + Sp.n, sizeof(double) * o->nr, hipMemcpyDeviceToHost));
+  });
+}
+
 int nc_output_destroy(nc_output o) {
   return guard([&] { delete o; });
 }
@@ -673,7 +684,7 @@ int nc_output_get(nc_output o, int32_t k, int32_t* states, double* p, double* si
     const auto& P = *o->snaps[k];
     const auto& Sp = *o->spaces[P.space];
     if (states && Sp.n)
-      NC_CHECK_HIP(hipMemcpy(states, Sp.states.p, sizeof(int32_t) * Sp.n * o->ns, hipMemcpyDeviceToHost));
+      NC_CHECK_HIP(hipMemcpy(states, Sp.states->p, sizeof(int32_t) * Sp.n * o->ns, hipMemcpyDeviceToHost));
     if (p && Sp.n) NC_CHECK_HIP(hipMemcpy(p, P.u.p, sizeof(double) * Sp.n, hipMemcpyDeviceToHost));
     if (sinks) NC_CHECK_HIP(hipMemcpy(sinks, P.u.p + Sp.n, sizeof(double) * o->nr, hipMemcpyDeviceToHost));
   });

@@ -28,6 +28,8 @@ struct nc_matrix_s {
   int64_t ld = 0;      // slot stride (n rounded up to 64)
   int K = 0;
   int rows_per_thread = 2;  // mat-vec rows per thread (1, 2 or 4; see nc_matrix_autotune)
+  uint64_t launch_gen = 0;  // bumped whenever the launch shape changes: graphs captured
+                            // with an older shape are re-captured by their owners
   nc::DevBuf<double> diag;
   nc::DevBuf<int32_t> col;
   nc::DevBuf<unsigned char> val;

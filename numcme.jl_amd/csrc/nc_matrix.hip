@@ -1118,6 +1118,7 @@ int nc_matrix_set_rows_per_thread(nc_matrix A, int32_t rows) {
   return guard([&] {
     NC_REQUIRE(A, NC_ERR_ARG, "null matrix");
     NC_REQUIRE(rows == 1 || rows == 2 || rows == 4, NC_ERR_ARG, "rows per thread must be 1, 2 or 4");
+    if (rows != A->rows_per_thread) A->launch_gen++;
     A->rows_per_thread = rows;
   });
 }
@@ -1126,13 +1127,15 @@ int nc_matrix_autotune(nc_matrix A, int32_t reps, double* best_ms) {
   return guard([&] {
     NC_REQUIRE(A && reps > 0, NC_ERR_ARG, "bad arguments");
     double best = 1e300;
-    int best_r = A->rows_per_thread;
+    const int old_r = A->rows_per_thread;
+    int best_r = old_r;
     for (int r : {1, 2, 4}) {
       A->rows_per_thread = r;
       const double ms = bench_matvec_cold(A, 0.0, reps);
       if (ms < best) best = ms, best_r = r;
     }
     A->rows_per_thread = best_r;
+    if (best_r != old_r) A->launch_gen++;
     if (best_ms) *best_ms = best;
   });
 }

@@ -1,6 +1,8 @@
 // Time integrators (internal): device-resident Krylov expv and Dormand-Prince 5(4).
 #pragma once
 
+#include <memory>
+
 #include "nc_sens.h"
 
 struct nc_solver_s {
@@ -40,6 +42,7 @@ struct nc_solver_s {
   bool started = false;
   hipGraph_t arnoldi_graph = nullptr;  // captured Arnoldi process (replayed every step)
   hipGraphExec_t arnoldi_exec = nullptr;
+  uint64_t arnoldi_gen = 0;  // the matrix's launch_gen when the graph was captured
   // RK
   nc::DevBuf<unsigned char> rkv;   // VT copy of the stage input
   nc::DevBuf<double> k;            // 7 x N stage derivatives (FP64)
@@ -48,6 +51,9 @@ struct nc_solver_s {
   int ksw = 0;  // 1: RK stages 0 and 6 have traded slots in k
   bool fsal_valid = false;
   nc_solve_stats st{};
+  // state list shared by the snapshot vectors handed out (nc_solver_get_fspvec)
+  std::shared_ptr<nc::DevBuf<int32_t>> snap_states;
+  uint64_t snap_space_uid = 0, snap_version = 0;
   ~nc_solver_s() {
     if (arnoldi_exec) (void)hipGraphExecDestroy(arnoldi_exec);
     if (arnoldi_graph) (void)hipGraphDestroy(arnoldi_graph);

@@ -21,6 +21,7 @@
 #include <limits>
 #include <type_traits>
 
+#include "nc_fspvec.h"
 #include "nc_solver.h"
 
 namespace nc {
@@ -875,6 +876,14 @@ static void arnoldi_run(nc_solver_s* S) {
     arnoldi_launch(S);
     return;
   }
+  if (S->arnoldi_exec && S->arnoldi_gen != S->A->launch_gen) {
+    // the matrix's launch shape changed since the capture (autotune, set_rows_per_thread)
+    NC_CHECK_HIP(hipStreamSynchronize(S->s));
+    (void)hipGraphExecDestroy(S->arnoldi_exec);
+    (void)hipGraphDestroy(S->arnoldi_graph);
+    S->arnoldi_exec = nullptr;
+    S->arnoldi_graph = nullptr;
+  }
   if (!S->arnoldi_exec) {
     hipGraph_t g = nullptr;
     NC_CHECK_HIP(hipStreamBeginCapture(S->s, hipStreamCaptureModeThreadLocal));
@@ -888,9 +897,35 @@ static void arnoldi_run(nc_solver_s* S) {
     NC_CHECK_HIP(hipStreamEndCapture(S->s, &g));
     NC_CHECK_HIP(hipGraphInstantiate(&S->arnoldi_exec, g, nullptr, nullptr, 0));
     S->arnoldi_graph = g;
+    S->arnoldi_gen = S->A->launch_gen;
   }
   NC_CHECK_HIP(hipGraphLaunch(S->arnoldi_exec, S->s));
 }
+
+// Kernel timing of one solver's advance. The generator may be shared by several solvers
+// (the flag and the event pool live on the matrix), so the solver's own time_kernels
+// setting applies only while it advances, and the timings recorded meanwhile are
+// credited to its own statistics.
+struct TimingScope {
+  nc_solver_s* S;
+  bool was;
+  double ms0;
+  int64_t n0;
+  explicit TimingScope(nc_solver_s* s_) : S(s_), was(s_->A->timing) {
+    S->A->timing = S->opts.time_kernels != 0;
+    ms0 = S->A->timed_ms;
+    n0 = S->A->timed_count;
+  }
+  ~TimingScope() {
+    try {
+      matvec_harvest_timing(S->A);
+    } catch (...) {
+    }
+    S->st.matvec_ms += S->A->timed_ms - ms0;
+    S->st.matvecs_timed += S->A->timed_count - n0;
+    S->A->timing = was;
+  }
+};
 
 // Sink-mass budget of the adaptive solve (fspsolve.jl:145-148): g(t) = sum(sinks) - c t.
 static double budget(const EventSpec* ev, double sink_sum, double t) {
@@ -898,6 +933,7 @@ static double budget(const EventSpec* ev, double sink_sum, double t) {
 }
 
 void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
+  TimingScope timing_scope(S);
   nc_matrix_s* A = S->A;
   const int m = S->m;
   const int64_t N = S->Np;
@@ -1070,6 +1106,7 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
 
 // ---- Dormand-Prince 5(4) ---------------------------------------------------------------
 void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
+  TimingScope timing_scope(S);
   nc_matrix_s* A = S->A;
   const int64_t N = S->Np;  // padded: rows beyond the true length stay zero
   const int R = A->nr;
@@ -1240,7 +1277,6 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
                  "the Krylov integrator needs a time-invariant generator; use NC_METHOD_RK45");
       S->method = method;
       S->m = S->opts.krylov_m;
-      A->timing = S->opts.time_kernels != 0;
       S->Np = S->blocks * S->Npb;
       S->u.alloc(S->Np);
       NC_CHECK_HIP(hipMemsetAsync(S->u.p, 0, sizeof(double) * S->Np, S->s));
@@ -1392,6 +1428,25 @@ int nc_solver_get_snapshot(nc_solver S, int32_t k, double* u) {
   });
 }
 
+int nc_solver_get_fspvec(nc_solver S, int32_t k, int32_t block, nc_space space, nc_fspvec* p, double* sinks) {
+  return guard([&] {
+    NC_REQUIRE(S && space && p && k >= 0 && k < S->n_snaps, NC_ERR_ARG, "snapshot index out of range");
+    NC_REQUIRE(block >= 0 && block < S->blocks, NC_ERR_ARG, "block index out of range");
+    NC_REQUIRE(space->n == S->A->n && space->nr == S->A->nr, NC_ERR_ARG,
+               "the state space does not match the solver's generator");
+    if (!S->snap_states || S->snap_space_uid != space->uid || S->snap_version != space->version) {
+      S->snap_states = share_states(space->states.p, space->n, space->ns, space->stream.s);
+      NC_CHECK_HIP(hipStreamSynchronize(space->stream.s));
+      S->snap_space_uid = space->uid;
+      S->snap_version = space->version;
+    }
+    const double* base = S->snaps.p + (size_t)k * S->Np + (size_t)block * S->Npb;
+    *p = fspvec_from_device(space->ns, space->n, S->snap_states, base, S->s);
+    if (sinks)
+      NC_CHECK_HIP(hipMemcpy(sinks, base + S->A->n, sizeof(double) * S->A->nr, hipMemcpyDeviceToHost));
+  });
+}
+
 int nc_solver_get_state(nc_solver S, double* t, double* u) {
   return guard([&] {
     NC_REQUIRE(S, NC_ERR_ARG, "null solver");
@@ -1404,8 +1459,6 @@ int nc_solver_stats(nc_solver S, nc_solve_stats* st) {
   return guard([&] {
     NC_REQUIRE(S && st, NC_ERR_ARG, "null argument");
     *st = S->st;
-    st->matvecs_timed = S->A->timed_count;
-    st->matvec_ms = S->A->timed_ms;
   });
 }
 

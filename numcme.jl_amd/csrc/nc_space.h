@@ -69,6 +69,7 @@ struct nc_space_s {
   nc::DevBuf<uint32_t> flags, ranks;
   nc::DevBuf<unsigned char> scan_tmp;
   uint64_t version = 0;  // bumped on every change of the state list
+  uint64_t uid = 0;      // unique per space in the process
 };
 
 namespace nc {

@@ -13,6 +13,8 @@
 //    states, so it runs one thread per (new state, reaction).
 //  * deleteat! (:276-331) is a keep-flag scan, a compaction with index remapping, a hash
 //    rebuild and a sink recomputation.
+#include <atomic>
+
 #include "nc_space.h"
 
 namespace nc {
@@ -377,6 +379,8 @@ nc_space_s* space_create(int ns, int nr, const int32_t* stoich, int64_t n_init, 
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
     throw Error(NC_ERR_NO_DEVICE, "no HIP device available");
   auto sp = new nc_space_s();
+  static std::atomic<uint64_t> next_uid{1};
+  sp->uid = next_uid++;
   try {
     sp->ns = ns;
     sp->nr = nr;

@@ -214,6 +214,15 @@ SIGNATURES = {
     "nc_output_info": [P, I32, PD, PI64, PI32],
     "nc_output_get": [P, I32, PI32, PD, PD],
     "nc_expm_dense": [I32, PD, PD],
+    "nc_fspvec_create": [I32, I64, PI32, PD, C.POINTER(P)],
+    "nc_fspvec_destroy": [P],
+    "nc_fspvec_size": [P, PI64, PI32],
+    "nc_fspvec_get": [P, PI32, PD],
+    "nc_fspvec_sum": [P, PD],
+    "nc_fspvec_marginal": [P, I32, PI32, C.POINTER(P)],
+    "nc_fspvec_dense": [P, PI64, I64, PD],
+    "nc_solver_get_fspvec": [P, I32, I32, P, C.POINTER(P), PD],
+    "nc_output_get_fspvec": [P, I32, I32, C.POINTER(P), PD],
 }
 
 _lib = None

@@ -181,23 +181,17 @@ def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: Ada
     out.space_ids = []  # index of the state space each snapshot lives on
     nsnap, nspace = C.c_int32(), C.c_int32()
     L.call("nc_output_count", res.handle, C.byref(nsnap), C.byref(nspace))
-    ns, nr = model.get_species_count(), model.get_reaction_count()
-    spaces = {}
+    nr = model.get_reaction_count()
     for k in range(nsnap.value):
         t, n, sid = C.c_double(), C.c_int64(), C.c_int32()
         L.call("nc_output_info", res.handle, k, C.byref(t), C.byref(n), C.byref(sid))
-        p = np.empty(max(n.value, 1))
         sinks = np.empty(nr)
-        states = None
-        if sid.value not in spaces:
-            states = np.empty((n.value, ns), dtype=np.int32)
-        L.call("nc_output_get", res.handle, k, None if states is None else L.ptr(states, C.c_int32),
-               L.ptr(p, C.c_double), L.ptr(sinks, C.c_double))
-        if states is not None:
-            spaces[sid.value] = states.astype(np.int64)
+        h = C.c_void_p()
+        # the snapshot stays on the device; snapshots of one space share its state list
+        L.call("nc_output_get_fspvec", res.handle, k, -1, C.byref(h), L.ptr(sinks, C.c_double))
         out.t.append(t.value)
         out.space_ids.append(sid.value)
-        out.p.append(FspVectorSparse(spaces[sid.value], p[: n.value], copy=False))
+        out.p.append(FspVectorSparse._from_device(h.value))
         out.sinks.append(sinks)
     out.stats = {f: getattr(st, f) for f, _ in L.nc_solve_stats._fields_}
     res.free()

@@ -255,6 +255,63 @@ def evaluate(e: Expr, t, x, p):
     return _eval_node(e.op, [evaluate(a, t, x, p) for a in e.args])
 
 
+def _factors(e: Expr, power, out):
+    """Multiplicative factors of ``e`` as ``(factor, +1 | -1)`` pairs (-1: a divisor)."""
+    if e.op == "MUL":
+        _factors(e.args[0], power, out)
+        _factors(e.args[1], power, out)
+    elif e.op == "DIV":
+        _factors(e.args[0], power, out)
+        _factors(e.args[1], -power, out)
+    elif e.op == "NEG":
+        out.append((Expr("CONST", value=-1.0), 1))
+        _factors(e.args[0], power, out)
+    else:
+        out.append((e, power))
+    return out
+
+
+def _product(factors):
+    num = [f for f, k in factors if k > 0]
+    den = [f for f, k in factors if k < 0]
+    out = ONE
+    for f in num:
+        out = _bin("MUL", out, f)
+    if den:
+        d = den[0]
+        for f in den[1:]:
+            d = _bin("MUL", d, f)
+        out = _bin("DIV", out, d)
+    return out
+
+
+def factor_time(e: Expr):
+    """Split a joint law ``f(t, x, p)`` into ``tfactor(t, p) * statefactor(x, p)``.
+
+    Catalyst hands every time-varying rate law over as a joint propensity
+    (``src/cmemodel/catalyst_interface.jl:21-27``), and a joint propensity makes the
+    generator re-evaluate every state's propensity whenever ``t`` changes
+    (``src/fspmatrix/sparse/fspsparsematrix.jl:154-166``). Most laws are products such as
+    ``k(t) * X`` or ``X / (1 + t)``; for those the matrix keeps the state factor and scales
+    it by the time factor at each launch, as for a ``SeparableTimeVaryingPropensity``.
+
+    ``e``'s product/quotient factors are grouped into those that read time (and no species)
+    and the rest. Returns ``("ti", f)`` when ``e`` does not read time, ``("sep", tfactor,
+    statefactor)`` when it factors, and ``None`` when a factor reads both time and the state
+    (e.g. ``x / (1 + t x)``). The factored product equals ``e`` up to the rounding of a
+    reordered product.
+    """
+    if not uses(e, "TIME"):
+        return ("ti", e)
+    tf, sf = [], []
+    for f, k in _factors(e, 1, []):
+        ut, us = uses(f, "TIME"), uses(f, "SPECIES")
+        if ut and us:
+            return None
+        (tf if ut else sf).append((f, k))
+    return ("sep", _product(tf), _product(sf))
+
+
 def params_used(e: Expr, acc=None):
     acc = set() if acc is None else acc
     if e.op == "PARAM":

@@ -111,27 +111,42 @@ def propensity(f, tfactor=None):
 
 
 class _Programs:
-    """Traced expressions of a model's propensities and their flattened programs."""
+    """Traced expressions of a model's propensities and their flattened programs.
 
-    def __init__(self, ns, nr, npar, props):
+    With ``factor_joint`` (the default), a joint law ``f(t, x, p)`` that is a product of a
+    time factor and a state factor is lowered as a separable propensity, and one that does
+    not read time as a time-invariant one (``expr.factor_time``); ``factored[r]`` records
+    the kind reaction ``r`` was lowered from. Other joint laws stay joint.
+    """
+
+    def __init__(self, ns, nr, npar, props, factor_joint=True):
         t, x, p = E.symbols(ns, npar)
-        self.xexpr, self.texpr, self.kinds = [], [], []
+        self.xexpr, self.texpr, self.kinds, self.factored = [], [], [], []
         for r, a in enumerate(props):
+            kind, lowered_from = a.kind if isinstance(a, Propensity) else None, None
             if isinstance(a, StandardTimeInvariantPropensity):
                 xe, te = E.trace(a.f, x, p), None
             elif isinstance(a, SeparableTimeVaryingPropensity):
                 xe, te = E.trace(a.statefactor, x, p), E.trace(a.tfactor, t, p)
             elif isinstance(a, JointTimeVaryingPropensity):
                 xe, te = E.trace(a.f, t, x, p), None
+                split = E.factor_time(xe) if factor_joint else None
+                if split is not None:
+                    lowered_from = L.NC_PROP_JOINT
+                    if split[0] == "ti":
+                        kind, xe = L.NC_PROP_TIME_INVARIANT, split[1]
+                    else:
+                        kind, te, xe = L.NC_PROP_SEPARABLE, split[1], split[2]
             else:
                 raise ArgumentError(L.NC_ERR_ARG, f"reaction {r}: unsupported propensity type {type(a).__name__}")
-            if isinstance(a, StandardTimeInvariantPropensity) and E.uses(xe, "TIME"):
+            if kind == L.NC_PROP_TIME_INVARIANT and E.uses(xe, "TIME"):
                 raise ArgumentError(L.NC_ERR_ARG, f"reaction {r}: time-invariant propensity uses time")
             if te is not None and E.uses(te, "SPECIES"):
                 raise ArgumentError(L.NC_ERR_ARG, f"reaction {r}: time factor reads the state")
             self.xexpr.append(xe)
             self.texpr.append(te)
-            self.kinds.append(a.kind)
+            self.kinds.append(kind)
+            self.factored.append(lowered_from)
 
     @staticmethod
     def flatten(pairs):
@@ -152,9 +167,12 @@ class CmeModel:
     """Stochastic reaction network model (cmemodel.jl:49-53).
 
     ``stoich_matrix`` is species x reactions (column r is the net change of reaction r).
+    ``factor_joint=False`` keeps every ``JointTimeVaryingPropensity`` joint on the device
+    (re-evaluated at every state when ``t`` changes) instead of lowering the factorable
+    ones to separable form.
     """
 
-    def __init__(self, stoich_matrix, propensities, parameters=()):
+    def __init__(self, stoich_matrix, propensities, parameters=(), factor_joint=True):
         S = np.asarray(stoich_matrix)
         if S.ndim != 2:
             raise ArgumentError(L.NC_ERR_ARG, "stoichiometry matrix must be 2-D (species x reactions)")
@@ -163,6 +181,7 @@ class CmeModel:
         self.stoich_matrix = S.astype(np.int64)
         self.propensities = list(propensities)
         self.parameters = np.asarray(parameters, dtype=np.float64).ravel()
+        self.factor_joint = bool(factor_joint)
         self._progs = None
         self._handle = None
 
@@ -193,7 +212,7 @@ class CmeModel:
     def programs(self):
         if self._progs is None:
             self._progs = _Programs(self.get_species_count(), self.get_reaction_count(), self.get_parameter_count(),
-                                    self.propensities)
+                                    self.propensities, self.factor_joint)
         return self._progs
 
     def device_model(self):

@@ -182,30 +182,29 @@ def solve_sens(model, initial_condition: ForwardSensFspInitialConditionSparse, t
     out = ForwardSensFspOutputSparse()
     nsnap, nspace = C.c_int32(), C.c_int32()
     L.call("nc_output_count", res.handle, C.byref(nsnap), C.byref(nspace))
-    spaces = {}
+    seen = set()
     for k in range(nsnap.value):
         t, n, sid = C.c_double(), C.c_int64(), C.c_int32()
         L.call("nc_output_info", res.handle, k, C.byref(t), C.byref(n), C.byref(sid))
         nn = n.value
-        p = np.empty(max(nn, 1))
-        sinks = np.empty(nr)
-        states = np.empty((nn, ns), dtype=np.int32) if sid.value not in spaces else None
-        L.call("nc_output_get", res.handle, k, None if states is None else L.ptr(states, C.c_int32),
-               L.ptr(p, C.c_double), L.ptr(sinks, C.c_double))
-        if states is not None:
-            spaces[sid.value] = states.astype(np.int64)
+        if sid.value not in seen:
+            seen.add(sid.value)
             if verbose and sid.value > 0:
                 print(f"At t = {round(t.value, 2)}: update sate space. New size: {nn} states.")
+        # every block stays on the device; the snapshots of one space share its state list
+        sinks = np.empty(nr)
+        h = C.c_void_p()
+        L.call("nc_output_get_fspvec", res.handle, k, -1, C.byref(h), L.ptr(sinks, C.c_double))
+        p = FspVectorSparse._from_device(h.value)
         Ss, ds = [], []
         for ip in range(npar):
-            s = np.empty(max(nn, 1))
             d = np.empty(nr)
-            L.call("nc_output_get_sens", res.handle, k, ip, L.ptr(s, C.c_double), L.ptr(d, C.c_double))
-            Ss.append(FspVectorSparse(spaces[sid.value], s[:nn], copy=False))
+            L.call("nc_output_get_fspvec", res.handle, k, ip, C.byref(h), L.ptr(d, C.c_double))
+            Ss.append(FspVectorSparse._from_device(h.value))
             ds.append(d)
         out.t.append(t.value)
         out.space_ids.append(sid.value)
-        out.p.append(FspVectorSparse(spaces[sid.value], p[:nn], copy=False))
+        out.p.append(p)
         out.sinks.append(sinks)
         out.S.append(Ss)
         out.dsinks.append(ds)

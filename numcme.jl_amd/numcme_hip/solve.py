@@ -63,14 +63,14 @@ class RK45:
 
 
 class FspOutputSliceSparse:
-    """``FspOutputSliceSparse`` (fspoutput.jl:181-185): fields ``t``, ``p``, ``sinks``."""
+    """``FspOutputSliceSparse`` (fspoutput.jl:30-34): fields ``t``, ``p``, ``sinks``."""
 
     def __init__(self, t, p, sinks):
         self.t, self.p, self.sinks = t, p, sinks
 
 
 class FspOutputSparse:
-    """``FspOutputSparse`` (fspoutput.jl:170-174): lists ``t``, ``p``, ``sinks``."""
+    """``FspOutputSparse`` (fspoutput.jl:19-23): lists ``t``, ``p``, ``sinks``."""
 
     def __init__(self):
         self.t, self.p, self.sinks = [], [], []
@@ -80,7 +80,7 @@ class FspOutputSparse:
         return len(self.t)
 
     def __getitem__(self, ind):
-        # fspoutput.jl:195-209 (0-based here)
+        # fspoutput.jl:44-58 (0-based here)
         if isinstance(ind, (list, tuple, np.ndarray)):
             return [self[i] for i in ind]
         if ind >= len(self.t):
@@ -122,10 +122,11 @@ def solve_fixed(model, p0, tspan, method, saveat=(), odeatol=1.0e-6, odertol=1.0
     if np.any(ts < t0) or np.any(ts > t1) or np.any(np.diff(ts) < 0):
         raise ArgumentError(L.NC_ERR_ARG, "saveat must be ascending and inside tspan")
     u0 = np.concatenate([p0.values, np.zeros(R)])  # fspsolve.jl:21
-    return _integrate(A, method.opts(odeatol, odertol), u0, t0, ts, space.get_states(), S)
+    return _integrate(A, method.opts(odeatol, odertol), u0, t0, ts, space)
 
 
-def _integrate(A, opts, u0, t0, ts, states, S):
+def _integrate(A, opts, u0, t0, ts, space):
+    """Run the solver and hand out every snapshot as a device-resident vector."""
     h = C.c_void_p()
     L.call("nc_solver_create", A.handle, C.byref(opts), C.byref(h))
     solver = _Solver(h.value)
@@ -134,12 +135,14 @@ def _integrate(A, opts, u0, t0, ts, states, S):
     tsa = L.f64(ts)
     L.call("nc_solver_run", solver.handle, len(tsa), L.ptr(tsa, C.c_double))
     out = FspOutputSparse()
-    buf = np.empty(A.rowcount)
+    R = space.get_sink_count()
     for k, t in enumerate(tsa):
-        L.call("nc_solver_get_snapshot", solver.handle, k, L.ptr(buf, C.c_double))
+        sinks = np.empty(R)
+        h = C.c_void_p()
+        L.call("nc_solver_get_fspvec", solver.handle, k, 0, space.handle, C.byref(h), L.ptr(sinks, C.c_double))
         out.t.append(float(t))
-        out.p.append(FspVectorSparse(states, buf[:S], copy=True))
-        out.sinks.append(buf[S:].copy())
+        out.p.append(FspVectorSparse._from_device(h.value))
+        out.sinks.append(sinks)
     st = L.nc_solve_stats()
     L.call("nc_solver_stats", solver.handle, C.byref(st))
     out.stats = {f: getattr(st, f) for f, _ in L.nc_solve_stats._fields_}

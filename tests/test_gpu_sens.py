@@ -92,6 +92,10 @@ def _fixed_ic(states, n_par):
     return nh.forwardsens_initial_condition(states, p0, [np.zeros(len(states)) for _ in range(n_par)])
 
 
+# FP32-storage bars per case, just above the measured worst relative error
+FP32_SENS_BAR = {"telegraph_sens": 1e-6, "gene4": 1e-5}
+
+
 @pytest.mark.parametrize("case,ts,kinks", [
     ("telegraph_sens", [0.0, 5.0, 12.0, 25.0, 40.0], [10.0]),
     ("gene4", [0.0, 60.0, 120.0, 300.0], []),
@@ -110,15 +114,20 @@ def test_sens_solve_matches_oracle(gpu, case, ts, kinks, precision):
     ref = solve_sens_fixed(StateSpace(S, states), models.to_oracle(props), th, gk(), ic.p, ic.S, ts,
                            breakpoints=kinks)
     assert out.stats["adaptations"] == 0 and len(out) == len(ts) + 1
-    bar = 1e-6 if precision == "fp64" else 1e-5
+    # relative to each sensitivity's largest entry; FP32 storage measures 6.5e-7 on the
+    # full-size C3 solve
+    bar = 1e-6 if precision == "fp64" else FP32_SENS_BAR[case]
+    worst, worst_sink = 0.0, 0.0
     for k in range(len(ts)):
         assert np.array_equal(out.p[k].states, ref.states)
         assert np.abs(out.p[k].values - ref.p[k]).sum() <= (1e-8 if precision == "fp64" else 1e-6)
         for ip in range(len(th)):
             want = ref.S[k][ip]
             scale = max(np.abs(want).max(), 1e-12)
-            assert np.abs(out.S[k][ip].values - want).max() <= bar * scale, (k, ip)
-            assert np.abs(out.dsinks[k][ip] - ref.dsinks[k][ip]).max() <= bar * max(scale, 1e-12) + 1e-14
+            worst = max(worst, np.abs(out.S[k][ip].values - want).max() / scale)
+            worst_sink = max(worst_sink, (np.abs(out.dsinks[k][ip] - ref.dsinks[k][ip]).max() - 1e-14) / scale)
+    print(f"{case} {precision}: worst relative sensitivity error {worst:.2e}, sinks {worst_sink:.2e}", flush=True)
+    assert worst <= bar and worst_sink <= bar
 
 
 def test_reference_adaptive_sens_solve(gpu):

@@ -209,3 +209,74 @@ def test_propensity_arity_dispatch():
     assert isinstance(sep, nh.SeparableTimeVaryingPropensity) and nh.istimeseparable(sep)
     with pytest.raises(nh.ArgumentError):
         nh.propensity(lambda a: a)
+
+
+def _joint_laws():
+    """(law, expected lowering) pairs for the joint -> separable factoring."""
+    sig = lambda t: (1.0 - E.exp(-6e-3 * t)) * E.exp(-9e-3 * t)
+    return [
+        (lambda t, x, p: E.max(0.0, 1.0 - E.sin(E.pi * t / 2)) * p[1] * x[1], "sep"),  # test_fspmat.jl:113
+        (lambda t, x, p: E.max(0.0, p[3] - 0.12 * sig(t)) * x[2], "sep"),  # the C3 k21(t) law
+        (lambda t, x, p: p[0] * x[0] / (1.0 + t), "sep"),
+        (lambda t, x, p: -(p[0] * (1.0 - t)) * (x[0] * x[1]) / (p[1] + x[0]), "sep"),
+        (lambda t, x, p: p[0] * x[0] + 0.0 * t, "joint"),  # a sum: not a product
+        (lambda t, x, p: p[0] * x[0] / (1.0 + t * x[0]), "joint"),
+        (lambda t, x, p: p[1] * x[0] * x[1], "ti"),  # reads no time
+    ]
+
+
+def test_joint_factoring_kinds_and_values():
+    rng = np.random.default_rng(21)
+    t, xs, ps = E.symbols(3, 4)
+    th = np.array([0.7, 1.3, 0.4, 2e-2])
+    X = rng.integers(0, 50, size=(40, 3))
+    for law, kind in _joint_laws():
+        e = E.trace(law, t, xs, ps)
+        split = E.factor_time(e)
+        if kind == "joint":
+            assert split is None
+            continue
+        assert split[0] == kind
+        for tt in (0.0, 0.37, 1.0, 5.5, 120.0):
+            for i in range(len(X)):
+                ref = run_postfix(E.compile_expr(e), tt, X[i], th)
+                if kind == "ti":
+                    got = run_postfix(E.compile_expr(split[1]), tt, X[i], th)
+                else:
+                    tf, sf = split[1], split[2]
+                    assert not E.uses(tf, "SPECIES") and not E.uses(sf, "TIME")
+                    got = run_postfix(E.compile_expr(tf), tt, X[i], th) * run_postfix(E.compile_expr(sf), tt, X[i], th)
+                # the factored product only reorders a product: a few ulps apart
+                assert abs(got - ref) <= 4 * np.finfo(float).eps * abs(ref), (kind, tt, X[i])
+
+
+def test_model_lowers_factorable_joint_laws():
+    S = np.array([[1, -1, 0, 0, 0, 0, 0], [0, 0, 1, -1, 0, 0, 0], [0, 0, 0, 0, 1, -1, 1]])
+    props = [nh.propensity(law) for law, _ in _joint_laws()]
+    m = nh.CmeModel(S, props, [0.7, 1.3, 0.4, 2e-2])
+    kinds = {"sep": L.NC_PROP_SEPARABLE, "joint": L.NC_PROP_JOINT, "ti": L.NC_PROP_TIME_INVARIANT}
+    assert m.programs.kinds == [kinds[k] for _, k in _joint_laws()]
+    assert m.programs.factored == [None if k == "joint" else L.NC_PROP_JOINT for _, k in _joint_laws()]
+    kept = nh.CmeModel(S, props, [0.7, 1.3, 0.4, 2e-2], factor_joint=False)
+    assert kept.programs.kinds == [L.NC_PROP_JOINT] * len(props)
+
+
+def test_factored_joint_gradients_vs_finite_differences():
+    # the separable gradient rule sf * dtf + tf * dsf (propensitygrad.jl:56-67) applied to a
+    # factored joint law must be the derivative of the joint law itself
+    law = lambda t, x, p: E.max(0.0, p[3] - 0.12 * (1.0 - E.exp(-6e-3 * t)) * E.exp(-9e-3 * t)) * p[1] * x[2]
+    S = np.array([[0], [0], [-1]])
+    th = np.array([0.7, 1.3, 0.4, 2e-2])
+    m = nh.CmeModelWithSensitivity(nh.CmeModel(S, [nh.propensity(law)], th))
+    assert m.cmemodel.programs.kinds == [L.NC_PROP_SEPARABLE]
+    X = np.array([[0, 0, 0], [1, 2, 3], [5, 0, 40]])
+    cols = [X[:, s] for s in range(3)]
+    for tt in (0.0, 30.0, 400.0):
+        g = m.propensity_gradient(0, cols, t=tt)
+        for k in range(len(th)):
+            h = 1e-6 * max(1.0, abs(th[k]))
+            tp, tm = th.copy(), th.copy()
+            tp[k] += h
+            tm[k] -= h
+            fd = (np.asarray(law(tt, cols, tp), float) - np.asarray(law(tt, cols, tm), float)) / (2 * h)
+            assert np.allclose(g[k], fd, rtol=1e-6, atol=1e-10), (tt, k)
