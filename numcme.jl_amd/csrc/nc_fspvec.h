@@ -9,12 +9,13 @@
 
 #include "nc_internal.h"
 
+// A vector owns no stream: a solve may hand out thousands of snapshots, and each operation
+// takes a stream from the library's cache for its own duration.
 struct nc_fspvec_s {
   int ns = 0;
   int64_t n = 0;
   std::shared_ptr<nc::DevBuf<int32_t>> states;  // may be shared with other vectors
   nc::DevBuf<double> values;
-  nc::Stream stream;
 };
 
 namespace nc {

@@ -158,8 +158,7 @@ __global__ void k_sum_final(int nb, const double* part, double* out) {
   if (threadIdx.x == 0) *out = sh[0];
 }
 
-void minmax(const nc_fspvec_s* v, std::vector<int32_t>& mn, std::vector<int32_t>& mx) {
-  hipStream_t s = v->stream.s;
+void minmax(const nc_fspvec_s* v, hipStream_t s, std::vector<int32_t>& mn, std::vector<int32_t>& mx) {
   const int ns = v->ns;
   mn.assign(ns, INT32_MAX);
   mx.assign(ns, INT32_MIN);
@@ -179,7 +178,8 @@ void minmax(const nc_fspvec_s* v, std::vector<int32_t>& mn, std::vector<int32_t>
 int bit_width(uint32_t x) { return x ? 32 - __builtin_clz(x) : 0; }
 
 nc_fspvec_s* marginal(const nc_fspvec_s* v, const std::vector<int32_t>& keep) {
-  hipStream_t s = v->stream.s;
+  Stream st;
+  hipStream_t s = st.s;
   const int64_t n = v->n;
   const int nk = (int)keep.size();
   auto out = std::make_unique<nc_fspvec_s>();
@@ -188,7 +188,7 @@ nc_fspvec_s* marginal(const nc_fspvec_s* v, const std::vector<int32_t>& keep) {
   NC_REQUIRE(n <= (int64_t)UINT32_MAX - 1, NC_ERR_OVERFLOW, "vector too long for the marginal");
   if (n == 0) return out.release();
   std::vector<int32_t> mn, mx;
-  minmax(v, mn, mx);
+  minmax(v, s, mn, mx);
   KeySpec ks{};
   ks.nk = nk;
   int bits = 0;
@@ -273,7 +273,8 @@ int nc_fspvec_create(int32_t ns, int64_t n, const int32_t* states, const double*
     v->ns = ns;
     v->n = n;
     v->states = std::make_shared<DevBuf<int32_t>>();
-    hipStream_t s = v->stream.s;
+    Stream st;
+    hipStream_t s = st.s;
     if (n > 0) {
       if (ns > 0) {
         v->states->alloc((size_t)n * ns);
@@ -302,7 +303,8 @@ int nc_fspvec_size(nc_fspvec v, int64_t* n, int32_t* ns) {
 int nc_fspvec_get(nc_fspvec v, int32_t* states, double* values) {
   return guard([&] {
     NC_REQUIRE(v, NC_ERR_ARG, "null vector");
-    hipStream_t s = v->stream.s;
+    Stream st;
+    hipStream_t s = st.s;
     if (v->n > 0) {
       if (states && v->ns > 0)
         NC_CHECK_HIP(hipMemcpyAsync(states, v->states->p, sizeof(int32_t) * v->n * v->ns, hipMemcpyDeviceToHost, s));
@@ -317,7 +319,8 @@ int nc_fspvec_sum(nc_fspvec v, double* total) {
     NC_REQUIRE(v && total, NC_ERR_ARG, "null argument");
     *total = 0.0;
     if (v->n == 0) return;
-    hipStream_t s = v->stream.s;
+    Stream st;
+    hipStream_t s = st.s;
     const int nb = (int)std::min<int64_t>(1024, grid_for(v->n, kB));
     DevBuf<double> part(nb + 1);
     k_sum_partial<<<nb, kB, 0, s>>>(v->n, v->values.p, part.p);
@@ -350,8 +353,10 @@ int nc_fspvec_dense(nc_fspvec v, int64_t* bounds, int64_t capacity, double* out)
   return guard([&] {
     NC_REQUIRE(v && bounds, NC_ERR_ARG, "null argument");
     NC_REQUIRE(v->n > 0, NC_ERR_ARG, "Cannot construct dense array from empty FspVector instance.");
+    Stream st;
+    hipStream_t s = st.s;
     std::vector<int32_t> mn, mx;
-    minmax(v, mn, mx);
+    minmax(v, s, mn, mx);
     int64_t total = 1;
     for (int sp = 0; sp < v->ns; ++sp) {
       NC_REQUIRE(mn[sp] >= 0, NC_ERR_ARG, "dense conversion needs non-negative states");
@@ -362,7 +367,6 @@ int nc_fspvec_dense(nc_fspvec v, int64_t* bounds, int64_t capacity, double* out)
     if (!out) return;
     NC_REQUIRE(capacity >= total, NC_ERR_ARG, "output buffer smaller than the dense array");
     NC_REQUIRE(v->n <= INT32_MAX, NC_ERR_OVERFLOW, "vector too long for dense conversion");
-    hipStream_t s = v->stream.s;
     DevBuf<double> dense(total);
     DevBuf<int32_t> last(total);
     DevBuf<int64_t> ddims(std::max(v->ns, 1));

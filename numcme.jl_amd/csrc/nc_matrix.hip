@@ -939,11 +939,6 @@ __global__ void k_to_vt(const double* in, VT* out, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) out[i] = (VT)in[i];
 }
-template <typename VT>
-__global__ void k_from_vt(const VT* in, double* out, int64_t n) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (double)in[i];
-}
 
 double bench_matvec(nc_matrix_s* A, double t, int reps) {
   hipStream_t s = A->stream.s;
@@ -1146,16 +1141,16 @@ int nc_matrix_matvec(nc_matrix A, double t, const double* v, double* out, int32_
     hipStream_t s = A->stream.s;
     const int64_t N = A->n_rows;
     DevBuf<double> dv(N), dout(N);
-    DevBuf<unsigned char> vv(N * A->vsize), vo(N * A->vsize);
     NC_CHECK_HIP(hipMemcpyAsync(dv.p, v, sizeof(double) * N, hipMemcpyHostToDevice, s));
     if (accumulate) NC_CHECK_HIP(hipMemcpyAsync(dout.p, out, sizeof(double) * N, hipMemcpyHostToDevice, s));
     if (A->precision == NC_FP32) {
-      k_to_vt<float><<<grid_for(N, 256), 256, 0, s>>>(dv.p, reinterpret_cast<float*>(vv.p), N);
-      if (accumulate) k_to_vt<float><<<grid_for(N, 256), 256, 0, s>>>(dout.p, reinterpret_cast<float*>(vo.p), N);
-      matvec_launch(A, t, vv.p, vo.p, accumulate != 0, s);
-      k_from_vt<float><<<grid_for(N, 256), 256, 0, s>>>(reinterpret_cast<float*>(vo.p), dout.p, N);
+      // FP32 storage rounds the matrix values and the input; the kernel accumulates and
+      // writes (or adds into) the output in FP64, as inside the integrators
+      DevBuf<float> vv(N);
+      k_to_vt<float><<<grid_for(N, 256), 256, 0, s>>>(dv.p, vv.p, N);
+      matvec_launch(A, t, vv.p, dout.p, accumulate != 0, s, true);
     } else {
-      matvec_launch(A, t, dv.p, dout.p, accumulate != 0, s);
+      matvec_launch(A, t, dv.p, dout.p, accumulate != 0, s, true);
     }
     NC_CHECK_HIP(hipGetLastError());
     NC_CHECK_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
