@@ -8,12 +8,17 @@
 // instance are its sinks (written by one workgroup per (instance, reaction) segment);
 // padding rows stay zero. Vectors use the same row space.
 //
-// Integrator: Dormand-Prince 5(4) with per-instance t, h and error control. One iteration
-// (accept/snapshot of the previous step, six stages with batched mat-vecs, the error
-// partials and their per-instance fold) is captured once as a HIP graph and replayed; the
-// per-instance step sizes and decisions live in device arrays the host rewrites between
-// replays, so a replay needs no argument changes. One host synchronisation per iteration
-// reads every instance's error norm.
+// Integrator: Dormand-Prince 5(4) with per-instance t, h and error control, all of it on
+// the device. One iteration is
+//   accept/snapshot of the previous step fused with stage 1, five more stages with their
+//   batched mat-vecs, the error partials, their per-instance fold, and the controller
+// The controller (one workgroup) decides every instance's accept, snapshot and next step
+// size, and lists the granules of the instances that have work in the next iteration;
+// every row kernel walks that list with a fixed grid, so an iteration costs what its
+// active instances cost (the large instances run longest: late iterations touch only
+// them). Iterations are captured in chunks as one HIP graph and replayed with no host
+// decision in between; the host reads one counter per chunk (the number of instances
+// still running), with the next chunk already queued.
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -24,7 +29,10 @@ namespace nc {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kGran = 256;  // rows of one ownership granule (instances are padded to it)
+constexpr int kGran = 256;       // rows of one ownership granule (instances are padded to it)
+constexpr int kCtlBlock = 1024;  // controller workgroup
+constexpr unsigned kMaxRowBlocks = 2048;  // fixed grid of the list-driven row kernels
+constexpr int kChunk = 8;        // iterations per captured graph
 
 template <typename T, int R>
 struct alignas(sizeof(T) * R) Pack {
@@ -46,6 +54,14 @@ __constant__ double c_a[7][6] = {
     {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84}};
 __constant__ double c_e[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525, -1.0 / 40};
 
+// Solve settings read by the captured kernels from device memory, so that one captured
+// chunk serves every solve.
+struct Params {
+  double atol, rtol, h0;
+  int64_t max_steps;  // steps + rejections per instance, 0 = unlimited
+  int32_t n_out;
+};
+
 struct Dev {
   const int32_t* gran_inst;  // owner instance of each granule
   const int64_t* row_off;    // first row of each instance
@@ -53,12 +69,19 @@ struct Dev {
   const double* h;           // step size of the current iteration (0: inactive)
   const int32_t* accept;     // 1: accept the last step
   const int32_t* snap;       // snapshot slot to fill after accepting, or -1
+  const int32_t* agran;      // granules of the instances with work in this iteration
+  const uint32_t* n_agran;   // their count
 };
 
-// Rows of thread t of block b: [(b * kBlock + t) * R, ... + R), inside one granule.
+// Work unit u of a list-driven kernel covers list entries u R .. u R + R - 1 (R granules,
+// kBlock R rows); thread t takes R consecutive rows of entry u R + t R / kGran. Returns
+// false for a unit entry past the end of the list.
 template <int R>
-__device__ inline int64_t first_row() {
-  return ((int64_t)blockIdx.x * kBlock + threadIdx.x) * R;
+__device__ inline bool unit_rows(const Dev& d, uint32_t n_list, int64_t u, int64_t& n0) {
+  const int64_t j = u * R + ((int64_t)threadIdx.x * R) / kGran;
+  if (j >= (int64_t)n_list) return false;
+  n0 = (int64_t)d.agran[j] * kGran + ((int64_t)threadIdx.x * R) % kGran;
+  return true;
 }
 
 // y_s = u + h_i sum_{j<s} a_sj k_j over the instance's rows: its VT copy, and at s = 6
@@ -67,31 +90,35 @@ __device__ inline int64_t first_row() {
 template <typename VT, int R, int STAGE>
 __global__ __launch_bounds__(kBlock) void k_bstage(Dev d, int64_t ld, const double* u, const double* k,
                                                    double* ytmp, VT* vin) {
-  const int64_t n0 = first_row<R>();
-  if (n0 >= ld) return;
-  const double h = d.h[d.gran_inst[n0 / kGran]];
-  if (h == 0.0) return;
-  Pack<double, R> y = *reinterpret_cast<const Pack<double, R>*>(u + n0);
-  // a_61 = 0: the last stage does not read k1
-  Pack<double, R> kj[STAGE > 0 ? STAGE : 1];
+  const uint32_t nl = *d.n_agran;
+  const int64_t units = ((int64_t)nl + R - 1) / R;
+  for (int64_t un = blockIdx.x; un < units; un += gridDim.x) {
+    int64_t n0;
+    if (!unit_rows<R>(d, nl, un, n0)) continue;
+    const double h = d.h[d.gran_inst[n0 / kGran]];
+    if (h == 0.0) continue;
+    Pack<double, R> y = *reinterpret_cast<const Pack<double, R>*>(u + n0);
+    // a_61 = 0: the last stage does not read k1
+    Pack<double, R> kj[STAGE > 0 ? STAGE : 1];
 #pragma unroll
-  for (int j = 0; j < STAGE; ++j)
-    if (!(STAGE == 6 && j == 1)) kj[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
-  double acc[R] = {};
+    for (int j = 0; j < STAGE; ++j)
+      if (!(STAGE == 6 && j == 1)) kj[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
+    double acc[R] = {};
 #pragma unroll
-  for (int j = 0; j < STAGE; ++j) {
-    if (STAGE == 6 && j == 1) continue;
+    for (int j = 0; j < STAGE; ++j) {
+      if (STAGE == 6 && j == 1) continue;
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] += c_a[STAGE][j] * kj[j].v[r];
+      for (int r = 0; r < R; ++r) acc[r] += c_a[STAGE][j] * kj[j].v[r];
+    }
+    Pack<VT, R> o;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      y.v[r] += h * acc[r];
+      o.v[r] = (VT)y.v[r];
+    }
+    if constexpr (STAGE == 6) *reinterpret_cast<Pack<double, R>*>(ytmp + n0) = y;
+    *reinterpret_cast<Pack<VT, R>*>(vin + n0) = o;
   }
-  Pack<VT, R> o;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    y.v[r] += h * acc[r];
-    o.v[r] = (VT)y.v[r];
-  }
-  if constexpr (STAGE == 6) *reinterpret_cast<Pack<double, R>*>(ytmp + n0) = y;
-  *reinterpret_cast<Pack<VT, R>*>(vin + n0) = o;
 }
 
 // out = A v over the state rows of active instances (R rows per thread). With the slot
@@ -125,106 +152,118 @@ __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, co
     if (threadIdx.x == 0) out[sg.row[seg]] = part[0] + part[1] + part[2] + part[3];
     return;
   }
-  const int64_t n0 = (((int64_t)blockIdx.x - sg.nseg) * kBlock + threadIdx.x) * R;
-  if (n0 >= ld) return;
-  const int inst = d.gran_inst[n0 / kGran];
-  if (d.h[inst] == 0.0) return;
-  const int64_t nend = d.row_off[inst] + d.nstates[inst];
-  if (n0 >= nend) return;
-  const Pack<double, R> dg = *reinterpret_cast<const Pack<double, R>*>(diag + n0);
-  const Pack<VT, R> vi = *reinterpret_cast<const Pack<VT, R>*>(v + n0);
-  double acc[R];
-  if constexpr (KT > 0) {
-    Pack<int32_t, R> c[KT];
-    Pack<VT, R> w[KT];
+  const uint32_t nl = *d.n_agran;
+  const int64_t units = ((int64_t)nl + R - 1) / R;
+  for (int64_t un = (int64_t)blockIdx.x - sg.nseg; un < units; un += (int64_t)gridDim.x - sg.nseg) {
+    int64_t n0;
+    if (!unit_rows<R>(d, nl, un, n0)) continue;
+    const int inst = d.gran_inst[n0 / kGran];
+    if (d.h[inst] == 0.0) continue;
+    const int64_t nend = d.row_off[inst] + d.nstates[inst];
+    if (n0 >= nend) continue;
+    const Pack<double, R> dg = *reinterpret_cast<const Pack<double, R>*>(diag + n0);
+    const Pack<VT, R> vi = *reinterpret_cast<const Pack<VT, R>*>(v + n0);
+    double acc[R];
+    if constexpr (KT > 0) {
+      Pack<int32_t, R> c[KT];
+      Pack<VT, R> w[KT];
 #pragma unroll
-    for (int s = 0; s < KT; ++s) {
-      c[s] = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
-      w[s] = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+      for (int s = 0; s < KT; ++s) {
+        c[s] = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
+        w[s] = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+      }
+      double gv[KT][R];
+#pragma unroll
+      for (int s = 0; s < KT; ++s) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) gv[s][r] = (double)v[c[s].v[r]];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
+#pragma unroll
+      for (int s = 0; s < KT; ++s) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += (double)w[s].v[r] * gv[s][r];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
+      for (int s = 0; s < K; ++s) {
+        const Pack<int32_t, R> c = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
+        const Pack<VT, R> w = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
+      }
     }
-    double gv[KT][R];
 #pragma unroll
-    for (int s = 0; s < KT; ++s) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) gv[s][r] = (double)v[c[s].v[r]];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
-#pragma unroll
-    for (int s = 0; s < KT; ++s) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] += (double)w[s].v[r] * gv[s][r];
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * (double)vi.v[r];
-    for (int s = 0; s < K; ++s) {
-      const Pack<int32_t, R> c = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
-      const Pack<VT, R> w = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
-#pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
-    }
+    for (int r = 0; r < R; ++r)
+      if (n0 + r < nend) out[n0 + r] = acc[r];
   }
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (n0 + r < nend) out[n0 + r] = acc[r];
 }
 
 // Error partials per granule: sum ((h sum_j e_j k_j) / (atol + rtol max(|u|, |y|)))^2.
-// A block covers R granules; granule j is waves [j W, (j + 1) W) with W = 4 / R.
+// A unit covers R list entries; entry j of the unit is waves [j W, (j + 1) W), W = 4 / R.
 template <int R>
-__global__ __launch_bounds__(kBlock) void k_berr(Dev d, int64_t ld, int64_t grans, double atol, double rtol,
-                                                 const double* u, const double* y, const double* k, double* partials) {
-  const int64_t n0 = first_row<R>();
-  double acc = 0.0;
-  const double h = n0 < ld ? d.h[d.gran_inst[n0 / kGran]] : 0.0;
-  if (h != 0.0) {
-    const Pack<double, R> uu = *reinterpret_cast<const Pack<double, R>*>(u + n0);
-    const Pack<double, R> yy = *reinterpret_cast<const Pack<double, R>*>(y + n0);
-    double e[R] = {};
-    // e_1 = 0 in Dormand-Prince 5(4): k1 is not read (a compile-time skip, so that the six
-    // loads go out together)
-    Pack<double, R> kj[7];
-#pragma unroll
-    for (int j = 0; j < 7; ++j)
-      if (j != 1) kj[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-      if (j == 1) continue;
-#pragma unroll
-      for (int r = 0; r < R; ++r) e[r] += c_e[j] * kj[j].v[r];
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double sc = atol + rtol * fmax(fabs(uu.v[r]), fabs(yy.v[r]));
-      const double q = h * e[r] / sc;
-      acc += q * q;
-    }
-  }
+__global__ __launch_bounds__(kBlock) void k_berr(Dev d, int64_t ld, const Params* par, const double* u,
+                                                 const double* y, const double* k, double* partials) {
   constexpr int W = (kBlock / 64) / R;
   __shared__ double part[kBlock / 64];
-  acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if ((int)threadIdx.x < R) {
-    const int64_t g = (int64_t)blockIdx.x * R + threadIdx.x;
-    if (g < grans) {
-      double sum = 0.0;
-      for (int w = 0; w < W; ++w) sum += part[threadIdx.x * W + w];
-      partials[g] = sum;
+  const double atol = par->atol, rtol = par->rtol;
+  const uint32_t nl = *d.n_agran;
+  const int64_t units = ((int64_t)nl + R - 1) / R;
+  for (int64_t un = blockIdx.x; un < units; un += gridDim.x) {
+    double acc = 0.0;
+    int64_t n0 = 0;
+    const bool ok = unit_rows<R>(d, nl, un, n0);
+    const double h = ok ? d.h[d.gran_inst[n0 / kGran]] : 0.0;
+    if (h != 0.0) {
+      const Pack<double, R> uu = *reinterpret_cast<const Pack<double, R>*>(u + n0);
+      const Pack<double, R> yy = *reinterpret_cast<const Pack<double, R>*>(y + n0);
+      double e[R] = {};
+      // e_1 = 0 in Dormand-Prince 5(4): k1 is not read (a compile-time skip, so that the
+      // six loads go out together)
+      Pack<double, R> kj[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j)
+        if (j != 1) kj[j] = *reinterpret_cast<const Pack<double, R>*>(k + (int64_t)j * ld + n0);
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        if (j == 1) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) e[r] += c_e[j] * kj[j].v[r];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double sc = atol + rtol * fmax(fabs(uu.v[r]), fabs(yy.v[r]));
+        const double q = h * e[r] / sc;
+        acc += q * q;
+      }
     }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if ((int)threadIdx.x < R) {
+      const int64_t j = un * R + threadIdx.x;
+      if (j < (int64_t)nl) {
+        double sum = 0.0;
+        for (int w = 0; w < W; ++w) sum += part[threadIdx.x * W + w];
+        partials[d.agran[j]] = sum;
+      }
+    }
+    __syncthreads();
   }
 }
 
 // Per-instance fold of the granule partials (granules of an instance are contiguous): one
 // wavefront per instance, lane l adding granules l, l + 64, ... in order, then a fixed
 // shuffle tree (deterministic)
-__global__ __launch_bounds__(kBlock) void k_bfold(int n_inst, const int64_t* gran_start, const double* partials,
-                                                  double* err) {
+__global__ __launch_bounds__(kBlock) void k_bfold(int n_inst, const int64_t* gran_start, const double* h,
+                                                  const double* partials, double* err) {
   const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n_inst) return;  // whole wavefronts leave together
+  if (h[i] == 0.0) return;  // not stepped in this iteration: its partials are stale
   double sum = 0.0;
   for (int64_t g = gran_start[i] + lane; g < gran_start[i + 1]; g += 64) sum += partials[g];
   sum = wave_sum(sum);
@@ -235,16 +274,21 @@ __global__ __launch_bounds__(kBlock) void k_bfold(int n_inst, const int64_t* gra
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_baccept(Dev d, int64_t ld, double* u, const double* y, double* k,
                                                     double* snaps) {
-  const int64_t n0 = first_row<R>();
-  if (n0 >= ld) return;
-  const int inst = d.gran_inst[n0 / kGran];
-  if (d.accept[inst]) {
-    *reinterpret_cast<Pack<double, R>*>(u + n0) = *reinterpret_cast<const Pack<double, R>*>(y + n0);
-    *reinterpret_cast<Pack<double, R>*>(k + n0) = *reinterpret_cast<const Pack<double, R>*>(k + 6 * ld + n0);
+  const uint32_t nl = *d.n_agran;
+  const int64_t units = ((int64_t)nl + R - 1) / R;
+  for (int64_t un = blockIdx.x; un < units; un += gridDim.x) {
+    int64_t n0;
+    if (!unit_rows<R>(d, nl, un, n0)) continue;
+    const int inst = d.gran_inst[n0 / kGran];
+    if (d.accept[inst]) {
+      *reinterpret_cast<Pack<double, R>*>(u + n0) = *reinterpret_cast<const Pack<double, R>*>(y + n0);
+      *reinterpret_cast<Pack<double, R>*>(k + n0) = *reinterpret_cast<const Pack<double, R>*>(k + 6 * ld + n0);
+    }
+    const int sl = d.snap[inst];
+    if (sl >= 0)
+      *reinterpret_cast<Pack<double, R>*>(snaps + (int64_t)sl * ld + n0) =
+          *reinterpret_cast<const Pack<double, R>*>(u + n0);
   }
-  const int sl = d.snap[inst];
-  if (sl >= 0)
-    *reinterpret_cast<Pack<double, R>*>(snaps + (int64_t)sl * ld + n0) = *reinterpret_cast<const Pack<double, R>*>(u + n0);
 }
 
 // k_baccept followed by stage 1 in one pass: the accepted (or kept) u and k0 stay in
@@ -252,27 +296,197 @@ __global__ __launch_bounds__(kBlock) void k_baccept(Dev d, int64_t ld, double* u
 template <typename VT, int R>
 __global__ __launch_bounds__(kBlock) void k_baccept_stage1(Dev d, int64_t ld, double* u, const double* y, double* k,
                                                            double* snaps, VT* vin) {
-  const int64_t n0 = first_row<R>();
-  if (n0 >= ld) return;
-  const int inst = d.gran_inst[n0 / kGran];
-  Pack<double, R> uu, k0;
-  if (d.accept[inst]) {
-    uu = *reinterpret_cast<const Pack<double, R>*>(y + n0);
-    k0 = *reinterpret_cast<const Pack<double, R>*>(k + 6 * ld + n0);
-    *reinterpret_cast<Pack<double, R>*>(u + n0) = uu;
-    *reinterpret_cast<Pack<double, R>*>(k + n0) = k0;
-  } else {
-    uu = *reinterpret_cast<const Pack<double, R>*>(u + n0);
-    k0 = *reinterpret_cast<const Pack<double, R>*>(k + n0);
-  }
-  const int sl = d.snap[inst];
-  if (sl >= 0) *reinterpret_cast<Pack<double, R>*>(snaps + (int64_t)sl * ld + n0) = uu;
-  const double h = d.h[inst];
-  if (h == 0.0) return;
-  Pack<VT, R> o;
+  const uint32_t nl = *d.n_agran;
+  const int64_t units = ((int64_t)nl + R - 1) / R;
+  for (int64_t un = blockIdx.x; un < units; un += gridDim.x) {
+    int64_t n0;
+    if (!unit_rows<R>(d, nl, un, n0)) continue;
+    const int inst = d.gran_inst[n0 / kGran];
+    Pack<double, R> uu, k0;
+    if (d.accept[inst]) {
+      uu = *reinterpret_cast<const Pack<double, R>*>(y + n0);
+      k0 = *reinterpret_cast<const Pack<double, R>*>(k + 6 * ld + n0);
+      *reinterpret_cast<Pack<double, R>*>(u + n0) = uu;
+      *reinterpret_cast<Pack<double, R>*>(k + n0) = k0;
+    } else {
+      uu = *reinterpret_cast<const Pack<double, R>*>(u + n0);
+      k0 = *reinterpret_cast<const Pack<double, R>*>(k + n0);
+    }
+    const int sl = d.snap[inst];
+    if (sl >= 0) *reinterpret_cast<Pack<double, R>*>(snaps + (int64_t)sl * ld + n0) = uu;
+    const double h = d.h[inst];
+    if (h == 0.0) continue;
+    Pack<VT, R> o;
 #pragma unroll
-  for (int r = 0; r < R; ++r) o.v[r] = (VT)(uu.v[r] + h * (c_a[1][0] * k0.v[r]));
-  *reinterpret_cast<Pack<VT, R>*>(vin + n0) = o;
+    for (int r = 0; r < R; ++r) o.v[r] = (VT)(uu.v[r] + h * (c_a[1][0] * k0.v[r]));
+    *reinterpret_cast<Pack<VT, R>*>(vin + n0) = o;
+  }
+}
+
+// ---- the step-size controller ----------------------------------------------------------
+// Per-instance state of the Dormand-Prince controller, in device memory.
+struct Ctl {
+  int n;
+  const Params* par;
+  const double* t_out;
+  const double* err;        // folded error sums of the iteration just run
+  const int64_t* nrows;     // error norm length per instance
+  const int64_t* gran_start;
+  double* t;                // current time
+  double* hc;               // controller step size
+  double* htry;             // step of the next iteration (0: inactive); Dev::h
+  int32_t* next;            // next output index
+  int32_t* done;
+  int32_t* clamped;         // htry was cut to hit an output time
+  int32_t* accept;          // Dev::accept
+  int32_t* snap;            // Dev::snap
+  int64_t* steps;
+  int64_t* rejected;
+  int32_t* status;          // 0 ok, 1 non-finite error, 2 step underflow, 3 max_steps
+  uint32_t* off;            // n + 1 list offsets (scratch)
+  int32_t* agran;           // Dev::agran of the next iteration
+  uint32_t* n_agran;        // Dev::n_agran
+  uint32_t* counters;       // [0] instances still running, [1] iterations with work
+};
+
+__device__ void control_one(const Ctl& c, int i, bool init, int n_out, double h0, int64_t max_steps) {
+  if (init) {
+    c.t[i] = n_out ? c.t_out[0] : 0.0;
+    c.next[i] = 1;
+    c.done[i] = n_out <= 1 ? 1 : 0;
+    c.steps[i] = 0;
+    c.rejected[i] = 0;
+    c.status[i] = 0;
+    c.accept[i] = 0;
+    c.snap[i] = n_out > 0 ? 0 : -1;  // the start is the first snapshot
+    const double tend = n_out ? c.t_out[n_out - 1] : 0.0;
+    // first step: a fraction of the span (the controller adapts it within a few steps)
+    c.hc[i] = h0 > 0 ? h0 : 1e-3 * fmax(1e-12, tend - c.t[i]);
+  } else if (c.htry[i] != 0.0) {
+    const double hs = c.htry[i];
+    const double err = sqrt(c.err[i] / (double)c.nrows[i]);
+    c.accept[i] = 0;
+    c.snap[i] = -1;
+    if (!isfinite(err)) {
+      c.status[i] = 1;
+      c.done[i] = 1;
+    } else {
+      double fac = err > 0 ? 0.9 * pow(err, -0.2) : 5.0;
+      double h;
+      if (err <= 1.0) {
+        c.accept[i] = 1;
+        c.steps[i]++;
+        double t = c.t[i] + hs;
+        const double target = c.t_out[c.next[i]];
+        if (t >= target - 1e-13 * fmax(1.0, fabs(target))) {
+          t = target;
+          c.snap[i] = c.next[i];
+          if (++c.next[i] >= n_out) c.done[i] = 1;
+        }
+        c.t[i] = t;
+        fac = fmin(5.0, fmax(0.2, fac));
+        h = (!c.clamped[i] || fac < 1.0) ? hs * fac : c.hc[i];
+      } else {
+        c.rejected[i]++;
+        h = hs * fmax(0.2, fac);
+      }
+      c.hc[i] = h;
+      if (!c.done[i] && !(h > 1e-14 * fmax(1.0, fabs(c.t[i])))) c.status[i] = 2, c.done[i] = 1;
+      if (!c.done[i] && max_steps && c.steps[i] + c.rejected[i] >= max_steps) c.status[i] = 3, c.done[i] = 1;
+    }
+  } else {
+    c.accept[i] = 0;
+    c.snap[i] = -1;
+  }
+  // the step of the next iteration, cut to the next output time
+  if (c.done[i]) {
+    c.htry[i] = 0.0;
+  } else {
+    const double target = c.t_out[c.next[i]];
+    const double hh = fmin(c.hc[i], target - c.t[i]);
+    c.clamped[i] = hh < c.hc[i];
+    c.htry[i] = hh;
+  }
+}
+
+// Block-wide exclusive scan of one value per thread (kCtlBlock threads); returns the
+// thread's offset and the block total.
+__device__ uint32_t block_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int k = 0; k < kCtlBlock / 64; ++k) {
+      const uint32_t s = sh[k];
+      sh[k] = run;
+      run += s;
+    }
+    sh[kCtlBlock / 64] = run;
+  }
+  __syncthreads();
+  const uint32_t res = sh[w] + x - v;
+  total = sh[kCtlBlock / 64];
+  __syncthreads();
+  return res;
+}
+
+// One workgroup: every instance's controller step, then the granule list of the instances
+// with work in the next iteration (a step to take, or an accept / snapshot to apply).
+__global__ __launch_bounds__(kCtlBlock) void k_bcontrol(Ctl c, int init) {
+  __shared__ uint32_t sh[kCtlBlock / 64 + 1];
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  const int n_out = c.par->n_out;
+  const double h0 = c.par->h0;
+  const int64_t max_steps = c.par->max_steps;
+  uint32_t running = 0;
+  for (int i = threadIdx.x; i < c.n; i += kCtlBlock) {
+    if (!init && c.htry[i] != 0.0) atomicOr(&any, 1);
+    control_one(c, i, init != 0, n_out, h0, max_steps);
+    running += c.htry[i] != 0.0;
+  }
+  // instance offsets into the list, in instance order
+  uint32_t carry = 0;
+  for (int base = 0; base < c.n; base += kCtlBlock) {
+    const int i = base + threadIdx.x;
+    uint32_t cnt = 0;
+    if (i < c.n && (c.htry[i] != 0.0 || c.accept[i] || c.snap[i] >= 0))
+      cnt = (uint32_t)(c.gran_start[i + 1] - c.gran_start[i]);
+    uint32_t tot;
+    const uint32_t o = block_scan(cnt, sh, tot);
+    if (i < c.n) c.off[i] = carry + o;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) c.off[c.n] = carry;
+  // running count: one wave sum per wave, then atomics (ordering-free: integers)
+  __shared__ uint32_t nrun;
+  if (threadIdx.x == 0) nrun = 0;
+  __syncthreads();
+  atomicAdd(&nrun, running);
+  __syncthreads();
+  // the list: entry w belongs to the instance whose [off, off + count) holds it
+  const uint32_t total = c.off[c.n];
+  for (uint32_t w = threadIdx.x; w < total; w += kCtlBlock) {
+    int lo = 0, hi = c.n - 1;  // last instance with off <= w
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (c.off[mid] <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    c.agran[w] = (int32_t)(c.gran_start[lo] + (w - c.off[lo]));
+  }
+  if (threadIdx.x == 0) {
+    *c.n_agran = total;
+    c.counters[0] = nrun;
+    if (any) c.counters[1]++;
+  }
 }
 
 // Copy one instance's slot-major ELL into the concatenated arrays.
@@ -298,6 +512,17 @@ __global__ void k_badd_off(int64_t n, const int32_t* src_s, int64_t off, int32_t
   if (i < n) src[i] = src_s[i] + (int32_t)off;
 }
 
+// Every instance listed, with step h (the autotuner's frozen iterations).
+__global__ void k_bfreeze(int n, double h, const int64_t* gran_start, double* htry, int32_t* accept, int32_t* snap,
+                          int32_t* agran, uint32_t* n_agran) {
+  const int64_t total = gran_start[n];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) agran[g] = (int32_t)g;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) htry[i] = h, accept[i] = 0, snap[i] = -1;
+  if (i == 0) *n_agran = (uint32_t)total;
+}
+
 }  // namespace
 }  // namespace nc
 
@@ -317,26 +542,27 @@ struct nc_batch_s {
   nc::DevBuf<int64_t> seg_row, seg_off;
   nc::DevBuf<unsigned char> sink_val;
   nc::DevBuf<int32_t> gran_inst;
-  nc::DevBuf<int64_t> d_row_off, d_nstates, d_gran_start;
+  nc::DevBuf<int64_t> d_row_off, d_nstates, d_gran_start, d_nrows;
   // integrator state
   nc::DevBuf<double> u, ytmp, k, partials, err, snaps;
   nc::DevBuf<unsigned char> vin;
-  nc::DevBuf<double> d_h;
-  nc::DevBuf<int32_t> d_accept, d_snap;
-  double* h_pin = nullptr;     // pinned mirrors of the per-instance controls
-  int32_t* acc_pin = nullptr;
-  int32_t* snap_pin = nullptr;
-  double* err_pin = nullptr;
+  // controller state (per instance) and the active granule list
+  nc::DevBuf<double> c_t, c_hc, d_h, t_out;
+  nc::DevBuf<int32_t> c_next, c_done, c_clamped, d_accept, d_snap, c_status;
+  nc::DevBuf<int64_t> c_steps, c_rej;
+  nc::DevBuf<uint32_t> c_off, n_agran, counters;
+  nc::DevBuf<int32_t> agran;
+  nc::DevBuf<nc::Params> par;
+  uint32_t* cnt_pin = nullptr;  // pinned copies of the running counter, one per chunk in flight
   int n_out = 0;
+  int n_out_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int graph_R = 0;
   ~nc_batch_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (h_pin) (void)hipHostFree(h_pin);
-    if (acc_pin) (void)hipHostFree(acc_pin);
-    if (snap_pin) (void)hipHostFree(snap_pin);
-    if (err_pin) (void)hipHostFree(err_pin);
+    if (cnt_pin) nc::pinned_free(cnt_pin);
   }
 };
 
@@ -431,10 +657,12 @@ nc_batch_s* batch_create(int n, const nc_matrix* mats) {
   B->gran_inst.alloc(B->grans);
   B->d_row_off.alloc(n);
   B->d_nstates.alloc(n);
+  B->d_nrows.alloc(n);
   B->d_gran_start.alloc(n + 1);
   NC_CHECK_HIP(hipMemcpyAsync(B->gran_inst.p, gran_inst.data(), sizeof(int32_t) * B->grans, hipMemcpyHostToDevice, s));
   NC_CHECK_HIP(hipMemcpyAsync(B->d_row_off.p, B->row_off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
   NC_CHECK_HIP(hipMemcpyAsync(B->d_nstates.p, B->nstates.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->d_nrows.p, B->nrows.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
   NC_CHECK_HIP(hipMemcpyAsync(B->d_gran_start.p, B->gran_start.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
   if (B->precision == NC_FP32)
     batch_build<float>(B.get(), mats);
@@ -447,27 +675,73 @@ nc_batch_s* batch_create(int n, const nc_matrix* mats) {
   B->vin.alloc(ld * B->vs);
   B->partials.alloc(B->grans);
   B->err.alloc(n);
-  B->d_h.alloc(n);
-  B->d_accept.alloc(n);
-  B->d_snap.alloc(n);
   NC_CHECK_HIP(hipMemsetAsync(B->u.p, 0, sizeof(double) * ld, s));
   NC_CHECK_HIP(hipMemsetAsync(B->ytmp.p, 0, sizeof(double) * ld, s));
   NC_CHECK_HIP(hipMemsetAsync(B->k.p, 0, sizeof(double) * 7 * ld, s));
   NC_CHECK_HIP(hipMemsetAsync(B->vin.p, 0, B->vs * ld, s));
-  NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->h_pin), sizeof(double) * n, hipHostMallocDefault));
-  NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->acc_pin), sizeof(int32_t) * n, hipHostMallocDefault));
-  NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->snap_pin), sizeof(int32_t) * n, hipHostMallocDefault));
-  NC_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->err_pin), sizeof(double) * n, hipHostMallocDefault));
+  NC_CHECK_HIP(hipMemsetAsync(B->partials.p, 0, sizeof(double) * B->grans, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->err.p, 0, sizeof(double) * n, s));
+  B->c_t.alloc(n);
+  B->c_hc.alloc(n);
+  B->d_h.alloc(n);
+  B->c_next.alloc(n);
+  B->c_done.alloc(n);
+  B->c_clamped.alloc(n);
+  B->d_accept.alloc(n);
+  B->d_snap.alloc(n);
+  B->c_status.alloc(n);
+  B->c_steps.alloc(n);
+  B->c_rej.alloc(n);
+  B->c_off.alloc(n + 1);
+  B->n_agran.alloc(1);
+  B->counters.alloc(2);
+  B->agran.alloc(B->grans);
+  B->par.alloc(1);
+  NC_CHECK_HIP(hipMemsetAsync(B->par.p, 0, sizeof(Params), s));
+  NC_CHECK_HIP(hipMemsetAsync(B->d_h.p, 0, sizeof(double) * n, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->c_clamped.p, 0, sizeof(int32_t) * n, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->n_agran.p, 0, sizeof(uint32_t), s));
+  NC_CHECK_HIP(hipMemsetAsync(B->counters.p, 0, sizeof(uint32_t) * 2, s));
+  B->cnt_pin = static_cast<uint32_t*>(pinned_alloc(sizeof(uint32_t) * 4));
   NC_CHECK_HIP(hipStreamSynchronize(s));
   return B.release();
 }
 
 Dev dev_view(nc_batch_s* B) {
-  return Dev{B->gran_inst.p, B->d_row_off.p, B->d_nstates.p, B->d_h.p, B->d_accept.p, B->d_snap.p};
+  return Dev{B->gran_inst.p, B->d_row_off.p, B->d_nstates.p, B->d_h.p, B->d_accept.p, B->d_snap.p,
+             B->agran.p,     B->n_agran.p};
 }
 
-// blocks of a launch covering every row with R rows per thread
-unsigned row_grid(const nc_batch_s* B, int R) { return (unsigned)((B->grans + R - 1) / R); }
+Ctl ctl_view(nc_batch_s* B) {
+  Ctl c{};
+  c.n = B->n;
+  c.par = B->par.p;
+  c.t_out = B->t_out.p;
+  c.err = B->err.p;
+  c.nrows = B->d_nrows.p;
+  c.gran_start = B->d_gran_start.p;
+  c.t = B->c_t.p;
+  c.hc = B->c_hc.p;
+  c.htry = B->d_h.p;
+  c.next = B->c_next.p;
+  c.done = B->c_done.p;
+  c.clamped = B->c_clamped.p;
+  c.accept = B->d_accept.p;
+  c.snap = B->d_snap.p;
+  c.steps = B->c_steps.p;
+  c.rejected = B->c_rej.p;
+  c.status = B->c_status.p;
+  c.off = B->c_off.p;
+  c.agran = B->agran.p;
+  c.n_agran = B->n_agran.p;
+  c.counters = B->counters.p;
+  return c;
+}
+
+// blocks of a list-driven launch: every unit of R granules once, at most kMaxRowBlocks
+unsigned row_grid(const nc_batch_s* B, int R) {
+  return (unsigned)std::min<int64_t>((B->grans + R - 1) / R, kMaxRowBlocks);
+}
 
 template <typename VT, int R>
 void launch_matvec(nc_batch_s* B, const VT* v, double* out) {
@@ -511,10 +785,15 @@ void launch_stage(nc_batch_s* B, int stage, unsigned g) {
   }
 }
 
-// accept/snapshot of the previous step (fused with stage 1), then one full DP5 attempt and
-// the error fold
+void launch_control(nc_batch_s* B, int init) {
+  k_bcontrol<<<1, kCtlBlock, 0, B->stream.s>>>(ctl_view(B), init);
+  NC_CHECK_HIP(hipGetLastError());
+}
+
+// accept/snapshot of the previous step (fused with stage 1), then one full DP5 attempt,
+// the error fold and (unless frozen) the controller
 template <typename VT, int R>
-void iteration_r(nc_batch_s* B, double atol, double rtol) {
+void iteration_r(nc_batch_s* B, bool control) {
   const Dev d = dev_view(B);
   hipStream_t s = B->stream.s;
   const unsigned g = row_grid(B, R);
@@ -524,26 +803,25 @@ void iteration_r(nc_batch_s* B, double atol, double rtol) {
     if (st > 1) launch_stage<VT, R>(B, st, g);
     launch_matvec<VT, R>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p + (int64_t)st * B->ld);
   }
-  k_berr<R><<<g, kBlock, 0, s>>>(d, B->ld, B->grans, atol, rtol, B->u.p, B->ytmp.p, B->k.p, B->partials.p);
-  k_bfold<<<(unsigned)((B->n + kBlock / 64 - 1) / (kBlock / 64)), kBlock, 0, s>>>(B->n, B->d_gran_start.p,
+  k_berr<R><<<g, kBlock, 0, s>>>(d, B->ld, B->par.p, B->u.p, B->ytmp.p, B->k.p, B->partials.p);
+  k_bfold<<<(unsigned)((B->n + kBlock / 64 - 1) / (kBlock / 64)), kBlock, 0, s>>>(B->n, B->d_gran_start.p, B->d_h.p,
                                                                                 B->partials.p, B->err.p);
   NC_CHECK_HIP(hipGetLastError());
+  if (control) launch_control(B, 0);
 }
 
 template <typename VT>
-void iteration(nc_batch_s* B, double atol, double rtol) {
+void iteration(nc_batch_s* B, bool control) {
   switch (B->rows_per_thread) {
-    case 1: iteration_r<VT, 1>(B, atol, rtol); break;
-    case 2: iteration_r<VT, 2>(B, atol, rtol); break;
-    default: iteration_r<VT, 4>(B, atol, rtol); break;
+    case 1: iteration_r<VT, 1>(B, control); break;
+    case 2: iteration_r<VT, 2>(B, control); break;
+    default: iteration_r<VT, 4>(B, control); break;
   }
 }
 
-// k0 = A u for every instance (stage 0 copies u into the mat-vec input)
+// k0 = A u for every listed instance (stage 0 copies u into the mat-vec input)
 template <typename VT, int R>
 void first_stage_r(nc_batch_s* B) {
-  hipStream_t s = B->stream.s;
-  (void)s;
   launch_stage<VT, R>(B, 0, row_grid(B, R));
   launch_matvec<VT, R>(B, reinterpret_cast<const VT*>(B->vin.p), B->k.p);
 }
@@ -561,13 +839,47 @@ void flush_accept(nc_batch_s* B) {
   hipStream_t s = B->stream.s;
   const int R = B->rows_per_thread;
   const Dev d = dev_view(B);
+  const unsigned g = row_grid(B, R);
   if (R == 1)
-    k_baccept<1><<<row_grid(B, 1), kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+    k_baccept<1><<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
   else if (R == 2)
-    k_baccept<2><<<row_grid(B, 2), kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+    k_baccept<2><<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
   else
-    k_baccept<4><<<row_grid(B, 4), kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
+    k_baccept<4><<<g, kBlock, 0, s>>>(d, B->ld, B->u.p, B->ytmp.p, B->k.p, B->snaps.p);
   NC_CHECK_HIP(hipGetLastError());
+}
+
+void drop_graph(nc_batch_s* B) {
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+}
+
+// kChunk iterations, each ending with the controller
+template <typename VT>
+void run_chunk(nc_batch_s* B, bool graphs) {
+  hipStream_t s = B->stream.s;
+  if (!graphs) {
+    for (int c = 0; c < kChunk; ++c) iteration<VT>(B, true);
+    return;
+  }
+  if (B->exec && B->graph_R != B->rows_per_thread) drop_graph(B);
+  if (!B->exec) {
+    NC_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int c = 0; c < kChunk; ++c) iteration<VT>(B, true);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    NC_CHECK_HIP(hipStreamEndCapture(s, &B->graph));
+    NC_CHECK_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    B->graph_R = B->rows_per_thread;
+  }
+  NC_CHECK_HIP(hipGraphLaunch(B->exec, s));
 }
 
 template <typename VT>
@@ -575,118 +887,84 @@ void batch_solve_t(nc_batch_s* B, const nc_solve_opts& o, const double* u0, int 
                    nc_solve_stats* st) {
   hipStream_t s = B->stream.s;
   const int n = B->n;
-  const double rtol = o.rtol > 0 ? o.rtol : 1e-8, atol = o.atol > 0 ? o.atol : 1e-14;
-  // initial state and snapshot storage
+  const bool graphs = o.no_graphs == 0;
+  Params* hp = static_cast<Params*>(host_stage(1));
+  *hp = Params{o.atol > 0 ? o.atol : 1e-14, o.rtol > 0 ? o.rtol : 1e-8, o.h0, o.max_steps, n_out};
+  NC_CHECK_HIP(hipMemcpyAsync(B->par.p, hp, sizeof(Params), hipMemcpyHostToDevice, s));
+  // initial state, output times and snapshot storage (the captured chunk holds the
+  // snapshot and output-time pointers: drop it with the old buffers)
   NC_CHECK_HIP(hipMemsetAsync(B->u.p, 0, sizeof(double) * B->ld, s));
   int64_t hoff = 0;
   for (int i = 0; i < n; ++i) {
     NC_CHECK_HIP(hipMemcpyAsync(B->u.p + B->row_off[i], u0 + hoff, sizeof(double) * B->nrows[i], hipMemcpyHostToDevice, s));
     hoff += B->nrows[i];
   }
-  if ((size_t)std::max(n_out, 1) * B->ld > B->snaps.n) {
-    // the captured iteration holds the snapshot pointer: drop it with the old buffer
-    if (B->exec) (void)hipGraphExecDestroy(B->exec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
-    B->exec = nullptr;
-    B->graph = nullptr;
-    B->snaps.alloc((size_t)std::max(n_out, 1) * B->ld);
+  if ((size_t)std::max(n_out, 1) * B->ld > B->snaps.n || n_out > B->n_out_cap) {
+    drop_graph(B);
+    B->snaps.reserve((size_t)std::max(n_out, 1) * B->ld);
+    B->t_out.reserve(std::max(n_out, 1));
+    B->n_out_cap = std::max(n_out, 1);
   }
   B->n_out = n_out;
-  // k0 = A u for every instance (all marked active with h = 1 for the launch)
-  for (int i = 0; i < n; ++i) B->h_pin[i] = 1.0, B->acc_pin[i] = 0, B->snap_pin[i] = n_out > 0 ? 0 : -1;
-  NC_CHECK_HIP(hipMemcpyAsync(B->d_h.p, B->h_pin, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  NC_CHECK_HIP(hipMemcpyAsync(B->d_accept.p, B->acc_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-  NC_CHECK_HIP(hipMemcpyAsync(B->d_snap.p, B->snap_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  if (n_out) NC_CHECK_HIP(hipMemcpyAsync(B->t_out.p, t_out, sizeof(double) * n_out, hipMemcpyHostToDevice, s));
+  NC_CHECK_HIP(hipMemsetAsync(B->counters.p, 0, sizeof(uint32_t) * 2, s));
+  launch_control(B, 1);  // initial controller state, first steps and list
   first_stage<VT>(B);
-  // per-instance controller state (the first output time is the start)
-  std::vector<double> t(n, n_out ? t_out[0] : 0.0), h(n, 0.0), h_try(n, 0.0);
-  std::vector<int> next(n, 1), done(n, 0), clamped(n, 0);
-  std::vector<int64_t> steps(n, 0), rejected(n, 0);
-  const double tend = n_out ? t_out[n_out - 1] : 0.0;
-  for (int i = 0; i < n; ++i) {
-    done[i] = n_out <= 1 ? 1 : 0;
-    // first step: a fraction of the span (the controller adapts it within a few steps)
-    h[i] = o.h0 > 0 ? o.h0 : 1e-3 * std::max(1e-12, tend - t[i]);
-  }
-  int64_t iters = 0, host_syncs = 0;
-  // snapshot slot 0 (the start) is filled by the first accept/snapshot kernel
-  for (;;) {
-    int active = 0;
-    for (int i = 0; i < n; ++i) {
-      if (done[i]) {
-        h_try[i] = 0.0;
-        continue;
-      }
-      const double target = t_out[next[i]];
-      double hh = std::min(h[i], target - t[i]);
-      clamped[i] = hh < h[i];
-      h_try[i] = hh;
-      ++active;
-    }
-    for (int i = 0; i < n; ++i) B->h_pin[i] = h_try[i];
-    NC_CHECK_HIP(hipMemcpyAsync(B->d_h.p, B->h_pin, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    NC_CHECK_HIP(hipMemcpyAsync(B->d_accept.p, B->acc_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-    NC_CHECK_HIP(hipMemcpyAsync(B->d_snap.p, B->snap_pin, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-    if (!active) {
-      // flush the last accepts and snapshots
-      flush_accept(B);
-      NC_CHECK_HIP(hipStreamSynchronize(s));
-      break;
-    }
-    NC_REQUIRE(!o.max_steps || iters < o.max_steps, NC_ERR_NUMERIC, "batched RK exceeded max_steps");
-    if (o.no_graphs) {
-      iteration<VT>(B, atol, rtol);
-    } else {
-      if (!B->exec) {
-        NC_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-          iteration<VT>(B, atol, rtol);
-        } catch (...) {
-          hipGraph_t g = nullptr;
-          (void)hipStreamEndCapture(s, &g);
-          if (g) (void)hipGraphDestroy(g);
-          throw;
+  NC_CHECK_HIP(hipMemcpyAsync(B->cnt_pin, B->counters.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipStreamSynchronize(s));
+  int64_t chunks = 0, host_syncs = 1;
+  // every instance ends by its own limits (status 2 or 3) first; this bound only keeps a
+  // pathological controller from queueing work forever (8M iterations without max_steps)
+  const int64_t max_chunks = o.max_steps > 0 ? o.max_steps / kChunk + 2 : (int64_t)1 << 20;
+  if (B->cnt_pin[0] != 0) {
+    // two chunks in flight: while one runs, the host reads the counter of the one before
+    hipEvent_t ev[2];
+    NC_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    NC_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    try {
+      for (;;) {
+        const int slot = (int)(chunks & 1);
+        run_chunk<VT>(B, graphs);
+        NC_CHECK_HIP(hipMemcpyAsync(B->cnt_pin + slot, B->counters.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        NC_CHECK_HIP(hipEventRecord(ev[slot], s));
+        ++chunks;
+        if (chunks >= 2) {
+          NC_CHECK_HIP(hipEventSynchronize(ev[slot ^ 1]));
+          ++host_syncs;
+          if (B->cnt_pin[slot ^ 1] == 0) break;
         }
-        NC_CHECK_HIP(hipStreamEndCapture(s, &B->graph));
-        NC_CHECK_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        // a bound every instance's own limits reach first (status 2 or 3 ends an instance)
+        NC_REQUIRE(chunks < max_chunks, NC_ERR_NUMERIC, "batched RK exceeded max_steps");
       }
-      NC_CHECK_HIP(hipGraphLaunch(B->exec, s));
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      (void)hipEventDestroy(ev[0]);
+      (void)hipEventDestroy(ev[1]);
+      throw;
     }
-    NC_CHECK_HIP(hipMemcpyAsync(B->err_pin, B->err.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
-    ++iters;
-    ++host_syncs;
-    for (int i = 0; i < n; ++i) {
-      B->acc_pin[i] = 0;
-      B->snap_pin[i] = -1;
-      if (h_try[i] == 0.0) continue;
-      const double err = std::sqrt(B->err_pin[i] / (double)B->nrows[i]);
-      NC_REQUIRE(std::isfinite(err), NC_ERR_NUMERIC, "non-finite error estimate in a batched instance");
-      double fac = err > 0 ? 0.9 * std::pow(err, -0.2) : 5.0;
-      if (err <= 1.0) {
-        B->acc_pin[i] = 1;
-        ++steps[i];
-        t[i] += h_try[i];
-        const double target = t_out[next[i]];
-        if (t[i] >= target - 1e-13 * std::max(1.0, std::fabs(target))) {
-          t[i] = target;
-          B->snap_pin[i] = next[i];
-          if (++next[i] >= n_out) done[i] = 1;
-        }
-        fac = std::min(5.0, std::max(0.2, fac));
-        if (!clamped[i] || fac < 1.0) h[i] = h_try[i] * fac;
-      } else {
-        ++rejected[i];
-        h[i] = h_try[i] * std::max(0.2, fac);
-      }
-      NC_REQUIRE(h[i] > 1e-14 * std::max(1.0, std::fabs(t[i])), NC_ERR_NUMERIC, "batched RK step size underflow");
-    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  }
+  // apply the accepts and snapshots the last controller asked for
+  flush_accept(B);
+  std::vector<int32_t> status(n);
+  std::vector<int64_t> steps(n), rej(n);
+  NC_CHECK_HIP(hipMemcpyAsync(status.data(), B->c_status.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipMemcpyAsync(steps.data(), B->c_steps.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipMemcpyAsync(rej.data(), B->c_rej.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipMemcpyAsync(B->cnt_pin + 2, B->counters.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i) {
+    if (status[i] == 1) throw Error(NC_ERR_NUMERIC, "non-finite error estimate in batched instance " + std::to_string(i));
+    if (status[i] == 2) throw Error(NC_ERR_NUMERIC, "batched RK step size underflow in instance " + std::to_string(i));
+    if (status[i] == 3) throw Error(NC_ERR_NUMERIC, "batched RK exceeded max_steps in instance " + std::to_string(i));
   }
   if (st) {
     *st = nc_solve_stats{};
-    for (int i = 0; i < n; ++i) st->steps += steps[i], st->rejected += rejected[i];
+    for (int i = 0; i < n; ++i) st->steps += steps[i], st->rejected += rej[i];
     st->host_syncs = host_syncs;
-    st->matvecs = iters * 6;  // batched mat-vec launches
+    st->matvecs = (int64_t)B->cnt_pin[2] * 6;  // batched mat-vec launches with work
   }
 }
 
@@ -709,11 +987,15 @@ int nc_batch_autotune(nc_batch B, int32_t reps, int32_t* rows_per_thread, double
   return guard([&] {
     NC_REQUIRE(B && reps >= 1, NC_ERR_ARG, "bad arguments");
     hipStream_t s = B->stream.s;
-    // every instance active with a tiny step; nothing accepted, nothing saved
-    for (int i = 0; i < B->n; ++i) B->h_pin[i] = 1e-9, B->acc_pin[i] = 0, B->snap_pin[i] = -1;
-    NC_CHECK_HIP(hipMemcpyAsync(B->d_h.p, B->h_pin, sizeof(double) * B->n, hipMemcpyHostToDevice, s));
-    NC_CHECK_HIP(hipMemcpyAsync(B->d_accept.p, B->acc_pin, sizeof(int32_t) * B->n, hipMemcpyHostToDevice, s));
-    NC_CHECK_HIP(hipMemcpyAsync(B->d_snap.p, B->snap_pin, sizeof(int32_t) * B->n, hipMemcpyHostToDevice, s));
+    // every instance listed with a tiny step and the controller left out: nothing is
+    // accepted or saved, and every iteration does the full batch's work
+    Params* hp = static_cast<Params*>(host_stage(1));
+    *hp = Params{1e-14, 1e-8, 0.0, 0, 0};
+    NC_CHECK_HIP(hipMemcpyAsync(B->par.p, hp, sizeof(Params), hipMemcpyHostToDevice, s));
+    k_bfreeze<<<grid_for(std::max<int64_t>(B->grans, B->n), 256), 256, 0, s>>>(
+        B->n, 1e-9, B->d_gran_start.p, B->d_h.p, B->d_accept.p, B->d_snap.p, B->agran.p, B->n_agran.p);
+    NC_CHECK_HIP(hipGetLastError());
+    if (!B->snaps.p) B->snaps.reserve(B->ld);  // the accept kernel takes a snapshot pointer
     hipEvent_t e0, e1;
     NC_CHECK_HIP(hipEventCreate(&e0));
     NC_CHECK_HIP(hipEventCreate(&e1));
@@ -723,9 +1005,9 @@ int nc_batch_autotune(nc_batch B, int32_t reps, int32_t* rows_per_thread, double
       B->rows_per_thread = r;
       auto it = [&] {
         if (B->precision == NC_FP32)
-          iteration<float>(B, 1e-14, 1e-8);
+          iteration<float>(B, false);
         else
-          iteration<double>(B, 1e-14, 1e-8);
+          iteration<double>(B, false);
       };
       it();  // warm-up
       NC_CHECK_HIP(hipEventRecord(e0, s));
@@ -739,11 +1021,7 @@ int nc_batch_autotune(nc_batch B, int32_t reps, int32_t* rows_per_thread, double
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     B->rows_per_thread = best_r;
-    // a captured iteration was recorded with the previous launch shape
-    if (B->exec) (void)hipGraphExecDestroy(B->exec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
-    B->exec = nullptr;
-    B->graph = nullptr;
+    drop_graph(B);  // a captured chunk was recorded with the previous launch shape
     if (rows_per_thread) *rows_per_thread = best_r;
     if (best_ms) *best_ms = best;
   });
@@ -753,12 +1031,7 @@ int nc_batch_set_rows_per_thread(nc_batch B, int32_t rows) {
   return guard([&] {
     NC_REQUIRE(B, NC_ERR_ARG, "null batch");
     NC_REQUIRE(rows == 1 || rows == 2 || rows == 4, NC_ERR_ARG, "rows per thread must be 1, 2 or 4");
-    if (rows != B->rows_per_thread) {
-      if (B->exec) (void)hipGraphExecDestroy(B->exec);
-      if (B->graph) (void)hipGraphDestroy(B->graph);
-      B->exec = nullptr;
-      B->graph = nullptr;
-    }
+    if (rows != B->rows_per_thread) drop_graph(B);
     B->rows_per_thread = rows;
   });
 }
