@@ -1,18 +1,20 @@
 """Benchmark: FSP solves/s on one MI355X, with the mat-vec roofline and a CPU baseline.
 
-Workload (one "step" = one complete solve): the two-species toggle switch on the fixed
-state space 0..1023 x 0..1023 (1,048,576 states + 4 sinks, 5,240,832 reference
-nonzeros), production 50/(1+(S/20)^2.5)+1, degradation 1 per molecule, initial state
-(0,0), outputs at t = 0,1,...,10, device Krylov expv with m = 30. Storage FP32 (matrix
-values and Krylov basis) with FP64 diagonal, accumulation and master state; the l1
-distance of every snapshot to the FP64 GPU solve is measured in the same run (the FP64
-path itself is checked against the exact oracle by the test suite).
+Workload (one "step" = one complete solve, from model upload to snapshot download): the
+two-species toggle switch on the fixed state space 0..1023 x 0..1023 (1,048,576 states +
+4 sinks, 5,240,832 reference nonzeros), production 50/(1+(S/20)^2.5)+1, degradation 1 per
+molecule, initial state (0,0), outputs at t = 0,1,...,10, device Krylov expv with m = 30.
+Each step traces and uploads the model, builds the state space, the generator and the
+solver on the GPU, integrates, and downloads the 11 snapshots. Storage FP32 (matrix values
+and Krylov basis) with FP64 diagonal, accumulation and master state; the l1 distance of
+every snapshot to the FP64 GPU solve at tolerance 1e-12 is measured in the same run (the
+FP64 path itself is checked against the exact oracle by the test suite).
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|fp64] [--no-cpu]
 
-Prints one JSON line. Inputs are resident on the GPU before the timed region; each
-timed solve restarts from the device-resident initial state and keeps its 11
-snapshots on the device.
+Prints one JSON line. Besides the headline, the line carries the other quantities the
+metric names: the batched form against the plain FP64 single-solve path (C4), the learned
+adapter against the plain FP64 adapter (C5), the SpMV roofline, and the CPU restatement.
 """
 from __future__ import annotations
 
@@ -98,32 +100,87 @@ class Solver:
         self.L.call("nc_solver_destroy", self.h)
 
 
-def cpu_baseline(model_theta, gpu_fp64_steps, max_steps=3):
-    """Oracle-built CSC matrix, C single-thread CSC mat-vec + Arnoldi; sample of Krylov steps."""
-    from oracle.cpu_ref import CscExpv, build
+def toggle_oracle_csc(theta, n=1024):
+    """The oracle's FP64 toggle generator on the n x n box (bench-side CPU restatement input)."""
     from oracle.fspmatrix import FspMatrix, TimeInvariant
     from oracle.statespace import StateSpace
 
-    build()
     S = np.array([[1, -1, 0, 0], [0, 0, 1, -1]])
-    th = model_theta
+    th = theta
     hill = lambda y: th[3] + th[0] / (1.0 + (y / th[1]) ** th[2])
     props = [TimeInvariant(lambda x, p: hill(x[1])), TimeInvariant(lambda x, p: th[4] * x[0]),
              TimeInvariant(lambda x, p: hill(x[0])), TimeInvariant(lambda x, p: th[4] * x[1])]
-    sp = StateSpace(S, box_states(1024))
-    A = FspMatrix(sp, props, th).to_csc(0.0)
-    ex = CscExpv(A, m=30, tol=1e-12)
+    return FspMatrix(StateSpace(S, box_states(n)), props, th).to_csc(0.0)
+
+
+def cpu_baseline(model_theta, krylov_tol, gpu_fp64_steps, st_fraction=0.05, log=None):
+    """The C restatement of the reference's CPU path (oracle/cpu_ref.c) on the host cores:
+    the same Krylov expv algorithm, FP64, on C2 (the full solve with the row-parallel CSR
+    mat-vec on every core; the first twentieth of the solve's steps with the reference's
+    one-thread CSC scatter, scaled by the step count), on C1 (full solve, one thread) and on
+    a sample of the C4 table (all cores)."""
+    from oracle import cpu_ref
+    from oracle.fspmatrix import FspMatrix
+    from oracle.statespace import StateSpace
+
+    say = log or (lambda m: None)
+    cpu_ref.build()
+    cores = cpu_ref.cpu_count()
+    ts = np.arange(0.0, 11.0)
+    A = toggle_oracle_csc(model_theta)
     u0 = np.zeros(A.shape[0])
     u0[0] = 1.0
-    _, steps, secs = ex.run(u0, 10.0, max_steps=max_steps)
-    per_step = secs / max(steps, 1)
+    # all cores, full solve
+    ex = cpu_ref.Expv(A, m=30, tol=krylov_tol, layout="csr", threads=cores)
+    _, steps_mt, secs_mt = ex.solve(u0, ts)
+    say(f"cpu: C2 full solve on {cores} threads {secs_mt:.1f}s ({steps_mt} steps)")
+    # one thread, the reference's CSC scatter, a quarter of the steps (from t = 0)
+    ex1 = cpu_ref.Expv(A, m=30, tol=krylov_tol, layout="csc")
+    k1 = max(1, int(round(st_fraction * steps_mt)))
+    _, st1, secs1 = ex1.run(u0, 10.0, max_steps=k1)
+    per_step1 = secs1 / max(st1, 1)
+    say(f"cpu: C2 one thread {st1} steps {secs1:.1f}s")
+    # C1: telegraph on 800 states, one thread, full solve
+    S1 = np.array([[-1, 1, 0, 0], [1, -1, 0, 0], [0, 0, 1, -1]])
+    from oracle.fspmatrix import TimeInvariant
+    th1 = (0.05, 0.1, 5.0, 0.05)
+    props1 = [TimeInvariant(lambda x, p: p[0] * x[0]), TimeInvariant(lambda x, p: p[1] * x[1]),
+              TimeInvariant(lambda x, p: p[2] * x[1]), TimeInvariant(lambda x, p: p[3] * x[2])]
+    st_c1 = [[1, 0, m] for m in range(400)] + [[0, 1, m] for m in range(400)]
+    A1 = FspMatrix(StateSpace(S1, st_c1), props1, np.asarray(th1)).to_csc(0.0)
+    w1 = np.zeros(A1.shape[0])
+    w1[0] = 1.0
+    _, c1_steps, c1_secs = cpu_ref.Expv(A1, m=30, tol=krylov_tol, layout="csc").solve(w1, np.arange(0.0, 201.0, 10.0))
+    # C4: 8 instances evenly spaced by size, all cores, full solves
+    table = json.loads(C4_TABLE.read_text())
+    order = np.argsort([t["states"] for t in table], kind="stable")
+    sample = order[np.linspace(0, len(order) - 1, 8).round().astype(int)]
+    c4_secs = 0.0
+    for i in sample:
+        Ai = c4_oracle_csc(table[i])
+        ui = np.zeros(Ai.shape[0])
+        ui[0] = 1.0
+        _, _, sec = cpu_ref.Expv(Ai, m=30, tol=krylov_tol, layout="csr", threads=cores).solve(ui, ts)
+        c4_secs += sec
+    say(f"cpu: C4 sample {c4_secs:.1f}s")
     return {
-        "value": 1.0 / (per_step * gpu_fp64_steps),
+        "value": 1.0 / secs_mt,
         "unit": "solves/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": f"{steps} Krylov steps (m=30, {steps * 31} CSC column-scatter mat-vecs + CGS2) of the same solve "
-                  f"in FP64, {secs:.2f} s, scaled by the {gpu_fp64_steps} steps of the full FP64 solve",
+        "label": "restatement, not reference (Julia is not available on the box)",
+        "sample": f"the full FP64 C2 solve ({steps_mt} Krylov steps, m=30, tol {krylov_tol:g}) with a row-parallel "
+                  f"CSR mat-vec and vector work on {cores} OpenMP threads, {secs_mt:.1f} s",
+        "one_thread_csc": {
+            "solves_per_s": 1.0 / (per_step1 * steps_mt), "cores": 1,
+            "sample": f"{st1} of the {steps_mt} Krylov steps ({st1 / steps_mt:.0%}) of the same FP64 solve with "
+                      f"the reference's CSC column-scatter mat-vec, {secs1:.1f} s, scaled by the step count"},
+        "c1_one_thread_solves_per_s": 1.0 / c1_secs,
+        "c1_sample": f"the full C1 solve (800 states, t = 0:10:200, {c1_steps} Krylov steps), one thread, CSC",
+        "c4_all_cores_solves_per_s": len(sample) / c4_secs,
+        "c4_sample": f"{len(sample)} C4 instances evenly spaced by size, full FP64 solves, {cores} threads, "
+                     f"{c4_secs:.1f} s",
+        "gpu_fp64_krylov_steps": gpu_fp64_steps,
     }
 
 
@@ -226,60 +283,96 @@ def c4_instance(nh, spec, precision):
     return A, u0
 
 
-def c4_batch(nh, L, n_plain=32, graphs=True):
-    """Config C4: 512 uneven instances in one batched FP32 solve against the plain FP64
-    single-solve path (timed on an evenly spaced sample by size, scaled to 512)."""
+def c4_oracle_csc(spec):
+    """The oracle's FP64 generator of one C4 instance (the CPU restatement's input)."""
+    from oracle.fspmatrix import FspMatrix, TimeInvariant
+    from oracle.statespace import StateSpace
+
+    th, nmax = np.asarray(spec["theta"], dtype=float), spec["nmax"]
+    if spec["kind"] == "telegraph":
+        S = np.array([[-1, 1, 0, 0], [1, -1, 0, 0], [0, 0, 1, -1]])
+        props = [TimeInvariant(lambda x, p: p[0] * x[0]), TimeInvariant(lambda x, p: p[1] * x[1]),
+                 TimeInvariant(lambda x, p: p[2] * x[1]), TimeInvariant(lambda x, p: p[3] * x[2])]
+        states = [[1, 0, m] for m in range(nmax + 1)] + [[0, 1, m] for m in range(nmax + 1)]
+    else:
+        S = np.array([[1, -1, 0, 0], [0, 0, 1, -1]])
+        hill = lambda y, p: p[3] + p[0] / (1.0 + (y / p[1]) ** p[2])
+        props = [TimeInvariant(lambda x, p: hill(x[1], p)), TimeInvariant(lambda x, p: p[4] * x[0]),
+                 TimeInvariant(lambda x, p: hill(x[0], p)), TimeInvariant(lambda x, p: p[4] * x[1])]
+        states = box_states(nmax + 1)
+    return FspMatrix(StateSpace(S, states), props, th).to_csc(0.0)
+
+
+def c4_batch(nh, L, krylov_tol, graphs=True, log=None):
+    """Config C4: the 512 uneven instances in one batched FP32 solve against the plain FP64
+    single-solve path, both timed on one clock: from the upload of every instance's state
+    space and generator to the download of every snapshot."""
+    say = log or (lambda m: None)
     table = json.loads(C4_TABLE.read_text())
     ts = np.arange(0.0, 11.0)
-    mats, u0s = [], []
-    for spec in table:
-        A, u0 = c4_instance(nh, spec, "fp32")
-        mats.append(A)
-        u0s.append(u0)
-    B = nh.BatchFsp(mats)
-    rows_per_thread, tune_ms = B.autotune(reps=3)
-    B.solve(u0s, ts, fetch=False, graphs=graphs)  # warm-up: captures the iteration graph
-    t0 = time.perf_counter()
-    B.solve(u0s, ts, fetch=False, graphs=graphs)
-    batch_s = time.perf_counter() - t0
-    stats = dict(B.stats)
-    order = np.argsort([t["states"] for t in table])
-    sample = order[np.linspace(0, len(order) - 1, n_plain).round().astype(int)]
-    plain = {"rk45": 0.0, "krylov": 0.0}
+
+    def batched():
+        t0 = time.perf_counter()
+        mats, u0s = [], []
+        for spec in table:
+            A, u0 = c4_instance(nh, spec, "fp32")
+            mats.append(A)
+            u0s.append(u0)
+        B = nh.BatchFsp(mats)
+        t1 = time.perf_counter()
+        B.solve(u0s, ts, fetch=False, graphs=graphs)
+        t2 = time.perf_counter()
+        snaps = [[B.snapshot(i, k) for k in range(len(ts))] for i in range(len(table))]
+        t3 = time.perf_counter()
+        return snaps, dict(B.stats), (t3 - t0, t1 - t0, t2 - t1, t3 - t2)
+
+    batched()  # warm-up (code paths, caches)
+    snaps, bstats, (b_total, b_setup, b_solve, b_fetch) = batched()
+    say(f"c4: batched {b_total:.2f}s (setup {b_setup:.2f}, solve {b_solve:.2f}, download {b_fetch:.2f})")
+
+    def plain(spec):
+        A64, u0 = c4_instance(nh, spec, "fp64")
+        o = L.nc_solve_opts()
+        L.call("nc_solve_opts_default", C.byref(o))
+        o.method, o.krylov_tol = L.NC_METHOD_KRYLOV, krylov_tol
+        o.no_graphs = 0 if graphs else 1
+        out = np.empty(len(ts) * len(u0))
+        L.call("nc_solve_fixed", A64.handle, C.byref(o), L.ptr(u0, C.c_double), len(ts), L.ptr(ts, C.c_double),
+               L.ptr(out, C.c_double), None)
+        return out.reshape(len(ts), len(u0))
+
+    plain(table[0])  # warm-up
     l1 = 0.0
-    for i in sample:
-        A64, u0 = c4_instance(nh, table[i], "fp64")
-        for method in ("rk45", "krylov"):
-            o = L.nc_solve_opts()
-            L.call("nc_solve_opts_default", C.byref(o))
-            o.method = L.NC_METHOD_RK45 if method == "rk45" else L.NC_METHOD_KRYLOV
-            o.rtol, o.atol = 1e-8, 1e-14
-            o.no_graphs = 0 if graphs else 1
-            out = np.empty(len(ts) * len(u0))
-            t1 = time.perf_counter()
-            L.call("nc_solve_fixed", A64.handle, C.byref(o), L.ptr(u0, C.c_double), len(ts), L.ptr(ts, C.c_double),
-                   L.ptr(out, C.c_double), None)
-            plain[method] += time.perf_counter() - t1
-            out = out.reshape(len(ts), len(u0))
-            for k in range(len(ts)):
-                l1 = max(l1, float(np.abs(B.snapshot(int(i), k) - out[k]).sum()))
-    best = min(plain, key=plain.get)
-    plain_est = plain[best] * len(table) / n_plain
+    t0 = time.perf_counter()
+    plain_out = []
+    for spec in table:
+        plain_out.append(plain(spec))
+    p_total = time.perf_counter() - t0
+    for i in range(len(table)):
+        for k in range(len(ts)):
+            l1 = max(l1, float(np.abs(snaps[i][k] - plain_out[i][k]).sum()))
+    say(f"c4: plain fp64 {p_total:.2f}s")
     sizes = [t["states"] for t in table]
     return {
         "workload": f"512 instances (256 telegraph + 256 toggle, seeded table {C4_TABLE.name}), "
-                    f"{min(sizes)}..{max(sizes)} states, {sum(sizes)} in total, t = 0:1:10, DP5(4) rtol 1e-8",
-        "batched_fp32_solve_s": batch_s,
-        "batched_solves_per_s": len(table) / batch_s,
-        "batched_iterations": stats["host_syncs"],
-        "batched_rows_per_thread": rows_per_thread,
-        "autotune_iteration_ms": tune_ms,
-        "plain_fp64_solves_per_s": len(table) / plain_est,
-        "plain_fp64_rk45_solves_per_s": len(table) * n_plain / (plain["rk45"] * len(table)),
-        "plain_fp64_krylov_solves_per_s": len(table) * n_plain / (plain["krylov"] * len(table)),
-        "plain_method": best,
-        "plain_sample": f"{n_plain} instances evenly spaced by size, timed one by one, scaled to 512",
-        "speedup_batched_vs_plain": plain_est / batch_s,
+                    f"{min(sizes)}..{max(sizes)} states, {sum(sizes)} in total, t = 0:1:10; batched DP5(4) rtol 1e-8 "
+                    f"in FP32 storage vs plain FP64 Krylov expv (m=30, tol {krylov_tol:g}) one instance at a time",
+        "clock": "upload of every state space and generator to the download of every snapshot, both legs",
+        "batched_solves_per_s": len(table) / b_total,
+        "batched_total_s": b_total,
+        "batched_setup_s": b_setup,
+        "batched_solve_s": b_solve,
+        "batched_download_s": b_fetch,
+        "batched_resident_solves_per_s": len(table) / b_solve,
+        "batched_iterations": bstats["matvecs"] // 6,
+        "batched_host_syncs": bstats["host_syncs"],
+        "batched_steps_total": bstats["steps"],
+        "batched_rejected_total": bstats["rejected"],
+        "plain_fp64_solves_per_s": len(table) / p_total,
+        "plain_fp64_total_s": p_total,
+        "plain_method": "krylov",
+        "plain_sample": "all 512 instances, one after another",
+        "speedup_batched_vs_plain_fp64": p_total / b_total,
         "max_l1_batched_fp32_vs_plain_fp64": l1,
     }
 
@@ -304,12 +397,16 @@ def _l1_between(a, b):
     return float(sum(abs(da.get(x, 0.0) - db.get(x, 0.0)) for x in set(da) | set(db)))
 
 
-def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=(2, 6, 12), learned_levels=8, log=None):
+C5_PLAIN_LEVELS = (2, 6, 12, 24, 48)
+
+
+def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=C5_PLAIN_LEVELS, learned_levels=8, log=None):
     """Config C5: the cascade from (0,0,0) with fsptol 1e-4 over [0, T], adaptive FSP with
     the learned adapter against plain SelectiveRStepAdapter(2, R), each at its best
     expansion depth (the learned adapter expands learned_levels levels from the states it
     keeps; the plain FP64 baseline is the fastest of plain_levels). The network is trained
-    on the device on records from plain runs over n_scalings seeded parameter scalings."""
+    on the device on records from plain runs over n_scalings seeded parameter scalings. The
+    plain depths are swept until the fastest lies inside the range (bracketed)."""
     def solve(model, adapter, ode, tend, records=None):
         alg = nh.AdaptiveFspSparse(ode_method=ode, space_adapter=adapter)
         t0 = time.perf_counter()
@@ -350,8 +447,10 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=(2, 6, 12), lea
         best = min(plain_runs, key=lambda m: plain_runs[m][1])
         out_p, t_p = plain_runs[best]
         l1_vs_plain = max(_l1_between(a, b) for a, b in zip(out_l64.p, out_p.p))
+        keys = list(plain_runs)
+        bracketed = 0 < keys.index(best) < len(keys) - 1
     else:
-        best, out_p, t_p, l1_vs_plain = None, None, float("nan"), float("nan")
+        best, out_p, t_p, l1_vs_plain, bracketed = None, None, float("nan"), float("nan"), False
     fwd_ms = net.bench(n=1 << 21, reps=20, train=False)
     trn_ms = net.bench(n=1 << 21, reps=10, train=True)
     fwd_tf = MLP_FLOP_FORWARD * (1 << 21) / (fwd_ms * 1e-3) / 1e12
@@ -373,6 +472,8 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=(2, 6, 12), lea
         "plain_fp64_best": best,
         "plain_fp64_times_s": {m: v[1] for m, v in plain_runs.items()},
         "speedup_learned_vs_plain_fp64": t_p / t_l,
+        "speedup_learned_fp64_vs_plain_fp64": t_p / t_l64,
+        "plain_fp64_minimum_bracketed": bracketed,
         "speedup_learned_vs_plain_fp64_R2": (plain_runs["rk45 R=2"][1] / t_l) if "rk45 R=2" in plain_runs else None,
         "learned_max_states": st_l["max_states"],
         "plain_max_states": st_p["max_states"],
@@ -414,17 +515,31 @@ def spmv_probe(nh, precision, reps):
           flush=True)
 
 
+def c2_end_to_end(nh, precision, krylov_tol, ts):
+    """One complete C2 solve from model upload to snapshot download: trace and upload the
+    model, build the state space, generator and solver, integrate, download every snapshot."""
+    model = toggle_model(nh)
+    states = box_states(1024)
+    p0 = np.zeros(len(states))
+    p0[0] = 1.0
+    out = nh.solve(model, nh.FspVectorSparse(states, p0, copy=False), (0.0, float(ts[-1])),
+                   nh.KrylovExpv(m=30, tol=krylov_tol, precision=precision), saveat=ts)
+    snaps = [np.concatenate([p.values, s]) for p, s in zip(out.p, out.sinks)]
+    return snaps, out.stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--krylov-tol", type=float, default=1e-12,
-                    help="expv local error tolerance per unit time of the timed solves (the FP64 "
-                         "accuracy reference always uses 1e-12)")
-    ap.add_argument("--no-graphs", action="store_true", help="launch Arnoldi kernels eagerly (for profilers)")
+    ap.add_argument("--krylov-tol", type=float, default=1e-9,
+                    help="expv local error tolerance per unit time of the timed solves: the loosest setting "
+                         "whose FP64 solve stays within 1e-8 (l1) of the exact one (the FP64 accuracy reference "
+                         "always uses 1e-12)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch kernels eagerly instead of replaying graphs")
     ap.add_argument("--no-c3", action="store_true", help="skip the forward-sensitivity (C3) measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched (C4) measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the learned-adapter (C5) measurement")
@@ -435,10 +550,11 @@ def main():
                          "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs one FSP solve per GPU on a single device; multi-GPU runs "
-                                   "are not supported (the state-index partition that would shard one solve "
-                                   "is documented with the library)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; runs on more than one GPU are not supported (how "
+                                   "a solve or a batch would shard is documented with the library)",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
+    say = lambda m: print(m, file=sys.stderr, flush=True)
 
     import numcme_hip as nh
     from numcme_hip import _lib as L
@@ -447,65 +563,54 @@ def main():
         spmv_probe(nh, args.precision, args.spmv_probe)
         return
 
-    t_build = time.perf_counter()
+    ts = np.arange(0.0, 11.0)
+    # FP64 reference at tolerance 1e-12 (accuracy check of the timed solves)
+    ref, _ = c2_end_to_end(nh, "fp64", 1e-12, ts)
+
+    # timed region: K complete solves, each from model upload to snapshot download
+    for _ in range(max(args.warmup, 0)):
+        c2_end_to_end(nh, args.precision, args.krylov_tol, ts)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        snaps, stats = c2_end_to_end(nh, args.precision, args.krylov_tol, ts)
+    elapsed = time.perf_counter() - t0
+    l1 = max(float(np.abs(snaps[k] - ref[k]).sum()) for k in range(len(ts)))
+    say(f"c2: {args.steps / elapsed:.3f} solves/s end to end, l1 vs fp64 {l1:.2e}")
+    # the plain FP64 path on the same clock and tolerance
+    t0 = time.perf_counter()
+    snaps64, stats64 = c2_end_to_end(nh, "fp64", args.krylov_tol, ts)
+    fp64_s = time.perf_counter() - t0
+    l1_64 = max(float(np.abs(snaps64[k] - ref[k]).sum()) for k in range(len(ts)))
+
+    # the same solves with inputs resident on the device (restart from the uploaded state)
     model = toggle_model(nh)
-    states = box_states(1024)
-    space = nh.StateSpaceSparse(model.stoich_matrix, states)
+    space = nh.StateSpaceSparse(model.stoich_matrix, box_states(1024))
+    t_build = time.perf_counter()
     A = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision=args.precision, model=model)
-    A64 = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision="fp64", model=model)
     build_s = time.perf_counter() - t_build
-    rows_per_thread, _ = A.autotune(reps=30)
-    A64.autotune(reps=30)
     info = A.info()
     N = info["n_rows"]
     u0 = np.zeros(N)
     u0[0] = 1.0
-    ts = np.arange(0.0, 11.0)
-
-    # FP64 reference solve on the GPU at tolerance 1e-12 (accuracy check of the timed
-    # solves); the plain FP64 path is timed at the timed solves' tolerance
-    s64 = Solver(L, A64, "fp64", False, graphs=not args.no_graphs)
-    s64.set_state(u0)
-    s64.solve(ts)
-    ref = [s64.snapshot(k, N) for k in range(len(ts))]
-    fp64_steps = s64.stats()["steps"]
-    if args.krylov_tol != 1e-12:
-        s64.free()
-        s64 = Solver(L, A64, "fp64", False, graphs=not args.no_graphs, tol=args.krylov_tol)
-        s64.set_state(u0)
-        s64.solve(ts)
-        fp64_steps = s64.stats()["steps"]
-    t0 = time.perf_counter()
-    s64.solve(ts)
-    fp64_solve_s = time.perf_counter() - t0
-    s64.free()
-
-    # timed region: K complete solves, Arnoldi processes replayed as HIP graphs
     sol = Solver(L, A, args.precision, False, graphs=not args.no_graphs, tol=args.krylov_tol)
     sol.set_state(u0)
-    for _ in range(max(args.warmup, 1)):
-        sol.solve(ts)
-    l1 = max(float(np.abs(sol.snapshot(k, N) - ref[k]).sum()) for k in range(len(ts)))
+    sol.solve(ts)
     t0 = time.perf_counter()
-    steps_stats = []
     for _ in range(args.steps):
         sol.solve(ts)
-        steps_stats.append(sol.stats())
-    elapsed = time.perf_counter() - t0
+    resident_s = (time.perf_counter() - t0) / args.steps
     sol.free()
 
     # mat-vec roofline: one more solve with HIP events recorded around every mat-vec
     # launch on the solver's stream (graphs off, since events cannot sit inside them)
     tim = Solver(L, A, args.precision, True, tol=args.krylov_tol)
     tim.set_state(u0)
-    before = tim.stats()
     tim.solve(ts)
-    after = tim.stats()
-    mv_ms = after["matvec_ms"] - before["matvec_ms"]
-    mv_n = after["matvecs_timed"] - before["matvecs_timed"]
+    st = tim.stats()
+    mv_ms, mv_n = st["matvec_ms"], st["matvecs_timed"]
     tim.free()
     back_to_back_ms = A.bench_matvec(reps=200)
-
+    rows_tuned, _ = A.autotune(reps=30)  # reported only: the timed solves use the default shape
     avg_mv_s = (mv_ms / mv_n) * 1e-3 if mv_n else float("nan")
     achieved = info["csr_bytes_per_matvec"] / avg_mv_s / 1e9
     line = {
@@ -525,22 +630,26 @@ def main():
             "workload": "toggle switch, fixed FSP space 0..1023^2 (1,048,576 states + 4 sinks, 5,240,832 nnz), "
                         "t=0..10 at 11 outputs, device Krylov expv m=30",
             "precision": args.precision,
-            "form": "plain single solve per launch sequence (FP32 storage); value = its solves/s, "
-                    "fp64_solves_per_s = the plain FP64 path",
+            "clock": "model upload (trace, state space, generator, solver) to the download of all 11 snapshots",
+            "value_is": "plain single-solve FP32-storage C2 solves/s at l1 <= 1e-6 (l1_vs_fp64_gpu); the batched "
+                        "and learned forms are metric_components",
             "krylov_tol": args.krylov_tol,
-            "krylov_steps_per_solve": steps_stats[-1]["steps"],
-            "matvecs_per_solve": steps_stats[-1]["matvecs"],
+            "krylov_steps_per_solve": stats["steps"],
+            "matvecs_per_solve": stats["matvecs"],
         },
         "l1_vs_fp64_gpu": l1,
-        "fp64_solves_per_s": 1.0 / fp64_solve_s,
-        "setup_s": build_s,
+        "fp64_solves_per_s": 1.0 / fp64_s,
+        "fp64_l1_vs_tol_1e-12": l1_64,
+        "fp64_krylov_steps_per_solve": stats64["steps"],
+        "resident_solves_per_s": 1.0 / resident_s,
+        "matrix_build_s": build_s,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": measured_traffic(rows_per_thread, args.precision),
+            "traffic": measured_traffic(info["rows_per_thread"], args.precision),
             "traffic_source": TRAFFIC_PROFILE.name,
             "kernel": "k_matvec (FSP SpMV)",
             "bytes_per_launch_csr": info["csr_bytes_per_matvec"],
@@ -548,20 +657,30 @@ def main():
             "avg_launch_us": avg_mv_s * 1e6,
             "launches_timed": mv_n,
             "back_to_back_launch_us": back_to_back_ms * 1e3,
-            "rows_per_thread": rows_per_thread,
+            "rows_per_thread": info["rows_per_thread"],
+            "autotuned_rows_per_thread": rows_tuned,
             "computed_col_slots": info["computed_col_slots"],
         },
         "cpu_baseline": None,
     }
+    comps = {}
     if not args.no_c3:
         line["c3_sensitivity"] = c3_sensitivity(nh, L)
     if not args.no_c4:
-        line["c4_batch"] = c4_batch(nh, L, graphs=not args.no_graphs)
+        c4 = c4_batch(nh, L, args.krylov_tol, graphs=not args.no_graphs, log=say)
+        line["c4_batch"] = c4
+        comps["batched_vs_plain_fp64"] = c4["speedup_batched_vs_plain_fp64"]
+        comps["batched_solves_per_s"] = c4["batched_solves_per_s"]
     if not args.no_c5:
-        plain = (2, 6, 12) if args.c5_plain == "all" else ()
-        line["c5_learned"] = c5_learned(nh, plain_levels=plain, log=lambda m: print(m, file=sys.stderr, flush=True))
+        plain = C5_PLAIN_LEVELS if args.c5_plain == "all" else ()
+        c5 = c5_learned(nh, plain_levels=plain, log=say)
+        line["c5_learned"] = c5
+        comps["learned_vs_plain_fp64"] = c5["speedup_learned_vs_plain_fp64"]
+        comps["learned_fp64_vs_plain_fp64"] = c5["speedup_learned_fp64_vs_plain_fp64"]
+    comps["spmv_hbm_GBps"] = achieved
+    line["metric_components"] = comps
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(model.parameters, fp64_steps)
+        line["cpu_baseline"] = cpu_baseline(model.parameters, args.krylov_tol, stats64["steps"], log=say)
     print(json.dumps(line), flush=True)
 
 

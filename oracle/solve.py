@@ -100,3 +100,51 @@ def _endpoint(A, u, a, b, rtol, atol):
 def expm_dense(M):
     """Dense matrix exponential (scipy, Pade 13 scaling-and-squaring; Al-Mohy & Higham 2009)."""
     return sla.expm(np.asarray(M, dtype=np.float64))
+
+
+def uniformization(A, u0, ts, tail=1e-14):
+    """Exact transient solution of du/dt = A u for a generator A (non-negative off-diagonal,
+    non-positive diagonal, columns summing to <= 0) at the ascending times ``ts`` (from
+    ``ts[0]``), by uniformization (Jensen 1953; Grassmann 1977): with L >= max |A_jj| and
+    P = I + A / L (a sub-stochastic matrix), u(t) = sum_k e^{-L t} (L t)^k / k! P^k u0.
+    Every term is non-negative for a non-negative u0, so there is no cancellation; the
+    series is cut where the Poisson weights left out sum to less than ``tail``. Intervals
+    between output times are advanced one after another.
+    """
+    import math
+
+    import scipy.sparse as sps
+    from scipy.stats import poisson
+
+    A = sps.csr_matrix(A)
+    L = float(np.abs(A.diagonal()).max()) * 1.0000001 or 1.0
+    P = sps.identity(A.shape[0], format="csr") + A / L
+    P = sps.csr_matrix(P)
+    u = np.asarray(u0, dtype=np.float64).copy()
+    out = [u.copy()]
+    for a, b in zip(ts[:-1], ts[1:]):
+        lt = L * (b - a)
+        if lt == 0.0:
+            out.append(u.copy())
+            continue
+        kmax = int(poisson.isf(tail, lt)) + 2
+        # Poisson weights by the recurrence w_k = w_{k-1} lt / k, scaled from the mode down
+        # and up so that no weight underflows before it is used
+        mode = int(lt)
+        logw_mode = -lt + mode * math.log(lt) - math.lgamma(mode + 1)
+        w = np.empty(kmax + 1)
+        w[mode] = 1.0 if mode <= kmax else 0.0
+        for k in range(mode + 1, kmax + 1):
+            w[k] = w[k - 1] * lt / k
+        for k in range(min(mode, kmax + 1) - 1, -1, -1):
+            w[k] = w[k + 1] * (k + 1) / lt
+        w *= math.exp(logw_mode)
+        acc = w[0] * u
+        term = u
+        for k in range(1, kmax + 1):
+            term = P @ term
+            if w[k] != 0.0:
+                acc += w[k] * term
+        u = acc
+        out.append(u.copy())
+    return out

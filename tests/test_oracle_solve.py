@@ -92,3 +92,51 @@ def test_time_varying_mass_conservation():
     out = solve_fixed(S, models.to_oracle(props), th, states, p0, ts, breakpoints=kinks)
     for k in range(len(ts)):
         assert abs(out.p[k].sum() + out.sinks[k].sum() - 1) < 1e-10
+
+
+def _toggle_generator(bounds, th=None):
+    from oracle.fspmatrix import FspMatrix
+    from oracle.statespace import StateSpace
+    S, props, th0 = models.toggle() if th is None else models.toggle(th)
+    A = FspMatrix(StateSpace(S, models.box_states(bounds)), models.to_oracle(props), th0).to_csc(0.0)
+    u0 = np.zeros(A.shape[0])
+    u0[0] = 1.0
+    return A, u0
+
+
+def test_uniformization_vs_dense_expm_and_expm_multiply():
+    # the exact method of the large batched parity cases, pinned by two independent exact ones
+    from oracle import cpu_ref
+    from oracle.solve import uniformization
+    import scipy.sparse.linalg as spla
+
+    ts = np.arange(0.0, 6.0)
+    A, u0 = _toggle_generator([24, 24])
+    U = uniformization(A, u0, ts)
+    Uc = cpu_ref.uniformization(A, u0, ts)
+    for k, t in enumerate(ts):
+        ref = sla.expm(A.toarray() * t) @ u0
+        assert np.abs(U[k] - ref).sum() < 1e-12
+        assert np.abs(Uc[k] - U[k]).sum() < 1e-14
+    # a slow, widely spread instance (gamma = 0.05) against expm_multiply
+    A, u0 = _toggle_generator([119, 119], (30.0, 10.0, 2.5, 1.0, 0.05))
+    Uc = cpu_ref.uniformization(A, u0, ts)
+    u = u0
+    for k in range(1, len(ts)):
+        u = spla.expm_multiply(A * (ts[k] - ts[k - 1]), u)
+        assert np.abs(Uc[k] - u).sum() < 1e-11
+        assert abs(Uc[k].sum() - 1.0) < 1e-12  # the sinks hold the rest (none reached here)
+
+
+def test_c_krylov_restatement_matches_exact():
+    # the timed CPU restatement (one-thread CSC scatter, all-thread CSR) computes the solution
+    from oracle import cpu_ref
+
+    ts = np.array([0.0, 0.5, 2.0])
+    A, u0 = _toggle_generator([40, 40])
+    ref = cpu_ref.uniformization(A, u0, ts)
+    for layout, th in (("csc", 1), ("csr", cpu_ref.cpu_count())):
+        W, steps, _ = cpu_ref.Expv(A, m=30, tol=1e-12, layout=layout, threads=th).solve(u0, ts)
+        assert steps > 0
+        for k in range(len(ts)):
+            assert np.abs(W[k] - ref[k]).sum() < 1e-10
