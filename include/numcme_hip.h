@@ -175,6 +175,13 @@ int nc_matrix_info(nc_matrix A, nc_matrix_info_t* info);
 /* matvec!(out, t, A, v) (accumulate == 0) and matvecadd!(out, t, A, v) (accumulate != 0)
  * — fspsparsematrix.jl:196-247. v, out: host arrays of n_rows doubles. */
 int nc_matrix_matvec(nc_matrix A, double t, const double* v, double* out, int32_t accumulate);
+/* nc_matrix_matvec without any allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, code that forbids
+ * hipMalloc). The input, its FP32 copy and the output are staged in a fixed device array
+ * of 48 MiB per device, which limits the generator to about 2.5M rows (FP32 storage) or
+ * 3.1M rows (FP64); larger ones fail with NC_ERR_ARG. Every call on a device shares that
+ * array, so calls that overlap on one device race; results equal nc_matrix_matvec's. */
+int nc_matrix_matvec_noalloc(nc_matrix A, double t, const double* v, double* out, int32_t accumulate);
 /* The generator A(t) as COO triplets (row, col, value) in the stored precision, for
  * inspection. Call with rows == NULL to get *nnz only. */
 int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t* cols, double* vals);

@@ -73,6 +73,8 @@ struct nc_matrix_s {
 };
 
 namespace nc {
+// Bytes of the fixed device scratch behind nc_matrix_matvec_noalloc (per device).
+constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;
 // FspMatrixSparse{RealT}(space, propensities; parameters) for the space's current states
 nc_matrix_s* matrix_create(nc_space_s* sp, nc_model_s* m, int precision);
 // Bring time-dependent values up to date for time t (joint refresh, on the matrix stream).

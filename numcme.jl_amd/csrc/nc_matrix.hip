@@ -16,6 +16,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <mutex>
+
 #include "nc_matrix.h"
 
 nc_matrix_s::~nc_matrix_s() {
@@ -940,6 +942,38 @@ __global__ void k_to_vt(const double* in, VT* out, int64_t n) {
   if (i < n) out[i] = (VT)in[i];
 }
 
+}  // namespace
+
+// ---- fixed device scratch of the allocation-free mat-vec -------------------------------
+// nc_matrix_matvec_noalloc stages its vectors here instead of allocating: the input
+// (FP64), its FP32 copy for an FP32 matrix, and the FP64 output, each region aligned to
+// 256 bytes. Every device the library runs on holds its own copy of the array until the
+// process ends. In host code the array's name is not a device address: the address comes
+// from hipGetSymbolAddress, taken once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per device
+
+unsigned char* scratch_on_current_device() {
+  int dev = 0;
+  NC_CHECK_HIP(hipGetDevice(&dev));
+  NC_REQUIRE(dev >= 0 && dev < kMaxDevices, NC_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    NC_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    NC_REQUIRE(p, NC_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
 double bench_matvec(nc_matrix_s* A, double t, int reps) {
   hipStream_t s = A->stream.s;
   const int64_t N = A->n_rows;
@@ -1154,6 +1188,37 @@ int nc_matrix_matvec(nc_matrix A, double t, const double* v, double* out, int32_
     }
     NC_CHECK_HIP(hipGetLastError());
     NC_CHECK_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    A->ev_used = 0;
+  });
+}
+
+int nc_matrix_matvec_noalloc(nc_matrix A, double t, const double* v, double* out, int32_t accumulate) {
+  return guard([&] {
+    NC_REQUIRE(A && v && out, NC_ERR_ARG, "null argument");
+    const int64_t N = A->n_rows;
+    const bool fp32 = A->precision == NC_FP32;
+    const size_t b64 = align256(sizeof(double) * (size_t)N), b32 = align256(sizeof(float) * (size_t)N);
+    const size_t need = 2 * b64 + (fp32 ? b32 : 0);
+    NC_REQUIRE(need <= kNoAllocScratchBytes, NC_ERR_ARG,
+               "nc_matrix_matvec_noalloc: " + std::to_string(N) + " rows need " + std::to_string(need) +
+                   " bytes of scratch, more than its fixed " + std::to_string(kNoAllocScratchBytes) +
+                   " (about 2.5M rows at FP32 storage, 3.1M at FP64); use nc_matrix_matvec");
+    unsigned char* base = scratch_on_current_device();
+    double* dv = reinterpret_cast<double*>(base);
+    double* dout = reinterpret_cast<double*>(base + b64);
+    hipStream_t s = A->stream.s;
+    NC_CHECK_HIP(hipMemcpyAsync(dv, v, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    if (accumulate) NC_CHECK_HIP(hipMemcpyAsync(dout, out, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    if (fp32) {
+      float* vf = reinterpret_cast<float*>(base + 2 * b64);
+      k_to_vt<float><<<grid_for(N, 256), 256, 0, s>>>(dv, vf, N);
+      matvec_launch(A, t, vf, dout, accumulate != 0, s, true);
+    } else {
+      matvec_launch(A, t, dv, dout, accumulate != 0, s, true);
+    }
+    NC_CHECK_HIP(hipGetLastError());
+    NC_CHECK_HIP(hipMemcpyAsync(out, dout, sizeof(double) * N, hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
     A->ev_used = 0;
   });

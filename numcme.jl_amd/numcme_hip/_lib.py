@@ -158,6 +158,7 @@ SIGNATURES = {
     "nc_matrix_destroy": [P],
     "nc_matrix_info": [P, C.POINTER(nc_matrix_info_t)],
     "nc_matrix_matvec": [P, D, PD, PD, I32],
+    "nc_matrix_matvec_noalloc": [P, D, PD, PD, I32],
     "nc_matrix_export": [P, D, PI64, PI64, PI64, PD],
     "nc_matrix_bench_matvec": [P, D, I32, PD],
     "nc_matrix_set_rows_per_thread": [P, I32],
