@@ -345,8 +345,10 @@ def c4_batch(nh, L, krylov_tol, graphs=True, log=None):
     l1 = 0.0
     t0 = time.perf_counter()
     plain_out = []
-    for spec in table:
+    for i, spec in enumerate(table):
         plain_out.append(plain(spec))
+        if i % 128 == 127:
+            say(f"c4: plain fp64 {i + 1} instances, {time.perf_counter() - t0:.1f}s")
     p_total = time.perf_counter() - t0
     for i in range(len(table)):
         for k in range(len(ts)):

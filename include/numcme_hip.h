@@ -185,8 +185,9 @@ int nc_matrix_matvec_noalloc(nc_matrix A, double t, const double* v, double* out
 /* The generator A(t) as COO triplets (row, col, value) in the stored precision, for
  * inspection. Call with rows == NULL to get *nnz only. */
 int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t* cols, double* vals);
-/* Repeat the mat-vec kernel `reps` times on device-resident vectors and report the mean
- * kernel duration from HIP events on the matrix's stream. */
+/* Repeat the mat-vec kernel `reps` times on device-resident vectors (the integrators' form:
+ * stored-precision input, FP64 output) and report the mean kernel duration from HIP events
+ * on the matrix's stream. No reference counterpart. */
 int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms);
 /* Mat-vec launch shape: consecutive state rows per thread (1, 2 or 4). No reference
  * counterpart (the reference's CSC mul! has no launch shape). */

@@ -75,6 +75,19 @@ constexpr size_t kHostStageSlot = 4096;
 void* host_stage(int slot);
 hipStream_t stream_acquire();        // non-blocking stream, default priority
 void stream_release(hipStream_t s);  // synchronises it, then keeps it for reuse
+// What must be complete before a device block goes back to the cache: by default every
+// stream's work (the whole device). Inside a PrivateStreams scope, only the work the calling
+// thread has queued on the streams it holds.
+void release_sync();
+// For worker threads that create, use and free every object of their own (the threads of a
+// batch of independent solves): their buffer releases then wait for their own streams only,
+// so that the threads do not wait for each other whenever one of them frees a buffer.
+struct PrivateStreams {
+  PrivateStreams();
+  ~PrivateStreams();
+  PrivateStreams(const PrivateStreams&) = delete;
+  PrivateStreams& operator=(const PrivateStreams&) = delete;
+};
 
 // ---- device buffers -------------------------------------------------------------------
 template <class T>
@@ -111,8 +124,8 @@ struct DevBuf {
   }
   void release() {
     if (p) {
-      // work queued on any stream may still read the buffer
-      (void)hipDeviceSynchronize();
+      // queued work may still read the buffer
+      release_sync();
       pool_free(p);
     }
     p = nullptr;

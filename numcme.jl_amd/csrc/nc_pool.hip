@@ -9,6 +9,7 @@
 //
 // The caches live until the process ends and are never torn down (the HIP runtime may
 // already be gone when static destructors run).
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -152,33 +153,57 @@ void* host_stage(int slot) {
   return base + (size_t)slot * kHostStageSlot;
 }
 
+// The streams this host thread holds (taken here and not yet released), and whether its
+// buffer releases wait for these only (inside a PrivateStreams scope).
+thread_local std::vector<hipStream_t> tl_streams;
+thread_local int tl_private = 0;
+
 hipStream_t stream_acquire() {
   Cache& c = cache();
   const int dev = current_device();
+  hipStream_t s = nullptr;
   {
     std::lock_guard<std::mutex> lk(c.mu);
     auto& v = c.streams[dev];
     if (!v.empty()) {
-      hipStream_t s = v.back();
+      s = v.back();
       v.pop_back();
-      return s;
     }
   }
-  hipStream_t s = nullptr;
-  NC_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  std::lock_guard<std::mutex> lk(c.mu);
-  c.stream_dev[s] = dev;
+  if (!s) {
+    NC_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.stream_dev[s] = dev;
+  }
+  if (std::find(tl_streams.begin(), tl_streams.end(), s) == tl_streams.end()) tl_streams.push_back(s);
   return s;
 }
 
 void stream_release(hipStream_t s) {
   if (!s) return;
   (void)hipStreamSynchronize(s);
+  for (size_t k = 0; k < tl_streams.size(); ++k)
+    if (tl_streams[k] == s) {
+      tl_streams[k] = tl_streams.back();
+      tl_streams.pop_back();
+      break;
+    }
   Cache& c = cache();
   std::lock_guard<std::mutex> lk(c.mu);
   auto it = c.stream_dev.find(s);
   if (it == c.stream_dev.end()) return;
   c.streams[it->second].push_back(s);
 }
+
+void release_sync() {
+  if (!tl_private) {
+    (void)hipDeviceSynchronize();
+    return;
+  }
+  for (hipStream_t s : tl_streams) (void)hipStreamSynchronize(s);
+}
+
+PrivateStreams::PrivateStreams() { ++tl_private; }
+PrivateStreams::~PrivateStreams() { --tl_private; }
 
 }  // namespace nc

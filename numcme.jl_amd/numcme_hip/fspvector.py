@@ -63,29 +63,28 @@ class FspVectorSparse:
                     vals[i - 1] = v
         return cls(states, vals, copy=False)
 
-    # ---- host view (downloaded on first access) ------------------------------------------
-    def _download(self):
-        st = np.empty((self._n, self._ns), dtype=np.int32)
-        vals = np.empty(self._n, dtype=np.float64)
-        L.call("nc_fspvec_get", self._dev.handle, L.ptr(st, C.c_int32) if st.size else None,
-               L.ptr(vals, C.c_double) if vals.size else None)
-        self._states, self._values = st.astype(np.int64), vals
-
+    # ---- host view (each part downloaded on its first access) ------------------------------
     @property
     def states(self):
         if self._states is None:
-            self._download()
+            st = np.empty((self._n, self._ns), dtype=np.int32)
+            if st.size:
+                L.call("nc_fspvec_get", self._dev.handle, L.ptr(st, C.c_int32), None)
+            self._states = st.astype(np.int64)
         return self._states
 
     @property
     def values(self):
         if self._values is None:
-            self._download()
+            vals = np.empty(self._n, dtype=np.float64)
+            if vals.size:
+                L.call("nc_fspvec_get", self._dev.handle, None, L.ptr(vals, C.c_double))
+            self._values = vals
         return self._values
 
     def on_device(self):
-        """True while the vector has not been downloaded to the host."""
-        return self._states is None
+        """True while neither the states nor the values have been downloaded to the host."""
+        return self._states is None and self._values is None
 
     def _device(self):
         if self._dev is None:

@@ -17,15 +17,21 @@ Modules and the reference code each one restates:
   ``_update_sparsematrix!``, ``matvec!``/``matvecadd!``).
 * ``sensmatrix``  — ``src/forwardsensfspmatrix/forwardsensfspmatrixsparse/
   sensfspmatrixsparse.jl:31-142``.
-* ``adapters``    — ``src/transientcme/sparse/rstepadapters.jl:23-110`` and
-  ``src/forwardsenscme/sparse/fsspaceadapterssparse.jl:21-60``.
-* ``solve``       — ``src/transientcme/sparse/fspsolve.jl:10-41,105-197`` with
-  an exact integrator (``scipy.sparse.linalg.expm_multiply`` /
-  tight-tolerance DOP853) in place of CVODE.
-* ``fspvector``   — ``src/fspvector/fspvector.jl:28-129``.
-* ``cpu_ref.c``   — C restatement of the reference CPU hot loop (CSC column
-  scatter SpMV as in SparseArrays ``mul!`` plus a Krylov ``expv`` integrator),
-  used only as the timed CPU baseline.
+* ``adaptive``    — ``src/transientcme/sparse/rstepadapters.jl:23-110``,
+  ``src/forwardsenscme/sparse/fsspaceadapterssparse.jl:21-60`` and the adaptive driver of
+  ``src/transientcme/sparse/fspsolve.jl:105-197``.
+* ``solve``       — ``src/transientcme/sparse/fspsolve.jl:10-41`` with an exact
+  integrator (``scipy.sparse.linalg.expm_multiply``, dense ``expm``, tight-tolerance DOP853,
+  or uniformization) in place of CVODE.
+* ``fspvector``   — ``src/fspvector/fspvector.jl:57-129`` (``sum(p)``, ``sum(p, dims)``,
+  ``Array(p)``).
+* ``learned``     — boundary choice, features and labels of the learned adapter (no
+  reference counterpart).
+* ``mlp``         — the adapter network in FP64, with BF16 rounding where the kernels round.
+* ``cpu_ref.c`` / ``cpu_ref.py`` — C restatement of the reference CPU hot loop (CSC column
+  scatter SpMV as in SparseArrays ``mul!``, and a row-parallel CSR variant on all cores)
+  with the Krylov ``expv`` integrator, the timed CPU baseline; and the uniformization
+  series used for large exact solutions in the tests.
 
 Parity pins for every function are listed in the module docstrings.
 """
