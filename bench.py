@@ -394,9 +394,25 @@ def cascade_model(nh, theta):
 
 
 def _l1_between(a, b):
-    da = {tuple(x): v for x, v in zip(a.states.tolist(), a.values)}
-    db = {tuple(x): v for x, v in zip(b.states.tolist(), b.values)}
-    return float(sum(abs(da.get(x, 0.0) - db.get(x, 0.0)) for x in set(da) | set(db)))
+    """l1 distance of two sparse vectors over the union of their states (each state list
+    holds every state once)."""
+    sa, sb = np.asarray(a.states, dtype=np.int64), np.asarray(b.states, dtype=np.int64)
+    ns = sa.shape[1]
+    m = int(max(sa.max(initial=0), sb.max(initial=0))) + 1
+    if sa.min(initial=0) < 0 or sb.min(initial=0) < 0 or m ** ns >= 2 ** 62:
+        da = {tuple(x): v for x, v in zip(sa.tolist(), a.values)}
+        db = {tuple(x): v for x, v in zip(sb.tolist(), b.values)}
+        return float(sum(abs(da.get(x, 0.0) - db.get(x, 0.0)) for x in set(da) | set(db)))
+
+    def key(s):
+        k = np.zeros(s.shape[0], dtype=np.int64)
+        for j in range(ns):
+            k = k * m + s[:, j]
+        return k
+
+    _, inv = np.unique(np.concatenate([key(sa), key(sb)]), return_inverse=True)
+    d = np.bincount(inv, weights=np.concatenate([a.values, -np.asarray(b.values)]))
+    return float(np.abs(d).sum())
 
 
 C5_PLAIN_LEVELS = (2, 6, 12, 24, 48)
@@ -453,6 +469,7 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=C5_PLAIN_LEVELS
         bracketed = 0 < keys.index(best) < len(keys) - 1
     else:
         best, out_p, t_p, l1_vs_plain, bracketed = None, None, float("nan"), float("nan"), False
+    batch = c5_batched(nh, net, learned_levels, best, T=T, log=say) if plain_runs else None
     fwd_ms = net.bench(n=1 << 21, reps=20, train=False)
     trn_ms = net.bench(n=1 << 21, reps=10, train=True)
     fwd_tf = MLP_FLOP_FORWARD * (1 << 21) / (fwd_ms * 1e-3) / 1e12
@@ -486,6 +503,7 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=C5_PLAIN_LEVELS
         "final_sink_mass_plain": float(out_p.sinks[-1].sum()) if out_p is not None else None,
         "max_l1_learned_fp32_vs_fp64": l1_prec,
         "max_l1_learned_vs_plain_fp64": l1_vs_plain,
+        "batched": batch,
         "mlp_roofline": {
             "bound": "mfma", "unit": "TFLOP/s", "peak": BF16_DENSE_PEAK_TFLOPS,
             "forward_achieved": fwd_tf, "forward_frac": fwd_tf / BF16_DENSE_PEAK_TFLOPS,
@@ -493,6 +511,93 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=C5_PLAIN_LEVELS
             "flop_per_sample_forward": MLP_FLOP_FORWARD, "samples_per_launch": 1 << 21,
             "forward_launch_ms": fwd_ms, "train_launch_ms": trn_ms,
         },
+    }
+
+
+C5_BATCH_INSTANCES = 16
+C5_BATCH_WORKERS = 4
+
+
+def c5_batched(nh, net, learned_levels, best, T=5.0, n_inst=C5_BATCH_INSTANCES, workers=C5_BATCH_WORKERS,
+               log=None):
+    """The batched + learned form on C5: a sweep of n_inst parameter points around C5's
+    (every rate scaled by its own factor from [0.8, 1.25], seed 1; the network was trained
+    on seed 0's scalings from [0.5, 2] over a shorter horizon), solved by
+    nc_solve_adaptive_batch with the learned adapter, against the plain FP64 single-solve
+    path (SelectiveRStepAdapter at the best depth of the bracketed sweep, one solve after
+    another). Both legs run on one clock: model upload to the download of every snapshot's
+    states and values. (Over t = 5 the full [0.5, 2] range reaches spaces of 10^8 states
+    at its corners: a faster production and slower decay of C grow it without bound.)"""
+    say = log or (lambda m: None)
+    rng = np.random.default_rng(1)
+    thetas = [np.asarray(C5_THETA) * np.exp(rng.uniform(np.log(0.8), np.log(1.25), size=6)) for _ in range(n_inst)]
+    rk = lambda prec: nh.RK45(rtol=1e-6, atol=1e-10, precision=prec)
+    p0 = nh.FspVectorSparse([[0, 0, 0]], [1.0])
+    saveat = np.linspace(0.0, T, 6)
+    plain_R = int(best.split("R=")[1])
+
+    def fetch(outs):
+        for o in outs:
+            for p in o.p:
+                p.states, p.values  # download
+        return outs
+
+    def batched(prec, nworkers):
+        alg = nh.AdaptiveFspSparse(ode_method=rk(prec),
+                                   space_adapter=nh.LearnedAdapter(net, 2, learned_levels, False, threshold=0.3))
+        t0 = time.perf_counter()
+        outs = fetch(nh.solve_adaptive_batch([cascade_model(nh, th) for th in thetas], p0, (0.0, T), alg,
+                                             saveat=saveat, fsptol=1e-4, workers=nworkers))
+        return outs, time.perf_counter() - t0
+
+    def singles(adapter, prec, tag):
+        alg = nh.AdaptiveFspSparse(ode_method=rk(prec), space_adapter=adapter)
+        t0 = time.perf_counter()
+        outs = []
+        for i, th in enumerate(thetas):
+            outs.append(fetch([nh.solve(cascade_model(nh, th), p0, (0.0, T), alg, saveat=saveat, fsptol=1e-4)])[0])
+            if i % 4 == 3:
+                say(f"c5 batch: {tag} {i + 1} instances, {time.perf_counter() - t0:.1f}s")
+        return outs, time.perf_counter() - t0
+
+    out_s, t_s = singles(nh.LearnedAdapter(net, 2, learned_levels, False, threshold=0.3), "fp32", "learned single")
+    say("c5 batch: learned states per instance " + " ".join(str(o.stats["max_states"]) for o in out_s))
+    batched("fp32", workers)  # warm-up of the worker threads' caches
+    out_b, t_b = batched("fp32", workers)
+    say(f"c5 batch: batched learned fp32 {t_b:.2f}s")
+    out_b64, t_b64 = batched("fp64", workers)
+    say(f"c5 batch: batched learned fp64 {t_b64:.2f}s")
+    out_p, t_p = singles(nh.SelectiveRStepAdapter(2, plain_R, False), "fp64", f"plain fp64 R={plain_R}")
+    same = all(np.array_equal(a.p[k].values, b.p[k].values) and np.array_equal(a.p[k].states, b.p[k].states)
+               for a, b in zip(out_b, out_s) for k in range(len(a.p)))
+    l1_prec = max(_l1_between(a.p[k], b.p[k]) for a, b in zip(out_b, out_b64) for k in range(len(a.p)))
+    l1_plain = max(_l1_between(a.p[k], b.p[k]) for a, b in zip(out_b64, out_p) for k in range(len(a.p)))
+    sink_ok = all(o.sinks[k].sum() <= 1e-4 * o.t[k] / T + 1e-13 for o in out_b + out_b64 + out_p
+                  for k in range(len(o.t)))
+    return {
+        "workload": f"{n_inst} parameter points around C5 (each rate x exp(U[log 0.8, log 1.25]), seed 1), "
+                    f"fsptol 1e-4, t in [0, {T:g}] with 6 outputs",
+        "clock": "model upload to the download of every snapshot, both legs",
+        "workers": workers,
+        "batched_learned_fp32_s": t_b,
+        "batched_learned_fp64_s": t_b64,
+        "single_learned_fp32_s": t_s,
+        "plain_fp64_single_s": t_p,
+        "plain_adapter": f"SelectiveRStepAdapter(2, {plain_R}, false), the best depth of the C5 sweep",
+        "batched_solves_per_s": n_inst / t_b,
+        "batched_fp64_solves_per_s": n_inst / t_b64,
+        "plain_fp64_solves_per_s": n_inst / t_p,
+        "speedup_batched_learned_vs_plain_fp64": t_p / t_b,
+        "speedup_batched_learned_fp64_vs_plain_fp64": t_p / t_b64,
+        "speedup_batching_alone": t_s / t_b,
+        "batched_equals_single_solves": bool(same),
+        "sink_bound_held_every_snapshot": bool(sink_ok),
+        "max_l1_batched_fp32_vs_fp64": l1_prec,
+        "max_l1_batched_fp64_vs_plain_fp64": l1_plain,
+        "max_states_per_instance": [int(o.stats["max_states"]) for o in out_b],
+        "plain_max_states_per_instance": [int(o.stats["max_states"]) for o in out_p],
+        "learned_adaptations": int(sum(o.stats["learned_adaptations"] for o in out_b)),
+        "fallback_adaptations": int(sum(o.stats["fallback_adaptations"] for o in out_b)),
     }
 
 
@@ -679,6 +784,10 @@ def main():
         line["c5_learned"] = c5
         comps["learned_vs_plain_fp64"] = c5["speedup_learned_vs_plain_fp64"]
         comps["learned_fp64_vs_plain_fp64"] = c5["speedup_learned_fp64_vs_plain_fp64"]
+        if c5["batched"]:
+            comps["batched_learned_vs_plain_fp64"] = c5["batched"]["speedup_batched_learned_vs_plain_fp64"]
+            comps["batched_learned_fp64_vs_plain_fp64"] = c5["batched"]["speedup_batched_learned_fp64_vs_plain_fp64"]
+            comps["batched_learned_solves_per_s"] = c5["batched"]["batched_solves_per_s"]
     comps["spmv_hbm_GBps"] = achieved
     line["metric_components"] = comps
     if not args.no_cpu:

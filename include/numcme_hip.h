@@ -373,6 +373,22 @@ int nc_solve_adaptive(nc_model model, int64_t n0, const int32_t* states0, const 
 int nc_solve_sens_adaptive(nc_model model, int64_t n0, const int32_t* states0, const double* p0,
                            const double* S0, double t0, double t1, int32_t n_save, const double* saveat,
                            const nc_adaptive_opts* opts, nc_output* out, nc_solve_stats* st);
+/* Many independent adaptive solves at once (a parameter sweep). No reference counterpart:
+ * the reference solves one CmeModel per solve call, so a sweep is a loop over
+ * fspsolve.jl:105-197; each instance here gives exactly what nc_solve_adaptive gives for
+ * it. Instance i solves models[i] from its initial distribution: n0[i] states, which follow
+ * those of instances 0..i-1 in states0 (n0[i] x ns_i row-major) and p0; every instance
+ * shares tspan, saveat and opts (opts->records must be null). n_workers host threads
+ * (<= 0: 4) each run whole solves on streams of their own, so that one solve's small
+ * launches, host decisions and synchronisations overlap the others' device work; they
+ * launch kernels directly (opts->ode.no_graphs is taken as 1), since a stream capture in
+ * one thread does not survive the allocations of the others. An instance that runs out of
+ * device memory next to the others is solved again, alone, after them. On success outs[i]
+ * (and stats[i], if stats is not null) belong to instance i; on an error no output is
+ * returned and the message names the first instance that failed. */
+int nc_solve_adaptive_batch(int32_t n_inst, const nc_model* models, const int64_t* n0, const int32_t* states0,
+                            const double* p0, double t0, double t1, int32_t n_save, const double* saveat,
+                            const nc_adaptive_opts* opts, int32_t n_workers, nc_output* outs, nc_solve_stats* stats);
 int nc_output_destroy(nc_output o);
 /* Number of snapshots, and of state spaces (adaptations + 1). */
 int nc_output_count(nc_output o, int32_t* n_snapshots, int32_t* n_spaces);

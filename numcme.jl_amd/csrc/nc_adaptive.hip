@@ -17,10 +17,15 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <system_error>
+#include <thread>
+#include <vector>
 
 #include "nc_fspvec.h"
 #include "nc_learned.h"
@@ -200,6 +205,10 @@ void check_adapter(const nc_adapter* ad) {
 
 // SelectiveRStep and the learned adapter built on it expand only along reactions with a
 // positive sink derivative and drop with ">".
+// Worker threads of nc_solve_adaptive_batch by default: one per hardware queue HIP gives a
+// process by default, so that the workers' launches do not queue behind each other.
+constexpr int kBatchWorkers = 4;
+
 bool selective_kind(int kind) { return kind != NC_ADAPTER_RSTEP; }
 
 std::vector<int32_t> all_reactions(int nr) {
@@ -627,6 +636,111 @@ int nc_solve_sens_adaptive(nc_model model, int64_t n0, const int32_t* states0, c
     auto O = std::make_unique<nc_output_s>();
     solve_adaptive_impl(model, model->np, n0, states0, p0, S0, t0, t1, n_save, saveat, o, O.get(), st);
     *out = O.release();
+  });
+}
+
+int nc_solve_adaptive_batch(int32_t n_inst, const nc_model* models, const int64_t* n0, const int32_t* states0,
+                            const double* p0, double t0, double t1, int32_t n_save, const double* saveat,
+                            const nc_adaptive_opts* opts, int32_t n_workers, nc_output* outs, nc_solve_stats* stats) {
+  return guard([&] {
+    NC_REQUIRE(n_inst >= 0, NC_ERR_ARG, "negative instance count");
+    if (n_inst == 0) return;
+    NC_REQUIRE(models && n0 && states0 && p0 && outs, NC_ERR_ARG, "null argument");
+    nc_adaptive_opts o{};
+    if (opts)
+      o = *opts;
+    else
+      nc_adaptive_opts_default(&o);
+    NC_REQUIRE(!o.records, NC_ERR_ARG, "training records come from single solves (nc_solve_adaptive)");
+    // The workers launch eagerly: a stream capture in one thread is invalidated by the
+    // allocations and synchronisations of the others (HIP does not keep thread-local
+    // captures apart), so no worker captures graphs.
+    o.ode.no_graphs = 1;
+    // instance i's initial states and probabilities start at these offsets
+    std::vector<int64_t> st_off(n_inst + 1, 0), p_off(n_inst + 1, 0);
+    for (int32_t i = 0; i < n_inst; ++i) {
+      NC_REQUIRE(models[i], NC_ERR_ARG, "null model of instance " + std::to_string(i));
+      NC_REQUIRE(n0[i] >= 1, NC_ERR_ARG, "empty initial distribution of instance " + std::to_string(i));
+      p_off[i + 1] = p_off[i] + n0[i];
+      st_off[i + 1] = st_off[i] + n0[i] * models[i]->ns;
+    }
+    int dev = 0;
+    NC_CHECK_HIP(hipGetDevice(&dev));
+    const int W = std::max(1, std::min<int>(n_inst, n_workers > 0 ? n_workers : kBatchWorkers));
+    std::vector<std::unique_ptr<nc_output_s>> res(n_inst);
+    std::vector<nc_solve_stats> st(n_inst);
+    std::atomic<int32_t> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex err_mu;
+    int err_code = NC_OK;
+    std::string err_msg;
+    std::vector<int32_t> deferred;  // ran out of device memory next to the others: run alone
+    auto fail = [&](int32_t i, int code, const char* what) {
+      std::lock_guard<std::mutex> lk(err_mu);
+      if (err_code == NC_OK) {
+        err_code = code;
+        err_msg = "instance " + std::to_string(i) + ": " + what;
+      }
+      failed = true;
+    };
+    // Each worker runs whole solves, one instance at a time, on streams of its own; every
+    // object a solve creates is created, used and freed on its worker.
+    auto solve_one = [&](int32_t i) {
+      auto O = std::make_unique<nc_output_s>();
+      solve_adaptive_impl(models[i], 0, n0[i], states0 + st_off[i], p0 + p_off[i], nullptr, t0, t1, n_save, saveat, o,
+                          O.get(), &st[i]);
+      res[i] = std::move(O);
+    };
+    auto work = [&] {
+      if (hipSetDevice(dev) != hipSuccess) {
+        fail(-1, NC_ERR_HIP, "hipSetDevice failed on a worker thread");
+        return;
+      }
+      PrivateStreams scope;
+      for (;;) {
+        const int32_t i = next.fetch_add(1);
+        if (i >= n_inst || failed) break;
+        try {
+          solve_one(i);
+        } catch (const Error& e) {
+          if (e.code == NC_ERR_ALLOC && W > 1) {
+            std::lock_guard<std::mutex> lk(err_mu);
+            deferred.push_back(i);
+          } else {
+            fail(i, e.code, e.what());
+          }
+        } catch (const std::bad_alloc&) {
+          fail(i, NC_ERR_ALLOC, "host allocation failed");
+        } catch (const std::exception& e) {
+          fail(i, NC_ERR_INTERNAL, e.what());
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(W - 1);
+    try {
+      for (int w = 1; w < W; ++w) pool.emplace_back(work);
+    } catch (const std::system_error&) {
+      // fewer threads than asked for: the ones started (and this one) take every instance
+    }
+    work();
+    for (auto& th : pool) th.join();
+    if (failed) throw Error(err_code, err_msg);  // the finished outputs are freed with res
+    // instances whose device memory ran out while others ran: again, one at a time, with
+    // the cached blocks returned to the driver first
+    std::sort(deferred.begin(), deferred.end());
+    for (int32_t i : deferred) {
+      pool_trim();
+      try {
+        solve_one(i);
+      } catch (const Error& e) {
+        throw Error(e.code, "instance " + std::to_string(i) + ": " + e.what());
+      }
+    }
+    for (int32_t i = 0; i < n_inst; ++i) {
+      outs[i] = res[i].release();
+      if (stats) stats[i] = st[i];
+    }
   });
 }
 

@@ -146,11 +146,20 @@ void pinned_free(void* p) {
   c.live_host.erase(it);
 }
 
+// A thread's staging block goes back to the pinned cache when the thread ends (a worker of
+// a batch of solves: its queued copies completed when its objects were freed).
+struct StageBlock {
+  unsigned char* p = nullptr;
+  ~StageBlock() {
+    if (p) pinned_free(p);
+  }
+};
+
 void* host_stage(int slot) {
-  thread_local unsigned char* base = nullptr;  // kept until the process ends
-  if (!base) base = static_cast<unsigned char*>(pinned_alloc(16 * kHostStageSlot));
+  thread_local StageBlock blk;
+  if (!blk.p) blk.p = static_cast<unsigned char*>(pinned_alloc(16 * kHostStageSlot));
   if (slot < 0 || slot >= 16) throw Error(NC_ERR_INTERNAL, "bad host staging slot");
-  return base + (size_t)slot * kHostStageSlot;
+  return blk.p + (size_t)slot * kHostStageSlot;
 }
 
 // The streams this host thread holds (taken here and not yet released), and whether its
@@ -181,6 +190,28 @@ hipStream_t stream_acquire() {
 
 void stream_release(hipStream_t s) {
   if (!s) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    // a capture that failed part-way and was not ended: the stream cannot serve again, so
+    // it is ended and destroyed instead of going back to the cache
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    (void)hipGetLastError();
+    for (size_t k = 0; k < tl_streams.size(); ++k)
+      if (tl_streams[k] == s) {
+        tl_streams[k] = tl_streams.back();
+        tl_streams.pop_back();
+        break;
+      }
+    Cache& c = cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.stream_dev.erase(s);
+    return;
+  }
   (void)hipStreamSynchronize(s);
   for (size_t k = 0; k < tl_streams.size(); ++k)
     if (tl_streams[k] == s) {

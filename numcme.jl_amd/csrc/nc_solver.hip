@@ -892,6 +892,7 @@ static void arnoldi_run(nc_solver_s* S) {
     } catch (...) {
       (void)hipStreamEndCapture(S->s, &g);
       if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
       throw;
     }
     NC_CHECK_HIP(hipStreamEndCapture(S->s, &g));

@@ -12,7 +12,8 @@ from .model import (CmeModel, CmeModelWithSensitivity, JointTimeVaryingPropensit
                     SeparableTimeVaryingPropensity, StandardTimeInvariantPropensity, istimeseparable,
                     istimevarying, propensity)
 from .solve import RK45, FspOutputSliceSparse, FspOutputSparse, KrylovExpv, solve
-from .adaptive import AdaptiveFspSparse, LearnedAdapter, RStepAdapter, SelectiveRStepAdapter, adapt_, init_
+from .adaptive import (AdaptiveFspSparse, LearnedAdapter, RStepAdapter, SelectiveRStepAdapter, adapt_, init_,
+                       solve_adaptive_batch)
 from .batch import BatchFsp
 from .mlp import AdapterMlp
 from .learned import TrainingRecords, boundary_features, train_adapter

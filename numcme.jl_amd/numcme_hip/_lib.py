@@ -207,6 +207,7 @@ SIGNATURES = {
     "nc_adaptive_opts_default": [C.POINTER(nc_adaptive_opts)],
     "nc_solve_adaptive": [P, I64, PI32, PD, D, D, I32, PD, C.POINTER(nc_adaptive_opts), C.POINTER(P),
                           C.POINTER(nc_solve_stats)],
+    "nc_solve_adaptive_batch": [I32, P, PI64, PI32, PD, D, D, I32, PD, C.POINTER(nc_adaptive_opts), I32, P, P],
     "nc_solve_sens_adaptive": [P, I64, PI32, PD, PD, D, D, I32, PD, C.POINTER(nc_adaptive_opts), C.POINTER(P),
                                C.POINTER(nc_solve_stats)],
     "nc_output_get_sens": [P, I32, I32, PD, PD],

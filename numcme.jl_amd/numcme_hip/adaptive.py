@@ -137,11 +137,8 @@ def _saveat_points(saveat, tspan):
     return np.asarray(list(saveat), dtype=np.float64)
 
 
-def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: AdaptiveFspSparse, saveat=(),
-                   fsptol=1.0e-6, odeatol=1.0e-6, odertol=1.0e-4, verbose=False, records=None):
-    """The adaptive solve. ``verbose`` prints every space update as the reference does;
-    ``records`` (a ``TrainingRecords``) collects learned-adapter training records."""
-    from .solve import RK45, FspOutputSparse, KrylovExpv
+def _adaptive_opts(alg, fsptol, odeatol, odertol, verbose, records):
+    from .solve import RK45, KrylovExpv
 
     if not isinstance(alg.space_adapter, (RStepAdapter, LearnedAdapter)):
         raise ArgumentError(L.NC_ERR_ARG,
@@ -164,19 +161,22 @@ def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: Ada
         o.precision = L.NC_FP32 if ode.precision == "fp32" else L.NC_FP64
     else:
         raise ArgumentError(L.NC_ERR_ARG, "ode_method must be KrylovExpv, RK45 or None")
+    return o
 
-    ts = _saveat_points(saveat, tspan)
+
+def _initial(model, initial_distribution):
     X = L.i32(np.asarray(initial_distribution.states).reshape(-1, model.get_species_count()))
     p0 = L.f64(initial_distribution.values)
     if X.shape[0] != p0.size:
         raise ArgumentError(L.NC_ERR_ARG, "initial distribution states and values differ in length")
-    h = C.c_void_p()
-    st = L.nc_solve_stats()
-    tsp = ts if ts.size else np.zeros(1)
-    L.call("nc_solve_adaptive", model.device_model().handle, X.shape[0], L.ptr(X, C.c_int32), L.ptr(p0, C.c_double),
-           float(tspan[0]), float(tspan[1]), int(ts.size), L.ptr(L.f64(tsp), C.c_double), C.byref(o), C.byref(h),
-           C.byref(st))
-    res = _Output(h.value)
+    return X, p0
+
+
+def _output(res, model, st):
+    """FspOutputSparse of an nc_output (an ``_Output``, freed before returning); the
+    snapshots stay on the device, and those of one space share its state list."""
+    from .solve import FspOutputSparse
+
     out = FspOutputSparse()
     out.space_ids = []  # index of the state space each snapshot lives on
     nsnap, nspace = C.c_int32(), C.c_int32()
@@ -187,7 +187,6 @@ def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: Ada
         L.call("nc_output_info", res.handle, k, C.byref(t), C.byref(n), C.byref(sid))
         sinks = np.empty(nr)
         h = C.c_void_p()
-        # the snapshot stays on the device; snapshots of one space share its state list
         L.call("nc_output_get_fspvec", res.handle, k, -1, C.byref(h), L.ptr(sinks, C.c_double))
         out.t.append(t.value)
         out.space_ids.append(sid.value)
@@ -196,6 +195,56 @@ def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: Ada
     out.stats = {f: getattr(st, f) for f, _ in L.nc_solve_stats._fields_}
     res.free()
     return out
+
+
+def solve_adaptive(model, initial_distribution: FspVectorSparse, tspan, alg: AdaptiveFspSparse, saveat=(),
+                   fsptol=1.0e-6, odeatol=1.0e-6, odertol=1.0e-4, verbose=False, records=None):
+    """The adaptive solve. ``verbose`` prints every space update as the reference does;
+    ``records`` (a ``TrainingRecords``) collects learned-adapter training records."""
+    o = _adaptive_opts(alg, fsptol, odeatol, odertol, verbose, records)
+    ts = _saveat_points(saveat, tspan)
+    X, p0 = _initial(model, initial_distribution)
+    h = C.c_void_p()
+    st = L.nc_solve_stats()
+    tsp = ts if ts.size else np.zeros(1)
+    L.call("nc_solve_adaptive", model.device_model().handle, X.shape[0], L.ptr(X, C.c_int32), L.ptr(p0, C.c_double),
+           float(tspan[0]), float(tspan[1]), int(ts.size), L.ptr(L.f64(tsp), C.c_double), C.byref(o), C.byref(h),
+           C.byref(st))
+    return _output(_Output(h.value), model, st)
+
+
+def solve_adaptive_batch(models, initial_distributions, tspan, alg: AdaptiveFspSparse, saveat=(), fsptol=1.0e-6,
+                         odeatol=1.0e-6, odertol=1.0e-4, workers=0):
+    """Adaptive solves of many instances at once (a parameter sweep; no reference
+    counterpart, where a sweep is a loop of ``solve`` calls): ``models[i]`` from
+    ``initial_distributions[i]`` (one distribution for all, if a single FspVectorSparse is
+    given), all over the same ``tspan``, ``saveat`` and ``alg``. ``workers`` host threads of
+    the library (0: 4) each run whole solves on streams of their own. Returns one
+    ``FspOutputSparse`` per instance, each what ``solve`` gives for it."""
+    models = list(models)
+    n = len(models)
+    if isinstance(initial_distributions, FspVectorSparse):
+        initial_distributions = [initial_distributions] * n
+    initial_distributions = list(initial_distributions)
+    if len(initial_distributions) != n:
+        raise ArgumentError(L.NC_ERR_ARG, "one initial distribution per model")
+    o = _adaptive_opts(alg, fsptol, odeatol, odertol, False, None)
+    ts = _saveat_points(saveat, tspan)
+    if n == 0:
+        return []
+    parts = [_initial(m, d) for m, d in zip(models, initial_distributions)]
+    X = np.ascontiguousarray(np.concatenate([x.ravel() for x, _ in parts]), dtype=np.int32)
+    p0 = np.ascontiguousarray(np.concatenate([p for _, p in parts]), dtype=np.float64)
+    n0 = np.array([p.size for _, p in parts], dtype=np.int64)
+    handles = (C.c_void_p * n)(*[m.device_model().handle for m in models])
+    outs = (C.c_void_p * n)()
+    stats = (L.nc_solve_stats * n)()
+    tsp = ts if ts.size else np.zeros(1)
+    L.call("nc_solve_adaptive_batch", n, handles, L.ptr(n0, C.c_int64), L.ptr(X, C.c_int32), L.ptr(p0, C.c_double),
+           float(tspan[0]), float(tspan[1]), int(ts.size), L.ptr(L.f64(tsp), C.c_double), C.byref(o), int(workers),
+           outs, stats)
+    owned = [_Output(outs[i]) for i in range(n)]  # freed even if a conversion fails
+    return [_output(res, m, stats[i]) for i, (res, m) in enumerate(zip(owned, models))]
 
 
 class _Output(L.Handle):
