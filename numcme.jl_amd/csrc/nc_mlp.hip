@@ -1,16 +1,25 @@
 // Learned space adapter network: an 8 -> 64 -> 64 -> 1 MLP with tanh-form GELU hidden
-// layers and a keep/add logit, computed with BF16 MFMA (v_mfma_f32_16x16x16_bf16) and FP32
-// accumulation. Training (binary cross-entropy on the logit, Adam on FP32 master weights)
-// runs on the device with the forward and backward kernels below. No reference
-// counterpart: NumCME.jl's adapters are rule-based.
+// layers and a keep/add logit, computed with BF16 MFMA and FP32 accumulation: gfx950's
+// v_mfma_f32_16x16x32_bf16 for every product with a contraction of 32 or more, and
+// v_mfma_f32_16x16x16_bf16 for the first layer (8 input features, padded to 16). Training
+// (binary cross-entropy on the logit, Adam on FP32 master weights) runs on the device with
+// the forward and backward kernels below. No reference counterpart: NumCME.jl's adapters
+// are rule-based.
 //
 // Parameters (FP32 master copy, NP = 4801 floats): W1 [8][64] (input-major), b1 [64],
 // W2 [64][64] (input-major), b2 [64], w3 [64], b3. Every MFMA operand is rounded to BF16
 // (round to nearest even) when it is loaded into registers.
 //
-// Workgroups have 4 waves; a wave works on 16 samples at a time. For
-// v_mfma_f32_16x16x16_bf16 a lane l holds A[l % 16][4 (l / 16) + j], B[4 (l / 16) + j][l % 16]
-// and D[4 (l / 16) + j][l % 16], j = 0..3.
+// Lane layouts (lane l, g = l / 16, i = l % 16):
+//   16x16x16: A[i][4 g + j], B[4 g + j][i], j = 0..3;
+//   16x16x32: A[i][k(g, j)], B[k(g, j)][i], j = 0..7, with the same K slot k(g, j) in A and
+//             B (which K index a slot stands for is the program's choice: a contraction
+//             only needs A and B to agree);
+//   both:     D[4 g + j][i], j = 0..3.
+// Activations are kept transposed (units x samples): a layer's D tile, lane (g, sample i)
+// holding units 16 t + 4 g + j, is the next layer's B operand with no data movement, the
+// K slots of a 32-unit chunk q being k(g, j) = 32 q + 16 (j / 4) + 4 g + j % 4, i.e. the
+// D tiles 2q and 2q+1 side by side.
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -24,7 +33,6 @@ namespace {
 constexpr int kF = 8, kH = 64;
 constexpr int OW1 = 0, OB1 = OW1 + kF * kH, OW2 = OB1 + kH, OB2 = OW2 + kH * kH, OW3 = OB2 + kH, OB3 = OW3 + kH;
 constexpr int kThreads = 256;
-constexpr int kSamples = 64;  // per training group (one workgroup at a time)
 constexpr unsigned kFwdMaxBlocks = 4096, kTrainMaxBlocks = 512;
 
 // forward: 4 waves per workgroup, 16 samples per wave and tile
@@ -32,9 +40,9 @@ unsigned fwd_grid(int64_t n) {
   const int64_t tiles = (n + 15) / 16;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, kFwdMaxBlocks));
 }
-// training: one 64-sample group per workgroup and iteration
+// training: 32-sample chunks, one per wave and iteration, 4 waves per workgroup
 unsigned train_grid(int64_t n) {
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kSamples - 1) / kSamples, kTrainMaxBlocks));
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 127) / 128, kTrainMaxBlocks));
 }
 
 typedef short v4s __attribute__((ext_vector_type(4)));
@@ -46,35 +54,44 @@ typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
 __device__ inline short bf16(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 __device__ inline v4s bf16x4(v4f x) { return __builtin_bit_cast(v4s, __builtin_convertvector(x, v4bf)); }
 
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
 __device__ inline v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ inline v4f mfma32(v8s a, v8s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
+// the K = 32 operand of two 16-row D tiles (see the lane layouts above)
+__device__ inline v8s cat(v4s lo, v4s hi) { return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7); }
+// K unit of slot (g, j) of chunk q
+__device__ inline int kslot(int q, int g, int j) { return 32 * q + 16 * (j >> 2) + 4 * g + (j & 3); }
 
 constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2/pi)
 // tanh-form GELU, 0.5 x (1 + tanh(u)) with u = c (x + 0.044715 x^3), written as
 // x * sigmoid(2u) (the same function: 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u))) on the
-// hardware exponential
-// sigmoid(2u) = 1 / (1 + 2^(-2 u log2(e))) on v_exp_f32 / v_rcp_f32 (about 1 ulp each)
+// hardware exponential and reciprocal (about 1 ulp each):
+// sigmoid(2u) = 1 / (1 + 2^(x (k + 0.044715 k x^2))), k = -2 c log2(e)
 constexpr float kM2CLog2e = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
 __device__ inline float sig2u(float x) {
-  const float t = __builtin_fmaf(0.044715f * x * x, x, x);
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(kM2CLog2e * t));
+  const float e = x * __builtin_fmaf(0.044715f * kM2CLog2e * x, x, kM2CLog2e);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(e));
 }
 __device__ inline float gelu(float x) { return x * sig2u(x); }
-// d/dx [x sigmoid(2u)] = sigmoid(2u) + 2 x sigmoid(2u) (1 - sigmoid(2u)) u'
-__device__ inline float gelu_d(float x) {
-  const float sg = sig2u(x);
-  return sg + 2.0f * x * sg * (1.0f - sg) * kGeluC * (1.0f + 3.0f * 0.044715f * x * x);
+// d/dx [x sigmoid(2u)] = sigmoid(2u) + 2 x sigmoid(2u) (1 - sigmoid(2u)) u', from the
+// forward pass's sigmoid sg
+__device__ inline float gelu_d(float x, float sg) {
+  return sg + 2.0f * x * sg * (1.0f - sg) * kGeluC * __builtin_fmaf(3.0f * 0.044715f * x, x, 1.0f);
 }
 
 // ---- forward: register-resident, transposed orientation --------------------------------
-// Each layer computes Z^T = W^T H^T (rows = units, columns = 16 samples). The MFMA result
-// D[4 (l/16) + j][l % 16] is then exactly the next layer's B operand B[k][n] (k = unit,
-// n = sample), so activations never leave registers; the weights are A operands kept in
-// VGPRs for the life of the wave, which walks over 16-sample tiles.
+// Each layer computes Z^T = W^T H^T (rows = units, columns = 16 samples); the weights are
+// A operands kept in VGPRs for the life of the wave, which walks over 16-sample tiles.
 __global__ __launch_bounds__(kThreads) void k_mlp_fwd(const float* __restrict__ X, int64_t n,
                                                       const float* __restrict__ P, float* __restrict__ logits) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  v4s a1[4], a2[4][4], a3[4];
+  v4s a1[4];
+  v8s a2[4][2], a3[2];
   float b1r[4][4], b2r[4][4];
   for (int t = 0; t < 4; ++t)
     for (int j = 0; j < 4; ++j) {
@@ -82,8 +99,12 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(const float* __restrict__ 
       a1[t][j] = k < kF ? bf16(P[OW1 + k * kH + 16 * t + li]) : (short)0;
       b1r[t][j] = P[OB1 + 16 * t + k];
       b2r[t][j] = P[OB2 + 16 * t + k];
-      a3[t][j] = li == 0 ? bf16(P[OW3 + 16 * t + k]) : (short)0;
-      for (int q = 0; q < 4; ++q) a2[t][q][j] = bf16(P[OW2 + (16 * q + k) * kH + 16 * t + li]);
+    }
+  for (int q = 0; q < 2; ++q)
+    for (int j = 0; j < 8; ++j) {
+      const int k = kslot(q, lk, j);
+      a3[q][j] = li == 0 ? bf16(P[OW3 + k]) : (short)0;
+      for (int t = 0; t < 4; ++t) a2[t][q][j] = bf16(P[OW2 + k * kH + 16 * t + li]);
     }
   const float b3 = P[OB3];
   const int64_t ntiles = (n + 15) / 16;
@@ -102,245 +123,229 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd(const float* __restrict__ 
     }
     for (int t = 0; t < 4; ++t) {
       v4f d{0, 0, 0, 0};
-      for (int q = 0; q < 4; ++q) d = mfma(a2[t][q], h1[q], d);
+      for (int q = 0; q < 2; ++q) d = mfma32(a2[t][q], cat(h1[2 * q], h1[2 * q + 1]), d);
       for (int j = 0; j < 4; ++j) d[j] = gelu(d[j] + b2r[t][j]);
       h2[t] = bf16x4(d);
     }
     v4f d3{0, 0, 0, 0};
-    for (int q = 0; q < 4; ++q) d3 = mfma(a3[q], h2[q], d3);
+    for (int q = 0; q < 2; ++q) d3 = mfma32(a3[q], cat(h2[2 * q], h2[2 * q + 1]), d3);
     if (lk == 0 && sidx < n) logits[sidx] = d3[0] + b3;
   }
 }
 
-// ---- training: forward and backward of 64-sample groups ---------------------------------
-struct Lds {
-  float w1[kF][kH];
-  float w2[kH][kH];
-  float b1[kH], b2[kH], w3[kH];
-  float b3;
-  float x[4][16][kF];
-  float z1[4][16][kH];  // pre-activation, later the hidden-1 gradient
-  float h1[4][16][kH];
-  float z2[4][16][kH];  // pre-activation, later the hidden-2 gradient
-  float h2[4][16][kH];
-  float d3[4][16];
-  float acc[kH][kH];    // one group's weight gradient (W2, then W1)
-  float vec[4][kH];     // per-wave vector gradients
-  float lred[kThreads];
-};
+// ---- training: forward and backward, register-resident -------------------------------
+// A wave takes 32-sample chunks (chunk w, w + waves, ... of the grid's waves, in order).
+// For each 16-sample half it runs the forward pass in the transposed orientation above,
+// the loss gradient dz, the output-layer and hidden-2 gradients G2^T = dz w3 * gelu'(Z2^T)
+// and the hidden-1 gradient G1^T = (W2 G2^T) * gelu'(Z1^T) (W2 as the A operand, G2^T's D
+// tiles as B), all in registers. The weight gradients contract over samples, which the D
+// tiles hold across lanes: H1, G2 and G1 are written to the wave's own LDS rows
+// [unit][sample] (BF16) and read back with lane = unit, K slots = the chunk's 32 samples,
+// so that dW2 = H1^T G2 and dW1 = X^T G1 are one 16x16x32 MFMA per output tile and chunk,
+// accumulated in registers. Only the wave's own lanes exchange data through its LDS rows,
+// so no workgroup barrier runs inside the loop. At the end the four waves' sums go
+// through LDS and are added in wave order into the workgroup's partial
+// grad_part[blockIdx.x][NP] (unscaled), its summed loss into loss_part[blockIdx.x].
+constexpr int kTP = 36;                    // LDS row pitch (BF16) of a transposed activation
+constexpr int kTWave = 3 * kH * kTP;       // H1, G2, G1 rows of one wave
+constexpr int kRed = kNP + 4;              // floats of one wave's partial sums (+ loss)
+constexpr size_t kTrainLds = std::max<size_t>(4 * kTWave * sizeof(short), 4 * kRed * sizeof(float));
 
-// One workgroup = 4 waves; a wave owns 16 samples of the current 64-sample group. The
-// workgroup walks over groups blockIdx.x, blockIdx.x + gridDim.x, ... and keeps the
-// gradient sums in registers (thread tid owns W2 entries tid + 256 r, W1 entries
-// tid + 256 r, and with tid < 64 entry tid of b1, b2 and w3), added group by group in
-// the same order on every run. It writes its gradient sum (unscaled) to
-// grad_part[blockIdx.x][NP] and its summed loss to loss_part[blockIdx.x].
+// orders a wave's LDS stores before its own lanes' later loads of them
+__device__ inline void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(kThreads) void k_mlp_train(const float* __restrict__ X, const float* __restrict__ Y,
                                                         int64_t n, const float* __restrict__ P,
                                                         float* __restrict__ grad_part, float* __restrict__ loss_part) {
-  __shared__ Lds s;
+  __shared__ __align__(16) unsigned char lds[kTrainLds];
+  __shared__ __align__(16) float s_b1[kH], s_b2[kH], s_w3[kH];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int li = lane & 15, lk = lane >> 4;  // MFMA lane coordinates
-  for (int i = tid; i < kF * kH; i += kThreads) s.w1[i / kH][i % kH] = P[OW1 + i];
-  for (int i = tid; i < kH * kH; i += kThreads) s.w2[i / kH][i % kH] = P[OW2 + i];
+  const int li = lane & 15, lk = lane >> 4;
+  short* tH1 = reinterpret_cast<short*>(lds) + wave * kTWave;
+  short* tG2 = tH1 + kH * kTP;
+  short* tG1 = tG2 + kH * kTP;
   if (tid < kH) {
-    s.b1[tid] = P[OB1 + tid];
-    s.b2[tid] = P[OB2 + tid];
-    s.w3[tid] = P[OW3 + tid];
+    s_b1[tid] = P[OB1 + tid];
+    s_b2[tid] = P[OB2 + tid];
+    s_w3[tid] = P[OW3 + tid];
   }
-  if (tid == 0) s.b3 = P[OB3];
-  float acc_w2[kH * kH / kThreads] = {};
-  float acc_w1[kF * kH / kThreads] = {};
-  float acc_b1 = 0.0f, acc_b2 = 0.0f, acc_w3 = 0.0f, acc_b3 = 0.0f, lsum = 0.0f;
-  const int64_t ngroups = (n + kSamples - 1) / kSamples;
-  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    __syncthreads();  // the previous group's reads of x, z, h, acc and vec are done
-    const int64_t s0 = grp * kSamples + wave * 16;
-    for (int i = lane; i < 16 * kF; i += 64) {
-      const int64_t m = s0 + i / kF;
-      s.x[wave][i / kF][i % kF] = m < n ? X[m * kF + i % kF] : 0.0f;
-    }
-    __syncthreads();
-    // ---- layer 1: Z1 = X W1 + b1 (K = 8 padded to 16) ----
-    v4s ax;
+  // weights as A operands: W1^T (16x16x16), W2^T (forward), W2 (G1), the w3 row
+  v4s a1[4];
+  v8s a2[4][2], a2b[4][2], a3[2];
+  for (int t = 0; t < 4; ++t)
     for (int j = 0; j < 4; ++j) {
       const int k = 4 * lk + j;
-      ax[j] = k < kF ? bf16(s.x[wave][li][k]) : (short)0;
+      a1[t][j] = k < kF ? bf16(P[OW1 + k * kH + 16 * t + li]) : (short)0;
     }
-    for (int t = 0; t < 4; ++t) {
-      v4s bw;
-      for (int j = 0; j < 4; ++j) {
-        const int k = 4 * lk + j;
-        bw[j] = k < kF ? bf16(s.w1[k][16 * t + li]) : (short)0;
-      }
-      const v4f d = mfma(ax, bw, v4f{0, 0, 0, 0});
-      for (int j = 0; j < 4; ++j) {
-        const int m = 4 * lk + j, c = 16 * t + li;
-        const float z = d[j] + s.b1[c];
-        s.z1[wave][m][c] = z;
-        s.h1[wave][m][c] = gelu(z);
+  for (int q = 0; q < 2; ++q)
+    for (int j = 0; j < 8; ++j) {
+      const int k = kslot(q, lk, j);
+      a3[q][j] = li == 0 ? bf16(P[OW3 + k]) : (short)0;
+      for (int t = 0; t < 4; ++t) {
+        a2[t][q][j] = bf16(P[OW2 + k * kH + 16 * t + li]);   // A[c][k] = W2[k][c]
+        a2b[t][q][j] = bf16(P[OW2 + (16 * t + li) * kH + k]);  // A[k'][c] = W2[k'][c]
       }
     }
-    __syncthreads();
-    // ---- layer 2: Z2 = H1 W2 + b2 ----
-    for (int t = 0; t < 4; ++t) {
-      v4f d{0, 0, 0, 0};
-      for (int q = 0; q < 4; ++q) {
-        v4s a, b;
-        for (int j = 0; j < 4; ++j) {
-          a[j] = bf16(s.h1[wave][li][16 * q + 4 * lk + j]);
-          b[j] = bf16(s.w2[16 * q + 4 * lk + j][16 * t + li]);
-        }
-        d = mfma(a, b, d);
-      }
-      for (int j = 0; j < 4; ++j) {
-        const int m = 4 * lk + j, c = 16 * t + li;
-        const float z = d[j] + s.b2[c];
-        s.z2[wave][m][c] = z;
-        s.h2[wave][m][c] = gelu(z);
-      }
-    }
-    __syncthreads();
-    // ---- layer 3: z3 = H2 w3 + b3 (N = 1 padded to 16: column 0 carries w3) ----
-    v4f d3{0, 0, 0, 0};
-    for (int q = 0; q < 4; ++q) {
-      v4s a, b;
-      for (int j = 0; j < 4; ++j) {
-        a[j] = bf16(s.h2[wave][li][16 * q + 4 * lk + j]);
-        b[j] = li == 0 ? bf16(s.w3[16 * q + 4 * lk + j]) : (short)0;
-      }
-      d3 = mfma(a, b, d3);
-    }
-    if (li == 0) {
-      for (int j = 0; j < 4; ++j) {
-        const int m = 4 * lk + j;
-        const int64_t g = s0 + m;
-        const float z = d3[j] + s.b3;
-        float dz = 0.0f;
-        if (g < n) {
-          const float y = Y[g];
-          dz = 1.0f / (1.0f + expf(-z)) - y;
-          lsum += fmaxf(z, 0.0f) - z * y + log1pf(expf(-fabsf(z)));
-        }
-        s.d3[wave][m] = dz;
-      }
-    }
-    __syncthreads();
-    // ---- output layer gradients: w3, b3 ----
-    {
-      float g = 0.0f;
-      for (int m = 0; m < 16; ++m) g += s.h2[wave][m][lane] * s.d3[wave][m];
-      s.vec[wave][lane] = g;
-    }
-    __syncthreads();
-    if (tid < kH) acc_w3 += s.vec[0][tid] + s.vec[1][tid] + s.vec[2][tid] + s.vec[3][tid];
-    if (tid == 0) {
-      float g = 0.0f;
-      for (int w = 0; w < 4; ++w)
-        for (int m = 0; m < 16; ++m) g += s.d3[w][m];
-      acc_b3 += g;
-    }
-    // ---- hidden-2 gradient G2 = (d3 w3^T) * gelu'(Z2), kept in z2 ----
-    for (int i = lane; i < 16 * kH; i += 64) {
-      const int m = i / kH, c = i % kH;
-      s.z2[wave][m][c] = s.d3[wave][m] * s.w3[c] * gelu_d(s.z2[wave][m][c]);
-    }
-    for (int i = tid; i < kH * kH; i += kThreads) s.acc[i / kH][i % kH] = 0.0f;
-    __syncthreads();
-    // dW2 = H1^T G2 (M = hidden-1 unit, K = samples, N = hidden-2 unit), waves added in order
-    for (int w = 0; w < 4; ++w) {
-      if (wave == w) {
-        for (int tm = 0; tm < 4; ++tm)
-          for (int tn = 0; tn < 4; ++tn) {
-            v4s a, b;
-            for (int j = 0; j < 4; ++j) {
-              a[j] = bf16(s.h1[w][4 * lk + j][16 * tm + li]);
-              b[j] = bf16(s.z2[w][4 * lk + j][16 * tn + li]);
-            }
-            const v4f d = mfma(a, b, v4f{0, 0, 0, 0});
-            for (int j = 0; j < 4; ++j) s.acc[16 * tm + 4 * lk + j][16 * tn + li] += d[j];
-          }
-      }
-      __syncthreads();
-    }
-    for (int r = 0; r < kH * kH / kThreads; ++r) {
-      const int i = tid + kThreads * r;
-      acc_w2[r] += s.acc[i / kH][i % kH];
-    }
-    {
-      float g = 0.0f;
-      for (int m = 0; m < 16; ++m) g += s.z2[wave][m][lane];
-      s.vec[wave][lane] = g;
-    }
-    __syncthreads();
-    if (tid < kH) acc_b2 += s.vec[0][tid] + s.vec[1][tid] + s.vec[2][tid] + s.vec[3][tid];
-    // ---- hidden-1 gradient G1 = (G2 W2^T) * gelu'(Z1), kept in z1 ----
-    for (int t = 0; t < 4; ++t) {
-      v4f d{0, 0, 0, 0};
-      for (int q = 0; q < 4; ++q) {
-        v4s a, b;
-        for (int j = 0; j < 4; ++j) {
-          a[j] = bf16(s.z2[wave][li][16 * q + 4 * lk + j]);
-          b[j] = bf16(s.w2[16 * t + li][16 * q + 4 * lk + j]);  // W2^T
-        }
-        d = mfma(a, b, d);
-      }
-      for (int j = 0; j < 4; ++j) {
-        const int m = 4 * lk + j, c = 16 * t + li;
-        s.z1[wave][m][c] = d[j] * gelu_d(s.z1[wave][m][c]);
-      }
-    }
-    __syncthreads();  // every thread has read acc (W2) and vec; z1 holds G1
-    // dW1 = X^T G1 (M = input feature, padded to 16), waves added in order
-    for (int i = tid; i < kF * kH; i += kThreads) s.acc[i / kH][i % kH] = 0.0f;
-    __syncthreads();
-    for (int w = 0; w < 4; ++w) {
-      if (wave == w) {
-        for (int tn = 0; tn < 4; ++tn) {
-          v4s a, b;
-          for (int j = 0; j < 4; ++j) {
-            a[j] = li < kF ? bf16(s.x[w][4 * lk + j][li]) : (short)0;
-            b[j] = bf16(s.z1[w][4 * lk + j][16 * tn + li]);
-          }
-          const v4f d = mfma(a, b, v4f{0, 0, 0, 0});
-          for (int j = 0; j < 4; ++j) {
-            const int r = 4 * lk + j;
-            if (r < kF) s.acc[r][16 * tn + li] += d[j];
-          }
-        }
-      }
-      __syncthreads();
-    }
-    for (int r = 0; r < kF * kH / kThreads; ++r) {
-      const int i = tid + kThreads * r;
-      acc_w1[r] += s.acc[i / kH][i % kH];
-    }
-    {
-      float g = 0.0f;
-      for (int m = 0; m < 16; ++m) g += s.z1[wave][m][lane];
-      s.vec[wave][lane] = g;
-    }
-    __syncthreads();
-    if (tid < kH) acc_b1 += s.vec[0][tid] + s.vec[1][tid] + s.vec[2][tid] + s.vec[3][tid];
+  const float b3 = P[OB3];
+  __syncthreads();
+  v4f gw2[4][4], gw1[4];
+  float gb1[4][4], gb2[4][4], gw3[4][4];
+  for (int t = 0; t < 4; ++t) {
+    gw1[t] = v4f{0, 0, 0, 0};
+    for (int u = 0; u < 4; ++u) gw2[t][u] = v4f{0, 0, 0, 0};
+    for (int j = 0; j < 4; ++j) gb1[t][j] = gb2[t][j] = gw3[t][j] = 0.0f;
   }
+  float gb3 = 0.0f, lsum = 0.0f;
+  const int64_t nchunks = (n + 31) / 32;
+  for (int64_t ch = (int64_t)blockIdx.x * 4 + wave; ch < nchunks; ch += (int64_t)gridDim.x * 4) {
+    for (int hf = 0; hf < 2; ++hf) {
+      const int64_t sidx = ch * 32 + hf * 16 + li;
+      const int col = hf * 16 + li;  // this lane's sample column in the LDS rows
+      v4s xb{0, 0, 0, 0};
+      if (lk < 2 && sidx < n) {
+        const float4 v = *reinterpret_cast<const float4*>(X + sidx * kF + 4 * lk);
+        xb = bf16x4(v4f{v.x, v.y, v.z, v.w});
+      }
+      // layer 1: Z1^T, kept with its GELU sigmoid; H1^T to LDS
+      float z1[4][4], s1[4][4];
+      v4s h1[4];
+      for (int t = 0; t < 4; ++t) {
+        const v4f d = mfma(a1[t], xb, v4f{0, 0, 0, 0});
+        const float4 b = *reinterpret_cast<const float4*>(s_b1 + 16 * t + 4 * lk);
+        const float bb[4] = {b.x, b.y, b.z, b.w};
+        v4f h;
+        for (int j = 0; j < 4; ++j) {
+          z1[t][j] = d[j] + bb[j];
+          s1[t][j] = sig2u(z1[t][j]);
+          h[j] = z1[t][j] * s1[t][j];
+        }
+        h1[t] = bf16x4(h);
+        for (int j = 0; j < 4; ++j) tH1[(16 * t + 4 * lk + j) * kTP + col] = h1[t][j];
+      }
+      // layer 2
+      float z2[4][4], s2[4][4];
+      v4s h2[4];
+      for (int t = 0; t < 4; ++t) {
+        v4f d{0, 0, 0, 0};
+        for (int q = 0; q < 2; ++q) d = mfma32(a2[t][q], cat(h1[2 * q], h1[2 * q + 1]), d);
+        const float4 b = *reinterpret_cast<const float4*>(s_b2 + 16 * t + 4 * lk);
+        const float bb[4] = {b.x, b.y, b.z, b.w};
+        v4f h;
+        for (int j = 0; j < 4; ++j) {
+          z2[t][j] = d[j] + bb[j];
+          s2[t][j] = sig2u(z2[t][j]);
+          h[j] = z2[t][j] * s2[t][j];
+        }
+        h2[t] = bf16x4(h);
+      }
+      // layer 3 and the loss gradient dz (row 0 of D: lanes with lk == 0, sample li)
+      v4f d3{0, 0, 0, 0};
+      for (int q = 0; q < 2; ++q) d3 = mfma32(a3[q], cat(h2[2 * q], h2[2 * q + 1]), d3);
+      float dz = 0.0f;
+      if (lk == 0 && sidx < n) {
+        const float z = d3[0] + b3, y = Y[sidx];
+        dz = 1.0f / (1.0f + expf(-z)) - y;
+        lsum += fmaxf(z, 0.0f) - z * y + log1pf(expf(-fabsf(z)));
+        gb3 += dz;
+      }
+      dz = __shfl(dz, li, 64);  // every lane: dz of its sample
+      // output layer (w3 with the unrounded H2) and hidden-2 gradient; G2^T to LDS
+      v4s g2[4];
+      for (int t = 0; t < 4; ++t) {
+        const float4 w = *reinterpret_cast<const float4*>(s_w3 + 16 * t + 4 * lk);
+        const float ww[4] = {w.x, w.y, w.z, w.w};
+        v4f g;
+        for (int j = 0; j < 4; ++j) {
+          gw3[t][j] += z2[t][j] * s2[t][j] * dz;
+          g[j] = dz * ww[j] * gelu_d(z2[t][j], s2[t][j]);
+          gb2[t][j] += g[j];
+        }
+        g2[t] = bf16x4(g);
+        for (int j = 0; j < 4; ++j) tG2[(16 * t + 4 * lk + j) * kTP + col] = g2[t][j];
+      }
+      // hidden-1 gradient G1^T = (W2 G2^T) * gelu'(Z1^T); G1^T to LDS
+      for (int t = 0; t < 4; ++t) {
+        v4f d{0, 0, 0, 0};
+        for (int q = 0; q < 2; ++q) d = mfma32(a2b[t][q], cat(g2[2 * q], g2[2 * q + 1]), d);
+        v4f g;
+        for (int j = 0; j < 4; ++j) {
+          g[j] = d[j] * gelu_d(z1[t][j], s1[t][j]);
+          gb1[t][j] += g[j];
+        }
+        const v4s gb = bf16x4(g);
+        for (int j = 0; j < 4; ++j) tG1[(16 * t + 4 * lk + j) * kTP + col] = gb[j];
+      }
+    }
+    wave_lds_fence();
+    // weight gradients over the chunk: operands with lane = unit 16 tu + li and the K slots
+    // k(g, j) -> sample 16 (j / 4) + 4 g + j % 4 of the chunk
+    auto rows = [&](const short* T, int tu) {
+      const short* r = T + (16 * tu + li) * kTP + 4 * lk;
+      return cat(*reinterpret_cast<const v4s*>(r), *reinterpret_cast<const v4s*>(r + 16));
+    };
+    v8s hT[4], gT[4];
+    for (int u = 0; u < 4; ++u) {
+      hT[u] = rows(tH1, u);
+      gT[u] = rows(tG2, u);
+    }
+    for (int tk = 0; tk < 4; ++tk)
+      for (int tc = 0; tc < 4; ++tc) gw2[tk][tc] = mfma32(hT[tk], gT[tc], gw2[tk][tc]);  // dW2[k][c]
+    v8s xa{0, 0, 0, 0, 0, 0, 0, 0};  // A[f][s] = X[s][f], f = li < 8
+    if (li < kF)
+      for (int j = 0; j < 8; ++j) {
+        const int64_t sm = ch * 32 + 16 * (j >> 2) + 4 * lk + (j & 3);
+        xa[j] = sm < n ? bf16(X[sm * kF + li]) : (short)0;
+      }
+    for (int u = 0; u < 4; ++u) gw1[u] = mfma32(xa, rows(tG1, u), gw1[u]);  // dW1[f][u]
+    wave_lds_fence();  // the next chunk's stores follow these loads
+  }
+  // the waves' sums into LDS (the transposed rows are no longer needed), then added in
+  // wave order
+  __syncthreads();
+  float* red = reinterpret_cast<float*>(lds) + wave * kRed;
+  for (int tk = 0; tk < 4; ++tk)
+    for (int tc = 0; tc < 4; ++tc)
+      for (int j = 0; j < 4; ++j) red[OW2 + (16 * tk + 4 * lk + j) * kH + 16 * tc + li] = gw2[tk][tc][j];
+  for (int u = 0; u < 4; ++u)
+    for (int j = 0; j < 4; ++j) {
+      const int f = 4 * lk + j;
+      if (f < kF) red[OW1 + f * kH + 16 * u + li] = gw1[u][j];
+    }
+  // bias and w3 sums: add over the 16 sample lanes of each lane group
+  for (int t = 0; t < 4; ++t)
+    for (int j = 0; j < 4; ++j) {
+      float v1 = gb1[t][j], v2 = gb2[t][j], v3 = gw3[t][j];
+      for (int off = 8; off > 0; off >>= 1) {
+        v1 += __shfl_xor(v1, off, 16);
+        v2 += __shfl_xor(v2, off, 16);
+        v3 += __shfl_xor(v3, off, 16);
+      }
+      if (li == 0) {
+        red[OB1 + 16 * t + 4 * lk + j] = v1;
+        red[OB2 + 16 * t + 4 * lk + j] = v2;
+        red[OW3 + 16 * t + 4 * lk + j] = v3;
+      }
+    }
+  {
+    float v3b = gb3, vl = lsum;
+    for (int off = 32; off > 0; off >>= 1) {
+      v3b += __shfl_xor(v3b, off, 64);
+      vl += __shfl_xor(vl, off, 64);
+    }
+    if (lane == 0) {
+      red[OB3] = v3b;
+      red[kNP] = vl;
+    }
+  }
+  __syncthreads();
+  const float* r0 = reinterpret_cast<const float*>(lds);
   float* gp = grad_part + (int64_t)blockIdx.x * kNP;
-  for (int r = 0; r < kH * kH / kThreads; ++r) gp[OW2 + tid + kThreads * r] = acc_w2[r];
-  for (int r = 0; r < kF * kH / kThreads; ++r) gp[OW1 + tid + kThreads * r] = acc_w1[r];
-  if (tid < kH) {
-    gp[OB1 + tid] = acc_b1;
-    gp[OB2 + tid] = acc_b2;
-    gp[OW3 + tid] = acc_w3;
-  }
-  if (tid == 0) gp[OB3] = acc_b3;
-  // the loss of the workgroup: lanes 0, 16, 32, 48 of every wave hold partial sums
-  __syncthreads();
-  s.lred[tid] = lsum;
-  __syncthreads();
-  if (tid == 0) {
-    float l = 0.0f;
-    for (int i = 0; i < kThreads; ++i) l += s.lred[i];
-    loss_part[blockIdx.x] = l;
-  }
+  for (int p = tid; p < kNP; p += kThreads) gp[p] = ((r0[p] + r0[kRed + p]) + r0[2 * kRed + p]) + r0[3 * kRed + p];
+  if (tid == 0) loss_part[blockIdx.x] = ((r0[kNP] + r0[kRed + kNP]) + r0[2 * kRed + kNP]) + r0[3 * kRed + kNP];
 }
 
 // grad[p] = sum_b part[b][p] / count (fixed order); loss likewise
