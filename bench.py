@@ -35,13 +35,14 @@ HBM_PEAK_GBPS = 8000.0
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_matvec_toggle1024_fp32.md"
 
 
-def measured_traffic(rows_per_thread, precision):
+def measured_traffic(rows_per_thread, precision, matrix_free):
     """HBM bytes per mat-vec launch from the committed rocprofv3 counter runs (FETCH_SIZE
-    plus WRITE_SIZE, calibrated on a copy kernel), for the launch shape in use; None when
-    no counter run covers it."""
+    plus WRITE_SIZE, calibrated on a copy kernel), for the mode and launch shape in use;
+    None when no counter run covers it."""
     if precision != "fp32" or not TRAFFIC_PROFILE.exists():
         return None
-    key = f"traffic_bytes_per_launch rows_per_thread={rows_per_thread}:"
+    mode = {0: "stored", 1: "computed_values", 2: "matrix_free"}[int(matrix_free)]
+    key = f"traffic_bytes_per_launch mode={mode} rows_per_thread={rows_per_thread}:"
     for line in TRAFFIC_PROFILE.read_text().splitlines():
         if line.startswith(key):
             return float(line[len(key):].strip())
@@ -608,18 +609,28 @@ def spmv_probe(nh, precision, reps):
     model = toggle_model(nh)
     space = nh.StateSpaceSparse(model.stoich_matrix, box_states(1024))
     A = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision=precision, model=model)
-    rows, _ = A.autotune(reps=10)
-    ms = A.bench_matvec(reps=reps)
+    times = {}
+    # every launch shape of the three modes (the counter rows carry the kernel's template
+    # arguments; the last one is 3 for computed values, 4 for the matrix-free form)
+    names = {0: "stored", 1: "computed_values", 2: "matrix_free"}
+    default = A.info()["matrix_free"]
+    for mode in (1, 2, 0):
+        A.set_matrix_free(mode)
+        for rows in ((1, 2) if mode else (1, 2, 4)):
+            A.set_rows_per_thread(rows)
+            times[f"{names[mode]} rows_per_thread={rows}"] = A.bench_matvec(reps=reps) * 1e3
+    A.set_matrix_free(default)
+    A.set_rows_per_thread(2)
     x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
     y = torch.empty_like(x)
     for _ in range(3):
         torch.mul(x, 1.0, out=y)  # reads 256 MiB, writes 256 MiB
     torch.cuda.synchronize()
     info = A.info()
-    print(json.dumps({"probe": "spmv", "precision": precision, "reps": reps, "rows_per_thread": rows,
-                      "mean_launch_us": ms * 1e3, "csr_bytes_per_matvec": info["csr_bytes_per_matvec"],
-                      "own_bytes_per_matvec": info["bytes_per_matvec"], "copy_bytes_each_way": x.numel() * 4}),
-          flush=True)
+    print(json.dumps({"probe": "spmv", "precision": precision, "reps": reps, "mean_launch_us": times,
+                      "csr_bytes_per_matvec": info["csr_bytes_per_matvec"],
+                      "own_bytes_per_matvec_default_mode": info["bytes_per_matvec"],
+                      "copy_bytes_each_way": x.numel() * 4}), flush=True)
 
 
 def c2_end_to_end(nh, precision, krylov_tol, ts):
@@ -655,6 +666,8 @@ def main():
     ap.add_argument("--spmv-probe", type=int, default=0, metavar="REPS",
                     help="only build the matrix, launch the mat-vec REPS times back to back and run a plain "
                          "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
+    ap.add_argument("--only", choices=["c4", "c5", "mlp"],
+                    help="run one leg alone and print its figures (for profiler runs); no contract line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; runs on more than one GPU are not supported (how "
@@ -668,6 +681,22 @@ def main():
 
     if args.spmv_probe:
         spmv_probe(nh, args.precision, args.spmv_probe)
+        return
+    if args.only == "mlp":
+        # the network kernels alone on 2,097,152 synthetic rows (HIP events per launch)
+        net = nh.AdapterMlp(seed=0)
+        fwd_ms = net.bench(n=1 << 21, reps=20, train=False)
+        trn_ms = net.bench(n=1 << 21, reps=10, train=True)
+        print(json.dumps({"mlp_forward_ms": fwd_ms, "mlp_train_ms": trn_ms,
+                          "forward_TFLOPs": MLP_FLOP_FORWARD * (1 << 21) / (fwd_ms * 1e-3) / 1e12,
+                          "train_TFLOPs": 3 * MLP_FLOP_FORWARD * (1 << 21) / (trn_ms * 1e-3) / 1e12}), flush=True)
+        return
+    if args.only == "c4":
+        print(json.dumps(c4_batch(nh, L, args.krylov_tol, graphs=not args.no_graphs, log=say)), flush=True)
+        return
+    if args.only == "c5":
+        print(json.dumps(c5_learned(nh, plain_levels=C5_PLAIN_LEVELS if args.c5_plain == "all" else (), log=say)),
+              flush=True)
         return
 
     ts = np.arange(0.0, 11.0)
@@ -710,15 +739,31 @@ def main():
 
     # mat-vec roofline: one more solve with HIP events recorded around every mat-vec
     # launch on the solver's stream (graphs off, since events cannot sit inside them)
-    tim = Solver(L, A, args.precision, True, tol=args.krylov_tol)
-    tim.set_state(u0)
-    tim.solve(ts)
-    st = tim.stats()
-    mv_ms, mv_n = st["matvec_ms"], st["matvecs_timed"]
-    tim.free()
+    def timed_solve():
+        tim = Solver(L, A, args.precision, True, tol=args.krylov_tol)
+        tim.set_state(u0)
+        tim.solve(ts)
+        st = tim.stats()
+        tim.free()
+        return (st["matvec_ms"] / st["matvecs_timed"]) * 1e-3 if st["matvecs_timed"] else float("nan"), \
+            st["matvecs_timed"]
+
+    avg_mv_s, mv_n = timed_solve()
     back_to_back_ms = A.bench_matvec(reps=200)
+    mf_on = bool(info["matrix_free"])
+    stored = None
+    if mf_on:
+        # the same kernel reading the stored generator, for comparison
+        A.set_matrix_free(0)
+        s_avg, _ = timed_solve()
+        s_info = A.info()
+        stored = {"avg_launch_us": s_avg * 1e6, "achieved_csr_equivalent": info["csr_bytes_per_matvec"] / s_avg / 1e9,
+                  "bytes_per_launch_own_layout": s_info["bytes_per_matvec"],
+                  "achieved_own_bytes": s_info["bytes_per_matvec"] / s_avg / 1e9,
+                  "traffic": measured_traffic(info["rows_per_thread"], args.precision, 0)}
+        A.set_matrix_free(info["matrix_free"])
     rows_tuned, _ = A.autotune(reps=30)  # reported only: the timed solves use the default shape
-    avg_mv_s = (mv_ms / mv_n) * 1e-3 if mv_n else float("nan")
+    A.set_rows_per_thread(info["rows_per_thread"])
     achieved = info["csr_bytes_per_matvec"] / avg_mv_s / 1e9
     line = {
         "metric": "FSP solves/s at \u21131\u22641e-6 (batched+learned vs plain FP64); SpMV HBM GB/s",
@@ -756,11 +801,15 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": measured_traffic(info["rows_per_thread"], args.precision),
+            "traffic": measured_traffic(info["rows_per_thread"], args.precision, info["matrix_free"]),
             "traffic_source": TRAFFIC_PROFILE.name,
-            "kernel": "k_matvec (FSP SpMV)",
+            "kernel": "k_matvec (FSP SpMV), " + ("computed values: off-diagonal values from per-reaction tables, "
+                                                 "columns from row offsets, the FP64 diagonal read; achieved "
+                                                 "counts the CSR-equivalent bytes" if mf_on else "stored generator"),
             "bytes_per_launch_csr": info["csr_bytes_per_matvec"],
             "bytes_per_launch_own_layout": info["bytes_per_matvec"],
+            "achieved_own_bytes": info["bytes_per_matvec"] / avg_mv_s / 1e9,
+            "stored_generator": stored,
             "avg_launch_us": avg_mv_s * 1e6,
             "launches_timed": mv_n,
             "back_to_back_launch_us": back_to_back_ms * 1e3,

@@ -163,6 +163,9 @@ typedef struct {
   int32_t rows_per_thread; /* mat-vec kernel: consecutive state rows per thread     */
   int32_t computed_col_slots; /* slots whose columns are row + constant offset in
                             every nonzero row, computed instead of loaded          */
+  int32_t matrix_free;   /* mat-vec mode (nc_matrix_set_matrix_free): 0 stored arrays,
+                            1 computed values with the stored diagonal, 2 matrix-free */
+  int32_t matrix_free_available; /* bit 0: mode 1 available, bit 1: mode 2 available */
 } nc_matrix_info_t;
 
 /* FspMatrixSparse{RealT}(space, propensities; parameters) — fspsparsematrix.jl:47-108.
@@ -189,6 +192,21 @@ int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t
  * stored-precision input, FP64 output) and report the mean kernel duration from HIP events
  * on the matrix's stream. No reference counterpart. */
 int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms);
+/* Computed mat-vecs. A generator on a box state space in lattice order (species 0 fastest,
+ * at most 4 species, at most 8 distinct stoichiometries with at most 4 reactions each)
+ * whose propensities are all time-invariant and each read at most one species keeps,
+ * besides its stored arrays, one table per reaction: the propensity over its species'
+ * range, evaluated by the build's own evaluator. In the integrators' mat-vec (FP64 output;
+ * other launches read the stored arrays) mode 1 then takes every off-diagonal value from
+ * the tables and every column from a row offset, reading the stored FP64 diagonal (every
+ * stoichiometry must hold one reaction changing one species). Mode 2 computes the
+ * diagonal as well and reads only v and the sink lists (matrix-free). Both give results
+ * bit-identical to the stored generator's (mode 0, the default: on MI355X it is the
+ * fastest, the computed forms being bound by their index arithmetic rather than by
+ * memory). Modes the generator does not qualify for fail with NC_ERR_UNSUPPORTED. No
+ * reference counterpart (the reference always stores its generator,
+ * fspsparsematrix.jl:47-108). */
+int nc_matrix_set_matrix_free(nc_matrix A, int32_t mode);
 /* Mat-vec launch shape: consecutive state rows per thread (1, 2 or 4). No reference
  * counterpart (the reference's CSC mul! has no launch shape). */
 int nc_matrix_set_rows_per_thread(nc_matrix A, int32_t rows);

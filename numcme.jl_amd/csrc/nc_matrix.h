@@ -54,6 +54,18 @@ struct nc_matrix_s {
   // spaces in lattice order): the mat-vec computes those columns instead of loading them
   uint32_t dia_mask = 0;
   std::vector<int32_t> dia_off;
+  // Computed mat-vecs: on box spaces in lattice order (species 0 fastest) whose
+  // propensities are all time-invariant and each read at most one species, every value is
+  // a lookup in its reaction's table (the propensity over its species' range, evaluated by
+  // the build's own program evaluator) and every column a row offset; mode 1 still reads
+  // the FP64 diagonal, mode 2 computes it too and reads only v (and the sink lists).
+  bool mf_ok = false;     // the build found the space and model eligible
+  bool mf_multi = false;  // some slot holds several reactions or changes several species
+  int mf_mode = 0;        // 0: stored arrays; 1: computed values, stored diagonal (needs
+                          // !mf_multi); 2: matrix-free (values and diagonal computed)
+  nc::DevBuf<double> mf_tab;            // concatenated per-reaction tables
+  nc::DevBuf<unsigned char> mf_info;    // box, groups and table offsets (MfInfo)
+  int64_t mf_table_entries = 0;
   nc::DevBuf<int32_t> states;      // kept for joint refreshes
   nc::DevBuf<int32_t> state_conn;  // kept for joint refreshes
   nc_model_s* model = nullptr;     // borrowed; must outlive the matrix

@@ -55,6 +55,26 @@ __device__ inline double xprog(const Progs& p, int r, double t, const int32_t* x
   return prog_eval(p.code + o, p.offs[2 * r + 1] - o, t, x, p.theta);
 }
 
+// Matrix-free mat-vec metadata (device memory; every kernel read is wave-uniform): the box
+// [lo, hi] in lattice order with species 0 fastest, the reaction groups (one slot each, in
+// slot order), each group leader's stoichiometry and column offset, the species each
+// reaction's propensity reads (-1: none) and where its table starts.
+constexpr int kMfSpecies = 4;
+constexpr int kMfSlots = 8;
+constexpr int kMfMembers = 4;  // reactions per group (one stoichiometry) in the matrix-free mode
+struct MfInfo {
+  int32_t ns, ng;
+  int32_t lo[kMfSpecies], hi[kMfSpecies];
+  int64_t stride[kMfSpecies];
+  int32_t start[kMfSlots + 1];
+  int32_t members[kMaxReactions];
+  int32_t nu[kMfSlots][kMfSpecies];
+  int64_t off[kMfSlots];  // column of the slot's predecessor: i - off
+  int32_t sp[kMaxReactions];
+  int64_t toff[kMaxReactions];
+  int32_t cs[kMfSlots];  // the one species the slot's stoichiometry changes (lean mode)
+};
+
 __global__ void k_build_count(int64_t n, int nr, const int32_t* state_conn, Groups g, TvList tv, int32_t* kmax,
                               unsigned long long* offdiag_total, unsigned long long* tv_pred) {
   // Grid-stride over the states with a bounded grid, reduced over the workgroup: one atomic
@@ -358,6 +378,8 @@ struct MatArgs {
   const int32_t* chunk_first;     // nr + 1: first chunk of each sink row
   double* chunk_part;
   unsigned* chunk_arrivals;       // chunk workgroups done in this launch, zero between launches
+  const MfInfo* mf;               // matrix-free mode (DM 3): metadata and tables
+  const double* mf_tab;
 };
 
 template <typename T, int R>
@@ -375,6 +397,245 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Computed-value rows (mode 1): the slot values come from the tables and the columns from
+// row offsets, as in the matrix-free form below, but the FP64 diagonal is read from the
+// stored generator. Every slot holds one reaction whose stoichiometry changes one species
+// (cs[k] by nu[k]), so a slot costs one comparison, one index and one table lookup. Same
+// arithmetic, and the same result bit for bit, as the stored kernel.
+template <typename VT, typename OT, int KT, int R>
+__device__ inline void mf_rows_values(const MatArgs<VT>& a, const VT* __restrict__ v, OT* __restrict__ out,
+                                      int64_t i0, int accumulate) {
+  const MfInfo* mi = a.mf;
+  const int ns = mi->ns;
+  const uint32_t n32 = (uint32_t)a.n, i032 = (uint32_t)i0;
+  int32_t lo[kMfSpecies], hi[kMfSpecies];
+#pragma unroll
+  for (int sx = 0; sx < kMfSpecies; ++sx) {
+    lo[sx] = sx < ns ? mi->lo[sx] : 0;
+    hi[sx] = sx < ns ? mi->hi[sx] : 0;
+  }
+  int32_t x[kMfSpecies];
+  {
+    uint32_t rem = i032;
+#pragma unroll
+    for (int sx = 0; sx < kMfSpecies; ++sx) {
+      x[sx] = 0;
+      if (sx + 1 < ns) {
+        const uint32_t e = (uint32_t)(hi[sx] - lo[sx] + 1), qt = rem / e;
+        x[sx] = lo[sx] + (int32_t)(rem - qt * e);
+        rem = qt;
+      } else if (sx + 1 == ns) {
+        x[sx] = lo[sx] + (int32_t)rem;
+      }
+    }
+  }
+  uint32_t cc[R][KT], iin[R][KT];
+  bool in[R][KT];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int cs = mi->cs[k];
+      const int m = mi->members[mi->start[k]];
+      const int spm = mi->sp[m];
+      const int32_t nu = mi->nu[k][cs];
+      int32_t xc = x[0], xm = x[0];
+#pragma unroll
+      for (int sx = 1; sx < kMfSpecies; ++sx) {
+        if (cs == sx) xc = x[sx];
+        if (spm == sx) xm = x[sx];
+      }
+      int32_t lc = lo[0], hc = hi[0], lm = lo[0], wm = hi[0] - lo[0];
+#pragma unroll
+      for (int sx = 1; sx < kMfSpecies; ++sx) {
+        if (cs == sx) lc = lo[sx], hc = hi[sx];
+        if (spm == sx) lm = lo[sx], wm = hi[sx] - lo[sx];
+      }
+      const int32_t yc = xc - nu;
+      const uint32_t i = i032 + r;
+      in[r][k] = yc >= lc && yc <= hc;
+      cc[r][k] = in[r][k] ? i - (uint32_t)mi->off[k] : i;
+      const int32_t ym = spm == cs ? yc : xm;
+      iin[r][k] = spm < 0 ? 0u : (uint32_t)min(max(ym - lm, 0), wm);  // clamped: no predecessor adds nothing
+    }
+    x[0] += 1;
+#pragma unroll
+    for (int sx = 0; sx + 1 < kMfSpecies; ++sx)
+      if (sx + 1 < ns && x[sx] > hi[sx]) {
+        x[sx] = lo[sx];
+        x[sx + 1] += 1;
+      }
+  }
+  double tin[R][KT], gv[R][KT], vi[R], dg[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t ic = min(i032 + r, n32 - 1);
+    vi[r] = (double)v[ic];
+    dg[r] = a.diag[ic];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      gv[r][k] = (double)v[cc[r][k]];
+      tin[r][k] = a.mf_tab[mi->toff[mi->members[mi->start[k]]] + iin[r][k]];
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t i = i032 + r;
+    if (i >= n32) continue;
+    double acc = __dmul_rn(dg[r], vi[r]);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) acc = __fma_rn(in[r][k] ? (double)(VT)tin[r][k] : 0.0, gv[r][k], acc);
+    out[i] = (OT)(accumulate ? (double)out[i] + acc : acc);
+  }
+}
+
+// Matrix-free rows [i0, i0 + R) of a box space in lattice order (species 0 fastest), MM
+// members per reaction group at most. The coordinates of row i0 come from one division per
+// species, those of the next rows from a step of species 0 with carries. Every index of
+// every row is computed first, then every table value and every gather is loaded in one
+// batch (indices clamped, unused members weighted out), then the arithmetic runs in the
+// build's order: the predecessor entry is the group's FP64 sum rounded to VT, the outflow
+// is added per member for leaks and sinks and per group otherwise, and the row sum is
+// diag * v_i followed by one fused multiply-add per slot, as in the stored kernel; the
+// result equals the stored generator's bit for bit.
+template <typename VT, typename OT, int KT, int R, int MM>
+__device__ inline void mf_rows(const MatArgs<VT>& a, const VT* __restrict__ v, OT* __restrict__ out, int64_t i0,
+                               int accumulate) {
+  const MfInfo* mi = a.mf;
+  const int ns = mi->ns;
+  int32_t lo[kMfSpecies], hi[kMfSpecies], nu[KT][kMfSpecies];
+  int32_t off[KT];
+  int32_t cnt[KT], spm[KT][MM];
+  const double* tb[KT][MM];
+#pragma unroll
+  for (int sx = 0; sx < kMfSpecies; ++sx) {
+    lo[sx] = sx < ns ? mi->lo[sx] : 0;
+    hi[sx] = sx < ns ? mi->hi[sx] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    off[k] = (int32_t)mi->off[k];
+#pragma unroll
+    for (int sx = 0; sx < kMfSpecies; ++sx) nu[k][sx] = sx < ns ? mi->nu[k][sx] : 0;
+    const int q0 = mi->start[k];
+    cnt[k] = mi->start[k + 1] - q0;
+#pragma unroll
+    for (int q = 0; q < MM; ++q) {
+      const int m = mi->members[q < cnt[k] ? q0 + q : q0];
+      spm[k][q] = mi->sp[m];
+      tb[k][q] = a.mf_tab + mi->toff[m];
+    }
+  }
+  // 32-bit index arithmetic: eligible spaces hold fewer than 2^31 states
+  const int32_t n32 = (int32_t)a.n, i032 = (int32_t)i0;
+  int32_t x[kMfSpecies];
+  {
+    uint32_t rem = (uint32_t)i032;
+#pragma unroll
+    for (int sx = 0; sx < kMfSpecies; ++sx) {
+      x[sx] = 0;
+      if (sx + 1 < ns) {
+        const uint32_t e = (uint32_t)(hi[sx] - lo[sx] + 1), qt = rem / e;
+        x[sx] = lo[sx] + (int32_t)(rem - qt * e);
+        rem = qt;
+      } else if (sx + 1 == ns) {
+        x[sx] = lo[sx] + (int32_t)rem;  // the slowest species needs no remainder
+      }
+    }
+  }
+  // indices and flags of every row
+  int32_t cc[R][KT];
+  int32_t iin[R][KT][MM], iout[R][KT][MM];
+  bool in[R][KT], leak[R][KT], sink[R][KT], live[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int32_t i = i032 + r;
+    live[r] = i < n32;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      bool pin = true, nonneg = true, sin = true;
+      int32_t y[kMfSpecies];
+#pragma unroll
+      for (int sx = 0; sx < kMfSpecies; ++sx) {
+        y[sx] = x[sx];
+        if (sx < ns && nu[k][sx] != 0) {
+          y[sx] = x[sx] - nu[k][sx];
+          const int32_t z = x[sx] + nu[k][sx];
+          pin = pin && y[sx] >= lo[sx] && y[sx] <= hi[sx];
+          nonneg = nonneg && z >= 0;
+          sin = sin && z >= lo[sx] && z <= hi[sx];
+        }
+      }
+      in[r][k] = pin;
+      leak[r][k] = !nonneg;
+      sink[r][k] = nonneg && !sin;
+      cc[r][k] = pin ? i - off[k] : i;
+#pragma unroll
+      for (int q = 0; q < MM; ++q) {
+        int32_t ys = 0, xs = 0, l = 0, h = 0;
+#pragma unroll
+        for (int sx = 0; sx < kMfSpecies; ++sx)
+          if (sx == spm[k][q]) ys = y[sx], xs = x[sx], l = lo[sx], h = hi[sx];
+        iin[r][k][q] = min(max(ys - l, 0), h - l);  // clamped: without a predecessor it adds nothing
+        iout[r][k][q] = min(max(xs - l, 0), h - l);
+      }
+    }
+    // the next row's coordinates
+    x[0] += 1;
+#pragma unroll
+    for (int sx = 0; sx + 1 < kMfSpecies; ++sx)
+      if (sx + 1 < ns && x[sx] > hi[sx]) {
+        x[sx] = lo[sx];
+        x[sx + 1] += 1;
+      }
+  }
+  // every load in one batch
+  double tin[R][KT][MM], tout[R][KT][MM], gv[R][KT], vi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    vi[r] = (double)v[min(i032 + r, n32 - 1)];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      gv[r][k] = (double)v[cc[r][k]];  // in the box: the row itself or its predecessor
+#pragma unroll
+      for (int q = 0; q < MM; ++q) {
+        tin[r][k][q] = tb[k][q][iin[r][k][q]];
+        tout[r][k][q] = tb[k][q][iout[r][k][q]];
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!live[r]) continue;
+    double d = 0.0;
+    VT wv[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      double ain = 0.0, aout = 0.0;
+#pragma unroll
+      for (int q = 0; q < MM; ++q) {
+        if (q >= cnt[k]) break;
+        ain += tin[r][k][q];
+        if (leak[r][k])
+          d += tout[r][k][q];  // no successor and no sink: the mass leaks
+        else if (sink[r][k])
+          d += (double)(VT)tout[r][k][q];
+        else
+          aout += tout[r][k][q];
+      }
+      if (!leak[r][k] && !sink[r][k]) d += (double)(VT)aout;
+      wv[k] = in[r][k] ? (VT)ain : (VT)0;
+    }
+    double acc = __dmul_rn(-d, vi[r]);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) acc = __fma_rn((double)wv[k], gv[r][k], acc);
+    const int32_t i = i032 + r;
+    out[i] = (OT)(accumulate ? (double)out[i] + acc : acc);
+  }
+}
+
 // Thread t owns state rows [R*t, R*t+R). Matrix streams are read with one R-wide vector
 // load per slot; v[i] and the predecessor gathers are scalar.
 template <typename VT, typename OT, int KT, int R, int DM>
@@ -390,6 +651,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
     const int64_t t = lb * kBlock + threadIdx.x;
     const int64_t i0 = t * R;
     if (i0 >= a.n) return;
+    if constexpr (DM == 3) {
+      mf_rows_values<VT, OT, KT, R>(a, v, out, i0, accumulate);
+      return;
+    } else if constexpr (DM == 4) {
+      mf_rows<VT, OT, KT, R, kMfMembers>(a, v, out, i0, accumulate);
+      return;
+    }
     // Loads are unconditional (clamped indices, selects) and issued in two batches: the
     // row's own entries and every slot's column and value, then every gather. A load
     // under a branch is waited for at the end of its block, which would make the slots
@@ -438,21 +706,23 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
         for (int r = 0; r < R; ++r) gv[k][r] = (double)v[c[k].v[r]];
       }
       __builtin_amdgcn_sched_barrier(0);
+      // diag * v_i, then one fused multiply-add per slot in slot order (written out, so
+      // that every mode of the kernel rounds alike)
 #pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * vi[r];
+      for (int r = 0; r < R; ++r) acc[r] = __dmul_rn(dg.v[r], vi[r]);
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] += (double)w[k].v[r] * gv[k][r];
+        for (int r = 0; r < R; ++r) acc[r] = __fma_rn((double)w[k].v[r], gv[k][r], acc[r]);
       }
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] = dg.v[r] * vi[r];
+      for (int r = 0; r < R; ++r) acc[r] = __dmul_rn(dg.v[r], vi[r]);
       for (int k = 0; k < K; ++k) {
         const Pack<int32_t, R> c = slot_cols(k);
         const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] += (double)w.v[r] * (double)v[c.v[r]];
+        for (int r = 0; r < R; ++r) acc[r] = __fma_rn((double)w.v[r], (double)v[c.v[r]], acc[r]);
       }
     }
     for (int q = 0; q < a.ntv; ++q) {
@@ -586,6 +856,32 @@ void dispatch_k(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, O
     launch_one<VT, OT, 0, R, 2>(c, a, vv, oo, acc);
 }
 
+// Launches of the computed modes (FP64 output, 1 or 2 rows per thread; DM 3: computed
+// values and the stored diagonal; DM 4: fully matrix-free).
+template <typename VT, int R, int DM>
+void dispatch_mf(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, double* oo, int acc) {
+  switch (K) {
+    case 1: launch_one<VT, double, 1, R, DM>(c, a, vv, oo, acc); break;
+    case 2: launch_one<VT, double, 2, R, DM>(c, a, vv, oo, acc); break;
+    case 3: launch_one<VT, double, 3, R, DM>(c, a, vv, oo, acc); break;
+    case 4: launch_one<VT, double, 4, R, DM>(c, a, vv, oo, acc); break;
+    case 5: launch_one<VT, double, 5, R, DM>(c, a, vv, oo, acc); break;
+    case 6: launch_one<VT, double, 6, R, DM>(c, a, vv, oo, acc); break;
+    case 7: launch_one<VT, double, 7, R, DM>(c, a, vv, oo, acc); break;
+    default: launch_one<VT, double, 8, R, DM>(c, a, vv, oo, acc); break;
+  }
+}
+
+template <typename VT>
+void dispatch_mf_r(int R, bool full, int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, double* oo,
+                   int acc) {
+  // full: the matrix-free form (DM 4); otherwise computed values with the stored diagonal (DM 3)
+  if (R == 1)
+    full ? dispatch_mf<VT, 1, 4>(K, c, a, vv, oo, acc) : dispatch_mf<VT, 1, 3>(K, c, a, vv, oo, acc);
+  else
+    full ? dispatch_mf<VT, 2, 4>(K, c, a, vv, oo, acc) : dispatch_mf<VT, 2, 3>(K, c, a, vv, oo, acc);
+}
+
 template <typename VT, typename OT>
 void dispatch_r(int R, int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
   if (R == 4)
@@ -622,20 +918,186 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   a.chunk_first = A->chunk_first.p;
   a.chunk_arrivals = A->chunk_arrivals.p;
   for (size_t k = 0; k < A->dia_off.size() && k < (size_t)kMaxReactions; ++k) a.dia_off[k] = A->dia_off[k];
+  a.mf = A->mf_mode ? reinterpret_cast<const MfInfo*>(A->mf_info.p) : nullptr;
+  a.mf_tab = A->mf_mode ? A->mf_tab.p : nullptr;
   const nc_model_s* m = A->model;
   for (int r = 0; r < A->nr; ++r) a.sink_scale[r] = m->kinds[r] == NC_PROP_SEPARABLE ? m->tfactor(r, t) : 1.0;
   for (size_t q = 0; q < A->tv_r.size(); ++q)
     a.tv_scale[q] = A->tv_kind[q] == NC_PROP_SEPARABLE ? m->tfactor(A->tv_r[q], t) : 1.0;
-  const int R = A->rows_per_thread;
+  // the matrix-free kernel takes 1 or 2 rows per thread (4 runs as 2)
+  const int R = (out_fp64 && A->mf_mode && A->rows_per_thread == 4) ? 2 : A->rows_per_thread;
   const int64_t row_threads = (A->n + R - 1) / R;
   const int64_t row_blocks = (row_threads + kBlock - 1) / kBlock;
   const LaunchCfg c{(unsigned)(row_blocks + (A->n_chunks ? A->n_chunks : A->nr)), s, e0, e1};
   const VT* vv = reinterpret_cast<const VT*>(v);
   const int acc = accumulate ? 1 : 0;
-  if (out_fp64)
+  if (out_fp64 && A->mf_mode)
+    dispatch_mf_r<VT>(R, A->mf_mode == 2, A->K, c, a, vv, reinterpret_cast<double*>(out), acc);
+  else if (out_fp64)
     dispatch_r<VT, double>(R, A->K, c, a, vv, reinterpret_cast<double*>(out), acc);
   else
     dispatch_r<VT, VT>(R, A->K, c, a, vv, reinterpret_cast<VT*>(out), acc);
+}
+
+// ---- matrix-free setup --------------------------------------------------------------
+// lohi[2s] = min, lohi[2s + 1] = max of species s over the states (grid-stride; one atomic
+// pair per species and workgroup)
+__global__ __launch_bounds__(kBlock) void k_mf_bounds(int64_t n, int ns, const int32_t* states, int32_t* lohi) {
+  __shared__ int32_t slo[kMfSpecies][kBlock / 64], shi[kMfSpecies][kBlock / 64];
+  int32_t lo[kMfSpecies], hi[kMfSpecies];
+  for (int sx = 0; sx < kMfSpecies; ++sx) lo[sx] = INT32_MAX, hi[sx] = INT32_MIN;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    for (int sx = 0; sx < kMfSpecies; ++sx)
+      if (sx < ns) {
+        const int32_t v = states[i * ns + sx];
+        lo[sx] = min(lo[sx], v);
+        hi[sx] = max(hi[sx], v);
+      }
+  for (int sx = 0; sx < kMfSpecies; ++sx) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[sx] = min(lo[sx], __shfl_xor(lo[sx], off, 64));
+      hi[sx] = max(hi[sx], __shfl_xor(hi[sx], off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) slo[sx][threadIdx.x >> 6] = lo[sx], shi[sx][threadIdx.x >> 6] = hi[sx];
+  }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)ns) {
+    const int sx = threadIdx.x;
+    int32_t l = slo[sx][0], h = shi[sx][0];
+    for (int w = 1; w < kBlock / 64; ++w) l = min(l, slo[sx][w]), h = max(h, shi[sx][w]);
+    atomicMin(lohi + 2 * sx, l);
+    atomicMax(lohi + 2 * sx + 1, h);
+  }
+}
+
+// *bad = 1 when some state is not at its lattice position (species 0 fastest)
+__global__ void k_mf_check(int64_t n, const int32_t* states, const MfInfo* mi, unsigned* bad) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ns = mi->ns;
+  uint64_t rem = (uint64_t)i;
+  bool ok = true;
+  for (int sx = 0; sx < ns; ++sx) {
+    const uint64_t e = (uint64_t)(mi->hi[sx] - mi->lo[sx] + 1);
+    ok = ok && states[i * ns + sx] == mi->lo[sx] + (int32_t)(rem % e);
+    rem /= e;
+  }
+  if (!ok) atomicOr(bad, 1u);
+}
+
+// Entry e of the tables: reaction r's propensity at coordinate lo + (e - toff[r]) of its
+// species (every other species at its lower bound; the program does not read them), or its
+// constant value (one entry) when it reads no species.
+__global__ void k_mf_tables(int64_t total, int nr, const MfInfo* mi, Progs pr, double* tab) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  int r = 0;
+  while (r + 1 < nr && mi->toff[r + 1] <= e) ++r;
+  int32_t x[kMaxSpecies];
+  for (int sx = 0; sx < mi->ns; ++sx) x[sx] = mi->lo[sx];
+  const int spm = mi->sp[r];
+  if (spm >= 0) x[spm] = mi->lo[spm] + (int32_t)(e - mi->toff[r]);
+  tab[e] = xprog(pr, r, 0.0, x);
+}
+
+// Decides matrix-free eligibility after the stored build and, when eligible, builds the
+// tables and turns the mode on.
+void mf_setup(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m, const Groups& g) {
+  A->mf_ok = false;
+  A->mf_mode = 0;
+  const int ns = sp->ns, nr = sp->nr;
+  const int64_t n = A->n;
+  if (n == 0 || n >= (int64_t)INT32_MAX || ns > kMfSpecies || A->K != g.ng || A->K < 1 || A->K > kMfSlots ||
+      !A->tv_r.empty())
+    return;
+  A->mf_multi = false;
+  for (int k = 0; k < g.ng; ++k) {
+    if (g.start[k + 1] - g.start[k] > kMfMembers) return;
+    A->mf_multi = A->mf_multi || g.start[k + 1] - g.start[k] > 1;
+  }
+  // every propensity reads at most one species
+  std::vector<int32_t> spr(nr, -1);
+  for (int r = 0; r < nr; ++r) {
+    const nc_instr* c = m->xprog(r);
+    for (int64_t k = 0; k < m->xlen(r); ++k) {
+      if (c[k].op != NC_OP_SPECIES) continue;
+      if (spr[r] < 0)
+        spr[r] = c[k].arg;
+      else if (spr[r] != c[k].arg)
+        return;
+    }
+  }
+  hipStream_t s = A->stream.s;
+  DevBuf<int32_t> lohi(2 * kMfSpecies);
+  int32_t* h = static_cast<int32_t*>(host_stage(5));
+  for (int sx = 0; sx < kMfSpecies; ++sx) h[2 * sx] = INT32_MAX, h[2 * sx + 1] = INT32_MIN;
+  NC_CHECK_HIP(hipMemcpyAsync(lohi.p, h, sizeof(int32_t) * 2 * kMfSpecies, hipMemcpyHostToDevice, s));
+  k_mf_bounds<<<std::min<unsigned>(grid_for(n, kBlock), 1024), kBlock, 0, s>>>(n, ns, sp->states.p, lohi.p);
+  NC_CHECK_HIP(hipGetLastError());
+  NC_CHECK_HIP(hipMemcpyAsync(h, lohi.p, sizeof(int32_t) * 2 * kMfSpecies, hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipStreamSynchronize(s));
+  MfInfo info{};
+  info.ns = ns;
+  info.ng = g.ng;
+  double cells = 1.0;
+  int64_t stride = 1;
+  for (int sx = 0; sx < ns; ++sx) {
+    info.lo[sx] = h[2 * sx];
+    info.hi[sx] = h[2 * sx + 1];
+    info.stride[sx] = stride;
+    const int64_t e = (int64_t)info.hi[sx] - info.lo[sx] + 1;
+    cells *= (double)e;
+    stride *= e;
+  }
+  if (cells != (double)n) return;  // not a full box
+  for (int k = 0; k <= g.ng; ++k) info.start[k] = g.start[k];
+  for (int q = 0; q < g.start[g.ng]; ++q) info.members[q] = g.members[q];
+  for (int k = 0; k < g.ng; ++k) {
+    int64_t off = 0;
+    int changed = 0;
+    info.cs[k] = 0;
+    for (int sx = 0; sx < ns; ++sx) {
+      info.nu[k][sx] = m->stoich[sx + g.leader[k] * ns];
+      off += (int64_t)info.nu[k][sx] * info.stride[sx];
+      if (info.nu[k][sx] != 0) {
+        info.cs[k] = sx;
+        ++changed;
+      }
+    }
+    info.off[k] = off;
+    // the lean form needs one changed species per slot (and one reaction per slot)
+    if (changed != 1) A->mf_multi = true;
+  }
+  int64_t total = 0;
+  for (int r = 0; r < nr; ++r) {
+    info.sp[r] = spr[r];
+    info.toff[r] = total;
+    total += spr[r] < 0 ? 1 : (int64_t)info.hi[spr[r]] - info.lo[spr[r]] + 1;
+  }
+  A->mf_info.alloc(sizeof(MfInfo));
+  NC_CHECK_HIP(hipMemcpyAsync(A->mf_info.p, &info, sizeof(MfInfo), hipMemcpyHostToDevice, s));
+  DevBuf<unsigned> bad(1);
+  NC_CHECK_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
+  const MfInfo* dinfo = reinterpret_cast<const MfInfo*>(A->mf_info.p);
+  k_mf_check<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, sp->states.p, dinfo, bad.p);
+  A->mf_tab.alloc((size_t)total);
+  Progs pr{m->d_code.p, m->d_offs.p, m->d_params.p};
+  k_mf_tables<<<grid_for(total, kBlock), kBlock, 0, s>>>(total, nr, dinfo, pr, A->mf_tab.p);
+  NC_CHECK_HIP(hipGetLastError());
+  unsigned* hb = reinterpret_cast<unsigned*>(h);
+  NC_CHECK_HIP(hipMemcpyAsync(hb, bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  NC_CHECK_HIP(hipStreamSynchronize(s));  // info (a stack copy) and the tables are complete
+  if (*hb) {
+    A->mf_tab.release();
+    A->mf_info.release();
+    return;  // not in lattice order
+  }
+  A->mf_table_entries = total;
+  A->mf_ok = true;
+  // opt-in: on MI355X the stored kernel (columns computed, values and diagonal streamed) is
+  // faster than both computed forms, which are bound by their index and select arithmetic
+  // (C2's 1024^2 toggle: 8.9 us stored, 12.7 us computed values, 41 us matrix-free)
+  A->mf_mode = 0;
 }
 
 template <typename VT>
@@ -859,12 +1321,16 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   }
   for (int q = 0; q < tv.n; ++q) nnz += n + (int64_t)counts[2 + q] + sinks_per_r[tv.r[q]];
   A->nnz_ref = nnz;
+  mf_setup(A, sp, m, g);
 }
 
 }  // namespace
 
 double matrix_bytes_per_matvec(const nc_matrix_s* A) {
   const double vs = (double)A->vsize;
+  if (A->mf_mode)  // computed values: v, out, the sink lists and the tables, plus the diagonal in mode 1
+    return (double)A->n_sink_entries * (4.0 + vs) + (double)(A->nr + 1) * 8.0 + 2.0 * (double)A->n_rows * vs +
+           8.0 * (double)A->mf_table_entries + (A->mf_mode == 1 ? 8.0 * (double)A->n : 0.0);
   const double n = (double)A->n;
   double b = n * 8.0                                  // diag
              + n * A->K * (4.0 + vs)                  // ELL slots
@@ -1144,6 +1610,22 @@ int nc_matrix_info(nc_matrix A, nc_matrix_info_t* info) {
     info->norm1 = A->norm1;
     info->rows_per_thread = A->rows_per_thread;
     info->computed_col_slots = __builtin_popcount(A->dia_mask);
+    info->matrix_free = A->mf_mode;
+    info->matrix_free_available = A->mf_ok ? (A->mf_multi ? 2 : 3) : 0;
+  });
+}
+
+int nc_matrix_set_matrix_free(nc_matrix A, int32_t mode) {
+  return guard([&] {
+    NC_REQUIRE(A, NC_ERR_ARG, "null matrix");
+    NC_REQUIRE(mode >= 0 && mode <= 2, NC_ERR_ARG, "mode must be 0 (stored), 1 (computed values) or 2 (matrix-free)");
+    NC_REQUIRE(mode == 0 || A->mf_ok, NC_ERR_UNSUPPORTED,
+               "no computed mat-vec for this generator: it needs a box state space in lattice order (species 0 "
+               "fastest, at most 4 species) and time-invariant propensities that each read at most one species");
+    NC_REQUIRE(mode != 1 || !A->mf_multi, NC_ERR_UNSUPPORTED,
+               "computed values need every stoichiometry to hold one reaction and change one species");
+    if (mode != A->mf_mode) A->launch_gen++;
+    A->mf_mode = mode;
   });
 }
 

@@ -70,6 +70,8 @@ class nc_matrix_info_t(C.Structure):
         ("norm1", C.c_double),
         ("rows_per_thread", C.c_int32),
         ("computed_col_slots", C.c_int32),
+        ("matrix_free", C.c_int32),
+        ("matrix_free_available", C.c_int32),
     ]
 
 
@@ -162,6 +164,7 @@ SIGNATURES = {
     "nc_matrix_export": [P, D, PI64, PI64, PI64, PD],
     "nc_matrix_bench_matvec": [P, D, I32, PD],
     "nc_matrix_set_rows_per_thread": [P, I32],
+    "nc_matrix_set_matrix_free": [P, I32],
     "nc_matrix_autotune": [P, I32, PD],
     "nc_solve_opts_default": [C.POINTER(nc_solve_opts)],
     "nc_solver_create": [P, C.POINTER(nc_solve_opts), C.POINTER(P)],

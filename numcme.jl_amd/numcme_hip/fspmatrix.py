@@ -101,6 +101,14 @@ class FspMatrixSparse(L.Handle):
     def set_rows_per_thread(self, rows):
         L.call("nc_matrix_set_rows_per_thread", self.handle, int(rows))
 
+    def set_matrix_free(self, mode):
+        """Mat-vec mode: 0 reads the stored generator; 1 takes the off-diagonal values from
+        per-reaction tables and the columns from row offsets (box spaces in lattice order,
+        time-invariant single-species propensities); 2 computes the diagonal too
+        (matrix-free). Results are bit-identical across modes; 0 is the default (the
+        fastest on MI355X)."""
+        L.call("nc_matrix_set_matrix_free", self.handle, int(mode))
+
     def autotune(self, reps=50):
         """Pick the fastest mat-vec launch shape for this matrix; returns (rows, ms)."""
         ms = C.c_double()
