@@ -61,9 +61,11 @@ def toggle_model(nh):
 
 
 def box_states(n):
-    a = np.arange(n)
-    s1, s2 = np.meshgrid(a, a, indexing="xy")  # species 0 fastest
-    return np.stack([s1.ravel(), s2.ravel()], axis=1)
+    """0..n-1 squared, species 0 fastest"""
+    out = np.empty((n * n, 2), dtype=np.int64)
+    out[:, 0] = np.tile(np.arange(n), n)
+    out[:, 1] = np.repeat(np.arange(n), n)
+    return out
 
 
 class Solver:
@@ -264,21 +266,32 @@ def c3_sensitivity(nh, L):
 C4_TABLE = ROOT / "tests" / "golden" / "c4_instances.json"
 
 
+_C4_MODELS = {}
+
+
 def c4_instance(nh, spec, precision):
-    """Space, generator and initial vector of one batched instance (telegraph or toggle)."""
+    """Space, generator and initial vector of one batched instance (telegraph or toggle).
+    The two networks are traced once; each instance takes its parameter values
+    (CmeModel.with_parameters) and uploads its own device model."""
     th = spec["theta"]
     nmax = spec["nmax"]
+    if spec["kind"] not in _C4_MODELS:
+        if spec["kind"] == "telegraph":
+            S = np.array([[-1, 1, 0, 0], [1, -1, 0, 0], [0, 0, 1, -1]])
+            props = [nh.propensity(lambda x, p: p[0] * x[0]), nh.propensity(lambda x, p: p[1] * x[1]),
+                     nh.propensity(lambda x, p: p[2] * x[1]), nh.propensity(lambda x, p: p[3] * x[2])]
+            _C4_MODELS["telegraph"] = nh.CmeModel(S, props, th)
+        else:
+            _C4_MODELS["toggle"] = toggle_model(nh).with_parameters(th)
+    model = _C4_MODELS[spec["kind"]].with_parameters(th)
     if spec["kind"] == "telegraph":
-        S = np.array([[-1, 1, 0, 0], [1, -1, 0, 0], [0, 0, 1, -1]])
-        props = [nh.propensity(lambda x, p: p[0] * x[0]), nh.propensity(lambda x, p: p[1] * x[1]),
-                 nh.propensity(lambda x, p: p[2] * x[1]), nh.propensity(lambda x, p: p[3] * x[2])]
-        states = np.array([[1, 0, m] for m in range(nmax + 1)] + [[0, 1, m] for m in range(nmax + 1)])
+        m = np.arange(nmax + 1)
+        one, zero = np.ones_like(m), np.zeros_like(m)
+        states = np.concatenate([np.stack([one, zero, m], axis=1), np.stack([zero, one, m], axis=1)])
     else:
-        model = toggle_model(nh)
-        S, props = model.stoich_matrix, model.propensities
         states = box_states(nmax + 1)
-    sp = nh.StateSpaceSparse(S, states)
-    A = nh.FspMatrixSparse(sp, props, th, precision=precision)
+    sp = nh.StateSpaceSparse(model.stoich_matrix, states)
+    A = nh.FspMatrixSparse(sp, model.propensities, model.parameters, precision=precision, model=model)
     u0 = np.zeros(len(states) + S.shape[1])
     u0[0] = 1.0  # telegraph (G_off, 0) / toggle (0, 0): the first state of the list
     return A, u0

@@ -48,11 +48,31 @@ struct Progs {
   const nc_instr* code;
   const int64_t* offs;  // 2*nr+1
   const double* theta;
+  int32_t reg;          // every program fits prog_eval_reg's stack
+  int32_t ns;
 };
 
 __device__ inline double xprog(const Progs& p, int r, double t, const int32_t* x) {
   const int64_t o = p.offs[2 * r];
   return prog_eval(p.code + o, p.offs[2 * r + 1] - o, t, x, p.theta);
+}
+
+// xprog with the species values from xs(index): the register evaluator when every program
+// fits it, else the array evaluator on a copy of the state
+template <bool REG, class XS>
+__device__ inline double xprog_f(const Progs& p, int r, double t, XS xs) {
+  const int64_t o = p.offs[2 * r];
+  if constexpr (REG) {
+    return prog_eval_reg(p.code + o, p.offs[2 * r + 1] - o, t, xs, p.theta);
+  } else {
+    int32_t x[kMaxSpecies];
+    for (int sx = 0; sx < p.ns; ++sx) x[sx] = xs(sx);
+    return prog_eval(p.code + o, p.offs[2 * r + 1] - o, t, x, p.theta);
+  }
+}
+
+Progs make_progs(const nc_model_s* m) {
+  return Progs{m->d_code.p, m->d_offs.p, m->d_params.p, m->max_depth <= kRegStack ? 1 : 0, m->ns};
 }
 
 // Matrix-free mat-vec metadata (device memory; every kernel read is wave-uniform): the box
@@ -144,7 +164,7 @@ __global__ void k_build_count(int64_t n, int nr, const int32_t* state_conn, Grou
 // are unchanged, and on lattice-ordered spaces each slot's column is row + constant.
 // Otherwise a row's entries are packed into its first slots.
 // One row of the build; returns |diag| (0 for a padding row).
-template <typename VT>
+template <typename VT, bool REG>
 __device__ double build_fill_row(int64_t i, int64_t n, int64_t ld, int ns, int nr, int K, int positional,
                                  const int32_t* states, const int32_t* state_conn, const uint32_t* sink_mask,
                                  const int32_t* stoich, const Groups& g, const TvList& tv, const Progs& pr,
@@ -155,8 +175,8 @@ __device__ double build_fill_row(int64_t i, int64_t n, int64_t ld, int ns, int n
     for (int q = 0; q < tv.n; ++q) tvd[q * ld + i] = (VT)0, tvv[q * ld + i] = (VT)0, tvc[q * ld + i] = 0;
     return 0.0;
   }
-  int32_t x[kMaxSpecies], y[kMaxSpecies];
-  for (int s = 0; s < ns; ++s) x[s] = states[i * ns + s];
+  const int32_t* xrow = states + i * ns;
+  auto xs = [&](int a) { return xrow[a]; };
   const uint32_t smask = sink_mask[i];
   int slot = 0;
   double d = 0.0;
@@ -165,9 +185,9 @@ __device__ double build_fill_row(int64_t i, int64_t n, int64_t ld, int ns, int n
     // incoming entry of this row from the group's predecessor x - s_g
     const int32_t j = state_conn[i * nr + lead];
     if (j >= 0) {
-      for (int s = 0; s < ns; ++s) y[s] = x[s] - stoich[s + lead * ns];
+      auto ys = [&](int a) { return xrow[a] - stoich[a + lead * ns]; };
       double a = 0.0;
-      for (int m = g.start[k]; m < g.start[k + 1]; ++m) a += xprog(pr, g.members[m], 0.0, y);
+      for (int m = g.start[k]; m < g.start[k + 1]; ++m) a += xprog_f<REG>(pr, g.members[m], 0.0, ys);
       col[slot * ld + i] = j;
       val[slot * ld + i] = (VT)a;
       ++slot;
@@ -179,15 +199,15 @@ __device__ double build_fill_row(int64_t i, int64_t n, int64_t ld, int ns, int n
     // this row's outflow through the group: the diagonal is minus the stored column
     // entries, so each stored column sums to zero exactly in real arithmetic
     bool nonneg = true;
-    for (int s = 0; s < ns; ++s) nonneg = nonneg && (x[s] + stoich[s + lead * ns] >= 0);
+    for (int s = 0; s < ns; ++s) nonneg = nonneg && (xrow[s] + stoich[s + lead * ns] >= 0);
     if (!nonneg) {
       // no successor and no sink: the mass leaks, as in the reference (:129-132)
-      for (int m = g.start[k]; m < g.start[k + 1]; ++m) d += xprog(pr, g.members[m], 0.0, x);
+      for (int m = g.start[k]; m < g.start[k + 1]; ++m) d += xprog_f<REG>(pr, g.members[m], 0.0, xs);
     } else if (smask & (1u << lead)) {
-      for (int m = g.start[k]; m < g.start[k + 1]; ++m) d += (double)(VT)xprog(pr, g.members[m], 0.0, x);
+      for (int m = g.start[k]; m < g.start[k + 1]; ++m) d += (double)(VT)xprog_f<REG>(pr, g.members[m], 0.0, xs);
     } else {
       double a = 0.0;
-      for (int m = g.start[k]; m < g.start[k + 1]; ++m) a += xprog(pr, g.members[m], 0.0, x);
+      for (int m = g.start[k]; m < g.start[k + 1]; ++m) a += xprog_f<REG>(pr, g.members[m], 0.0, xs);
       d += (double)(VT)a;
     }
   }
@@ -201,10 +221,10 @@ __device__ double build_fill_row(int64_t i, int64_t n, int64_t ld, int ns, int n
     const int r = tv.r[q];
     const int32_t j = state_conn[i * nr + r];
     if (tv.kind[q] == NC_PROP_SEPARABLE) {
-      tvd[q * ld + i] = -(VT)xprog(pr, r, 0.0, x);
+      tvd[q * ld + i] = -(VT)xprog_f<REG>(pr, r, 0.0, xs);
       if (j >= 0) {
-        for (int s = 0; s < ns; ++s) y[s] = x[s] - stoich[s + r * ns];
-        tvv[q * ld + i] = (VT)xprog(pr, r, 0.0, y);
+        auto ys = [&](int a) { return xrow[a] - stoich[a + r * ns]; };
+        tvv[q * ld + i] = (VT)xprog_f<REG>(pr, r, 0.0, ys);
       } else {
         tvv[q * ld + i] = (VT)0;
       }
@@ -219,8 +239,8 @@ __device__ double build_fill_row(int64_t i, int64_t n, int64_t ld, int ns, int n
 
 // The build's rows, one per thread, and the largest |diag| (nonnegative doubles order as
 // their bit patterns), reduced over the wavefront before its one atomic.
-template <typename VT>
-__global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int positional, const int32_t* states,
+template <typename VT, bool REG>
+__global__ __launch_bounds__(kBlock) void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int positional, const int32_t* states,
                              const int32_t* state_conn, const uint32_t* sink_mask, const int32_t* stoich, Groups g,
                              TvList tv, Progs pr, double* diag, int32_t* col, VT* val, VT* tvd, int32_t* tvc,
                              VT* tvv, unsigned long long* diag_max_bits) {
@@ -229,7 +249,7 @@ __global__ void k_build_fill(int64_t n, int64_t ld, int ns, int nr, int K, int p
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double dabs = 0.0;
   if (i < ld)
-    dabs = build_fill_row<VT>(i, n, ld, ns, nr, K, positional, states, state_conn, sink_mask, stoich, g, tv, pr, diag,
+    dabs = build_fill_row<VT, REG>(i, n, ld, ns, nr, K, positional, states, state_conn, sink_mask, stoich, g, tv, pr, diag,
                               col, val, tvd, tvc, tvv);
   for (int off = 32; off > 0; off >>= 1) dabs = fmax(dabs, __shfl_down(dabs, off, 64));
   __shared__ double s_d[kBlock / 64];
@@ -305,18 +325,17 @@ __global__ void k_sink_flags(int64_t n, int nr, const uint32_t* sink_mask, uint3
   flags[t] = (sink_mask[j] >> r) & 1u;
 }
 
-template <typename VT>
-__global__ void k_sink_fill(int64_t n, int ns, int nr, const int32_t* states, const uint32_t* flags,
+template <typename VT, bool REG>
+__global__ __launch_bounds__(kBlock) void k_sink_fill(int64_t n, int ns, int nr, const int32_t* states, const uint32_t* flags,
                             const uint32_t* pos, const int32_t* kinds, Progs pr, int32_t* sink_src, VT* sink_val) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n * nr || !flags[t]) return;
   const int r = (int)(t / n);
   const int64_t j = t % n;
-  int32_t x[kMaxSpecies];
-  for (int s = 0; s < ns; ++s) x[s] = states[j * ns + s];
+  const int32_t* xrow = states + j * ns;
   const uint32_t e = pos[t];
   sink_src[e] = (int32_t)j;
-  sink_val[e] = kinds[r] == NC_PROP_JOINT ? (VT)0 : (VT)xprog(pr, r, 0.0, x);
+  sink_val[e] = kinds[r] == NC_PROP_JOINT ? (VT)0 : (VT)xprog_f<REG>(pr, r, 0.0, [&](int a) { return xrow[a]; });
 }
 
 // _update_sparsematrix! (fspsparsematrix.jl:154-166) for every joint reaction at time t.
@@ -1010,6 +1029,10 @@ void mf_setup(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m, const Groups& g) {
   if (n == 0 || n >= (int64_t)INT32_MAX || ns > kMfSpecies || A->K != g.ng || A->K < 1 || A->K > kMfSlots ||
       !A->tv_r.empty())
     return;
+  // a box in lattice order has every slot's column at one offset (checked by the build's
+  // column detection): anything else needs no further look
+  const uint32_t all_slots = A->K >= 32 ? 0xffffffffu : ((1u << A->K) - 1u);
+  if ((A->dia_mask & all_slots) != all_slots) return;
   A->mf_multi = false;
   for (int k = 0; k < g.ng; ++k) {
     if (g.start[k + 1] - g.start[k] > kMfMembers) return;
@@ -1081,7 +1104,7 @@ void mf_setup(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m, const Groups& g) {
   const MfInfo* dinfo = reinterpret_cast<const MfInfo*>(A->mf_info.p);
   k_mf_check<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, sp->states.p, dinfo, bad.p);
   A->mf_tab.alloc((size_t)total);
-  Progs pr{m->d_code.p, m->d_offs.p, m->d_params.p};
+  const Progs pr = make_progs(m);
   k_mf_tables<<<grid_for(total, kBlock), kBlock, 0, s>>>(total, nr, dinfo, pr, A->mf_tab.p);
   NC_CHECK_HIP(hipGetLastError());
   unsigned* hb = reinterpret_cast<unsigned*>(h);
@@ -1110,7 +1133,7 @@ void joint_refresh(nc_matrix_s* A, double t) {
   TvList tv{};
   tv.n = (int32_t)A->tv_r.size();
   for (int q = 0; q < tv.n; ++q) tv.r[q] = A->tv_r[q], tv.kind[q] = A->tv_kind[q];
-  Progs pr{m->d_code.p, m->d_offs.p, m->d_params.p};
+  const Progs pr = make_progs(m);
   hipStream_t s = A->stream.s;
   if (A->n)
     k_joint_refresh<VT><<<grid_for(A->n, kBlock), kBlock, 0, s>>>(
@@ -1195,14 +1218,16 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   A->tvd.alloc(std::max<int64_t>((int64_t)tv.n * ld, 1) * vs);
   A->tvv.alloc(std::max<int64_t>((int64_t)tv.n * ld, 1) * vs);
   A->tvc.alloc(std::max<int64_t>((int64_t)tv.n * ld, 1));
-  Progs pr{m->d_code.p, m->d_offs.p, m->d_params.p};
+  const Progs pr = make_progs(m);
   DevBuf<unsigned long long> d_dmax(1);
   NC_CHECK_HIP(hipMemsetAsync(d_dmax.p, 0, sizeof(unsigned long long), s));
-  if (ld)
-    k_build_fill<VT><<<grid_for(ld, kBlock), kBlock, 0, s>>>(
-        n, ld, ns, nr, K, K == g.ng ? 1 : 0, sp->states.p, sp->state_conn.p, sp->sink_mask.p, sp->d_stoich.p, g, tv, pr, A->diag.p,
-        A->col.p, reinterpret_cast<VT*>(A->val.p), reinterpret_cast<VT*>(A->tvd.p), A->tvc.p,
+  if (ld) {
+    auto fill = pr.reg ? k_build_fill<VT, true> : k_build_fill<VT, false>;
+    fill<<<grid_for(ld, kBlock), kBlock, 0, s>>>(
+        n, ld, ns, nr, K, K == g.ng ? 1 : 0, sp->states.p, sp->state_conn.p, sp->sink_mask.p, sp->d_stoich.p, g, tv, pr,
+        A->diag.p, A->col.p, reinterpret_cast<VT*>(A->val.p), reinterpret_cast<VT*>(A->tvd.p), A->tvc.p,
         reinterpret_cast<VT*>(A->tvv.p), d_dmax.p);
+  }
   NC_CHECK_HIP(hipGetLastError());
   // slots with one column offset in every nonzero row (read back with the final sync)
   const int kd = std::min<int>(K, kMaxReactions);
@@ -1282,10 +1307,11 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   A->sink_val.alloc(std::max<uint32_t>(total, 1) * vs);
   DevBuf<int32_t> d_kinds(nr);
   NC_CHECK_HIP(hipMemcpyAsync(d_kinds.p, m->kinds.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, s));
-  if (n && total)
-    k_sink_fill<VT><<<grid_for(n * nr, kBlock), kBlock, 0, s>>>(n, ns, nr, sp->states.p, flags.p, pos.p, d_kinds.p,
-                                                                pr, A->sink_src.p,
-                                                                reinterpret_cast<VT*>(A->sink_val.p));
+  if (n && total) {
+    auto sfill = pr.reg ? k_sink_fill<VT, true> : k_sink_fill<VT, false>;
+    sfill<<<grid_for(n * nr, kBlock), kBlock, 0, s>>>(n, ns, nr, sp->states.p, flags.p, pos.p, d_kinds.p, pr,
+                                                      A->sink_src.p, reinterpret_cast<VT*>(A->sink_val.p));
+  }
   NC_CHECK_HIP(hipGetLastError());
   bool any_joint = false;
   for (int k : A->tv_kind) any_joint = any_joint || k == NC_PROP_JOINT;

@@ -48,6 +48,80 @@ __host__ __device__ inline double prog_eval(const nc_instr* code, int64_t len, d
   return sp > 0 ? st[sp - 1] : 0.0;
 }
 
+// Register-resident evaluator for programs whose stack stays within kRegStack entries (the
+// model records its deepest program). A program's instructions are the same for every lane
+// of a wave, so the stack pointer is wave-uniform: the stack slots are eight named
+// registers picked by scalar branches, and nothing goes to per-lane scratch memory (the
+// array stack of prog_eval does, once per inlined call). Species values come from xs(index).
+// The same operations in the same order as prog_eval, so the same results.
+constexpr int kRegStack = 8;
+template <class XS>
+__device__ inline double prog_eval_reg(const nc_instr* code, int64_t len, double t, XS xs, const double* theta) {
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0, r4 = 0.0, r5 = 0.0, r6 = 0.0, r7 = 0.0;
+  auto get = [&](int k) -> double {
+    switch (k) {
+      case 0: return r0;
+      case 1: return r1;
+      case 2: return r2;
+      case 3: return r3;
+      case 4: return r4;
+      case 5: return r5;
+      case 6: return r6;
+      default: return r7;
+    }
+  };
+  auto put = [&](int k, double v) {
+    switch (k) {
+      case 0: r0 = v; break;
+      case 1: r1 = v; break;
+      case 2: r2 = v; break;
+      case 3: r3 = v; break;
+      case 4: r4 = v; break;
+      case 5: r5 = v; break;
+      case 6: r6 = v; break;
+      default: r7 = v; break;
+    }
+  };
+  int sp = 0;
+  for (int64_t k = 0; k < len; ++k) {
+    const nc_instr in = code[k];
+    switch (in.op) {
+      case NC_OP_CONST: put(sp++, in.c); break;
+      case NC_OP_PARAM: put(sp++, theta[in.arg]); break;
+      case NC_OP_SPECIES: put(sp++, (double)xs(in.arg)); break;
+      case NC_OP_TIME: put(sp++, t); break;
+      case NC_OP_ADD: { const double b = get(--sp); put(sp - 1, get(sp - 1) + b); } break;
+      case NC_OP_SUB: { const double b = get(--sp); put(sp - 1, get(sp - 1) - b); } break;
+      case NC_OP_MUL: { const double b = get(--sp); put(sp - 1, get(sp - 1) * b); } break;
+      case NC_OP_DIV: { const double b = get(--sp); put(sp - 1, get(sp - 1) / b); } break;
+      case NC_OP_POW: { const double b = get(--sp); put(sp - 1, pow(get(sp - 1), b)); } break;
+      case NC_OP_NEG: put(sp - 1, -get(sp - 1)); break;
+      case NC_OP_EXP: put(sp - 1, exp(get(sp - 1))); break;
+      case NC_OP_LOG: put(sp - 1, log(get(sp - 1))); break;
+      case NC_OP_SIN: put(sp - 1, sin(get(sp - 1))); break;
+      case NC_OP_COS: put(sp - 1, cos(get(sp - 1))); break;
+      case NC_OP_SQRT: put(sp - 1, sqrt(get(sp - 1))); break;
+      case NC_OP_ABS: put(sp - 1, fabs(get(sp - 1))); break;
+      case NC_OP_MAX: { const double b = get(--sp), a = get(sp - 1); put(sp - 1, (b < a) ? a : b); } break;
+      case NC_OP_MIN: { const double b = get(--sp), a = get(sp - 1); put(sp - 1, (b < a) ? b : a); } break;
+      case NC_OP_LT: { const double b = get(--sp); put(sp - 1, (get(sp - 1) < b) ? 1.0 : 0.0); } break;
+      case NC_OP_LE: { const double b = get(--sp); put(sp - 1, (get(sp - 1) <= b) ? 1.0 : 0.0); } break;
+      case NC_OP_GT: { const double b = get(--sp); put(sp - 1, (get(sp - 1) > b) ? 1.0 : 0.0); } break;
+      case NC_OP_GE: { const double b = get(--sp); put(sp - 1, (get(sp - 1) >= b) ? 1.0 : 0.0); } break;
+      case NC_OP_SELECT: {
+        sp -= 2;
+        const double c = get(sp - 1), a = get(sp), b = get(sp + 1);
+        put(sp - 1, (c != 0.0) ? a : b);
+      } break;
+      default: break;
+    }
+  }
+  return sp > 0 ? get(sp - 1) : 0.0;
+}
+
+// Deepest stack a (valid) program reaches.
+int prog_max_depth(const nc_instr* code, int64_t len);
+
 // Host-side validation: stack discipline, operand indices. `allow_x` / `allow_t` say
 // whether the program may read species / time. Empty programs are allowed only when
 // `allow_empty`.
@@ -68,6 +142,7 @@ struct nc_model_s {
   nc::DevBuf<int64_t> d_offs;
   nc::DevBuf<double> d_params;
   // gradients (CmeModelWithSensitivity)
+  int max_depth = 0;  // deepest stack of the propensity programs (prog_eval_reg up to kRegStack)
   bool has_grad = false;
   std::vector<uint8_t> pattern;  // nr x np
   std::vector<int64_t> doffs;    // 2*nr*np+1

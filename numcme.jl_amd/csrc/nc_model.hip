@@ -1,6 +1,8 @@
 // Model object: stoichiometry, parameters and propensity programs (CmeModel,
 // src/cmemodel/cmemodel.jl:49-53; propensity kinds src/cmemodel/propensity.jl:50-153;
 // gradients src/cmemodel/propensitygrad.jl:16-67).
+#include <algorithm>
+
 #include "nc_model.h"
 #include "nc_space.h"
 
@@ -44,6 +46,21 @@ void prog_validate(const nc_instr* code, int64_t len, int ns, int np, bool allow
     NC_REQUIRE(depth <= NC_PROG_STACK, NC_ERR_ARG, w + ": stack deeper than NC_PROG_STACK");
   }
   NC_REQUIRE(depth == 1, NC_ERR_ARG, w + ": program must leave exactly one value");
+}
+
+int prog_max_depth(const nc_instr* code, int64_t len) {
+  int depth = 0, peak = 0;
+  for (int64_t k = 0; k < len; ++k) {
+    switch (code[k].op) {
+      case NC_OP_CONST: case NC_OP_PARAM: case NC_OP_SPECIES: case NC_OP_TIME: ++depth; break;
+      case NC_OP_NEG: case NC_OP_EXP: case NC_OP_LOG: case NC_OP_SIN: case NC_OP_COS: case NC_OP_SQRT: case NC_OP_ABS:
+        break;
+      case NC_OP_SELECT: depth -= 2; break;
+      default: --depth; break;  // binary operations
+    }
+    peak = std::max(peak, depth);
+  }
+  return peak;
 }
 
 namespace {
@@ -101,6 +118,8 @@ int nc_model_create(int32_t ns, int32_t nr, const int32_t* stoich, int32_t np, c
       m->kinds.assign(kinds, kinds + nr);
       m->offs.assign(offs, offs + 2 * nr + 1);
       m->code.assign(code, code + total);
+      for (int k = 0; k < 2 * nr; ++k)
+        m->max_depth = std::max(m->max_depth, prog_max_depth(code + offs[k], offs[k + 1] - offs[k]));
       m->d_stoich.alloc((size_t)ns * nr);
       NC_CHECK_HIP(hipMemcpy(m->d_stoich.p, stoich, sizeof(int32_t) * ns * nr, hipMemcpyHostToDevice));
       m->d_code.alloc(std::max<int64_t>(total, 1));

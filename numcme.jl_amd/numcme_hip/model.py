@@ -183,6 +183,7 @@ class CmeModel:
         self.parameters = np.asarray(parameters, dtype=np.float64).ravel()
         self.factor_joint = bool(factor_joint)
         self._progs = None
+        self._flat = None
         self._handle = None
 
     def get_parameters(self):
@@ -207,7 +208,27 @@ class CmeModel:
         return (f"Stochastic reaction network with {self.get_species_count()} species, "
                 f"{self.get_reaction_count()} reactions and {self.get_parameter_count()} parameters.")
 
+    def with_parameters(self, parameters):
+        """The same network with other parameter values (a parameter sweep): the traced
+        programs do not depend on the values, so they are shared, not traced again."""
+        theta = np.asarray(parameters, dtype=np.float64).ravel()
+        if len(theta) != self.get_parameter_count():
+            raise ArgumentError(L.NC_ERR_ARG, "with_parameters needs as many values as the model has parameters")
+        m = CmeModel.__new__(CmeModel)
+        m.stoich_matrix, m.propensities, m.factor_joint = self.stoich_matrix, self.propensities, self.factor_joint
+        m.parameters = theta
+        m._progs = self.programs
+        m._flat = self._flatten()
+        m._handle = None
+        return m
+
     # ---- device model -----------------------------------------------------------------
+    def _flatten(self):
+        if getattr(self, "_flat", None) is None:
+            pr = self.programs
+            self._flat = _Programs.flatten(list(zip(pr.xexpr, pr.texpr)))
+        return self._flat
+
     @property
     def programs(self):
         if self._progs is None:
@@ -219,7 +240,7 @@ class CmeModel:
         """The library model handle (created on first use, kept with this object)."""
         if self._handle is None:
             pr = self.programs
-            offs, code = _Programs.flatten(list(zip(pr.xexpr, pr.texpr)))
+            offs, code = self._flatten()
             S = L.i32(self.stoich_matrix.T.copy())  # column-major ns x nr == row-major nr x ns
             theta = L.f64(self.parameters if len(self.parameters) else np.zeros(1))
             kinds = L.i32(pr.kinds)

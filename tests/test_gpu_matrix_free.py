@@ -46,7 +46,8 @@ def test_toggle_box_bit_identical_and_vs_oracle(gpu, precision, bounds):
     A.set_matrix_free(1)
     assert A.info()["bytes_per_matvec"] < 0.5 * inf["csr_bytes_per_matvec"]
     A.set_matrix_free(2)
-    assert A.info()["bytes_per_matvec"] < 0.2 * inf["csr_bytes_per_matvec"]
+    # what is left is reading v and writing out (16 of the CSR form's ~80 bytes a row in FP64)
+    assert A.info()["bytes_per_matvec"] < 0.25 * inf["csr_bytes_per_matvec"]
     A.set_matrix_free(0)
     rng = np.random.default_rng(len(states))
     v = rng.standard_normal(A.rowcount)

@@ -280,3 +280,21 @@ def test_factored_joint_gradients_vs_finite_differences():
             tm[k] -= h
             fd = (np.asarray(law(tt, cols, tp), float) - np.asarray(law(tt, cols, tm), float)) / (2 * h)
             assert np.allclose(g[k], fd, rtol=1e-6, atol=1e-10), (tt, k)
+
+
+def test_with_parameters_shares_programs():
+    # a parameter sweep re-uses the traced programs; only the values change
+    S, props, th = models.toggle()
+    m = nh.CmeModel(S, models.to_product(props), th)
+    th2 = np.asarray(th, float) * 1.5
+    m2 = m.with_parameters(th2)
+    assert m2.programs is m.programs and np.array_equal(m2.parameters, th2)
+    assert np.array_equal(m.parameters, np.asarray(th, float))
+    X = np.array([[0, 0], [3, 7], [40, 2]])
+    for r, (_, f) in enumerate(props):
+        for x in X:
+            ref = f(x, th2)
+            got = run_postfix(E.compile_expr(m2.programs.xexpr[r]), 0.0, x, th2)
+            assert abs(got - ref) <= 4 * np.finfo(float).eps * abs(ref)
+    with pytest.raises(nh.ArgumentError):
+        m.with_parameters(th2[:-1])
