@@ -54,6 +54,7 @@ struct nc_matrix_s {
   // spaces in lattice order): the mat-vec computes those columns instead of loading them
   uint32_t dia_mask = 0;
   std::vector<int32_t> dia_off;
+  int64_t max_reach = 0;  // largest |column - row| over the slots' nonzero entries
   // Computed mat-vecs: on box spaces in lattice order (species 0 fastest) whose
   // propensities are all time-invariant and each read at most one species, every value is
   // a lookup in its reaction's table (the propensity over its species' range, evaluated by

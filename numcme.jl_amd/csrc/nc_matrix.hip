@@ -1325,9 +1325,11 @@ void build_t(nc_matrix_s* A, nc_space_s* sp, nc_model_s* m) {
   NC_CHECK_HIP(hipStreamSynchronize(s));
   const unsigned long long dmax_bits = *dmax_h;
   A->dia_mask = 0;
+  A->max_reach = 0;
   A->dia_off.assign(kd, 0);
   for (int k = 0; n && k < kd; ++k) {
     const int32_t lo = dia_h[2 * k], hi = dia_h[2 * k + 1];
+    if (lo <= hi) A->max_reach = std::max<int64_t>(A->max_reach, std::max(std::abs((int64_t)lo), std::abs((int64_t)hi)));
     if (lo > hi || lo == hi) {  // no nonzero entry, or one offset throughout
       A->dia_mask |= 1u << k;
       A->dia_off[k] = lo > hi ? 0 : lo;

@@ -39,6 +39,8 @@ struct nc_sensmatrix_s {
   nc::DevBuf<unsigned char> meta_dev;  // per-pair metadata read by the kernel
   nc::DevBuf<unsigned char> terms_dev;  // derivative terms of the mat-vec's row part, by wave
   int32_t tstart[5] = {0, 0, 0, 0, 0};  // terms of wave w: tstart[w] .. tstart[w+1)
+  int32_t bstart[5] = {0, 0, 0, 0, 0};  // vector blocks of wave w: bstart[w] .. bstart[w+1)
+  int32_t fold = 1;                     // row-tile fold of the mat-vec's launch
   nc::DevBuf<unsigned char> ddiag, dpred, dsink;
   nc::DevBuf<int32_t> dcol;
   nc::DevBuf<int32_t> states, state_conn;  // kept for joint refreshes

@@ -43,6 +43,7 @@ template <typename VT>
 struct SensArgs {
   int64_t n, ld, vstride, ostride;
   int32_t K, ntv, nr, nb;
+  int32_t fold;  // tile fold (1: none)
   const double* diag;
   const int32_t* col;
   const VT* val;
@@ -59,7 +60,10 @@ struct SensArgs {
   const PairMeta* meta;  // pairs sorted by parameter
   const Term* terms;     // derivative terms of the row part, grouped by wave
   int32_t tstart[kBlock / 64 + 1];  // terms of wave w: tstart[w] .. tstart[w+1)
+  int32_t bstart[kBlock / 64 + 1];  // blocks of wave w: bstart[w] .. bstart[w+1)
   int32_t pstart[kMaxSensBlocks + 1];  // pairs of block b (b >= 1): pstart[b] .. pstart[b+1)
+  uint32_t dia_mask;                   // bit k: slot k's column is i + dia_off[k]
+  int32_t dia_off[kMaxReactions];
   double tv_scale[kMaxReactions];
   double sink_scale[kMaxReactions];
   double dscale[kMaxPairs];   // multiplies the pair's state-factor derivative term
@@ -71,19 +75,155 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// The row part of one wave: blocks b0 .. b0 + M - 1 of the 64 rows i of its tile. Every
+// load count is exact (no padded slot, block or term): the kernel is bound by how many
+// loads it keeps in flight, so a padded load costs as much as a useful one. The loads of a
+// row go out in batches whose addresses do not depend on each other (the slot columns and
+// values with the row's own entries, then every gather for them; the time-varying slots two
+// at a time; the derivative terms up to four at a time).
+template <typename VT, int M>
+__device__ __forceinline__ void sens_tv(const SensArgs<VT>& a, const VT* const (&vb)[M], const double (&vi)[M],
+                                        double (&acc)[M], int q, int nq, int64_t i) {
+  const int64_t ld = a.ld;
+  if (nq == 2) {
+    int32_t c[2];
+    double d[2], w[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      c[u] = a.tvc[(q + u) * ld + i];
+      d[u] = (double)a.tvd[(q + u) * ld + i];
+      w[u] = (double)a.tvv[(q + u) * ld + i];
+    }
+    double g[2][M];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int kk = 0; kk < M; ++kk) g[u][kk] = (double)vb[kk][c[u]];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int kk = 0; kk < M; ++kk) acc[kk] += a.tv_scale[q + u] * (d[u] * vi[kk] + w[u] * g[u][kk]);
+  } else {
+    const int32_t c = a.tvc[q * ld + i];
+    const double d = (double)a.tvd[q * ld + i], w = (double)a.tvv[q * ld + i];
+    double g[M];
+#pragma unroll
+    for (int kk = 0; kk < M; ++kk) g[kk] = (double)vb[kk][c];
+#pragma unroll
+    for (int kk = 0; kk < M; ++kk) acc[kk] += a.tv_scale[q] * (d * vi[kk] + w * g[kk]);
+  }
+}
+
+// NT derivative terms dA_p v_0 (pair p's block is the wave's accumulator T.kk)
+template <typename VT, int M, int NT>
+__device__ __forceinline__ void sens_terms(const SensArgs<VT>& a, const Term* s_terms, const VT* __restrict__ v,
+                                           double v0, double (&acc)[M], int t0, int64_t i) {
+  const int64_t ld = a.ld;
+  int32_t c[NT], kk_of[NT];
+  double d[NT], w[NT], g[NT], sc[NT];
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const Term T = s_terms[t0 + u];
+    const bool st = T.kind == 0;
+    sc[u] = st ? a.dscale[T.pair] : a.sfscale[T.pair];
+    kk_of[u] = T.kk;
+    const int32_t* cp = st ? a.dcol + (int64_t)T.cslot * ld : a.tvc + (int64_t)T.idx * ld;
+    const VT* dp = st ? a.ddiag + (int64_t)T.idx * ld : a.tvd + (int64_t)T.idx * ld;
+    const VT* wp = st ? a.dpred + (int64_t)T.idx * ld : a.tvv + (int64_t)T.idx * ld;
+    c[u] = cp[i];
+    d[u] = (double)dp[i];
+    w[u] = (double)wp[i];
+  }
+#pragma unroll
+  for (int u = 0; u < NT; ++u) g[u] = (double)v[c[u]];
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const double add = sc[u] * (d[u] * v0 + w[u] * g[u]);
+#pragma unroll
+    for (int kk = 0; kk < M; ++kk) acc[kk] += kk == kk_of[u] ? add : 0.0;
+  }
+}
+
+template <typename VT, int M, int K>
+__device__ __forceinline__ void sens_rows(const SensArgs<VT>& a, const Term* s_terms, const VT* __restrict__ v,
+                                          double* __restrict__ out, int wave, int64_t i) {
+  const int64_t ld = a.ld, vs = a.vstride;
+  const int b0 = a.bstart[wave];
+  const VT* vb[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) vb[k] = v + (int64_t)(b0 + k) * vs;
+  double acc[M], vi[M];
+  const double dg = a.diag[i];
+  const double v0 = (double)v[i];
+#pragma unroll
+  for (int k = 0; k < M; ++k) vi[k] = (double)vb[k][i];
+  // the slot's column: computed (i + dia_off, clamped: rows whose value is 0 may point
+  // anywhere) or loaded
+  auto col = [&](int k) -> int32_t {
+    if ((a.dia_mask >> k) & 1u) {
+      const int64_t j = i + a.dia_off[k];
+      return (int32_t)(j < 0 ? 0 : (j >= a.n ? a.n - 1 : j));
+    }
+    return a.col[k * ld + i];
+  };
+  if constexpr (K > 0) {
+    int32_t c[K];
+    double w[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      c[k] = col(k);
+      w[k] = (double)a.val[k * ld + i];
+    }
+#pragma unroll
+    for (int kk = 0; kk < M; ++kk) acc[kk] = dg * vi[kk];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+      for (int kk = 0; kk < M; ++kk) acc[kk] += w[k] * (double)vb[kk][c[k]];
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < M; ++kk) acc[kk] = dg * vi[kk];
+    for (int k = 0; k < a.K; ++k) {
+      const int32_t c = col(k);
+      const double w = (double)a.val[k * ld + i];
+#pragma unroll
+      for (int kk = 0; kk < M; ++kk) acc[kk] += w * (double)vb[kk][c];
+    }
+  }
+  int q = 0;
+  for (; q + 2 <= a.ntv; q += 2) sens_tv<VT, M>(a, vb, vi, acc, q, 2, i);
+  if (q < a.ntv) sens_tv<VT, M>(a, vb, vi, acc, q, 1, i);
+  int t = a.tstart[wave];
+  const int t1 = a.tstart[wave + 1];
+  for (; t + 4 <= t1; t += 4) sens_terms<VT, M, 4>(a, s_terms, v, v0, acc, t, i);
+  switch (t1 - t) {
+    case 3: sens_terms<VT, M, 3>(a, s_terms, v, v0, acc, t, i); break;
+    case 2: sens_terms<VT, M, 2>(a, s_terms, v, v0, acc, t, i); break;
+    case 1: sens_terms<VT, M, 1>(a, s_terms, v, v0, acc, t, i); break;
+    default: break;
+  }
+#pragma unroll
+  for (int kk = 0; kk < M; ++kk) out[(int64_t)(b0 + kk) * a.ostride + i] = acc[kk];
+}
+
 // One workgroup per tile of 64 rows; its four waves share the tile's rows and split the
-// P + 1 vector blocks (wave w owns blocks w, w + 4, ...), so a thread holds MBW <= 8
-// block accumulators instead of P + 1. A's slot arrays are read by the four waves of one
-// compute unit (one HBM read, the rest from its caches). A row's loads are issued in
-// batches whose addresses do not depend on each other (all slot columns and values, then
-// every gather for them; the derivative terms TB at a time), so a thread waits for a few
-// memory round trips instead of one per slot and term. The derivative terms dA_p v_0 of a
-// wave's blocks come from a host-built list (Term). The nb x nr sink rows come first in
-// the grid, one workgroup each, so that their longer reductions overlap the row tiles.
-template <typename VT, int MBW, int K>
+// P + 1 vector blocks in contiguous chunks (MH = ceil((P + 1) / 4) blocks for the first
+// waves, MH - 1 for the rest), so a thread holds at most 8 block accumulators instead of
+// P + 1. A's slot arrays are read by the four waves of one compute unit (one HBM read, the
+// rest from its caches). The derivative terms dA_p v_0 of a wave's blocks come from a
+// host-built list (Term), staged in LDS. The nb x nr sink rows come first in the grid, one
+// workgroup each, so that their longer reductions overlap the row tiles.
+//
+// Tile order: tiles go to the XCDs in contiguous ranges (blockIdx.x modulo 8 selects the
+// XCD), and within a range a fold F > 1 interleaves the F segments of tiles / F tiles each,
+// so that rows one segment apart (a slow species' neighbours, such as the gene states of a
+// one-hot model) are processed by the same XCD at about the same time and their gathers
+// hit its L2 instead of going back to memory.
+template <typename VT, int MH, int K>
 __global__ __launch_bounds__(kBlock) void k_sens_matvec(SensArgs<VT> a, const VT* __restrict__ v,
                                                         double* __restrict__ out) {
-  constexpr int W = kBlock / 64, KK = K > 0 ? K : 1, TB = 4;
+  constexpr int W = kBlock / 64;
   const int nb = a.nb;
   const int64_t nsink = (int64_t)nb * a.nr, nsink8 = (nsink + 7) & ~int64_t(7);
   if ((int64_t)blockIdx.x >= nsink8) {
@@ -93,97 +233,20 @@ __global__ __launch_bounds__(kBlock) void k_sens_matvec(SensArgs<VT> a, const VT
     for (int t = threadIdx.x; t < nterms; t += kBlock) s_terms[t] = a.terms[t];
     __syncthreads();
     const int64_t tiles = (a.n + 63) / 64;
-    // XCD-contiguous tiles, as in the plain mat-vec (the sink slots are padded to a
-    // multiple of 8 so that b and blockIdx.x share their XCD)
     const int64_t b = (int64_t)blockIdx.x - nsink8, q8 = tiles >> 3, r8 = tiles & 7, x = b & 7;
-    const int64_t tile = x * q8 + (x < r8 ? x : r8) + (b >> 3);
+    int64_t tile = x * q8 + (x < r8 ? x : r8) + (b >> 3);
+    if (a.fold > 1) {
+      const int64_t seg = tiles / a.fold;  // a.fold * seg <= tiles; the rest keep their place
+      if (tile < a.fold * seg) tile = (tile % a.fold) * seg + tile / a.fold;
+    }
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t i = tile * 64 + (threadIdx.x & 63);
     if (i >= a.n) return;
-    const int64_t ld = a.ld, vs = a.vstride;
-    // Every load below is unconditional (block indices clamped to nb - 1, term and
-    // time-varying slot indices to their last entry, padding weighted by zero), so the
-    // loads of a batch are issued back to back; accumulators beyond nb are never stored.
-    const VT* vb[MBW];
-#pragma unroll
-    for (int k = 0; k < MBW; ++k) vb[k] = v + (int64_t)min(wave + W * k, nb - 1) * vs;
-    double acc[MBW], vi[MBW];
-    const double dg = a.diag[i];
-    const double v0 = (double)v[i];
-#pragma unroll
-    for (int k = 0; k < MBW; ++k) vi[k] = (double)vb[k][i];
-#pragma unroll
-    for (int k = 0; k < MBW; ++k) acc[k] = dg * vi[k];
-    if constexpr (K > 0) {
-      int32_t c[KK];
-      double w[KK];
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        c[k] = a.col[k * ld + i];
-        w[k] = (double)a.val[k * ld + i];
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-#pragma unroll
-        for (int kk = 0; kk < MBW; ++kk) acc[kk] += w[k] * (double)vb[kk][c[k]];
-      }
-    } else {
-      for (int k = 0; k < a.K; ++k) {
-        const int32_t c = a.col[k * ld + i];
-        const double w = (double)a.val[k * ld + i];
-#pragma unroll
-        for (int kk = 0; kk < MBW; ++kk) acc[kk] += w * (double)vb[kk][c];
-      }
-    }
-    for (int q0 = 0; q0 < a.ntv; q0 += TB) {
-      int32_t c[TB];
-      double d[TB], w[TB], sq[TB];
-#pragma unroll
-      for (int u = 0; u < TB; ++u) {
-        const int q = min(q0 + u, a.ntv - 1);
-        sq[u] = q0 + u < a.ntv ? a.tv_scale[q] : 0.0;
-        c[u] = a.tvc[q * ld + i];
-        d[u] = (double)a.tvd[q * ld + i];
-        w[u] = (double)a.tvv[q * ld + i];
-      }
-#pragma unroll
-      for (int u = 0; u < TB; ++u) {
-#pragma unroll
-        for (int kk = 0; kk < MBW; ++kk) acc[kk] += sq[u] * (d[u] * vi[kk] + w[u] * (double)vb[kk][c[u]]);
-      }
-    }
-    // dA_p v_0 into the blocks p + 1 this wave owns
-    const int t1 = a.tstart[wave + 1];
-    for (int t0 = a.tstart[wave]; t0 < t1; t0 += TB) {
-      int32_t c[TB], kk_of[TB];
-      double d[TB], w[TB], g[TB], sc[TB];
-#pragma unroll
-      for (int u = 0; u < TB; ++u) {
-        const Term T = s_terms[min(t0 + u, t1 - 1)];
-        const bool st = T.kind == 0;
-        sc[u] = t0 + u < t1 ? (st ? a.dscale[T.pair] : a.sfscale[T.pair]) : 0.0;
-        kk_of[u] = T.kk;
-        const int32_t* cp = st ? a.dcol + (int64_t)T.cslot * ld : a.tvc + (int64_t)T.idx * ld;
-        const VT* dp = st ? a.ddiag + (int64_t)T.idx * ld : a.tvd + (int64_t)T.idx * ld;
-        const VT* wp = st ? a.dpred + (int64_t)T.idx * ld : a.tvv + (int64_t)T.idx * ld;
-        c[u] = cp[i];
-        d[u] = (double)dp[i];
-        w[u] = (double)wp[i];
-      }
-#pragma unroll
-      for (int u = 0; u < TB; ++u) g[u] = (double)v[c[u]];
-#pragma unroll
-      for (int u = 0; u < TB; ++u) {
-        const double add = sc[u] * (d[u] * v0 + w[u] * g[u]);
-#pragma unroll
-        for (int kk = 0; kk < MBW; ++kk) acc[kk] += kk == kk_of[u] ? add : 0.0;
-      }
-    }
-#pragma unroll
-    for (int kk = 0; kk < MBW; ++kk) {
-      const int bb = wave + W * kk;
-      if (bb < nb) out[bb * a.ostride + i] = acc[kk];
-    }
+    const int nblk = a.bstart[wave + 1] - a.bstart[wave];
+    if (nblk == MH)
+      sens_rows<VT, MH, K>(a, s_terms, v, out, wave, i);
+    else if constexpr (MH > 1)
+      if (nblk == MH - 1) sens_rows<VT, MH - 1, K>(a, s_terms, v, out, wave, i);
   } else if ((int64_t)blockIdx.x < nsink) {
     // sink row r of block bb: one workgroup, deterministic tree
     const int sb = (int)blockIdx.x;
@@ -354,7 +417,13 @@ void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, 
   a.dsink = reinterpret_cast<const VT*>(SA->dsink.p);
   a.meta = reinterpret_cast<const PairMeta*>(SA->meta_dev.p);
   a.terms = reinterpret_cast<const Term*>(SA->terms_dev.p);
-  for (int w = 0; w <= kBlock / 64; ++w) a.tstart[w] = SA->tstart[w];
+  for (int w = 0; w <= kBlock / 64; ++w) {
+    a.tstart[w] = SA->tstart[w];
+    a.bstart[w] = SA->bstart[w];
+  }
+  a.fold = SA->fold;
+  a.dia_mask = A->dia_mask;
+  for (size_t k = 0; k < A->dia_off.size() && k < (size_t)kMaxReactions; ++k) a.dia_off[k] = A->dia_off[k];
   matrix_scales(A, t, a.sink_scale, a.tv_scale);
   for (int b = 0; b <= SA->P + 1; ++b) a.pstart[b] = SA->pstart[b];
   for (size_t k = 0; k < SA->pairs.size(); ++k) {
@@ -374,8 +443,8 @@ void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, 
       kern<<<grid, kBlock, 0, s>>>(a, vv, out);
   };
   static_assert(kMaxSensBlocks <= 8 * (kBlock / 64), "blocks per wave");
-  auto by_k = [&](auto mbw) {
-    constexpr int M = decltype(mbw)::value;
+  auto by_k = [&](auto mh) {
+    constexpr int M = decltype(mh)::value;
     switch (a.K) {
       case 1: go(k_sens_matvec<VT, M, 1>); break;
       case 2: go(k_sens_matvec<VT, M, 2>); break;
@@ -388,14 +457,17 @@ void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, 
       default: go(k_sens_matvec<VT, M, 0>); break;
     }
   };
-  if (a.nb <= 4)
-    by_k(std::integral_constant<int, 1>{});
-  else if (a.nb <= 8)
-    by_k(std::integral_constant<int, 2>{});
-  else if (a.nb <= 16)
-    by_k(std::integral_constant<int, 4>{});
-  else
-    by_k(std::integral_constant<int, 8>{});
+  // the most blocks a wave owns (the dia_mask bits of slots past kMaxReactions stay clear)
+  switch ((a.nb + 3) / 4) {
+    case 1: by_k(std::integral_constant<int, 1>{}); break;
+    case 2: by_k(std::integral_constant<int, 2>{}); break;
+    case 3: by_k(std::integral_constant<int, 3>{}); break;
+    case 4: by_k(std::integral_constant<int, 4>{}); break;
+    case 5: by_k(std::integral_constant<int, 5>{}); break;
+    case 6: by_k(std::integral_constant<int, 6>{}); break;
+    case 7: by_k(std::integral_constant<int, 7>{}); break;
+    default: by_k(std::integral_constant<int, 8>{}); break;
+  }
   NC_CHECK_HIP(hipGetLastError());
 }
 
@@ -483,19 +555,23 @@ nc_sensmatrix_s* sensmatrix_create(nc_space_s* sp, nc_model_s* m, int precision)
   hipStream_t s = A->stream.s;
   SA->meta_dev.alloc(meta.size() * sizeof(PairMeta));
   NC_CHECK_HIP(hipMemcpyAsync(SA->meta_dev.p, meta.data(), meta.size() * sizeof(PairMeta), hipMemcpyHostToDevice, s));
-  // the row part's derivative terms, grouped by the wave that owns the pair's block
-  // (block b = p + 1 belongs to wave b % 4, accumulator b / 4)
+  // the row part's derivative terms, grouped by the wave that owns the pair's block: the
+  // P + 1 blocks go to the four waves in contiguous chunks, the first (P + 1) % 4 waves
+  // taking one block more (block b = p + 1 is accumulator b - bstart[w] of its wave w)
   {
     constexpr int W = kBlock / 64;
+    const int nb = SA->P + 1, base = nb / W, extra = nb % W;
+    SA->bstart[0] = 0;
+    for (int w = 0; w < W; ++w) SA->bstart[w + 1] = SA->bstart[w] + base + (w < extra ? 1 : 0);
     std::vector<Term> terms;
     for (int w = 0; w < W; ++w) {
       SA->tstart[w] = (int32_t)terms.size();
       for (size_t k = 0; k < SA->pairs.size(); ++k) {
         const SensPair& p = SA->pairs[k];
         const int bb = p.p + 1;
-        if (bb % W != w) continue;
+        if (bb < SA->bstart[w] || bb >= SA->bstart[w + 1]) continue;
         Term T{};
-        T.kk = (int8_t)(bb / W);
+        T.kk = (int8_t)(bb - SA->bstart[w]);
         T.pair = (int16_t)k;
         if (p.arr >= 0) {
           T.kind = 0;
@@ -516,6 +592,17 @@ nc_sensmatrix_s* sensmatrix_create(nc_space_s* sp, nc_model_s* m, int precision)
     if (terms.empty()) terms.push_back(Term{0, 0, -1, 0, 0, 0, 0});
     SA->terms_dev.alloc(terms.size() * sizeof(Term));
     NC_CHECK_HIP(hipMemcpyAsync(SA->terms_dev.p, terms.data(), terms.size() * sizeof(Term), hipMemcpyHostToDevice, s));
+  }
+  // tile fold: when the generator reaches rows farther apart than an XCD's contiguous share
+  // of the tiles (a slow species with few levels, such as the gene state of a one-hot
+  // model, placed last in lattice order), interleave the segments one reach apart
+  {
+    const int64_t tiles = (A->n + 63) / 64, reach_tiles = (A->max_reach + 63) / 64;
+    SA->fold = 1;
+    if (reach_tiles > 0 && reach_tiles >= tiles / 8) {
+      const int64_t f = tiles / reach_tiles;
+      if (f >= 2 && f <= 64) SA->fold = (int32_t)f;
+    }
   }
   if (SA->has_joint && sp->n) {
     SA->states.alloc((size_t)sp->n * sp->ns);
