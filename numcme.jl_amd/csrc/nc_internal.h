@@ -144,8 +144,10 @@ inline unsigned grid_for(int64_t n, int block) {
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, DevBuf<unsigned char>& tmp,
                         hipStream_t s, uint32_t* total_host);
 
-// Dense n x n row-major matrix exponential on the host (Pade 13 scaling and squaring).
+// Dense n x n row-major matrix exponential on the host (Pade 13 scaling and squaring),
+// and its first column alone (exp(A) e1, what a Krylov step uses).
 void expm_dense(int n, const double* a, double* out);
+void expm_first_column(int n, const double* a, double* out);
 
 // Stream with library-owned lifetime; default priority only.
 struct Stream {

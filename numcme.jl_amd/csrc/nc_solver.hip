@@ -986,7 +986,7 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
     }
     const double avnorm = hb[L.misc(MISC_AVN)];
     const int mxf = mb + k1;  // size of the exponentiated matrix
-    std::vector<double> Hm((size_t)mxf * mxf, 0.0), F((size_t)mxf * mxf);
+    std::vector<double> Hm((size_t)mxf * mxf, 0.0), F(mxf);  // F: first column of exp(tau Hm)
     auto exp_tau = [&](double tau, std::vector<double>& out) {
       for (int i = 0; i < mxf; ++i)
         for (int jj = 0; jj < mxf; ++jj) {
@@ -995,7 +995,7 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
           if (k1 != 0 && i == m + 1 && jj == m) v = 1.0;
           Hm[(size_t)i * mxf + jj] = tau * v;
         }
-      expm_dense(mxf, Hm.data(), out.data());
+      expm_first_column(mxf, Hm.data(), out.data());
     };
     double err_loc = btol, xm = 1.0 / m;
     for (int ireject = 0;; ++ireject) {
@@ -1004,8 +1004,8 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
         err_loc = btol;
         break;
       }
-      const double phi1 = std::fabs(S->beta * F[(size_t)m * mxf + 0]);
-      const double phi2 = std::fabs(S->beta * F[(size_t)(m + 1) * mxf + 0] * avnorm);
+      const double phi1 = std::fabs(S->beta * F[m]);
+      const double phi2 = std::fabs(S->beta * F[m + 1] * avnorm);
       if (phi1 > 10.0 * phi2) {
         err_loc = phi2;
         xm = 1.0 / m;
@@ -1027,7 +1027,7 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
       double tot = 0.0;
       for (int r = 0; r < R; ++r) {
         double v = Fm[0] * hb[L.sinkv(m + 1, r, R)];
-        for (int i = 1; i < mx; ++i) v += S->beta * Fm[(size_t)i * mxf] * hb[L.sc(i)] * hb[L.sinkv(i, r, R)];
+        for (int i = 1; i < mx; ++i) v += S->beta * Fm[i] * hb[L.sc(i)] * hb[L.sinkv(i, r, R)];
         tot += v;
       }
       return tot;
@@ -1038,7 +1038,7 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
     if (ev && R > 0 && budget(ev, sinks0, S->t) <= 0.0 && budget(ev, sink_sum(F), S->t + t_step) > 0.0) {
       // the budget is crossed inside this step: locate the crossing on the exact Krylov
       // solution w(tau) by bisection (the reference's callback root-finds to eps)
-      std::vector<double> Ft((size_t)mxf * mxf);
+      std::vector<double> Ft(mxf);
       double lo = 0.0, hi = t_step;
       for (int it = 0; it < 200 && hi - lo > 4.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, S->t + hi); ++it) {
         const double mid = 0.5 * (lo + hi);
@@ -1055,7 +1055,7 @@ void krylov_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_ste
     // w_new = c0/beta * w + sum_{i>=1} c_i sc_i V_i ; c = beta * F[:,0]
     double* coef = S->hcoef;  // pinned; the previous upload completed at the last sync
     std::fill(coef, coef + m + 2, 0.0);
-    for (int i = 1; i < mx; ++i) coef[i - 1] = S->beta * F[(size_t)i * mxf + 0];
+    for (int i = 1; i < mx; ++i) coef[i - 1] = S->beta * F[i];
     coef[m + 1] = F[0];  // (beta*F00)/beta
     NC_CHECK_HIP(hipMemcpyAsync(S->coef.p, coef, sizeof(double) * (m + 2), hipMemcpyHostToDevice, s));
     S->t += t_step;

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
+#include <vector>
 
 #include "nc_internal.h"
 
@@ -36,55 +38,42 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, DevBuf<uns
 // ---- dense matrix exponential (host, FP64) --------------------------------------------
 // Scaling and squaring with the [13/13] Pade approximant and the theta_13 threshold of
 // N. J. Higham, "The scaling and squaring method for the matrix exponential revisited",
-// SIAM J. Matrix Anal. Appl. 26(4), 2005. Matrices here are at most (m+2)x(m+2) Krylov
-// Hessenberg matrices, so plain triple loops suffice.
+// SIAM J. Matrix Anal. Appl. 26(4), 2005. The Krylov integrator calls it at least once per
+// step on its (m+2)x(m+2) Hessenberg matrix, between a device synchronisation and the next
+// launch, so it is on the solve's critical path: matrices of at most kExpmMax rows are
+// padded to a multiple of 8 and multiplied with a row of the product held in vector
+// registers (AVX2 and FMA when the host has them, checked once; plain vectors otherwise).
+// The Krylov step needs only the first column, which saves the last squaring.
 namespace {
 
-void matmul(int n, const double* a, const double* b, double* c) {
+constexpr int kExpmMax = 64;
+namespace expm_avx2 {
+#pragma clang attribute push(__attribute__((target("avx2,fma"))), apply_to = function)
+#include "nc_expm_impl.h"
+#pragma clang attribute pop
+}  // namespace expm_avx2
+
+namespace expm_base {
+#include "nc_expm_impl.h"
+}  // namespace expm_base
+
+bool host_has_avx2() {
+  static const bool yes = __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+  return yes;
+}
+
+// the plain algorithm for matrices above kExpmMax rows (not used by the integrators)
+void matmul_plain(int n, const double* a, const double* b, double* c) {
   std::fill(c, c + (size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i)
     for (int k = 0; k < n; ++k) {
-      double aik = a[i * n + k];
+      const double aik = a[i * n + k];
       if (aik == 0.0) continue;
-      const double* brow = b + k * n;
-      double* crow = c + i * n;
-      for (int j = 0; j < n; ++j) crow[j] += aik * brow[j];
+      for (int j = 0; j < n; ++j) c[i * n + j] += aik * b[k * n + j];
     }
 }
 
-// Solve Q X = P in place (X overwrites P); Gaussian elimination with partial pivoting.
-void solve_inplace(int n, std::vector<double>& q, std::vector<double>& p) {
-  for (int col = 0; col < n; ++col) {
-    int piv = col;
-    double best = std::fabs(q[col * n + col]);
-    for (int r = col + 1; r < n; ++r)
-      if (std::fabs(q[r * n + col]) > best) best = std::fabs(q[r * n + col]), piv = r;
-    if (best == 0.0) throw Error(NC_ERR_NUMERIC, "singular Pade denominator in expm");
-    if (piv != col) {
-      for (int j = 0; j < n; ++j) std::swap(q[col * n + j], q[piv * n + j]), std::swap(p[col * n + j], p[piv * n + j]);
-    }
-    double inv = 1.0 / q[col * n + col];
-    for (int r = col + 1; r < n; ++r) {
-      double f = q[r * n + col] * inv;
-      if (f == 0.0) continue;
-      for (int j = col; j < n; ++j) q[r * n + j] -= f * q[col * n + j];
-      for (int j = 0; j < n; ++j) p[r * n + j] -= f * p[col * n + j];
-    }
-  }
-  for (int col = n - 1; col >= 0; --col) {
-    double inv = 1.0 / q[col * n + col];
-    for (int j = 0; j < n; ++j) p[col * n + j] *= inv;
-    for (int r = 0; r < col; ++r) {
-      double f = q[r * n + col];
-      if (f == 0.0) continue;
-      for (int j = 0; j < n; ++j) p[r * n + j] -= f * p[col * n + j];
-    }
-  }
-}
-
-}  // namespace
-
-void expm_dense(int n, const double* a, double* out) {
+void expm_large(int n, const double* a, double* out) {
   static const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
                                129060195264000.0,   10559470521600.0,    670442572800.0,     33522128640.0,
                                1323241920.0,        40840800.0,          960960.0,           16380.0,
@@ -102,29 +91,85 @@ void expm_dense(int n, const double* a, double* out) {
   const double scale = std::ldexp(1.0, -sq);
   std::vector<double> A(nn), A2(nn), A4(nn), A6(nn), U(nn), V(nn), T(nn), W(nn);
   for (size_t k = 0; k < nn; ++k) A[k] = a[k] * scale;
-  matmul(n, A.data(), A.data(), A2.data());
-  matmul(n, A2.data(), A2.data(), A4.data());
-  matmul(n, A4.data(), A2.data(), A6.data());
-  // U = A [A6 (b13 A6 + b11 A4 + b9 A2) + b7 A6 + b5 A4 + b3 A2 + b1 I]
+  matmul_plain(n, A.data(), A.data(), A2.data());
+  matmul_plain(n, A2.data(), A2.data(), A4.data());
+  matmul_plain(n, A4.data(), A2.data(), A6.data());
   for (size_t k = 0; k < nn; ++k) T[k] = b[13] * A6[k] + b[11] * A4[k] + b[9] * A2[k];
-  matmul(n, A6.data(), T.data(), W.data());
+  matmul_plain(n, A6.data(), T.data(), W.data());
   for (size_t k = 0; k < nn; ++k) W[k] += b[7] * A6[k] + b[5] * A4[k] + b[3] * A2[k];
   for (int i = 0; i < n; ++i) W[(size_t)i * n + i] += b[1];
-  matmul(n, A.data(), W.data(), U.data());
-  // V = A6 (b12 A6 + b10 A4 + b8 A2) + b6 A6 + b4 A4 + b2 A2 + b0 I
+  matmul_plain(n, A.data(), W.data(), U.data());
   for (size_t k = 0; k < nn; ++k) T[k] = b[12] * A6[k] + b[10] * A4[k] + b[8] * A2[k];
-  matmul(n, A6.data(), T.data(), V.data());
+  matmul_plain(n, A6.data(), T.data(), V.data());
   for (size_t k = 0; k < nn; ++k) V[k] += b[6] * A6[k] + b[4] * A4[k] + b[2] * A2[k];
   for (int i = 0; i < n; ++i) V[(size_t)i * n + i] += b[0];
-  // (V - U) X = (V + U)
   std::vector<double> Q(nn), P(nn);
   for (size_t k = 0; k < nn; ++k) Q[k] = V[k] - U[k], P[k] = V[k] + U[k];
-  solve_inplace(n, Q, P);
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    double best = std::fabs(Q[col * n + col]);
+    for (int r = col + 1; r < n; ++r)
+      if (std::fabs(Q[r * n + col]) > best) best = std::fabs(Q[r * n + col]), piv = r;
+    if (best == 0.0) throw Error(NC_ERR_NUMERIC, "singular Pade denominator in expm");
+    if (piv != col)
+      for (int j = 0; j < n; ++j) std::swap(Q[col * n + j], Q[piv * n + j]), std::swap(P[col * n + j], P[piv * n + j]);
+    const double inv = 1.0 / Q[col * n + col];
+    for (int r = col + 1; r < n; ++r) {
+      const double f = Q[r * n + col] * inv;
+      if (f == 0.0) continue;
+      for (int j = col; j < n; ++j) Q[r * n + j] -= f * Q[col * n + j];
+      for (int j = 0; j < n; ++j) P[r * n + j] -= f * P[col * n + j];
+    }
+  }
+  for (int col = n - 1; col >= 0; --col) {
+    const double inv = 1.0 / Q[col * n + col];
+    for (int j = 0; j < n; ++j) P[col * n + j] *= inv;
+    for (int r = 0; r < col; ++r) {
+      const double f = Q[r * n + col];
+      if (f == 0.0) continue;
+      for (int j = 0; j < n; ++j) P[r * n + j] -= f * P[col * n + j];
+    }
+  }
   for (int k = 0; k < sq; ++k) {
-    matmul(n, P.data(), P.data(), T.data());
+    matmul_plain(n, P.data(), P.data(), T.data());
     std::swap(P, T);
   }
   std::copy(P.begin(), P.end(), out);
+}
+
+// one work area per host thread, on the heap (the arrays are 256 KB in all); both kernel
+// sets use the same layout
+struct alignas(64) ExpmArea {
+  unsigned char bytes[sizeof(expm_base::ExpmWork)];
+};
+static_assert(sizeof(expm_base::ExpmWork) == sizeof(expm_avx2::ExpmWork), "work area layouts differ");
+
+void expm_small(int n, const double* a, double* out, bool first_column) {
+  static thread_local std::unique_ptr<ExpmArea> area;
+  if (!area) area = std::make_unique<ExpmArea>();
+  if (host_has_avx2())
+    expm_avx2::expm_small(n, a, out, first_column, *reinterpret_cast<expm_avx2::ExpmWork*>(area.get()));
+  else
+    expm_base::expm_small(n, a, out, first_column, *reinterpret_cast<expm_base::ExpmWork*>(area.get()));
+}
+
+}  // namespace
+
+void expm_dense(int n, const double* a, double* out) {
+  if (n > kExpmMax)
+    expm_large(n, a, out);
+  else
+    expm_small(n, a, out, false);
+}
+
+void expm_first_column(int n, const double* a, double* out) {
+  if (n > kExpmMax) {
+    std::vector<double> full((size_t)n * n);
+    expm_large(n, a, full.data());
+    for (int i = 0; i < n; ++i) out[i] = full[(size_t)i * n];
+    return;
+  }
+  expm_small(n, a, out, true);
 }
 
 }  // namespace nc
