@@ -292,7 +292,7 @@ def c4_instance(nh, spec, precision):
         states = box_states(nmax + 1)
     sp = nh.StateSpaceSparse(model.stoich_matrix, states)
     A = nh.FspMatrixSparse(sp, model.propensities, model.parameters, precision=precision, model=model)
-    u0 = np.zeros(len(states) + S.shape[1])
+    u0 = np.zeros(len(states) + model.stoich_matrix.shape[1])
     u0[0] = 1.0  # telegraph (G_off, 0) / toggle (0, 0): the first state of the list
     return A, u0
 
@@ -432,6 +432,58 @@ def _l1_between(a, b):
 C5_PLAIN_LEVELS = (2, 6, 12, 24, 48)
 
 
+def c5_rk(nh, prec):
+    """C5's DP5(4) settings"""
+    return nh.RK45(rtol=1e-6, atol=1e-10, precision=prec)
+
+
+def c5_train(nh, T_rec=1.5, n_scalings=64, say=lambda m: None):
+    """The learned adapter's network: records from plain SelectiveRStepAdapter(2, 2) solves
+    of the cascade over n_scalings seeded parameter scalings, then device training."""
+    rng = np.random.default_rng(0)
+    rec = nh.TrainingRecords(capacity=1_000_000, per_adaptation=1024, label_mass=1e-10)
+    t0 = time.perf_counter()
+    for _ in range(n_scalings):
+        th = np.asarray(C5_THETA) * np.exp(rng.uniform(np.log(0.5), np.log(2.0), size=6))
+        alg = nh.AdaptiveFspSparse(ode_method=c5_rk(nh, "fp32"), space_adapter=nh.SelectiveRStepAdapter(2, 2, False))
+        nh.solve(cascade_model(nh, th), nh.FspVectorSparse([[0, 0, 0]], [1.0]), (0.0, T_rec), alg,
+                 saveat=np.linspace(0.0, T_rec, 6), fsptol=1e-4, records=rec)
+    gen_s = time.perf_counter() - t0
+    X, Y = rec.arrays()
+    say(f"c5: {len(X)} records in {gen_s:.1f}s")
+    t0 = time.perf_counter()
+    net, loss = nh.train_adapter(X, Y, epochs=20, batch=4096, lr=2e-3, seed=0)
+    train_s = time.perf_counter() - t0
+    acc = float(np.mean((net.forward(X) > 0) == (Y > 0.5)))
+    say(f"c5: trained in {train_s:.1f}s, loss {loss:.4f}, accuracy {acc:.4f}")
+    return net, (X, Y, gen_s, train_s, loss, acc)
+
+
+def c5_solve_only(nh, net_file, learned_levels=8, log=None):
+    """Two learned FP32 solves of C5 with a network saved by an earlier call (which trains
+    it and stores its parameters in net_file): the profiler view of one learned solve."""
+    say = log or (lambda m: None)
+    path = Path(net_file)
+    if not path.exists():
+        net, _ = c5_train(nh, say=say)
+        np.save(path, net.params)
+        return {"trained": str(path)}
+    net = nh.AdapterMlp(0)
+    net.params = np.load(path)
+    model = cascade_model(nh, C5_THETA)
+    res = []
+    for _ in range(2):
+        alg = nh.AdaptiveFspSparse(ode_method=c5_rk(nh, "fp32"),
+                                   space_adapter=nh.LearnedAdapter(net, 2, learned_levels, False, threshold=0.3))
+        t0 = time.perf_counter()
+        out = nh.solve(model, nh.FspVectorSparse([[0, 0, 0]], [1.0]), (0.0, 5.0), alg,
+                       saveat=np.linspace(0.0, 5.0, 6), fsptol=1e-4)
+        for p in out.p:
+            p.values
+        res.append({"solve_s": time.perf_counter() - t0, **out.stats})
+    return {"learned_fp32_solves": res}
+
+
 def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=C5_PLAIN_LEVELS, learned_levels=8, log=None):
     """Config C5: the cascade from (0,0,0) with fsptol 1e-4 over [0, T], adaptive FSP with
     the learned adapter against plain SelectiveRStepAdapter(2, R), each at its best
@@ -446,22 +498,9 @@ def c5_learned(nh, T=5.0, T_rec=1.5, n_scalings=64, plain_levels=C5_PLAIN_LEVELS
                        saveat=np.linspace(0.0, tend, 6), fsptol=1e-4, records=records)
         return out, time.perf_counter() - t0
 
-    rk = lambda prec: nh.RK45(rtol=1e-6, atol=1e-10, precision=prec)
+    rk = lambda prec: c5_rk(nh, prec)
     say = log or (lambda m: None)
-    rng = np.random.default_rng(0)
-    rec = nh.TrainingRecords(capacity=1_000_000, per_adaptation=1024, label_mass=1e-10)
-    t0 = time.perf_counter()
-    for _ in range(n_scalings):
-        th = np.asarray(C5_THETA) * np.exp(rng.uniform(np.log(0.5), np.log(2.0), size=6))
-        solve(cascade_model(nh, th), nh.SelectiveRStepAdapter(2, 2, False), rk("fp32"), T_rec, rec)
-    gen_s = time.perf_counter() - t0
-    X, Y = rec.arrays()
-    say(f"c5: {len(X)} records in {gen_s:.1f}s")
-    t0 = time.perf_counter()
-    net, loss = nh.train_adapter(X, Y, epochs=20, batch=4096, lr=2e-3, seed=0)
-    train_s = time.perf_counter() - t0
-    acc = float(np.mean((net.forward(X) > 0) == (Y > 0.5)))
-    say(f"c5: trained in {train_s:.1f}s, loss {loss:.4f}, accuracy {acc:.4f}")
+    net, (X, Y, gen_s, train_s, loss, acc) = c5_train(nh, T_rec, n_scalings, say)
 
     model = cascade_model(nh, C5_THETA)
     learned = nh.LearnedAdapter(net, 2, learned_levels, False, threshold=0.3)
@@ -646,6 +685,28 @@ def spmv_probe(nh, precision, reps):
                       "copy_bytes_each_way": x.numel() * 4}), flush=True)
 
 
+def sens_probe(nh, reps):
+    """C3's sensitivity mat-vec launched reps times back to back in FP32 and in FP64 storage
+    (t = 300, inside the pulse), plus the same copy kernel of known size as spmv_probe, for
+    rocprofv3 --pmc runs."""
+    import torch
+
+    model = gene4_model(nh)
+    space = nh.StateSpaceSparse(model.get_stoich_matrix(), gene4_states(255))
+    res = {}
+    for prec in ("fp32", "fp64"):
+        SA = nh.ForwardSensFspMatrixSparse(model, space, precision=prec)
+        res[prec] = {"mean_launch_us": SA.bench_matvec(reps=reps, t=300.0) * 1e3,
+                     "own_bytes_per_matvec": SA.bytes_per_matvec}
+        del SA
+    x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "sens", "reps": reps, **res, "copy_bytes_each_way": x.numel() * 4}), flush=True)
+
+
 def c2_end_to_end(nh, precision, krylov_tol, ts):
     """One complete C2 solve from model upload to snapshot download: trace and upload the
     model, build the state space, generator and solver, integrate, download every snapshot."""
@@ -679,8 +740,14 @@ def main():
     ap.add_argument("--spmv-probe", type=int, default=0, metavar="REPS",
                     help="only build the matrix, launch the mat-vec REPS times back to back and run a plain "
                          "256 MiB copy kernel three times (the calibration pair for HBM counter runs)")
-    ap.add_argument("--only", choices=["c4", "c5", "mlp"],
-                    help="run one leg alone and print its figures (for profiler runs); no contract line")
+    ap.add_argument("--sens-probe", type=int, default=0, metavar="REPS",
+                    help="only build C3's sensitivity operator in FP32 and FP64, launch its mat-vec REPS times back "
+                         "to back each, and run the 256 MiB copy kernel three times (counter runs)")
+    ap.add_argument("--only", choices=["c3", "c4", "c5", "c5-solve", "mlp"],
+                    help="run one leg alone and print its figures (for profiler runs); no contract line. "
+                         "c5-solve: train C5's network and save it to --net-file, or, when that file exists, "
+                         "load it and run two learned FP32 solves")
+    ap.add_argument("--net-file", default="gpurun_out/c5_net.npy", help="network parameters for --only c5-solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; runs on more than one GPU are not supported (how "
@@ -695,6 +762,9 @@ def main():
     if args.spmv_probe:
         spmv_probe(nh, args.precision, args.spmv_probe)
         return
+    if args.sens_probe:
+        sens_probe(nh, args.sens_probe)
+        return
     if args.only == "mlp":
         # the network kernels alone on 2,097,152 synthetic rows (HIP events per launch)
         net = nh.AdapterMlp(seed=0)
@@ -704,8 +774,14 @@ def main():
                           "forward_TFLOPs": MLP_FLOP_FORWARD * (1 << 21) / (fwd_ms * 1e-3) / 1e12,
                           "train_TFLOPs": 3 * MLP_FLOP_FORWARD * (1 << 21) / (trn_ms * 1e-3) / 1e12}), flush=True)
         return
+    if args.only == "c3":
+        print(json.dumps(c3_sensitivity(nh, L)), flush=True)
+        return
     if args.only == "c4":
         print(json.dumps(c4_batch(nh, L, args.krylov_tol, graphs=not args.no_graphs, log=say)), flush=True)
+        return
+    if args.only == "c5-solve":
+        print(json.dumps(c5_solve_only(nh, args.net_file, log=say)), flush=True)
         return
     if args.only == "c5":
         print(json.dumps(c5_learned(nh, plain_levels=C5_PLAIN_LEVELS if args.c5_plain == "all" else (), log=say)),
