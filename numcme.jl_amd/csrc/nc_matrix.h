@@ -98,6 +98,26 @@ void matrix_scales(const nc_matrix_s* A, double t, double* sink_scale, double* t
 // n_rows VT values, or FP64 values when out_fp64 is set.
 void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
                    bool out_fp64 = false);
+// A DP5(4) stage input formed in the mat-vec's epilogue from the stage vector the launch
+// produces: y = u + h (sum_{j: bit j of mask} a[j] k[j] + a_new k_new), written as the
+// next mat-vec's input copy vin (VT, a buffer other than the one the launch reads) and, when
+// ytmp is set, in FP64 there (the last stage, y_{n+1}). State and sink rows are written;
+// padding rows are not.
+struct RkEpilogue {
+  const double* k[6];
+  double a[6];
+  double a_new;
+  int mask;
+  double h;
+  const double* u;
+  double* ytmp;
+  void* vin;
+};
+// Whether the matrix's mat-vec kernel takes the epilogue (the stored generator at 2 rows per
+// thread); otherwise the stage pass runs on its own.
+bool matvec_rk_fusable(const nc_matrix_s* A);
+// out = A(t) v (FP64) and the epilogue's stage input
+void matvec_launch_rk(nc_matrix_s* A, double t, const void* v, double* out, hipStream_t s, const RkEpilogue& epi);
 // When kernel timing is on, hand out the next pair of HIP events for a dispatch-stamped
 // launch (the caller increments ev_used after the launch); false when timing is off.
 bool matvec_timing_slot(nc_matrix_s* A, hipEvent_t* e0, hipEvent_t* e1);

@@ -399,6 +399,15 @@ struct MatArgs {
   unsigned* chunk_arrivals;       // chunk workgroups done in this launch, zero between launches
   const MfInfo* mf;               // matrix-free mode (DM 3): metadata and tables
   const double* mf_tab;
+  // DP5(4) epilogue (EPI kernels): the next stage's input from the stage vector just
+  // formed, y = u + h (sum_{j: bit j of rk_mask} rk_a[j] k_j + rk_a[6] k_new)
+  const double* rk_k[6];  // k_j (a duplicate of rk_k[0] where the mask bit is clear)
+  double rk_a[7];
+  double rk_h;
+  const double* rk_u;
+  double* rk_ytmp;        // the last stage: y_{n+1} in FP64 too (else null)
+  VT* rk_vin;             // the next mat-vec's input copy (not the buffer this launch reads)
+  int32_t rk_mask;
 };
 
 template <typename T, int R>
@@ -655,9 +664,41 @@ __device__ inline void mf_rows(const MatArgs<VT>& a, const VT* __restrict__ v, O
   }
 }
 
+// The DP5(4) stage combination for rows i .. i + R - 1 (rows >= nw are not written: they
+// belong to the sink workgroups), from the stage vector values knew the mat-vec has just
+// formed: the same fused multiply-adds in the same order as the separate stage pass, so
+// the result is the same to the bit. The k loads are unconditional (a clear mask bit
+// points at k_0 again and its term is skipped by a select).
+template <typename VT, int R>
+__device__ inline void rk_epilogue(const MatArgs<VT>& a, int64_t i, int64_t nw, const double* knew) {
+  double kv[6][R], uv[R];
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+#pragma unroll
+    for (int r = 0; r < R; ++r) kv[j][r] = a.rk_k[j][i + r];
+#pragma unroll
+  for (int r = 0; r < R; ++r) uv[r] = a.rk_u[i + r];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double t = __fma_rn(a.rk_a[j], kv[j][r], acc);
+      acc = ((a.rk_mask >> j) & 1) ? t : acc;
+    }
+    acc = __fma_rn(a.rk_a[6], knew[r], acc);
+    const double y = __fma_rn(a.rk_h, acc, uv[r]);
+    if (i + r < nw) {
+      a.rk_vin[i + r] = (VT)y;
+      if (a.rk_ytmp) a.rk_ytmp[i + r] = y;
+    }
+  }
+}
+
 // Thread t owns state rows [R*t, R*t+R). Matrix streams are read with one R-wide vector
-// load per slot; v[i] and the predecessor gathers are scalar.
-template <typename VT, typename OT, int KT, int R, int DM>
+// load per slot; v[i] and the predecessor gathers are scalar. EPI: the launch also forms
+// the next DP5(4) stage input (rk_epilogue) for its rows, state and sink rows alike.
+template <typename VT, typename OT, int KT, int R, int DM, bool EPI = false>
 __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __restrict__ v, OT* __restrict__ out,
                                                    int accumulate) {
   const int64_t row_threads = (a.n + R - 1) / R;
@@ -757,6 +798,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
       const int64_t i = i0 + r;
       if (i < a.n) out[i] = (OT)(accumulate ? (double)out[i] + acc[r] : acc[r]);
     }
+    if constexpr (EPI) rk_epilogue<VT, R>(a, i0, a.n, acc);
   } else {
     const int c = (int)((int64_t)blockIdx.x - row_blocks);
     const int r = a.n_chunks ? a.chunk_row[c] : c;
@@ -804,6 +846,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
         s *= a.sink_scale[r];
         const int64_t o = a.n + r;
         out[o] = (OT)(accumulate ? (double)out[o] + s : s);
+        if constexpr (EPI) rk_epilogue<VT, 1>(a, o, o + 1, &s);
       }
     }
     if (a.n_chunks) {
@@ -822,6 +865,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
             s *= a.sink_scale[q];
             const int64_t o = a.n + q;
             out[o] = (OT)(accumulate ? (double)out[o] + s : s);
+            if constexpr (EPI) rk_epilogue<VT, 1>(a, o, o + 1, &s);
           }
         }
         if (threadIdx.x == 0) __hip_atomic_store(a.chunk_arrivals, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -839,12 +883,41 @@ struct LaunchCfg {
   hipEvent_t e0, e1;
 };
 
-template <typename VT, typename OT, int K, int R, int DM>
+template <typename VT, typename OT, int K, int R, int DM, bool EPI = false>
 void launch_one(const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
   if (c.e0)
-    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo, acc);
+    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM, EPI>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo,
+                          acc);
   else
-    k_matvec<VT, OT, K, R, DM><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+    k_matvec<VT, OT, K, R, DM, EPI><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+}
+
+// The epilogue kernels: FP64 output, 2 rows per thread, stored generator (loaded, computed
+// or mixed columns)
+template <typename VT, int DM>
+void dispatch_epi_kd(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, double* oo) {
+  switch (K) {
+    case 1: launch_one<VT, double, 1, 2, DM, true>(c, a, vv, oo, 0); break;
+    case 2: launch_one<VT, double, 2, 2, DM, true>(c, a, vv, oo, 0); break;
+    case 3: launch_one<VT, double, 3, 2, DM, true>(c, a, vv, oo, 0); break;
+    case 4: launch_one<VT, double, 4, 2, DM, true>(c, a, vv, oo, 0); break;
+    case 5: launch_one<VT, double, 5, 2, DM, true>(c, a, vv, oo, 0); break;
+    case 6: launch_one<VT, double, 6, 2, DM, true>(c, a, vv, oo, 0); break;
+    case 7: launch_one<VT, double, 7, 2, DM, true>(c, a, vv, oo, 0); break;
+    case 8: launch_one<VT, double, 8, 2, DM, true>(c, a, vv, oo, 0); break;
+    default: launch_one<VT, double, 0, 2, 2, true>(c, a, vv, oo, 0); break;
+  }
+}
+
+template <typename VT>
+void dispatch_epi(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, double* oo) {
+  const uint32_t all = K >= 32 ? 0xffffffffu : ((1u << K) - 1u);
+  if (a.dia_mask == 0u)
+    dispatch_epi_kd<VT, 0>(K, c, a, vv, oo);
+  else if ((a.dia_mask & all) == all)
+    dispatch_epi_kd<VT, 1>(K, c, a, vv, oo);
+  else
+    launch_one<VT, double, 0, 2, 2, true>(c, a, vv, oo, 0);
 }
 
 template <typename VT, typename OT, int R, int DM>
@@ -913,7 +986,7 @@ void dispatch_r(int R, int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT
 
 template <typename VT>
 void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
-                     bool out_fp64, hipEvent_t e0, hipEvent_t e1) {
+                     bool out_fp64, hipEvent_t e0, hipEvent_t e1, const RkEpilogue* epi = nullptr) {
   MatArgs<VT> a{};
   a.n = A->n;
   a.ld = A->ld;
@@ -950,6 +1023,22 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
   const LaunchCfg c{(unsigned)(row_blocks + (A->n_chunks ? A->n_chunks : A->nr)), s, e0, e1};
   const VT* vv = reinterpret_cast<const VT*>(v);
   const int acc = accumulate ? 1 : 0;
+  if (epi) {
+    if (!matvec_rk_fusable(A) || accumulate || !out_fp64) throw Error(NC_ERR_INTERNAL, "mat-vec epilogue not available");
+    for (int j = 0; j < 6; ++j) {
+      const bool on = (epi->mask >> j) & 1;
+      a.rk_k[j] = on ? epi->k[j] : epi->k[0];
+      a.rk_a[j] = on ? epi->a[j] : 0.0;
+    }
+    a.rk_a[6] = epi->a_new;
+    a.rk_h = epi->h;
+    a.rk_u = epi->u;
+    a.rk_ytmp = epi->ytmp;
+    a.rk_vin = reinterpret_cast<VT*>(epi->vin);
+    a.rk_mask = epi->mask;
+    dispatch_epi<VT>(A->K, c, a, vv, reinterpret_cast<double*>(out));
+    return;
+  }
   if (out_fp64 && A->mf_mode)
     dispatch_mf_r<VT>(R, A->mf_mode == 2, A->K, c, a, vv, reinterpret_cast<double*>(out), acc);
   else if (out_fp64)
@@ -1402,6 +1491,31 @@ void matvec_launch(nc_matrix_s* A, double t, const void* v, void* out, bool accu
     launch_matvec_t<float>(A, t, v, out, accumulate, s, out_fp64, e0, e1);
   else
     launch_matvec_t<double>(A, t, v, out, accumulate, s, out_fp64, e0, e1);
+  NC_CHECK_HIP(hipGetLastError());
+  if (timed) A->ev_used++;
+  A->launches++;
+}
+
+bool matvec_rk_fusable(const nc_matrix_s* A) { return A->mf_mode == 0 && A->rows_per_thread == 2; }
+
+void matvec_launch_rk(nc_matrix_s* A, double t, const void* v, double* out, hipStream_t s, const RkEpilogue& epi) {
+  if (A->precision == NC_FP32)
+    joint_refresh<float>(A, t);
+  else
+    joint_refresh<double>(A, t);
+  if (s != A->stream.s) {
+    hipEvent_t e;
+    NC_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    NC_CHECK_HIP(hipEventRecord(e, A->stream.s));
+    NC_CHECK_HIP(hipStreamWaitEvent(s, e, 0));
+    NC_CHECK_HIP(hipEventDestroy(e));
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = matvec_timing_slot(A, &e0, &e1);
+  if (A->precision == NC_FP32)
+    launch_matvec_t<float>(A, t, v, out, false, s, true, e0, e1, &epi);
+  else
+    launch_matvec_t<double>(A, t, v, out, false, s, true, e0, e1, &epi);
   NC_CHECK_HIP(hipGetLastError());
   if (timed) A->ev_used++;
   A->launches++;

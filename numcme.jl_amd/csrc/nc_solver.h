@@ -45,6 +45,7 @@ struct nc_solver_s {
   uint64_t arnoldi_gen = 0;  // the matrix's launch_gen when the graph was captured
   // RK
   nc::DevBuf<unsigned char> rkv;   // VT copy of the stage input
+  nc::DevBuf<unsigned char> rkv2;  // the other stage-input buffer (epilogue-formed stages)
   nc::DevBuf<double> k;            // 7 x N stage derivatives (FP64)
   nc::DevBuf<double> ytmp;         // N
   double h = 0.0;

@@ -591,14 +591,17 @@ void launch_vec(VecArgs<BT> a, const nc_solver_s* S, hipStream_t s) {
 }
 
 // ---- Dormand-Prince 5(4) ------------------------------------------------------------
-__constant__ double c_dp_a[7][6] = {
-    {0, 0, 0, 0, 0, 0},
-    {1.0 / 5, 0, 0, 0, 0, 0},
-    {3.0 / 40, 9.0 / 40, 0, 0, 0, 0},
-    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0},
-    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0},
-    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0},
-    {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84}};
+#define NC_DP_A                                                                  \
+  {{0, 0, 0, 0, 0, 0},                                                           \
+   {1.0 / 5, 0, 0, 0, 0, 0},                                                     \
+   {3.0 / 40, 9.0 / 40, 0, 0, 0, 0},                                             \
+   {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0},                                   \
+   {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0},        \
+   {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0}, \
+   {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84}}
+__constant__ double c_dp_a[7][6] = NC_DP_A;
+const double h_dp_a[7][6] = NC_DP_A;  // the same table for the host (mat-vec epilogues)
+#undef NC_DP_A
 const double h_dp_c[7] = {0, 1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
 // b - b* (5th minus embedded 4th order weights)
 __constant__ double c_dp_e[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525,
@@ -1125,14 +1128,43 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
     else
       k_to_vt<double><<<g, kBlock, 0, s>>>(src, reinterpret_cast<double*>(S->rkv.p), N);
   };
+  // With a stored generator, the mat-vec that forms k_s also forms stage s + 1's input in
+  // its epilogue (the stage pass runs on its own for stage 1 and for the other operators);
+  // the stage inputs alternate between two buffers, since a mat-vec gathers from its input
+  // while its epilogue writes the next one.
+  const bool fuse = !S->SA && S->rkv2.p && matvec_rk_fusable(A);
   auto stages = [&](double h) {
+    unsigned char* cur = S->rkv.p;
+    unsigned char* nxt = S->rkv2.p;
     for (int stage = 1; stage < 7; ++stage) {
-      if (fp32)
-        launch_rk_stage<float>(stage, g2, s, N, h, S->u.p, k, S->ksw, S->ytmp.p, reinterpret_cast<float*>(S->rkv.p));
-      else
-        launch_rk_stage<double>(stage, g2, s, N, h, S->u.p, k, S->ksw, S->ytmp.p, reinterpret_cast<double*>(S->rkv.p));
-      NC_CHECK_HIP(hipGetLastError());
-      op_apply(S, S->t + h_dp_c[stage] * h, S->rkv.p, kp(stage));
+      if (stage == 1 || !fuse) {
+        if (fp32)
+          launch_rk_stage<float>(stage, g2, s, N, h, S->u.p, k, S->ksw, S->ytmp.p, reinterpret_cast<float*>(cur));
+        else
+          launch_rk_stage<double>(stage, g2, s, N, h, S->u.p, k, S->ksw, S->ytmp.p, reinterpret_cast<double*>(cur));
+        NC_CHECK_HIP(hipGetLastError());
+      }
+      const double ts = S->t + h_dp_c[stage] * h;
+      if (fuse && stage < 6) {
+        const int next = stage + 1;
+        RkEpilogue e{};
+        for (int j = 0; j < stage; ++j) {
+          if (next == 6 && j == 1) continue;  // a_61 = 0: the last stage does not read k1
+          e.mask |= 1 << j;
+          e.k[j] = kp(j);
+          e.a[j] = h_dp_a[next][j];
+        }
+        e.k[0] = kp(0);
+        e.a_new = h_dp_a[next][stage];
+        e.h = h;
+        e.u = S->u.p;
+        e.ytmp = next == 6 ? S->ytmp.p : nullptr;
+        e.vin = nxt;
+        matvec_launch_rk(A, ts, cur, kp(stage), s, e);
+        std::swap(cur, nxt);
+      } else {
+        op_apply(S, ts, cur, kp(stage));
+      }
       S->st.matvecs++;
     }
   };
@@ -1302,6 +1334,10 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
         S->k.alloc((size_t)7 * S->Np);
         S->ytmp.alloc(S->Np);
         NC_CHECK_HIP(hipMemsetAsync(S->rkv.p, 0, S->Np * S->vs, S->s));
+        if (!SA) {
+          S->rkv2.alloc(S->Np * S->vs);
+          NC_CHECK_HIP(hipMemsetAsync(S->rkv2.p, 0, S->Np * S->vs, S->s));
+        }
         NC_CHECK_HIP(hipMemsetAsync(S->k.p, 0, sizeof(double) * 7 * S->Np, S->s));
         NC_CHECK_HIP(hipMemsetAsync(S->ytmp.p, 0, sizeof(double) * S->Np, S->s));
       }
