@@ -53,6 +53,10 @@ namespace nc {
 nc_sensmatrix_s* sensmatrix_create(nc_space_s* sp, nc_model_s* m, int precision);
 // out_b = (A v_b) for block 0 and A v_b + dA_{b-1} v_0 for the others; v holds P+1 VT
 // blocks with stride vstride, out P+1 FP64 blocks with stride ostride.
+// With epi set (vstride == ostride, the stage vectors in the same block layout), the
+// launch also forms the next DP5(4) stage input in its epilogue (see RkEpilogue).
 void sens_matvec_launch(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, double* out, int64_t ostride,
-                        hipStream_t s);
+                        hipStream_t s, const RkEpilogue* epi = nullptr);
+// Whether the epilogue kernels cover this operator (at most 16 blocks).
+bool sens_rk_fusable(const nc_sensmatrix_s* SA);
 }  // namespace nc

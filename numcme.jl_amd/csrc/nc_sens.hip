@@ -68,7 +68,39 @@ struct SensArgs {
   double sink_scale[kMaxReactions];
   double dscale[kMaxPairs];   // multiplies the pair's state-factor derivative term
   double sfscale[kMaxPairs];  // multiplies the time-factor derivative term (separable)
+  // DP5(4) epilogue (EPI kernels): the next stage input of row i of block b, index
+  // b * rk_stride + i of the stage vectors, u, ytmp and vin
+  const double* rk_k[6];  // k_j (a duplicate of rk_k[0] where the mask bit is clear)
+  double rk_a[7];
+  double rk_h;
+  const double* rk_u;
+  double* rk_ytmp;        // the last stage: y_{n+1} in FP64 too (else null)
+  VT* rk_vin;
+  int64_t rk_stride;
+  int32_t rk_mask;
 };
+
+// The DP5(4) stage combination of one (block, row) entry at index e of the stage vectors,
+// from the value knew the mat-vec has just formed: the fused multiply-adds of the separate
+// stage pass, in its order (the k loads are unconditional; a clear mask bit points at k_0
+// and its term is skipped by a select).
+template <typename VT>
+__device__ inline void sens_rk_epilogue(const SensArgs<VT>& a, int64_t e, double knew) {
+  double kv[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) kv[j] = a.rk_k[j][e];
+  const double uv = a.rk_u[e];
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const double t = __fma_rn(a.rk_a[j], kv[j], acc);
+    acc = ((a.rk_mask >> j) & 1) ? t : acc;
+  }
+  acc = __fma_rn(a.rk_a[6], knew, acc);
+  const double y = __fma_rn(a.rk_h, acc, uv);
+  a.rk_vin[e] = (VT)y;
+  if (a.rk_ytmp) a.rk_ytmp[e] = y;
+}
 
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -144,7 +176,7 @@ __device__ __forceinline__ void sens_terms(const SensArgs<VT>& a, const Term* s_
   }
 }
 
-template <typename VT, int M, int K>
+template <typename VT, int M, int K, bool EPI>
 __device__ __forceinline__ void sens_rows(const SensArgs<VT>& a, const Term* s_terms, const VT* __restrict__ v,
                                           double* __restrict__ out, int wave, int64_t i) {
   const int64_t ld = a.ld, vs = a.vstride;
@@ -205,6 +237,10 @@ __device__ __forceinline__ void sens_rows(const SensArgs<VT>& a, const Term* s_t
   }
 #pragma unroll
   for (int kk = 0; kk < M; ++kk) out[(int64_t)(b0 + kk) * a.ostride + i] = acc[kk];
+  if constexpr (EPI) {
+#pragma unroll
+    for (int kk = 0; kk < M; ++kk) sens_rk_epilogue<VT>(a, (int64_t)(b0 + kk) * a.rk_stride + i, acc[kk]);
+  }
 }
 
 // One workgroup per tile of 64 rows; its four waves share the tile's rows and split the
@@ -220,7 +256,7 @@ __device__ __forceinline__ void sens_rows(const SensArgs<VT>& a, const Term* s_t
 // so that rows one segment apart (a slow species' neighbours, such as the gene states of a
 // one-hot model) are processed by the same XCD at about the same time and their gathers
 // hit its L2 instead of going back to memory.
-template <typename VT, int MH, int K>
+template <typename VT, int MH, int K, bool EPI = false>
 __global__ __launch_bounds__(kBlock) void k_sens_matvec(SensArgs<VT> a, const VT* __restrict__ v,
                                                         double* __restrict__ out) {
   constexpr int W = kBlock / 64;
@@ -244,9 +280,9 @@ __global__ __launch_bounds__(kBlock) void k_sens_matvec(SensArgs<VT> a, const VT
     if (i >= a.n) return;
     const int nblk = a.bstart[wave + 1] - a.bstart[wave];
     if (nblk == MH)
-      sens_rows<VT, MH, K>(a, s_terms, v, out, wave, i);
+      sens_rows<VT, MH, K, EPI>(a, s_terms, v, out, wave, i);
     else if constexpr (MH > 1)
-      if (nblk == MH - 1) sens_rows<VT, MH - 1, K>(a, s_terms, v, out, wave, i);
+      if (nblk == MH - 1) sens_rows<VT, MH - 1, K, EPI>(a, s_terms, v, out, wave, i);
   } else if ((int64_t)blockIdx.x < nsink) {
     // sink row r of block bb: one workgroup, deterministic tree
     const int sb = (int)blockIdx.x;
@@ -309,6 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_sens_matvec(SensArgs<VT> a, const VT
       double sum = 0.0;
       for (int w = 0; w < kBlock / 64; ++w) sum += part[w];
       out[bb * a.ostride + a.n + r] = sum;
+      if constexpr (EPI) sens_rk_epilogue<VT>(a, bb * a.rk_stride + a.n + r, sum);
     }
   }
 }
@@ -390,7 +427,7 @@ void fill_pairs(nc_sensmatrix_s* SA, const int32_t* states, const int32_t* state
 
 template <typename VT>
 void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, double* out, int64_t ostride,
-                 hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                 hipStream_t s, hipEvent_t e0, hipEvent_t e1, const RkEpilogue* epi) {
   nc_matrix_s* A = SA->A;
   nc_model_s* m = SA->model;
   SensArgs<VT> a{};
@@ -443,6 +480,47 @@ void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, 
       kern<<<grid, kBlock, 0, s>>>(a, vv, out);
   };
   static_assert(kMaxSensBlocks <= 8 * (kBlock / 64), "blocks per wave");
+  if (epi) {
+    // the stage epilogue kernels (FP32 storage, at most 4 blocks per wave: P + 1 <= 16)
+    if (!sens_rk_fusable(SA) || !std::is_same<VT, float>::value)
+      throw Error(NC_ERR_INTERNAL, "sensitivity mat-vec epilogue not available");
+    if constexpr (std::is_same<VT, float>::value) {
+      for (int j = 0; j < 6; ++j) {
+        const bool on = (epi->mask >> j) & 1;
+        a.rk_k[j] = on ? epi->k[j] : epi->k[0];
+        a.rk_a[j] = on ? epi->a[j] : 0.0;
+      }
+      a.rk_a[6] = epi->a_new;
+      a.rk_h = epi->h;
+      a.rk_u = epi->u;
+      a.rk_ytmp = epi->ytmp;
+      a.rk_vin = reinterpret_cast<VT*>(epi->vin);
+      a.rk_mask = epi->mask;
+      a.rk_stride = vstride;
+      auto by_k_epi = [&](auto mh) {
+        constexpr int M = decltype(mh)::value;
+        switch (a.K) {
+          case 1: go(k_sens_matvec<VT, M, 1, true>); break;
+          case 2: go(k_sens_matvec<VT, M, 2, true>); break;
+          case 3: go(k_sens_matvec<VT, M, 3, true>); break;
+          case 4: go(k_sens_matvec<VT, M, 4, true>); break;
+          case 5: go(k_sens_matvec<VT, M, 5, true>); break;
+          case 6: go(k_sens_matvec<VT, M, 6, true>); break;
+          case 7: go(k_sens_matvec<VT, M, 7, true>); break;
+          case 8: go(k_sens_matvec<VT, M, 8, true>); break;
+          default: go(k_sens_matvec<VT, M, 0, true>); break;
+        }
+      };
+      switch ((a.nb + 3) / 4) {
+        case 1: by_k_epi(std::integral_constant<int, 1>{}); break;
+        case 2: by_k_epi(std::integral_constant<int, 2>{}); break;
+        case 3: by_k_epi(std::integral_constant<int, 3>{}); break;
+        default: by_k_epi(std::integral_constant<int, 4>{}); break;
+      }
+      NC_CHECK_HIP(hipGetLastError());
+    }
+    return;
+  }
   auto by_k = [&](auto mh) {
     constexpr int M = decltype(mh)::value;
     switch (a.K) {
@@ -473,8 +551,12 @@ void launch_sens(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, 
 
 }  // namespace
 
+// FP32 storage only: with FP64 stage-input copies the epilogue kernel measured 3% slower
+// over the C3 solve than the separate stage passes (8% faster in FP32 storage)
+bool sens_rk_fusable(const nc_sensmatrix_s* SA) { return SA->P + 1 <= 16 && SA->A->precision == NC_FP32; }
+
 void sens_matvec_launch(nc_sensmatrix_s* SA, double t, const void* v, int64_t vstride, double* out, int64_t ostride,
-                        hipStream_t s) {
+                        hipStream_t s, const RkEpilogue* epi) {
   nc_matrix_s* A = SA->A;
   matrix_prepare(A, t);
   if (SA->has_joint && t != SA->t_cache) {
@@ -494,9 +576,9 @@ void sens_matvec_launch(nc_sensmatrix_s* SA, double t, const void* v, int64_t vs
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const bool timed = matvec_timing_slot(A, &e0, &e1);
   if (A->precision == NC_FP32)
-    launch_sens<float>(SA, t, v, vstride, out, ostride, s, e0, e1);
+    launch_sens<float>(SA, t, v, vstride, out, ostride, s, e0, e1, epi);
   else
-    launch_sens<double>(SA, t, v, vstride, out, ostride, s, e0, e1);
+    launch_sens<double>(SA, t, v, vstride, out, ostride, s, e0, e1, epi);
   if (timed) A->ev_used++;
   A->launches++;
 }

@@ -1128,11 +1128,12 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
     else
       k_to_vt<double><<<g, kBlock, 0, s>>>(src, reinterpret_cast<double*>(S->rkv.p), N);
   };
-  // With a stored generator, the mat-vec that forms k_s also forms stage s + 1's input in
-  // its epilogue (the stage pass runs on its own for stage 1 and for the other operators);
+  // The mat-vec that forms k_s also forms stage s + 1's input in its epilogue (the stored
+  // generator at 2 rows per thread, and the sensitivity operator up to 16 blocks; the stage
+  // pass runs on its own for stage 1 and for the computed modes);
   // the stage inputs alternate between two buffers, since a mat-vec gathers from its input
   // while its epilogue writes the next one.
-  const bool fuse = !S->SA && S->rkv2.p && matvec_rk_fusable(A);
+  const bool fuse = S->rkv2.p && (S->SA ? sens_rk_fusable(S->SA) : matvec_rk_fusable(A));
   auto stages = [&](double h) {
     unsigned char* cur = S->rkv.p;
     unsigned char* nxt = S->rkv2.p;
@@ -1160,7 +1161,10 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
         e.u = S->u.p;
         e.ytmp = next == 6 ? S->ytmp.p : nullptr;
         e.vin = nxt;
-        matvec_launch_rk(A, ts, cur, kp(stage), s, e);
+        if (S->SA)
+          sens_matvec_launch(S->SA, ts, cur, S->Npb, kp(stage), S->Npb, s, &e);
+        else
+          matvec_launch_rk(A, ts, cur, kp(stage), s, e);
         std::swap(cur, nxt);
       } else {
         op_apply(S, ts, cur, kp(stage));
@@ -1334,10 +1338,8 @@ static nc_solver_s* solver_create_impl(nc_matrix_s* A, const nc_solve_opts* opts
         S->k.alloc((size_t)7 * S->Np);
         S->ytmp.alloc(S->Np);
         NC_CHECK_HIP(hipMemsetAsync(S->rkv.p, 0, S->Np * S->vs, S->s));
-        if (!SA) {
-          S->rkv2.alloc(S->Np * S->vs);
-          NC_CHECK_HIP(hipMemsetAsync(S->rkv2.p, 0, S->Np * S->vs, S->s));
-        }
+        S->rkv2.alloc(S->Np * S->vs);
+        NC_CHECK_HIP(hipMemsetAsync(S->rkv2.p, 0, S->Np * S->vs, S->s));
         NC_CHECK_HIP(hipMemsetAsync(S->k.p, 0, sizeof(double) * 7 * S->Np, S->s));
         NC_CHECK_HIP(hipMemsetAsync(S->ytmp.p, 0, sizeof(double) * S->Np, S->s));
       }

@@ -181,3 +181,37 @@ def test_sens_matvec_twelve_slots(gpu, order, precision):
     want = ref.matvec(0.0, vin)
     tol = 1e-13 if precision == "fp64" else 2e-6
     assert np.abs(out - want).max() <= tol * np.abs(want).max()
+
+
+def _birth_death(n_par):
+    # one species, birth k and death gamma x; with one parameter gamma is fixed at 0.5
+    S = np.array([[1, -1]])
+    if n_par == 2:
+        props = [("ti", lambda x, p: p[0] + 0.0 * x[0]), ("ti", lambda x, p: p[1] * x[0])]
+        th = np.array([12.0, 0.7])
+    else:
+        props = [("ti", lambda x, p: p[0] + 0.0 * x[0]), ("ti", lambda x, p: 0.5 * x[0])]
+        th = np.array([12.0])
+    return S, props, th
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("n_par", [1, 2])
+def test_sens_matvec_few_blocks(gpu, n_par, precision):
+    # two or three vector blocks: some of the four waves of a workgroup own no block
+    from oracle.sensmatrix import DTimeInvariant
+    S, props, th = _birth_death(n_par)
+    grads = {(0, 0): DTimeInvariant(lambda x, p: 1.0 + 0.0 * x[0])}
+    if n_par == 2:
+        grads[(1, 1)] = DTimeInvariant(lambda x, p: 1.0 * x[0])
+    states = [[i] for i in range(0, 300)]
+    model = nh.CmeModelWithSensitivity(nh.CmeModel(S, models.to_product(props), th))
+    SA = nh.ForwardSensFspMatrixSparse(model, nh.StateSpaceSparse(S, states), precision=precision)
+    ref = SensFspMatrix(StateSpace(S, states), models.to_oracle(props), th, grads)
+    assert SA.rowcount == ref.n and SA.parameter_count == n_par
+    v = np.random.default_rng(3).random(ref.n * (n_par + 1))
+    out = SA.matvec(0.0, v)
+    vin = v if precision == "fp64" else v.astype(np.float32).astype(np.float64)
+    want = ref.matvec(0.0, vin)
+    tol = 1e-13 if precision == "fp64" else 2e-6
+    assert np.abs(out - want).max() <= tol * np.abs(want).max()
