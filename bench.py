@@ -269,10 +269,9 @@ C4_TABLE = ROOT / "tests" / "golden" / "c4_instances.json"
 _C4_MODELS = {}
 
 
-def c4_instance(nh, spec, precision):
-    """Space, generator and initial vector of one batched instance (telegraph or toggle).
-    The two networks are traced once; each instance takes its parameter values
-    (CmeModel.with_parameters) and uploads its own device model."""
+def c4_model_states(nh, spec):
+    """The model (parameter values of the instance, networks traced once) and the state
+    list of one batched instance (telegraph or toggle)."""
     th = spec["theta"]
     nmax = spec["nmax"]
     if spec["kind"] not in _C4_MODELS:
@@ -290,6 +289,14 @@ def c4_instance(nh, spec, precision):
         states = np.concatenate([np.stack([one, zero, m], axis=1), np.stack([zero, one, m], axis=1)])
     else:
         states = box_states(nmax + 1)
+    return model, states
+
+
+def c4_instance(nh, spec, precision):
+    """Space, generator and initial vector of one batched instance (telegraph or toggle).
+    The two networks are traced once; each instance takes its parameter values
+    (CmeModel.with_parameters) and uploads its own device model."""
+    model, states = c4_model_states(nh, spec)
     sp = nh.StateSpaceSparse(model.stoich_matrix, states)
     A = nh.FspMatrixSparse(sp, model.propensities, model.parameters, precision=precision, model=model)
     u0 = np.zeros(len(states) + model.stoich_matrix.shape[1])
@@ -326,13 +333,18 @@ def c4_batch(nh, L, krylov_tol, graphs=True, log=None):
     ts = np.arange(0.0, 11.0)
 
     def batched():
+        # the models and state lists, then every space and generator built by the
+        # library's worker threads straight into the batch (nc_batch_create_models)
         t0 = time.perf_counter()
-        mats, u0s = [], []
+        models, states, u0s = [], [], []
         for spec in table:
-            A, u0 = c4_instance(nh, spec, "fp32")
-            mats.append(A)
+            model, st = c4_model_states(nh, spec)
+            models.append(model)
+            states.append(st)
+            u0 = np.zeros(len(st) + model.stoich_matrix.shape[1])
+            u0[0] = 1.0  # telegraph (G_off, 0) / toggle (0, 0): the first state of the list
             u0s.append(u0)
-        B = nh.BatchFsp(mats)
+        B = nh.BatchFsp.from_models(models, states, "fp32")
         t1 = time.perf_counter()
         B.solve(u0s, ts, fetch=False, graphs=graphs)
         t2 = time.perf_counter()
@@ -374,6 +386,8 @@ def c4_batch(nh, L, krylov_tol, graphs=True, log=None):
                     f"{min(sizes)}..{max(sizes)} states, {sum(sizes)} in total, t = 0:1:10; batched DP5(4) rtol 1e-8 "
                     f"in FP32 storage vs plain FP64 Krylov expv (m=30, tol {krylov_tol:g}) one instance at a time",
         "clock": "upload of every state space and generator to the download of every snapshot, both legs",
+        "batched_setup": "nc_batch_create_models: spaces and generators built on 4 library worker threads; the "
+                         "plain leg builds each instance's space and generator before its solve, one at a time",
         "batched_solves_per_s": len(table) / b_total,
         "batched_total_s": b_total,
         "batched_setup_s": b_setup,

@@ -301,6 +301,15 @@ typedef struct nc_batch_s* nc_batch;
 /* Build a batch from n generators (all of one precision, time-invariant). The matrices are
  * copied; they may be destroyed afterwards. */
 int nc_batch_create(int32_t n, const nc_matrix* mats, nc_batch* out);
+/* Build a batch straight from models and explicit state lists: instance i is the generator
+ * of models[i] (time-invariant) on the n_states[i] states of states[i] (row-major, ns int32
+ * each). The state spaces and generators (StateSpaceSparse and FspMatrixSparse of the
+ * reference, sparsestatespace.jl:103-144 and fspsparsematrix.jl:47-108) are built on
+ * n_workers host threads of the library (0: 4), each on streams of its own, then copied
+ * into the batch and freed. Errors name the instance. No reference counterpart (the
+ * reference builds one model at a time). */
+int nc_batch_create_models(int32_t n, const nc_model* models, const int32_t* const* states, const int64_t* n_states,
+                           int32_t precision, int32_t n_workers, nc_batch* out);
 int nc_batch_destroy(nc_batch B);
 /* Total device rows (padded) and each instance's n_rows (instance_rows: n entries). */
 int nc_batch_info(nc_batch B, int64_t* total_rows, int64_t* instance_rows);

@@ -20,8 +20,12 @@
 // decision in between; the host reads one counter per chunk (the number of instances
 // still running), with the next chunk already queued.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
 
 #include "nc_matrix.h"
 
@@ -33,6 +37,7 @@ constexpr int kGran = 256;       // rows of one ownership granule (instances are
 constexpr int kCtlBlock = 1024;  // controller workgroup
 constexpr unsigned kMaxRowBlocks = 2048;  // fixed grid of the list-driven row kernels
 constexpr int kChunk = 8;        // iterations per captured graph
+constexpr int kBuildWorkers = 4;  // host threads of nc_batch_create_models
 
 template <typename T, int R>
 struct alignas(sizeof(T) * R) Pack {
@@ -971,6 +976,93 @@ void batch_solve_t(nc_batch_s* B, const nc_solve_opts& o, const double* u0, int 
 }  // namespace
 
 extern "C" {
+
+int nc_batch_create_models(int32_t n, const nc_model* models, const int32_t* const* states, const int64_t* n_states,
+                           int32_t precision, int32_t n_workers, nc_batch* out) {
+  return guard([&] {
+    NC_REQUIRE(out && models && states && n_states && n >= 1, NC_ERR_ARG, "bad arguments");
+    NC_REQUIRE(precision == NC_FP32 || precision == NC_FP64, NC_ERR_ARG, "precision must be NC_FP32 or NC_FP64");
+    for (int32_t i = 0; i < n; ++i) {
+      NC_REQUIRE(models[i], NC_ERR_ARG, "null model of instance " + std::to_string(i));
+      NC_REQUIRE(n_states[i] >= 1 && states[i], NC_ERR_ARG, "empty state list of instance " + std::to_string(i));
+    }
+    int dev = 0;
+    NC_CHECK_HIP(hipGetDevice(&dev));
+    const int W = std::max(1, std::min<int>(n, n_workers > 0 ? n_workers : kBuildWorkers));
+    std::vector<std::unique_ptr<nc_space_s>> spaces(n);
+    std::vector<std::unique_ptr<nc_matrix_s>> mats(n);
+    std::atomic<int32_t> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex err_mu;
+    int err_code = NC_OK;
+    std::string err_msg;
+    auto fail = [&](int32_t i, int code, const char* what) {
+      std::lock_guard<std::mutex> lk(err_mu);
+      if (err_code == NC_OK) {
+        err_code = code;
+        err_msg = i >= 0 ? "instance " + std::to_string(i) + ": " + what : std::string(what);
+      }
+      failed = true;
+    };
+    // each worker builds whole instances (state space, then generator) on streams of its own
+    auto work = [&] {
+      if (hipSetDevice(dev) != hipSuccess) {
+        fail(-1, NC_ERR_HIP, "hipSetDevice failed on a worker thread");
+        return;
+      }
+      PrivateStreams scope;
+      for (;;) {
+        const int32_t i = next.fetch_add(1);
+        if (i >= n || failed) break;
+        try {
+          nc_model_s* m = models[i];
+          spaces[i].reset(space_create(m->ns, m->nr, m->stoich.data(), n_states[i], states[i]));
+          mats[i].reset(matrix_create(spaces[i].get(), m, precision));
+          NC_CHECK_HIP(hipStreamSynchronize(spaces[i]->stream.s));
+          NC_CHECK_HIP(hipStreamSynchronize(mats[i]->stream.s));
+          streams_settled();  // this instance is complete: later releases need not wait for it
+        } catch (const Error& e) {
+          fail(i, e.code, e.what());
+        } catch (const std::bad_alloc&) {
+          fail(i, NC_ERR_ALLOC, "host allocation failed");
+        } catch (const std::exception& e) {
+          fail(i, NC_ERR_INTERNAL, e.what());
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(W - 1);
+    try {
+      for (int w = 1; w < W; ++w) pool.emplace_back(work);
+    } catch (const std::system_error&) {
+      // fewer threads than asked for: the ones started (and this one) take every instance
+    }
+    work();
+    for (auto& th : pool) th.join();
+    if (failed) {
+      NC_CHECK_HIP(hipDeviceSynchronize());
+      throw Error(err_code, err_msg);
+    }
+    std::vector<nc_matrix> raw(n);
+    for (int32_t i = 0; i < n; ++i) raw[i] = mats[i].get();
+    nc_batch_s* B = batch_create(n, raw.data());
+    // the batch has copied the generators (on its own stream): the instances go once the
+    // copies are done
+    const hipError_t e = hipStreamSynchronize(B->stream.s);
+    if (e != hipSuccess) {
+      delete B;
+      throw Error(NC_ERR_HIP, std::string("batch copy failed: ") + hipGetErrorString(e));
+    }
+    {
+      // every instance's work is complete (synchronised above), so their buffers go back
+      // to the cache without a device-wide synchronisation each
+      PrivateStreams scope;
+      mats.clear();
+      spaces.clear();
+    }
+    *out = B;
+  });
+}
 
 int nc_batch_create(int32_t n, const nc_matrix* mats, nc_batch* out) {
   return guard([&] {

@@ -88,6 +88,10 @@ struct PrivateStreams {
   PrivateStreams(const PrivateStreams&) = delete;
   PrivateStreams& operator=(const PrivateStreams&) = delete;
 };
+// For such a thread whose finished objects outlive their work (a builder that hands its
+// objects on): once their streams have been synchronised, the thread stops tracking them,
+// so that its later releases wait only for the streams of the work still in progress.
+void streams_settled();
 
 // ---- device buffers -------------------------------------------------------------------
 template <class T>

@@ -237,4 +237,6 @@ void release_sync() {
 PrivateStreams::PrivateStreams() { ++tl_private; }
 PrivateStreams::~PrivateStreams() { --tl_private; }
 
+void streams_settled() { tl_streams.clear(); }
+
 }  // namespace nc

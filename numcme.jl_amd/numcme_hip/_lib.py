@@ -184,6 +184,7 @@ SIGNATURES = {
     "nc_solver_create_sens": [P, C.POINTER(nc_solve_opts), C.POINTER(P)],
     "nc_solve_sens_fixed": [P, C.POINTER(nc_solve_opts), PD, I32, PD, PD, C.POINTER(nc_solve_stats)],
     "nc_batch_create": [I32, C.POINTER(P), C.POINTER(P)],
+    "nc_batch_create_models": [I32, C.POINTER(P), C.POINTER(C.c_void_p), PI64, I32, I32, C.POINTER(P)],
     "nc_batch_destroy": [P],
     "nc_batch_info": [P, PI64, PI64],
     "nc_batch_solve": [P, C.POINTER(nc_solve_opts), PD, I32, PD, C.POINTER(nc_solve_stats)],

@@ -23,16 +23,43 @@ class BatchFsp(L.Handle):
         arr = (C.c_void_p * len(self.matrices))(*[m.handle for m in self.matrices])
         h = C.c_void_p()
         L.call("nc_batch_create", len(self.matrices), arr, C.byref(h))
-        super().__init__(h.value)
+        self._init_handle(h.value, len(self.matrices))
+
+    @classmethod
+    def from_models(cls, models, states, precision="fp32", workers=0):
+        """The batch of the generators of ``models[i]`` on the explicit state lists
+        ``states[i]`` (rows of species counts), built by worker threads of the library
+        (``nc_batch_create_models``) without a matrix object per instance."""
+        models, states = list(models), [np.ascontiguousarray(np.asarray(s, dtype=np.int32)) for s in states]
+        if len(models) != len(states) or not models:
+            raise ArgumentError(L.NC_ERR_ARG, "one state list per model, at least one instance")
+        for m, st in zip(models, states):
+            if st.ndim != 2 or st.shape[1] != m.get_species_count():
+                raise ArgumentError(L.NC_ERR_ARG, "each state list must have one column per species")
+        n = len(models)
+        hm = (C.c_void_p * n)(*[m.device_model().handle for m in models])
+        hs = (C.c_void_p * n)(*[st.ctypes.data for st in states])
+        counts = np.array([len(st) for st in states], dtype=np.int64)
+        prec = L.NC_FP32 if precision == "fp32" else L.NC_FP64
+        h = C.c_void_p()
+        self = cls.__new__(cls)
+        self.matrices = []
+        L.call("nc_batch_create_models", n, hm, hs, L.ptr(counts, C.c_int64), prec, int(workers), C.byref(h))
+        self._init_handle(h.value, n)
+        return self
+
+    def _init_handle(self, handle, n):
+        L.Handle.__init__(self, handle)
+        self.n_instances = n
         total = C.c_int64()
-        rows = np.empty(len(self.matrices), dtype=np.int64)
+        rows = np.empty(n, dtype=np.int64)
         L.call("nc_batch_info", self.handle, C.byref(total), L.ptr(rows, C.c_int64))
         self.total_rows = total.value
         self.rowcounts = rows
         self.stats = None
 
     def __len__(self):
-        return len(self.matrices)
+        return self.n_instances
 
     def autotune(self, reps=3):
         """Pick the kernels' rows per thread (1, 2 or 4) by timing batched iterations;

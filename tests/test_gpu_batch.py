@@ -113,3 +113,40 @@ def test_batch_twelve_slots_vs_oracle(gpu, precision, bar):
         for k in range(len(ts)):
             assert np.abs(snap[k][:n] - ref.p[k]).sum() <= bar
             assert np.abs(snap[k][n:] - ref.sinks[k]).sum() <= bar
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_from_models_equals_from_matrices(gpu, precision):
+    # nc_batch_create_models (spaces and generators built by worker threads of the library)
+    # gives the batch of the same generators: every snapshot bit-identical
+    inst, mats, u0s = _instances(precision)
+    cms = [nh.CmeModel(S, models.to_product(props), th) for S, props, th, _ in inst]
+    B1 = nh.BatchFsp(mats)
+    for workers in (1, 3):
+        B2 = nh.BatchFsp.from_models(cms, [np.asarray(st) for _, _, _, st in inst], precision, workers=workers)
+        assert len(B2) == len(B1) and np.array_equal(B2.rowcounts, B1.rowcounts)
+        ts = np.array([0.0, 1.0, 3.0])
+        s1 = B1.solve(u0s, ts, rtol=1e-9, atol=1e-15)
+        s2 = B2.solve(u0s, ts, rtol=1e-9, atol=1e-15)
+        for a, b in zip(s1, s2):
+            for k in range(len(ts)):
+                assert np.array_equal(a[k], b[k])
+
+
+def test_batch_from_models_errors(gpu):
+    S, props, th = models.telegraph((0.1, 0.1, 5.0, 1.0))
+    cm = nh.CmeModel(S, models.to_product(props), th)
+    good = np.asarray(models.telegraph_states(20))
+    empty = np.zeros((0, 3), dtype=np.int32)
+    with pytest.raises(nh.ArgumentError, match="instance 1"):
+        nh.BatchFsp.from_models([cm, cm, cm], [good, empty, good], "fp32", workers=2)
+    with pytest.raises(nh.ArgumentError):
+        nh.BatchFsp.from_models([cm, cm], [good], "fp32")
+    # a time-varying generator is built by the workers, then refused by the batch
+    Stv, props_tv, th_tv = models.telegraph_tv("sep")
+    tv = nh.CmeModel(Stv, models.to_product(props_tv), th_tv)
+    with pytest.raises(nh.NumCmeHipError, match="time-invariant"):
+        nh.BatchFsp.from_models([cm, tv], [good, np.asarray(models.box_states([1, 1, 20]))], "fp32", workers=2)
+    # the library is usable after the failure
+    B = nh.BatchFsp.from_models([cm, cm], [good, good], "fp32")
+    assert len(B) == 2
