@@ -306,9 +306,8 @@ int nc_fspvec_get(nc_fspvec v, int32_t* states, double* values) {
     Stream st;
     hipStream_t s = st.s;
     if (v->n > 0) {
-      if (states && v->ns > 0)
-        NC_CHECK_HIP(hipMemcpyAsync(states, v->states->p, sizeof(int32_t) * v->n * v->ns, hipMemcpyDeviceToHost, s));
-      if (values) NC_CHECK_HIP(hipMemcpyAsync(values, v->values.p, sizeof(double) * v->n, hipMemcpyDeviceToHost, s));
+      if (states && v->ns > 0) copy_d2h(states, v->states->p, sizeof(int32_t) * v->n * v->ns, s);
+      if (values) copy_d2h(values, v->values.p, sizeof(double) * v->n, s);
     }
     NC_CHECK_HIP(hipStreamSynchronize(s));
   });

@@ -92,6 +92,9 @@ struct PrivateStreams {
 // objects on): once their streams have been synchronised, the thread stops tracking them,
 // so that its later releases wait only for the streams of the work still in progress.
 void streams_settled();
+// Copy bytes from device memory to pageable host memory on stream s and wait for it
+// (large copies go through pinned staging chunks with the transfers overlapped).
+void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // ---- device buffers -------------------------------------------------------------------
 template <class T>

@@ -10,6 +10,7 @@
 // The caches live until the process ends and are never torn down (the HIP runtime may
 // already be gone when static destructors run).
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -238,5 +239,47 @@ PrivateStreams::PrivateStreams() { ++tl_private; }
 PrivateStreams::~PrivateStreams() { --tl_private; }
 
 void streams_settled() { tl_streams.clear(); }
+
+// Device-to-host copy into pageable memory through two pinned chunks: the DMA of chunk
+// c + 1 runs while the host copies chunk c out of its pinned buffer (a direct copy to
+// pageable memory measured 6-9 GB/s here for snapshots of 8 MB).
+void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  constexpr size_t kChunk = size_t(4) << 20;
+  if (bytes < (size_t(1) << 20)) {
+    NC_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+  };
+  thread_local Events ev;  // one pair per host thread, kept until the process ends
+  for (auto& e : ev.e)
+    if (!e) NC_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  unsigned char* buf = static_cast<unsigned char*>(pinned_alloc(2 * kChunk));
+  try {
+    const size_t n = (bytes + kChunk - 1) / kChunk;
+    const auto* sp = static_cast<const unsigned char*>(src);
+    auto* dp = static_cast<unsigned char*>(dst);
+    auto issue = [&](size_t c) {
+      const size_t off = c * kChunk, len = std::min(kChunk, bytes - off);
+      NC_CHECK_HIP(hipMemcpyAsync(buf + (c & 1) * kChunk, sp + off, len, hipMemcpyDeviceToHost, s));
+      NC_CHECK_HIP(hipEventRecord(ev.e[c & 1], s));
+    };
+    issue(0);
+    for (size_t c = 0; c < n; ++c) {
+      // buffer (c + 1) & 1 was emptied by the host copy of chunk c - 1 just before
+      if (c + 1 < n) issue(c + 1);
+      NC_CHECK_HIP(hipEventSynchronize(ev.e[c & 1]));
+      const size_t off = c * kChunk, len = std::min(kChunk, bytes - off);
+      std::memcpy(dp + off, buf + (c & 1) * kChunk, len);
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    pinned_free(buf);
+    throw;
+  }
+  pinned_free(buf);
+}
 
 }  // namespace nc
