@@ -761,7 +761,7 @@ def main():
                     help="run one leg alone and print its figures (for profiler runs); no contract line. "
                          "c5-solve: train C5's network and save it to --net-file, or, when that file exists, "
                          "load it and run two learned FP32 solves")
-    ap.add_argument("--net-file", default="gpurun_out/c5_net.npy", help="network parameters for --only c5-solve")
+    ap.add_argument("--net-file", default="c5_net.npy", help="network parameters for --only c5-solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; runs on more than one GPU are not supported (how "
