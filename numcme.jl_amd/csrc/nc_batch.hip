@@ -139,10 +139,16 @@ struct SinkSegs {
   const int32_t* src;   // source state rows of the entries
 };
 
-template <typename VT, int R, int KT>
+// DM 1: instances whose every slot column is the row plus a constant (dia_inst[inst] = 1,
+// a space in lattice order; the offsets are dia_off[inst * Kb + s]) compute those columns,
+// clamped to the instance's states (rows whose value is 0 may point anywhere), instead of
+// loading them; the other instances load theirs. A wave's rows lie in one granule, so the
+// instance, its kind and its offsets are uniform and the choice is a scalar branch.
+template <typename VT, int R, int KT, int DM>
 __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, const double* diag, const int32_t* col,
                                                     const VT* val, const VT* v, double* out, SinkSegs sg,
-                                                    const VT* sval) {
+                                                    const VT* sval, const int32_t* dia_off, const int32_t* dia_inst,
+                                                    int Kb) {
   if ((int)blockIdx.x < sg.nseg) {
     // sink row of one (instance, reaction) segment: one workgroup, deterministic tree
     const int seg = blockIdx.x;
@@ -162,9 +168,9 @@ __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, co
   for (int64_t un = (int64_t)blockIdx.x - sg.nseg; un < units; un += (int64_t)gridDim.x - sg.nseg) {
     int64_t n0;
     if (!unit_rows<R>(d, nl, un, n0)) continue;
-    const int inst = d.gran_inst[n0 / kGran];
+    const int inst = __builtin_amdgcn_readfirstlane(d.gran_inst[n0 / kGran]);
     if (d.h[inst] == 0.0) continue;
-    const int64_t nend = d.row_off[inst] + d.nstates[inst];
+    const int64_t nbeg = d.row_off[inst], nend = nbeg + d.nstates[inst];
     if (n0 >= nend) continue;
     const Pack<double, R> dg = *reinterpret_cast<const Pack<double, R>*>(diag + n0);
     const Pack<VT, R> vi = *reinterpret_cast<const Pack<VT, R>*>(v + n0);
@@ -172,10 +178,23 @@ __global__ __launch_bounds__(kBlock) void k_bmatvec(Dev d, int64_t ld, int K, co
     if constexpr (KT > 0) {
       Pack<int32_t, R> c[KT];
       Pack<VT, R> w[KT];
+      if (DM == 1 && dia_inst[inst]) {
 #pragma unroll
-      for (int s = 0; s < KT; ++s) {
-        c[s] = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
-        w[s] = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+        for (int s = 0; s < KT; ++s) {
+          const int64_t off = dia_off[(int64_t)inst * Kb + s];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int64_t j = n0 + r + off;
+            c[s].v[r] = (int32_t)(j < nbeg ? nbeg : (j >= nend ? nend - 1 : j));
+          }
+          w[s] = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < KT; ++s) {
+          c[s] = *reinterpret_cast<const Pack<int32_t, R>*>(col + (int64_t)s * ld + n0);
+          w[s] = *reinterpret_cast<const Pack<VT, R>*>(val + (int64_t)s * ld + n0);
+        }
       }
       double gv[KT][R];
 #pragma unroll
@@ -542,6 +561,10 @@ struct nc_batch_s {
   nc::DevBuf<double> diag;
   nc::DevBuf<int32_t> col;
   nc::DevBuf<unsigned char> val;
+  // instances whose slot columns are the row plus a constant (dia_inst[i] = 1) and their
+  // offsets (n x K): the mat-vec computes those columns instead of loading them
+  bool any_dia = false;
+  nc::DevBuf<int32_t> dia_off, dia_inst;
   int nseg = 0;
   nc::DevBuf<int32_t> seg_inst, sink_src;
   nc::DevBuf<int64_t> seg_row, seg_off;
@@ -586,6 +609,23 @@ void batch_build(nc_batch_s* B, const nc_matrix_s* const* mats) {
           A->n, A->ld, A->K, B->K, B->row_off[i], B->ld, A->diag.p, A->col.p, reinterpret_cast<const VT*>(A->val.p),
           B->diag.p, B->col.p, reinterpret_cast<VT*>(B->val.p));
     NC_CHECK_HIP(hipGetLastError());
+  }
+  // computed columns: the instances in lattice order (each of their slots a constant offset)
+  {
+    std::vector<int32_t> offs((size_t)B->n * std::max(B->K, 1), 0), flag(B->n, 0);
+    for (int i = 0; i < B->n; ++i) {
+      const nc_matrix_s* A = mats[i];
+      const uint32_t need = A->K >= 32 ? 0xffffffffu : ((1u << A->K) - 1u);
+      if (A->K > (int)A->dia_off.size() || (A->dia_mask & need) != need) continue;
+      flag[i] = 1;
+      B->any_dia = true;
+      for (int k = 0; k < A->K; ++k) offs[(size_t)i * B->K + k] = A->dia_off[k];
+    }
+    B->dia_off.alloc(offs.size());
+    B->dia_inst.alloc(flag.size());
+    NC_CHECK_HIP(hipMemcpyAsync(B->dia_off.p, offs.data(), sizeof(int32_t) * offs.size(), hipMemcpyHostToDevice, s));
+    NC_CHECK_HIP(hipMemcpyAsync(B->dia_inst.p, flag.data(), sizeof(int32_t) * flag.size(), hipMemcpyHostToDevice, s));
+    NC_CHECK_HIP(hipStreamSynchronize(s));
   }
   // sink segments: (instance, reaction), entries concatenated
   std::vector<int32_t> seg_inst;
@@ -756,19 +796,26 @@ void launch_matvec(nc_batch_s* B, const VT* v, double* out) {
   const unsigned grid = row_grid(B, R) + (unsigned)B->nseg;
   auto go = [&](auto kern) {
     kern<<<grid, kBlock, 0, s>>>(d, B->ld, B->K, B->diag.p, B->col.p, reinterpret_cast<const VT*>(B->val.p), v, out,
-                                 sg, reinterpret_cast<const VT*>(B->sink_val.p));
+                                 sg, reinterpret_cast<const VT*>(B->sink_val.p), B->dia_off.p, B->dia_inst.p, B->K);
   };
-  switch (B->K) {
-    case 1: go(k_bmatvec<VT, R, 1>); break;
-    case 2: go(k_bmatvec<VT, R, 2>); break;
-    case 3: go(k_bmatvec<VT, R, 3>); break;
-    case 4: go(k_bmatvec<VT, R, 4>); break;
-    case 5: go(k_bmatvec<VT, R, 5>); break;
-    case 6: go(k_bmatvec<VT, R, 6>); break;
-    case 7: go(k_bmatvec<VT, R, 7>); break;
-    case 8: go(k_bmatvec<VT, R, 8>); break;
-    default: go(k_bmatvec<VT, R, 0>); break;
-  }
+  auto by_k = [&](auto dm) {
+    constexpr int DM = decltype(dm)::value;
+    switch (B->K) {
+      case 1: go(k_bmatvec<VT, R, 1, DM>); break;
+      case 2: go(k_bmatvec<VT, R, 2, DM>); break;
+      case 3: go(k_bmatvec<VT, R, 3, DM>); break;
+      case 4: go(k_bmatvec<VT, R, 4, DM>); break;
+      case 5: go(k_bmatvec<VT, R, 5, DM>); break;
+      case 6: go(k_bmatvec<VT, R, 6, DM>); break;
+      case 7: go(k_bmatvec<VT, R, 7, DM>); break;
+      case 8: go(k_bmatvec<VT, R, 8, DM>); break;
+      default: go(k_bmatvec<VT, R, 0, 0>); break;
+    }
+  };
+  if (B->any_dia)
+    by_k(std::integral_constant<int, 1>{});
+  else
+    by_k(std::integral_constant<int, 0>{});
   NC_CHECK_HIP(hipGetLastError());
 }
 
