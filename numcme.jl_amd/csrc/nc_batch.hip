@@ -495,16 +495,16 @@ __global__ __launch_bounds__(kCtlBlock) void k_bcontrol(Ctl c, int init) {
   __syncthreads();
   atomicAdd(&nrun, running);
   __syncthreads();
-  // the list: entry w belongs to the instance whose [off, off + count) holds it
+  // the list: instance i's granules at entries [off[i], off[i + 1]), written by one wave
+  // per instance (instances dealt to the waves in turn), 64 consecutive entries at a time
   const uint32_t total = c.off[c.n];
-  for (uint32_t w = threadIdx.x; w < total; w += kCtlBlock) {
-    int lo = 0, hi = c.n - 1;  // last instance with off <= w
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (c.off[mid] <= w) lo = mid;
-      else hi = mid - 1;
+  {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = wave; i < c.n; i += kCtlBlock / 64) {
+      const uint32_t o0 = c.off[i], o1 = c.off[i + 1];
+      const int64_t g0 = c.gran_start[i];
+      for (uint32_t w = o0 + lane; w < o1; w += 64) c.agran[w] = (int32_t)(g0 + (w - o0));
     }
-    c.agran[w] = (int32_t)(c.gran_start[lo] + (w - c.off[lo]));
   }
   if (threadIdx.x == 0) {
     *c.n_agran = total;
