@@ -50,6 +50,10 @@ enum { NC_FP64 = 0, NC_FP32 = 1 };
 int nc_last_error(char* buf, size_t len);
 /* Library version and the HIP device it runs on. */
 int nc_version(int32_t* major, int32_t* minor);
+/* Return the device blocks the library keeps cached for reuse (freed objects' memory) to
+ * the driver; synchronises the device. No reference counterpart (Julia's GC plays this
+ * part for the reference's arrays). */
+int nc_memory_trim(void);
 int nc_device_name(char* buf, size_t len);
 
 /* ======================================================================================

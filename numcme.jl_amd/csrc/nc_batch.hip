@@ -1067,7 +1067,12 @@ int nc_batch_create_models(int32_t n, const nc_model* models, const int32_t* con
           mats[i].reset(matrix_create(spaces[i].get(), m, precision));
           NC_CHECK_HIP(hipStreamSynchronize(spaces[i]->stream.s));
           NC_CHECK_HIP(hipStreamSynchronize(mats[i]->stream.s));
-          streams_settled();  // this instance is complete: later releases need not wait for it
+          // this instance is complete and only read by the batch's copies from here on:
+          // its streams serve the next instance (a worker keeps a few streams in use, not
+          // two per instance), and later releases need not wait for it
+          spaces[i]->stream.release_now();
+          mats[i]->stream.release_now();
+          streams_settled();
         } catch (const Error& e) {
           fail(i, e.code, e.what());
         } catch (const std::bad_alloc&) {

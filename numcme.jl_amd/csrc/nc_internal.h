@@ -163,6 +163,12 @@ struct Stream {
   ~Stream() { if (s) stream_release(s); }
   Stream(const Stream&) = delete;
   Stream& operator=(const Stream&) = delete;
+  // Give the stream back before the owner goes (its work is complete, and the owner will
+  // only be read by work on other streams from here on); s becomes the null stream.
+  void release_now() {
+    if (s) stream_release(s);
+    s = nullptr;
+  }
 };
 
 }  // namespace nc

@@ -23,6 +23,11 @@ namespace {
 constexpr size_t kSmallGrain = 4096;
 constexpr size_t kLargeGrain = size_t(2) << 20;
 constexpr size_t kCacheCap = size_t(16) << 30;
+// Streams kept for reuse per device. Every object holds a stream while it lives, so a user
+// holding many objects at once (a batch built from hundreds of generators) makes as many;
+// keeping them all afterwards would slow every later device-wide synchronisation, which
+// visits each stream.
+constexpr size_t kStreamCacheCap = 32;
 
 size_t round_bytes(size_t b) {
   const size_t g = b > (size_t(1) << 20) ? kLargeGrain : kSmallGrain;
@@ -221,10 +226,18 @@ void stream_release(hipStream_t s) {
       break;
     }
   Cache& c = cache();
-  std::lock_guard<std::mutex> lk(c.mu);
-  auto it = c.stream_dev.find(s);
-  if (it == c.stream_dev.end()) return;
-  c.streams[it->second].push_back(s);
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.stream_dev.find(s);
+    if (it == c.stream_dev.end()) return;
+    auto& v = c.streams[it->second];
+    if (v.size() < kStreamCacheCap) {
+      v.push_back(s);
+      return;
+    }
+    c.stream_dev.erase(it);
+  }
+  (void)hipStreamDestroy(s);  // beyond the cache's cap (its work is complete)
 }
 
 void release_sync() {

@@ -186,6 +186,13 @@ int nc_last_error(char* buf, size_t len) {
   return NC_OK;
 }
 
+int nc_memory_trim(void) {
+  return guard([&] {
+    NC_CHECK_HIP(hipDeviceSynchronize());
+    pool_trim();
+  });
+}
+
 int nc_version(int32_t* major, int32_t* minor) {
   if (major) *major = 0;
   if (minor) *minor = 1;

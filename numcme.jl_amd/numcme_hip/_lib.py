@@ -141,6 +141,7 @@ PU8 = C.POINTER(C.c_uint8)
 # name -> argtypes (all functions return int status)
 SIGNATURES = {
     "nc_last_error": [C.c_char_p, C.c_size_t],
+    "nc_memory_trim": [],
     "nc_version": [PI32, PI32],
     "nc_device_name": [C.c_char_p, C.c_size_t],
     "nc_space_create": [I32, I32, PI32, I64, PI32, C.POINTER(P)],
