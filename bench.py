@@ -33,17 +33,20 @@ sys.path.insert(0, str(ROOT / "numcme.jl_amd"))
 
 HBM_PEAK_GBPS = 8000.0
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_matvec_toggle1024_fp32.md"
+PROJECTION_PROFILE = ROOT / "profiles" / "k_matvec_projection_toggle1024_fp32.md"
 
 
-def measured_traffic(rows_per_thread, precision, matrix_free):
+def measured_traffic(rows_per_thread, precision, matrix_free, projection=False):
     """HBM bytes per mat-vec launch from the committed rocprofv3 counter runs (FETCH_SIZE
-    plus WRITE_SIZE, calibrated on a copy kernel), for the mode and launch shape in use;
-    None when no counter run covers it."""
-    if precision != "fp32" or not TRAFFIC_PROFILE.exists():
+    plus WRITE_SIZE, calibrated on a copy kernel), for the mode and launch shape in use
+    (projection: the mat-vec with the Krylov projection epilogue, averaged over a solve's
+    launches); None when no counter run covers it."""
+    prof = PROJECTION_PROFILE if projection else TRAFFIC_PROFILE
+    if precision != "fp32" or not prof.exists():
         return None
-    mode = {0: "stored", 1: "computed_values", 2: "matrix_free"}[int(matrix_free)]
+    mode = "projection" if projection else {0: "stored", 1: "computed_values", 2: "matrix_free"}[int(matrix_free)]
     key = f"traffic_bytes_per_launch mode={mode} rows_per_thread={rows_per_thread}:"
-    for line in TRAFFIC_PROFILE.read_text().splitlines():
+    for line in prof.read_text().splitlines():
         if line.startswith(key):
             return float(line[len(key):].strip())
     return None
@@ -687,6 +690,11 @@ def spmv_probe(nh, precision, reps):
             times[f"{names[mode]} rows_per_thread={rows}"] = A.bench_matvec(reps=reps) * 1e3
     A.set_matrix_free(default)
     A.set_rows_per_thread(2)
+    # the mat-vec with its Krylov projection epilogue over the basis sizes of one Krylov
+    # step at m = 30 (1..30 vectors for the columns, 1 for the norm of A v_m), reps
+    # launches each
+    proj = {nv: A.bench_projection(nv, reps=reps) * 1e3 for nv in range(1, 31)}
+    proj_step_mean = (sum(proj.values()) + proj[1]) / 31
     x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
     y = torch.empty_like(x)
     for _ in range(3):
@@ -694,6 +702,7 @@ def spmv_probe(nh, precision, reps):
     torch.cuda.synchronize()
     info = A.info()
     print(json.dumps({"probe": "spmv", "precision": precision, "reps": reps, "mean_launch_us": times,
+                      "projection_mean_launch_us": proj, "projection_step_mix_mean_us": proj_step_mean,
                       "csr_bytes_per_matvec": info["csr_bytes_per_matvec"],
                       "own_bytes_per_matvec_default_mode": info["bytes_per_matvec"],
                       "copy_bytes_each_way": x.numel() * 4}), flush=True)
@@ -730,8 +739,14 @@ def c2_end_to_end(nh, precision, krylov_tol, ts):
     p0[0] = 1.0
     out = nh.solve(model, nh.FspVectorSparse(states, p0, copy=False), (0.0, float(ts[-1])),
                    nh.KrylovExpv(m=30, tol=krylov_tol, precision=precision), saveat=ts)
-    snaps = [np.concatenate([p.values, s]) for p, s in zip(out.p, out.sinks)]
+    # the download of every snapshot: (probabilities, sink values) per output time
+    snaps = [(p.values, s) for p, s in zip(out.p, out.sinks)]
     return snaps, out.stats
+
+
+def snap_l1(a, b):
+    """l1 distance of two downloaded snapshots (probabilities and sinks)"""
+    return float(np.abs(a[0] - b[0]).sum() + np.abs(np.asarray(a[1]) - np.asarray(b[1])).sum())
 
 
 def main():
@@ -813,13 +828,13 @@ def main():
     for _ in range(args.steps):
         snaps, stats = c2_end_to_end(nh, args.precision, args.krylov_tol, ts)
     elapsed = time.perf_counter() - t0
-    l1 = max(float(np.abs(snaps[k] - ref[k]).sum()) for k in range(len(ts)))
+    l1 = max(snap_l1(snaps[k], ref[k]) for k in range(len(ts)))
     say(f"c2: {args.steps / elapsed:.3f} solves/s end to end, l1 vs fp64 {l1:.2e}")
     # the plain FP64 path on the same clock and tolerance
     t0 = time.perf_counter()
     snaps64, stats64 = c2_end_to_end(nh, "fp64", args.krylov_tol, ts)
     fp64_s = time.perf_counter() - t0
-    l1_64 = max(float(np.abs(snaps64[k] - ref[k]).sum()) for k in range(len(ts)))
+    l1_64 = max(snap_l1(snaps64[k], ref[k]) for k in range(len(ts)))
 
     # the same solves with inputs resident on the device (restart from the uploaded state)
     model = toggle_model(nh)
@@ -841,9 +856,11 @@ def main():
     sol.free()
 
     # mat-vec roofline: one more solve with HIP events recorded around every mat-vec
-    # launch on the solver's stream (graphs off, since events cannot sit inside them)
-    def timed_solve():
-        tim = Solver(L, A, args.precision, True, tol=args.krylov_tol)
+    # launch on the solver's stream (graphs off, since events cannot sit inside them):
+    # mode 1 times the launches as the solves run them (the mat-vec with its projection
+    # epilogue), mode 2 the plain mat-vec (the projection sums then run as a separate pass)
+    def timed_solve(mode=1):
+        tim = Solver(L, A, args.precision, mode, tol=args.krylov_tol)
         tim.set_state(u0)
         tim.solve(ts)
         st = tim.stats()
@@ -851,14 +868,24 @@ def main():
         return (st["matvec_ms"] / st["matvecs_timed"]) * 1e-3 if st["matvecs_timed"] else float("nan"), \
             st["matvecs_timed"]
 
-    avg_mv_s, mv_n = timed_solve()
-    back_to_back_ms = A.bench_matvec(reps=200)
     mf_on = bool(info["matrix_free"])
+    # the stored generator at 2 rows per thread (the default shape) carries the epilogue
+    projection = not mf_on and info["rows_per_thread"] == 2
+    avg_mv_s, mv_n = timed_solve(1)
+    avg_plain_s, plain_n = timed_solve(2) if projection else (avg_mv_s, mv_n)
+    back_to_back_ms = A.bench_matvec(reps=200)
+    # the epilogue kernel back to back over one Krylov step's basis sizes (1..30, then 1)
+    proj_b2b_ms = ((sum(A.bench_projection(nv, reps=20) for nv in range(1, 31)) + A.bench_projection(1, reps=20)) / 31
+                   if projection else None)
+    # the projection epilogue reads the column's basis vectors V_0..V_j at the state rows:
+    # j + 1 = 1..m for the m columns and 1 for the norm of A v_m, m + 1 launches a step
+    m_k = 30
+    proj_bytes = (m_k * (m_k + 1) // 2 + 1) / (m_k + 1) * info["n_states"] * (4 if args.precision == "fp32" else 8)
     stored = None
     if mf_on:
         # the same kernel reading the stored generator, for comparison
         A.set_matrix_free(0)
-        s_avg, _ = timed_solve()
+        s_avg, _ = timed_solve(2)
         s_info = A.info()
         stored = {"avg_launch_us": s_avg * 1e6, "achieved_csr_equivalent": info["csr_bytes_per_matvec"] / s_avg / 1e9,
                   "bytes_per_launch_own_layout": s_info["bytes_per_matvec"],
@@ -867,7 +894,11 @@ def main():
         A.set_matrix_free(info["matrix_free"])
     rows_tuned, _ = A.autotune(reps=30)  # reported only: the timed solves use the default shape
     A.set_rows_per_thread(info["rows_per_thread"])
-    achieved = info["csr_bytes_per_matvec"] / avg_mv_s / 1e9
+    # the dominant kernel of the timed solves: the mat-vec with its projection epilogue
+    # (CSR-equivalent mat-vec bytes plus the basis vectors it reads), or the plain mat-vec
+    launch_bytes = info["csr_bytes_per_matvec"] + (proj_bytes if projection else 0.0)
+    achieved = launch_bytes / avg_mv_s / 1e9
+    spmv_achieved = info["csr_bytes_per_matvec"] / avg_plain_s / 1e9
     line = {
         "metric": "FSP solves/s at \u21131\u22641e-6 (batched+learned vs plain FP64); SpMV HBM GB/s",
         "value": args.steps / elapsed,
@@ -904,18 +935,33 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": measured_traffic(info["rows_per_thread"], args.precision, info["matrix_free"]),
-            "traffic_source": TRAFFIC_PROFILE.name,
-            "kernel": "k_matvec (FSP SpMV), " + ("computed values: off-diagonal values from per-reaction tables, "
-                                                 "columns from row offsets, the FP64 diagonal read; achieved "
-                                                 "counts the CSR-equivalent bytes" if mf_on else "stored generator"),
+            "traffic": measured_traffic(info["rows_per_thread"], args.precision, info["matrix_free"], projection),
+            "traffic_source": (PROJECTION_PROFILE if projection else TRAFFIC_PROFILE).name,
+            "kernel": ("k_matvec with the Arnoldi projection epilogue (FSP SpMV, stored generator, plus the "
+                       "column's projection sums over 512-row blocks)" if projection else
+                       "k_matvec (FSP SpMV), " + ("computed values: off-diagonal values from per-reaction tables, "
+                                                  "columns from row offsets, the FP64 diagonal read; achieved "
+                                                  "counts the CSR-equivalent bytes" if mf_on else "stored generator")),
+            "bytes_per_launch": launch_bytes,
             "bytes_per_launch_csr": info["csr_bytes_per_matvec"],
+            "bytes_per_launch_projection": proj_bytes if projection else 0.0,
             "bytes_per_launch_own_layout": info["bytes_per_matvec"],
-            "achieved_own_bytes": info["bytes_per_matvec"] / avg_mv_s / 1e9,
             "stored_generator": stored,
             "avg_launch_us": avg_mv_s * 1e6,
             "launches_timed": mv_n,
-            "back_to_back_launch_us": back_to_back_ms * 1e3,
+            "back_to_back_step_mix_launch_us": proj_b2b_ms * 1e3 if projection else None,
+            "spmv": {
+                "kernel": "k_matvec (FSP SpMV), stored generator, timed in a solve whose projection sums run as "
+                          "a separate pass" if projection else "as above",
+                "avg_launch_us": avg_plain_s * 1e6,
+                "launches_timed": plain_n,
+                "achieved_csr_equivalent": spmv_achieved,
+                "frac": spmv_achieved / HBM_PEAK_GBPS,
+                "achieved_own_bytes": info["bytes_per_matvec"] / avg_plain_s / 1e9,
+                "traffic": measured_traffic(info["rows_per_thread"], args.precision, info["matrix_free"]),
+                "traffic_source": TRAFFIC_PROFILE.name,
+                "back_to_back_launch_us": back_to_back_ms * 1e3,
+            },
             "rows_per_thread": info["rows_per_thread"],
             "autotuned_rows_per_thread": rows_tuned,
             "computed_col_slots": info["computed_col_slots"],
@@ -940,7 +986,7 @@ def main():
             comps["batched_learned_vs_plain_fp64"] = c5["batched"]["speedup_batched_learned_vs_plain_fp64"]
             comps["batched_learned_fp64_vs_plain_fp64"] = c5["batched"]["speedup_batched_learned_fp64_vs_plain_fp64"]
             comps["batched_learned_solves_per_s"] = c5["batched"]["batched_solves_per_s"]
-    comps["spmv_hbm_GBps"] = achieved
+    comps["spmv_hbm_GBps"] = spmv_achieved
     line["metric_components"] = comps
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(model.parameters, args.krylov_tol, stats64["steps"], log=say)

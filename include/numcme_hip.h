@@ -196,6 +196,11 @@ int nc_matrix_export(nc_matrix A, double t, int64_t* nnz, int64_t* rows, int64_t
  * stored-precision input, FP64 output) and report the mean kernel duration from HIP events
  * on the matrix's stream. No reference counterpart. */
 int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms);
+/* The same for the mat-vec that the Krylov integrator runs with its projection epilogue:
+ * out = A v plus the per-block projection sums of nv basis vectors (1..32), v the last of
+ * them. NC_ERR_UNSUPPORTED for generators whose mat-vec has no such epilogue. No reference
+ * counterpart. */
+int nc_matrix_bench_projection(nc_matrix A, int32_t nv, int32_t reps, double* mean_ms);
 /* Computed mat-vecs. A generator on a box state space in lattice order (species 0 fastest,
  * at most 4 species, at most 8 distinct stoichiometries with at most 4 reactions each)
  * whose propensities are all time-invariant and each read at most one species keeps,
@@ -232,7 +237,10 @@ typedef struct {
   double rtol, atol;   /* RK tolerances (defaults 1e-8, 1e-14)                     */
   double h0;           /* first RK step, 0 = automatic                             */
   int64_t max_steps;   /* 0 = unlimited                                            */
-  int32_t time_kernels;/* record HIP events around every mat-vec launch            */
+  int32_t time_kernels;/* record HIP events around every mat-vec launch: 1 as the
+                          solve runs (Krylov: the mat-vec with its projection sums where
+                          the generator allows), 2 plain mat-vecs (the projection sums by
+                          a separate pass)                                          */
   int32_t no_graphs;  /* 1: launch kernels directly instead of replaying HIP graphs */
 } nc_solve_opts;
 

@@ -75,8 +75,10 @@ struct nc_matrix_s {
   // reporting
   int64_t nnz_ref = 0, n_sink_entries = 0;
   double norm1 = 0.0;
-  // kernel timing (HIP events around mat-vec launches)
+  // kernel timing (HIP events around mat-vec launches); timing_plain: the launches timed
+  // are plain mat-vecs (the Krylov projection sums then run as a separate pass)
   bool timing = false;
+  bool timing_plain = false;
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
   double timed_ms = 0.0;
@@ -118,6 +120,22 @@ struct RkEpilogue {
 bool matvec_rk_fusable(const nc_matrix_s* A);
 // out = A(t) v (FP64) and the epilogue's stage input
 void matvec_launch_rk(nc_matrix_s* A, double t, const void* v, double* out, hipStream_t s, const RkEpilogue& epi);
+// Arnoldi projection sums computed by the mat-vec (nc_dual.h): besides out = A v, every
+// row workgroup writes the sums of its 512 rows, with v = V_{nv-1} and q = A v, into
+// part (value-major, nblk = the mat-vec's row workgroups). Time-invariant stored
+// generators with 2 rows per thread and 1..8 slots whose columns are all loaded or all
+// computed, unless plain mat-vecs are being timed; other generators take the separate
+// block pass.
+struct DualDots {
+  const void* V;     // V_0 .. V_{nv-1}, matrix precision, stride ld
+  int64_t ld;
+  int32_t nv;        // 1..32
+  const double* sc;  // scale of each V_i . q sum (null: 1)
+  double* part;
+  int64_t nblk;
+};
+bool matvec_dots_fusable(const nc_matrix_s* A);
+void matvec_launch_dots(nc_matrix_s* A, const void* v, double* out, hipStream_t s, const DualDots& d);
 // When kernel timing is on, hand out the next pair of HIP events for a dispatch-stamped
 // launch (the caller increments ev_used after the launch); false when timing is off.
 bool matvec_timing_slot(nc_matrix_s* A, hipEvent_t* e0, hipEvent_t* e1);

@@ -18,6 +18,7 @@
 
 #include <mutex>
 
+#include "nc_dual.h"
 #include "nc_matrix.h"
 
 nc_matrix_s::~nc_matrix_s() {
@@ -408,6 +409,12 @@ struct MatArgs {
   double* rk_ytmp;        // the last stage: y_{n+1} in FP64 too (else null)
   VT* rk_vin;             // the next mat-vec's input copy (not the buffer this launch reads)
   int32_t rk_mask;
+  // Arnoldi projection sums (DOT kernels): basis V_0..V_{dot_nv-1} with stride dot_ld
+  const VT* dot_V;
+  int64_t dot_ld;
+  int32_t dot_nv;
+  const double* dot_sc;
+  double* dot_part;
 };
 
 template <typename T, int R>
@@ -695,10 +702,99 @@ __device__ inline void rk_epilogue(const MatArgs<VT>& a, int64_t i, int64_t nw, 
   }
 }
 
+// Rows [i0, i0 + R) of the stored generator: out = A v (plus out when accumulating), the
+// FP64 row sums left in acc and v at the rows in vi (0 past the states).
+template <typename VT, typename OT, int KT, int R, int DM>
+__device__ __forceinline__ void stored_rows(const MatArgs<VT>& a, const VT* __restrict__ v, OT* __restrict__ out,
+                                            int64_t i0, int accumulate, double (&acc)[R], double (&vi)[R]) {
+  // Loads are unconditional (clamped indices, selects) and issued in two batches: the
+  // row's own entries and every slot's column and value, then every gather. A load
+  // under a branch is waited for at the end of its block, which would make the slots
+  // one memory round trip each.
+  const Pack<double, R> dg = ldpack<double, R>(a.diag + i0);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const double vr = (double)v[min(i0 + r, a.n - 1)];
+    vi[r] = i0 + r < a.n ? vr : 0.0;
+  }
+  const int K = KT > 0 ? KT : a.K;
+  // column of slot k for rows i0..i0+R-1: loaded, or i + dia_off[k] clamped to the
+  // states (rows whose value is 0 may point anywhere; they add nothing). DM fixes the
+  // kind for every slot (0 loaded, 1 computed) so the fixed-K kernels carry no branch;
+  // a matrix with both kinds uses the generic kernel (DM 2), which tests each slot.
+  auto slot_cols = [&](int k) {
+    Pack<int32_t, R> c;
+    const bool dia = DM == 1 || (DM == 2 && ((a.dia_mask >> k) & 1u));
+    if (dia) {
+      const int64_t hi = a.n - 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int64_t j = i0 + r + a.dia_off[k];
+        c.v[r] = (int32_t)(j < 0 ? 0 : (j > hi ? hi : j));
+      }
+    } else {
+      c = ldpack<int32_t, R>(a.col + k * a.ld + i0);
+    }
+    return c;
+  };
+  if constexpr (KT > 0) {
+    // values and columns (computed columns need no load), then every gather; with
+    // computed columns the gathers go out with the values, one round trip in all
+    Pack<int32_t, R> c[KT];
+    Pack<VT, R> w[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      w[k] = ldpack<VT, R>(a.val + k * a.ld + i0);
+      c[k] = slot_cols(k);
+    }
+    double gv[KT][R];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) gv[k][r] = (double)v[c[k].v[r]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // diag * v_i, then one fused multiply-add per slot in slot order (written out, so
+    // that every mode of the kernel rounds alike)
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = __dmul_rn(dg.v[r], vi[r]);
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = __fma_rn((double)w[k].v[r], gv[k][r], acc[r]);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = __dmul_rn(dg.v[r], vi[r]);
+    for (int k = 0; k < K; ++k) {
+      const Pack<int32_t, R> c = slot_cols(k);
+      const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = __fma_rn((double)w.v[r], (double)v[c.v[r]], acc[r]);
+    }
+  }
+  for (int q = 0; q < a.ntv; ++q) {
+    const Pack<VT, R> d = ldpack<VT, R>(a.tvd + q * a.ld + i0);
+    const Pack<int32_t, R> c = ldpack<int32_t, R>(a.tvc + q * a.ld + i0);
+    const Pack<VT, R> w = ldpack<VT, R>(a.tvv + q * a.ld + i0);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      acc[r] += a.tv_scale[q] * ((double)d.v[r] * vi[r] + (double)w.v[r] * (double)v[c.v[r]]);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = i0 + r;
+    if (i < a.n) out[i] = (OT)(accumulate ? (double)out[i] + acc[r] : acc[r]);
+  }
+}
+
 // Thread t owns state rows [R*t, R*t+R). Matrix streams are read with one R-wide vector
 // load per slot; v[i] and the predecessor gathers are scalar. EPI: the launch also forms
 // the next DP5(4) stage input (rk_epilogue) for its rows, state and sink rows alike.
-template <typename VT, typename OT, int KT, int R, int DM, bool EPI = false>
+// DOT: the launch also writes the Arnoldi projection sums of its 512 rows (nc_dual.h):
+// the rows' q = A v and v go through LDS to the layout of the block sums, so the
+// projection pass need not read q back.
+template <typename VT, typename OT, int KT, int R, int DM, bool EPI = false, bool DOT = false>
 __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __restrict__ v, OT* __restrict__ out,
                                                    int accumulate) {
   const int64_t row_threads = (a.n + R - 1) / R;
@@ -710,6 +806,30 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
     const int64_t lb = x * q + (x < r8 ? x : r8) + (b >> 3);
     const int64_t t = lb * kBlock + threadIdx.x;
     const int64_t i0 = t * R;
+    if constexpr (DOT) {
+      static_assert(R * kBlock == kDualRows && DM <= 2 && !EPI, "projection epilogue: 2 rows per thread, stored generator");
+      __shared__ double s_x[kDualRows], s_y[kDualRows];
+      double vi[R] = {0.0, 0.0}, acc[R] = {0.0, 0.0};
+      if (i0 < a.n) stored_rows<VT, OT, KT, R, DM>(a, v, out, i0, accumulate, acc, vi);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const bool live = i0 + r < a.n;  // vi is already 0 past the states
+        s_x[R * threadIdx.x + r] = live ? acc[r] : 0.0;
+        s_y[R * threadIdx.x + r] = vi[r];
+      }
+      __syncthreads();
+      const int lane = threadIdx.x & 63;
+      double xx[8], yy[8];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          xx[4 * h + r] = s_x[256 * h + 4 * lane + r];
+          yy[4 * h + r] = s_y[256 * h + 4 * lane + r];
+        }
+      dual_block<VT>(a.dot_V, a.dot_ld, a.dot_nv, lb * kDualRows, xx, yy, a.dot_sc, a.dot_part, row_blocks, lb);
+      return;
+    }
     if (i0 >= a.n) return;
     if constexpr (DM == 3) {
       mf_rows_values<VT, OT, KT, R>(a, v, out, i0, accumulate);
@@ -718,86 +838,8 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
       mf_rows<VT, OT, KT, R, kMfMembers>(a, v, out, i0, accumulate);
       return;
     }
-    // Loads are unconditional (clamped indices, selects) and issued in two batches: the
-    // row's own entries and every slot's column and value, then every gather. A load
-    // under a branch is waited for at the end of its block, which would make the slots
-    // one memory round trip each.
-    const Pack<double, R> dg = ldpack<double, R>(a.diag + i0);
     double vi[R], acc[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double vr = (double)v[min(i0 + r, a.n - 1)];
-      vi[r] = i0 + r < a.n ? vr : 0.0;
-    }
-    const int K = KT > 0 ? KT : a.K;
-    // column of slot k for rows i0..i0+R-1: loaded, or i + dia_off[k] clamped to the
-    // states (rows whose value is 0 may point anywhere; they add nothing). DM fixes the
-    // kind for every slot (0 loaded, 1 computed) so the fixed-K kernels carry no branch;
-    // a matrix with both kinds uses the generic kernel (DM 2), which tests each slot.
-    auto slot_cols = [&](int k) {
-      Pack<int32_t, R> c;
-      const bool dia = DM == 1 || (DM == 2 && ((a.dia_mask >> k) & 1u));
-      if (dia) {
-        const int64_t hi = a.n - 1;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int64_t j = i0 + r + a.dia_off[k];
-          c.v[r] = (int32_t)(j < 0 ? 0 : (j > hi ? hi : j));
-        }
-      } else {
-        c = ldpack<int32_t, R>(a.col + k * a.ld + i0);
-      }
-      return c;
-    };
-    if constexpr (KT > 0) {
-      // values and columns (computed columns need no load), then every gather; with
-      // computed columns the gathers go out with the values, one round trip in all
-      Pack<int32_t, R> c[KT];
-      Pack<VT, R> w[KT];
-#pragma unroll
-      for (int k = 0; k < KT; ++k) {
-        w[k] = ldpack<VT, R>(a.val + k * a.ld + i0);
-        c[k] = slot_cols(k);
-      }
-      double gv[KT][R];
-#pragma unroll
-      for (int k = 0; k < KT; ++k) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) gv[k][r] = (double)v[c[k].v[r]];
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      // diag * v_i, then one fused multiply-add per slot in slot order (written out, so
-      // that every mode of the kernel rounds alike)
-#pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] = __dmul_rn(dg.v[r], vi[r]);
-#pragma unroll
-      for (int k = 0; k < KT; ++k) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = __fma_rn((double)w[k].v[r], gv[k][r], acc[r]);
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] = __dmul_rn(dg.v[r], vi[r]);
-      for (int k = 0; k < K; ++k) {
-        const Pack<int32_t, R> c = slot_cols(k);
-        const Pack<VT, R> w = ldpack<VT, R>(a.val + k * a.ld + i0);
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = __fma_rn((double)w.v[r], (double)v[c.v[r]], acc[r]);
-      }
-    }
-    for (int q = 0; q < a.ntv; ++q) {
-      const Pack<VT, R> d = ldpack<VT, R>(a.tvd + q * a.ld + i0);
-      const Pack<int32_t, R> c = ldpack<int32_t, R>(a.tvc + q * a.ld + i0);
-      const Pack<VT, R> w = ldpack<VT, R>(a.tvv + q * a.ld + i0);
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        acc[r] += a.tv_scale[q] * ((double)d.v[r] * vi[r] + (double)w.v[r] * (double)v[c.v[r]]);
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int64_t i = i0 + r;
-      if (i < a.n) out[i] = (OT)(accumulate ? (double)out[i] + acc[r] : acc[r]);
-    }
+    stored_rows<VT, OT, KT, R, DM>(a, v, out, i0, accumulate, acc, vi);
     if constexpr (EPI) rk_epilogue<VT, R>(a, i0, a.n, acc);
   } else {
     const int c = (int)((int64_t)blockIdx.x - row_blocks);
@@ -883,13 +925,29 @@ struct LaunchCfg {
   hipEvent_t e0, e1;
 };
 
-template <typename VT, typename OT, int K, int R, int DM, bool EPI = false>
+template <typename VT, typename OT, int K, int R, int DM, bool EPI = false, bool DOT = false>
 void launch_one(const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
   if (c.e0)
-    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM, EPI>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv, oo,
-                          acc);
+    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM, EPI, DOT>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv,
+                          oo, acc);
   else
-    k_matvec<VT, OT, K, R, DM, EPI><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+    k_matvec<VT, OT, K, R, DM, EPI, DOT><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+}
+
+// The projection-epilogue kernels: FP64 output, 2 rows per thread, every slot's column
+// loaded (DM 0) or every slot's computed (DM 1), at most 8 slots
+template <typename VT, int DM>
+void dispatch_dots_kd(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, double* oo) {
+  switch (K) {
+    case 1: launch_one<VT, double, 1, 2, DM, false, true>(c, a, vv, oo, 0); break;
+    case 2: launch_one<VT, double, 2, 2, DM, false, true>(c, a, vv, oo, 0); break;
+    case 3: launch_one<VT, double, 3, 2, DM, false, true>(c, a, vv, oo, 0); break;
+    case 4: launch_one<VT, double, 4, 2, DM, false, true>(c, a, vv, oo, 0); break;
+    case 5: launch_one<VT, double, 5, 2, DM, false, true>(c, a, vv, oo, 0); break;
+    case 6: launch_one<VT, double, 6, 2, DM, false, true>(c, a, vv, oo, 0); break;
+    case 7: launch_one<VT, double, 7, 2, DM, false, true>(c, a, vv, oo, 0); break;
+    default: launch_one<VT, double, 8, 2, DM, false, true>(c, a, vv, oo, 0); break;
+  }
 }
 
 // The epilogue kernels: FP64 output, 2 rows per thread, stored generator (loaded, computed
@@ -986,7 +1044,8 @@ void dispatch_r(int R, int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT
 
 template <typename VT>
 void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
-                     bool out_fp64, hipEvent_t e0, hipEvent_t e1, const RkEpilogue* epi = nullptr) {
+                     bool out_fp64, hipEvent_t e0, hipEvent_t e1, const RkEpilogue* epi = nullptr,
+                     const DualDots* dots = nullptr) {
   MatArgs<VT> a{};
   a.n = A->n;
   a.ld = A->ld;
@@ -1037,6 +1096,22 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
     a.rk_vin = reinterpret_cast<VT*>(epi->vin);
     a.rk_mask = epi->mask;
     dispatch_epi<VT>(A->K, c, a, vv, reinterpret_cast<double*>(out));
+    return;
+  }
+  if (dots) {
+    if (!matvec_dots_fusable(A) || accumulate || !out_fp64 || R != 2)
+      throw Error(NC_ERR_INTERNAL, "mat-vec projection epilogue not available");
+    if (dots->nv < 1 || dots->nv > 32 || dots->nblk != row_blocks)
+      throw Error(NC_ERR_INTERNAL, "projection blocks do not match the mat-vec's row workgroups");
+    a.dot_V = reinterpret_cast<const VT*>(dots->V);
+    a.dot_ld = dots->ld;
+    a.dot_nv = dots->nv;
+    a.dot_sc = dots->sc;
+    a.dot_part = dots->part;
+    if (a.dia_mask == 0u)
+      dispatch_dots_kd<VT, 0>(A->K, c, a, vv, reinterpret_cast<double*>(out));
+    else
+      dispatch_dots_kd<VT, 1>(A->K, c, a, vv, reinterpret_cast<double*>(out));
     return;
   }
   if (out_fp64 && A->mf_mode)
@@ -1521,6 +1596,27 @@ void matvec_launch_rk(nc_matrix_s* A, double t, const void* v, double* out, hipS
   A->launches++;
 }
 
+bool matvec_dots_fusable(const nc_matrix_s* A) {
+  if (A->mf_mode != 0 || A->rows_per_thread != 2 || (A->timing && A->timing_plain) || !A->tv_r.empty() || A->K < 1 ||
+      A->K > 8)
+    return false;
+  const uint32_t all = (1u << A->K) - 1u;
+  return A->dia_mask == 0u || (A->dia_mask & all) == all;
+}
+
+void matvec_launch_dots(nc_matrix_s* A, const void* v, double* out, hipStream_t s, const DualDots& d) {
+  if (s != A->stream.s) throw Error(NC_ERR_INTERNAL, "projection mat-vec off the matrix stream");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = matvec_timing_slot(A, &e0, &e1);
+  if (A->precision == NC_FP32)
+    launch_matvec_t<float>(A, 0.0, v, out, false, s, true, e0, e1, nullptr, &d);
+  else
+    launch_matvec_t<double>(A, 0.0, v, out, false, s, true, e0, e1, nullptr, &d);
+  NC_CHECK_HIP(hipGetLastError());
+  if (timed) A->ev_used++;
+  A->launches++;
+}
+
 bool matvec_timing_slot(nc_matrix_s* A, hipEvent_t* e0, hipEvent_t* e1) {
   *e0 = *e1 = nullptr;
   if (!A->timing) return false;
@@ -1601,6 +1697,50 @@ double bench_matvec(nc_matrix_s* A, double t, int reps) {
   NC_CHECK_HIP(hipEventRecord(e1, s));
   NC_CHECK_HIP(hipEventSynchronize(e1));
   A->timing = was;
+  float ms = 0.f;
+  NC_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+// Mean duration of the mat-vec with the Krylov projection epilogue over nv basis vectors
+// (device-resident zero vectors in the solver's padded layout), launched reps times back
+// to back on the matrix's stream.
+double bench_projection(nc_matrix_s* A, int nv, int reps) {
+  const bool was = A->timing, was_plain = A->timing_plain;
+  A->timing = false;
+  A->timing_plain = false;
+  struct Restore {
+    nc_matrix_s* A;
+    bool t, p;
+    ~Restore() {
+      A->timing = t;
+      A->timing_plain = p;
+    }
+  } restore{A, was, was_plain};
+  if (!matvec_dots_fusable(A))
+    throw Error(NC_ERR_UNSUPPORTED, "this generator's mat-vec has no projection epilogue (it needs the stored "
+                                    "generator, 2 rows per thread, 1..8 slots, no time-varying reactions)");
+  NC_REQUIRE(nv >= 1 && nv <= 32, NC_ERR_ARG, "nv must be in 1..32");
+  const int64_t nblk = (A->n + kDualRows - 1) / kDualRows;
+  NC_REQUIRE(nblk >= 1 && nblk <= kDualMaxBlocks, NC_ERR_UNSUPPORTED, "state count outside the projection blocks' range");
+  hipStream_t s = A->stream.s;
+  const int64_t Np = (A->n_rows + 1023) / 1024 * 1024;
+  DevBuf<unsigned char> V((size_t)nv * Np * A->vsize);
+  DevBuf<double> o(Np), part((size_t)(2 * nv + 1) * nblk);
+  NC_CHECK_HIP(hipMemsetAsync(V.p, 0, (size_t)nv * Np * A->vsize, s));
+  NC_CHECK_HIP(hipMemsetAsync(o.p, 0, sizeof(double) * Np, s));
+  const DualDots d{V.p, Np, nv, nullptr, part.p, nblk};
+  const void* vlast = V.p + (size_t)(nv - 1) * Np * A->vsize;
+  matvec_launch_dots(A, vlast, o.p, s, d);  // warm-up
+  hipEvent_t e0, e1;
+  NC_CHECK_HIP(hipEventCreate(&e0));
+  NC_CHECK_HIP(hipEventCreate(&e1));
+  NC_CHECK_HIP(hipEventRecord(e0, s));
+  for (int k = 0; k < reps; ++k) matvec_launch_dots(A, vlast, o.p, s, d);
+  NC_CHECK_HIP(hipEventRecord(e1, s));
+  NC_CHECK_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   NC_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
@@ -1910,6 +2050,13 @@ int nc_matrix_bench_matvec(nc_matrix A, double t, int32_t reps, double* mean_ms)
   return guard([&] {
     NC_REQUIRE(A && mean_ms && reps > 0, NC_ERR_ARG, "bad arguments");
     *mean_ms = bench_matvec(A, t, reps);
+  });
+}
+
+int nc_matrix_bench_projection(nc_matrix A, int32_t nv, int32_t reps, double* mean_ms) {
+  return guard([&] {
+    NC_REQUIRE(A && mean_ms && reps > 0, NC_ERR_ARG, "bad arguments");
+    *mean_ms = bench_projection(A, nv, reps);
   });
 }
 

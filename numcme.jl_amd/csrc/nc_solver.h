@@ -27,6 +27,7 @@ struct nc_solver_s {
   nc::DevBuf<double> partials;
   nc::DevBuf<double> fold;  // folded totals of a vector pass and the fold's arrival counter
   unsigned red_blocks = 0;
+  int64_t dblk = 0;  // Krylov: 512-row projection blocks over the states (0: grid-stride passes)
   // Krylov
   int m = 30;
   nc::DevBuf<unsigned char> V;  // (m+1) x N, VT, unnormalised

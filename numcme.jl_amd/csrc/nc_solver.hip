@@ -21,6 +21,7 @@
 #include <limits>
 #include <type_traits>
 
+#include "nc_dual.h"
 #include "nc_fspvec.h"
 #include "nc_solver.h"
 
@@ -81,6 +82,8 @@ struct VecArgs {
   KLayout L;
   int j;
   double btol;
+  int64_t tail0, tail1;   // fold: rows [tail0, tail1) of V and xin (the sink rows) added to the
+                          // block sums of the 512-row projection passes
 };
 
 __device__ inline double wave_sum(double v) {
@@ -439,23 +442,52 @@ __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigne
   __shared__ int s_last;
   {
     const int k = blockIdx.x;
+    // thread t takes blocks t, t + 1024, t + 2048, t + 3072 (nblocks <= 4096), loaded
+    // together and added in that order
+    double pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const unsigned b = threadIdx.x + u * kFinBlock;
+      const double t = a.partials[(int64_t)k * nblocks + min(b, nblocks - 1)];
+      pv[u] = b < nblocks ? t : 0.0;
+    }
     double v = 0.0;
-    for (unsigned b = threadIdx.x; b < nblocks; b += kFinBlock) v += a.partials[(int64_t)k * nblocks + b];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v += pv[u];
     v = wave_sum(v);
     if (lane == 0) wsum[wave] = v;
+    // the rows past the blocks (sink rows of the 512-row projection passes), by wave 0 in
+    // row order; value k is V_k.x (times sc_k), V_{k-nv}.V_{nv-1} or x.x as in the blocks
+    double tl = 0.0;
+    if (wave == 0 && a.tail1 > a.tail0) {
+      const int nv = a.nv;
+      const BT* Vl = a.V + (int64_t)(nv - 1) * a.N;
+      for (int64_t o = a.tail0 + lane; o < a.tail1; o += 64) {
+        const double xo = a.xin[o];
+        if (k < nv)
+          tl = __fma_rn((double)a.V[(int64_t)k * a.N + o], xo, tl);
+        else if (k < 2 * nv)
+          tl = __fma_rn((double)a.V[(int64_t)(k - nv) * a.N + o], (double)Vl[o], tl);
+        else
+          tl = __fma_rn(xo, xo, tl);
+      }
+      tl = wave_sum(tl);
+      if (k < nv && a.sc) tl *= a.sc[k];
+    }
     __syncthreads();
     if (a.finish == FIN_FOLD) {
       // the totals are read by the next launch: its kernel boundary makes them visible
       if (threadIdx.x == 0) {
         double sum = 0.0;
         for (int w = 0; w < kFinBlock / 64; ++w) sum += wsum[w];
-        a.tot[k] = sum;
+        a.tot[k] = sum + tl;
       }
       return;
     }
     if (threadIdx.x == 0) {
       double sum = 0.0;
       for (int w = 0; w < kFinBlock / 64; ++w) sum += wsum[w];
+      sum += tl;
       __hip_atomic_store(reinterpret_cast<unsigned long long*>(a.tot) + k, (unsigned long long)__double_as_longlong(sum),
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __threadfence();
@@ -498,6 +530,31 @@ __global__ __launch_bounds__(kFinBlock) void k_vec_finish(VecArgs<BT> a, unsigne
       default: break;
     }
   }
+}
+
+// The projection sums of Arnoldi column nv - 1 over 512-row blocks from the stored
+// q = A V_{nv-1} (nc_dual.h): the same block sums the mat-vec's projection epilogue forms,
+// for generators whose mat-vec has none. One workgroup per block.
+template <typename BT>
+__global__ __launch_bounds__(kBlock) void k_dual_blocks(const BT* __restrict__ V, int64_t ld, int nv,
+                                                        const double* __restrict__ q, int64_t n, double* part) {
+  const int64_t blk = blockIdx.x, base = blk * kDualRows;
+  const int lane = threadIdx.x & 63;
+  const BT* Vl = V + (int64_t)(nv - 1) * ld;
+  double x[8], y[8];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t r = base + 256 * h + 4 * lane;
+    const Pack<double, 4> qx = *reinterpret_cast<const Pack<double, 4>*>(q + r);
+    const Pack<BT, 4> yl = *reinterpret_cast<const Pack<BT, 4>*>(Vl + r);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const bool live = r + rr < n;
+      x[4 * h + rr] = live ? qx.v[rr] : 0.0;
+      y[4 * h + rr] = live ? (double)yl.v[rr] : 0.0;
+    }
+  }
+  dual_block<BT>(V, ld, nv, base, x, y, nullptr, part, gridDim.x, blk);
 }
 
 template <typename BT, int MODE>
@@ -584,7 +641,8 @@ void launch_vec(VecArgs<BT> a, const nc_solver_s* S, hipStream_t s) {
       break;
   }
   if (a.finish != FIN_NONE) {
-    if (nvals > kPartStrideMax || !a.tot || !a.counter) throw Error(NC_ERR_INTERNAL, "vector pass fold not set up");
+    if (nvals > kPartStrideMax || !a.tot || !a.counter || grid > 4u * kFinBlock)
+      throw Error(NC_ERR_INTERNAL, "vector pass fold not set up");
     k_vec_finish<BT><<<nvals, kFinBlock, 0, s>>>(a, grid, nvals);
   }
   NC_CHECK_HIP(hipGetLastError());
@@ -747,8 +805,16 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
   // within 1% of each other since the passes issue their basis loads in batches)
   const unsigned vb = (S->method == NC_METHOD_KRYLOV && S->vs == 4) ? 1024u : kVecBlocks;
   S->red_blocks = std::min<unsigned>(vb, grid_for(S->Np / 4, kBlock));
+  // Krylov on one block: the projection sums go over 512-row blocks of the states (one
+  // workgroup each, the mat-vec's own row workgroups), folded with the sink rows
+  S->dblk = 0;
+  if (S->method == NC_METHOD_KRYLOV && !S->SA && S->blocks == 1) {
+    const int64_t nb = (S->A->n + kDualRows - 1) / kDualRows;
+    if (nb >= 1 && nb <= kDualMaxBlocks) S->dblk = nb;
+  }
   // the RK error pass puts 3 R gathered sink values behind its partials
-  S->partials.reserve(std::max<size_t>((size_t)S->red_blocks * kPartStrideMax, S->red_blocks + 3 * kMaxReactions));
+  S->partials.reserve(std::max<size_t>({(size_t)S->red_blocks * kPartStrideMax, (size_t)S->dblk * kPartStrideMax,
+                                        (size_t)S->red_blocks + 3 * kMaxReactions}));
   if (!S->fold.p) {
     // folded totals, then the fold's arrival counter (zero between passes)
     S->fold.alloc(kPartStrideMax + 1);
@@ -795,12 +861,69 @@ static void copy_to_host(nc_solver_s* S, double* host, const double* dev) {
 // Projecting with the Gram matrix of the stored vectors makes V_{j+1} orthogonal to them
 // to FP64 rounding in one projection: the work a second classical Gram-Schmidt pass
 // would do, without a third pass over V.
+// q = A V_{nv-1} and the projection sums of V_0..V_{nv-1} (vbase) over 512-row blocks:
+// by the mat-vec's projection epilogue, or by the block pass over the stored q (the same
+// arithmetic), then folded with the sink rows by `finish`.
+template <typename BT>
+static void dual_blocks_pass(nc_solver_s* S, const BT* vbase, int nv, int finish) {
+  nc_matrix_s* A = S->A;
+  const BT* vlast = vbase + (int64_t)(nv - 1) * S->Np;
+  if (matvec_dots_fusable(A)) {
+    const DualDots d{vbase, S->Np, nv, nullptr, S->partials.p, S->dblk};
+    matvec_launch_dots(A, vlast, S->q.p, S->s, d);
+  } else {
+    op_apply(S, 0.0, vlast, S->q.p);
+    k_dual_blocks<BT><<<(unsigned)S->dblk, kBlock, 0, S->s>>>(vbase, S->Np, nv, S->q.p, A->n, S->partials.p);
+    NC_CHECK_HIP(hipGetLastError());
+  }
+  VecArgs<BT> a{};
+  a.N = S->Np;
+  a.nv = nv;
+  a.V = vbase;
+  a.xin = S->q.p;
+  a.partials = S->partials.p;
+  a.kb = S->kbuf.p;
+  a.L = KLayout{S->m};
+  a.btol = kBreakdownTol;
+  a.finish = finish;
+  a.tot = S->fold.p;
+  a.counter = reinterpret_cast<unsigned*>(S->fold.p + kPartStrideMax);
+  a.tail0 = A->n;
+  a.tail1 = A->n + A->nr;
+  const int nvals = 2 * nv + 1;
+  if (nvals > kPartStrideMax || S->dblk > 4 * kFinBlock || a.tail1 > S->Np)
+    throw Error(NC_ERR_INTERNAL, "projection blocks not set up");
+  k_vec_finish<BT><<<nvals, kFinBlock, 0, S->s>>>(a, (unsigned)S->dblk, nvals);
+  NC_CHECK_HIP(hipGetLastError());
+}
+
 template <typename BT>
 static void arnoldi_iteration(nc_solver_s* S, int j) {
   const int64_t N = S->Np;
   const KLayout L{S->m};
   hipStream_t s = S->s;
   auto Vp = [&](int i) { return reinterpret_cast<BT*>(S->V.p + (size_t)i * N * S->vs); };
+  if (S->dblk) {
+    dual_blocks_pass<BT>(S, Vp(0), j + 1, FIN_FOLD);
+    VecArgs<BT> a{};
+    a.N = N;
+    a.nv = j + 1;
+    a.V = Vp(0);
+    a.partials = S->partials.p;
+    a.kb = S->kbuf.p;
+    a.gram = S->gram.p;
+    a.L = L;
+    a.j = j;
+    a.btol = kBreakdownTol;
+    a.vout_fp32 = std::is_same<BT, float>::value ? 1 : 0;
+    a.xin = S->q.p;
+    a.mode = VM_UPDATE;
+    a.arn_pre = 1;
+    a.vout = Vp(j + 1);
+    a.finish = FIN_NONE;
+    launch_vec(a, S, s);
+    return;
+  }
   op_apply(S, 0.0, Vp(j), S->q.p);
   VecArgs<BT> a{};
   a.N = N;
@@ -830,6 +953,10 @@ static void arnoldi_last(nc_solver_s* S) {
   // |V_m| (H(m, m-1), sc_m) and avnorm = |A v_m| in one VM_DUAL pass over V_m
   const KLayout L{S->m};
   BT* vm = reinterpret_cast<BT*>(S->V.p + (size_t)S->m * S->Np * S->vs);
+  if (S->dblk) {
+    dual_blocks_pass<BT>(S, vm, 1, FIN_LAST);
+    return;
+  }
   op_apply(S, 0.0, vm, S->q.p);
   VecArgs<BT> a{};
   a.N = S->Np;
@@ -912,11 +1039,12 @@ static void arnoldi_run(nc_solver_s* S) {
 // credited to its own statistics.
 struct TimingScope {
   nc_solver_s* S;
-  bool was;
+  bool was, was_plain;
   double ms0;
   int64_t n0;
-  explicit TimingScope(nc_solver_s* s_) : S(s_), was(s_->A->timing) {
+  explicit TimingScope(nc_solver_s* s_) : S(s_), was(s_->A->timing), was_plain(s_->A->timing_plain) {
     S->A->timing = S->opts.time_kernels != 0;
+    S->A->timing_plain = S->opts.time_kernels == 2;
     ms0 = S->A->timed_ms;
     n0 = S->A->timed_count;
   }
@@ -928,6 +1056,7 @@ struct TimingScope {
     S->st.matvec_ms += S->A->timed_ms - ms0;
     S->st.matvecs_timed += S->A->timed_count - n0;
     S->A->timing = was;
+    S->A->timing_plain = was_plain;
   }
 };
 

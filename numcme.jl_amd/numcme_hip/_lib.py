@@ -164,6 +164,7 @@ SIGNATURES = {
     "nc_matrix_matvec_noalloc": [P, D, PD, PD, I32],
     "nc_matrix_export": [P, D, PI64, PI64, PI64, PD],
     "nc_matrix_bench_matvec": [P, D, I32, PD],
+    "nc_matrix_bench_projection": [P, I32, I32, PD],
     "nc_matrix_set_rows_per_thread": [P, I32],
     "nc_matrix_set_matrix_free": [P, I32],
     "nc_matrix_autotune": [P, I32, PD],

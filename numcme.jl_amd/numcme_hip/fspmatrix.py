@@ -121,6 +121,13 @@ class FspMatrixSparse(L.Handle):
         L.call("nc_matrix_bench_matvec", self.handle, float(t), int(reps), C.byref(ms))
         return ms.value
 
+    def bench_projection(self, nv, reps=100):
+        """Mean time in ms of the Krylov mat-vec with its projection epilogue over nv basis
+        vectors (HIP events on the matrix stream)."""
+        ms = C.c_double()
+        L.call("nc_matrix_bench_projection", self.handle, int(nv), int(reps), C.byref(ms))
+        return ms.value
+
 
 def matvec(t, A: FspMatrixSparse, v):
     """``matvec(t, A, v)`` (fspsparsematrix.jl:254-258)."""
