@@ -327,10 +327,11 @@ def c4_oracle_csc(spec):
     return FspMatrix(StateSpace(S, states), props, th).to_csc(0.0)
 
 
-def c4_batch(nh, L, krylov_tol, graphs=True, log=None):
+def c4_batch(nh, L, krylov_tol, graphs=True, log=None, plain_leg=True):
     """Config C4: the 512 uneven instances in one batched FP32 solve against the plain FP64
     single-solve path, both timed on one clock: from the upload of every instance's state
-    space and generator to the download of every snapshot."""
+    space and generator to the download of every snapshot. plain_leg=False: the batched
+    solves alone (profiler runs)."""
     say = log or (lambda m: None)
     table = json.loads(C4_TABLE.read_text())
     ts = np.arange(0.0, 11.0)
@@ -358,6 +359,10 @@ def c4_batch(nh, L, krylov_tol, graphs=True, log=None):
     batched()  # warm-up (code paths, caches)
     snaps, bstats, (b_total, b_setup, b_solve, b_fetch) = batched()
     say(f"c4: batched {b_total:.2f}s (setup {b_setup:.2f}, solve {b_solve:.2f}, download {b_fetch:.2f})")
+    if not plain_leg:
+        return {"batched_total_s": b_total, "batched_setup_s": b_setup, "batched_solve_s": b_solve,
+                "batched_download_s": b_fetch, "batched_iterations": bstats["matvecs"] // 6,
+                "batched_host_syncs": bstats["host_syncs"]}
 
     def plain(spec):
         A64, u0 = c4_instance(nh, spec, "fp64")
@@ -764,6 +769,9 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the forward-sensitivity (C3) measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched (C4) measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the learned-adapter (C5) measurement")
+    ap.add_argument("--c4-plain", default="all", choices=["all", "none"],
+                    help="with --only c4: time the plain FP64 comparison too (none: the batched solves alone, "
+                         "for profiler runs)")
     ap.add_argument("--c5-plain", default="all", choices=["all", "none"],
                     help="time the plain FP64 adapters for C5 (none: learned only, for profiler runs)")
     ap.add_argument("--spmv-probe", type=int, default=0, metavar="REPS",
@@ -807,7 +815,8 @@ def main():
         print(json.dumps(c3_sensitivity(nh, L)), flush=True)
         return
     if args.only == "c4":
-        print(json.dumps(c4_batch(nh, L, args.krylov_tol, graphs=not args.no_graphs, log=say)), flush=True)
+        print(json.dumps(c4_batch(nh, L, args.krylov_tol, graphs=not args.no_graphs, log=say,
+                                  plain_leg=args.c4_plain == "all")), flush=True)
         return
     if args.only == "c5-solve":
         print(json.dumps(c5_solve_only(nh, args.net_file, log=say)), flush=True)
