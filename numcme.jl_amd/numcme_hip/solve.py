@@ -29,7 +29,10 @@ class KrylovExpv:
     m: int = 30
     tol: float = 1e-12
     precision: str = "fp64"
-    time_kernels: bool = False
+    # HIP events around every mat-vec launch: 1 (or True) as the solve runs them (the
+    # mat-vec with its projection epilogue where the generator has one), 2 plain mat-vecs
+    # (the projection sums then run as a separate pass; same results)
+    time_kernels: int = 0
     graphs: bool = True  # replay each Arnoldi process as a captured HIP graph
 
     def opts(self, odeatol, odertol):
