@@ -81,3 +81,23 @@ def test_bench_projection(gpu):
         A.bench_projection(4, reps=1)
     A.set_matrix_free(0)
     assert A.bench_projection(2, reps=1) > 0.0
+
+
+def test_large_space_grid_stride_pass(gpu):
+    # 1500 x 1450 = 2,175,000 states: past the 4,096 blocks of 512 rows, so the Krylov solve
+    # takes the grid-stride projection pass; checked against the device DP5(4) solve
+    S, props, th = models.toggle()
+    states = models.box_states([1499, 1449])
+    assert len(states) > 4096 * 512
+    model = nh.CmeModel(S, models.to_product(props), th)
+    space = nh.StateSpaceSparse(S, states)
+    A = nh.FspMatrixSparse(space, model.propensities, model.parameters, precision="fp64", model=model)
+    p0 = np.zeros(len(states))
+    p0[int(np.flatnonzero((states[:, 0] == 20) & (states[:, 1] == 5))[0])] = 1.0
+    u0 = np.concatenate([p0, np.zeros(space.get_sink_count())])
+    ts = np.array([0.0, 0.05, 0.1])
+    kr = _integrate(A, nh.KrylovExpv(m=30, tol=1e-12).opts(1e-6, 1e-4), u0, 0.0, ts, space)
+    rk = _integrate(A, nh.RK45(rtol=1e-10, atol=1e-16).opts(1e-6, 1e-4), u0, 0.0, ts, space)
+    for pk, pr, sk, sr in zip(kr.p, rk.p, kr.sinks, rk.sinks):
+        assert np.abs(pk.values - pr.values).sum() + np.abs(np.asarray(sk) - np.asarray(sr)).sum() < 1e-8
+    assert abs(kr.p[-1].values.sum() + np.sum(kr.sinks[-1]) - 1.0) < 1e-10
