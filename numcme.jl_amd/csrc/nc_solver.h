@@ -38,6 +38,8 @@ struct nc_solver_s {
   double* hbuf = nullptr;       // pinned host copy of kbuf
   double* hcoef = nullptr;      // pinned host staging of coef
   double* hpart = nullptr;      // pinned host copy of per-block partial sums
+  double* hpart_dev = nullptr;  // the same pinned block as a device address: the RK error
+                                // pass writes its partials there directly
   double beta = 0.0, t_new = 0.0, anorm = 0.0;
   bool beta_pending = false;  // beta was updated on the device and not yet read back
   bool started = false;

@@ -820,10 +820,12 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
     S->fold.alloc(kPartStrideMax + 1);
     NC_CHECK_HIP(hipMemsetAsync(S->fold.p, 0, sizeof(double) * (kPartStrideMax + 1), S->s));
   }
-  if (!S->hpart)
+  if (!S->hpart) {
     // partials (up to kMaxRedBlocks) and the RK step's 3 R gathered sink values, then the
     // carried sinks of u and the step's three sink sets (4 R) behind them
     S->hpart = static_cast<double*>(pinned_alloc(sizeof(double) * (kMaxRedBlocks + 7 * kMaxReactions)));
+    NC_CHECK_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&S->hpart_dev), S->hpart, 0));
+  }
 }
 
 // The right-hand-side operator: the FSP generator, or the block sensitivity operator.
@@ -1341,10 +1343,10 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
     stages(h);
     // after stage 6 the stage input is y + h*sum b_j k_j (row 7 of A): ytmp = y_{n+1}
     const int Rg = (ev && R > 0) ? R : 0;
-    k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->ksw, ns, Rg, S->partials.p);
+    // the partials and the gathered sinks go straight to the pinned host block: the step's
+    // read-back needs no copy launch, only the synchronisation
+    k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->ksw, ns, Rg, S->hpart_dev);
     NC_CHECK_HIP(hipGetLastError());
-    NC_CHECK_HIP(hipMemcpyAsync(S->hpart, S->partials.p, sizeof(double) * (S->red_blocks + 3 * Rg),
-                                hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
     for (int r = 0; r < Rg; ++r) {  // sinks of y_{n+1}, k0 and k6 behind the partials
       hs[R + r] = S->hpart[S->red_blocks + r];
