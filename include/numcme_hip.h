@@ -340,6 +340,9 @@ int nc_batch_autotune(nc_batch B, int32_t reps, int32_t* rows_per_thread, double
 int nc_batch_set_rows_per_thread(nc_batch B, int32_t rows);
 /* Copy snapshot k of instance inst (its n_rows doubles). */
 int nc_batch_get_snapshot(nc_batch B, int32_t inst, int32_t k, double* u);
+/* Copy snapshot k of every instance in one download, instance after instance (each its
+ * n_rows doubles, the layout of nc_batch_solve's u0). No reference counterpart. */
+int nc_batch_get_snapshots(nc_batch B, int32_t k, double* u);
 
 /* ======================================================================================
  * Adaptive FSP — replaces solve(model, p0, tspan, ::AdaptiveFspSparse; saveat, fsptol,

@@ -547,6 +547,18 @@ __global__ void k_bfreeze(int n, double h, const int64_t* gran_start, double* ht
   if (i == 0) *n_agran = (uint32_t)total;
 }
 
+// Snapshot rows of every instance packed back to back (instance i at pack_off[i]), one
+// workgroup per instance: the layout of the solve's initial vectors.
+__global__ __launch_bounds__(kBlock) void k_bpack(const double* __restrict__ snap, const int64_t* __restrict__ row_off,
+                                                  const int64_t* __restrict__ nrows,
+                                                  const int64_t* __restrict__ pack_off, double* __restrict__ out) {
+  const int i = blockIdx.x;
+  const double* src = snap + row_off[i];
+  double* dst = out + pack_off[i];
+  const int64_t n = nrows[i];
+  for (int64_t r = threadIdx.x; r < n; r += kBlock) dst[r] = src[r];
+}
+
 }  // namespace
 }  // namespace nc
 
@@ -1203,6 +1215,22 @@ int nc_batch_solve(nc_batch B, const nc_solve_opts* opts, const double* u0, int3
       batch_solve_t<float>(B, o, u0, n_out, t_out, st);
     else
       batch_solve_t<double>(B, o, u0, n_out, t_out, st);
+  });
+}
+
+int nc_batch_get_snapshots(nc_batch B, int32_t k, double* u) {
+  return guard([&] {
+    NC_REQUIRE(B && u && k >= 0 && k < B->n_out, NC_ERR_ARG, "index out of range");
+    hipStream_t s = B->stream.s;
+    std::vector<int64_t> off(B->n);
+    int64_t total = 0;
+    for (int i = 0; i < B->n; ++i) off[i] = total, total += B->nrows[i];
+    DevBuf<int64_t> d_off(B->n);
+    DevBuf<double> packed(total);
+    NC_CHECK_HIP(hipMemcpyAsync(d_off.p, off.data(), sizeof(int64_t) * B->n, hipMemcpyHostToDevice, s));
+    k_bpack<<<B->n, kBlock, 0, s>>>(B->snaps.p + (size_t)k * B->ld, B->d_row_off.p, B->d_nrows.p, d_off.p, packed.p);
+    NC_CHECK_HIP(hipGetLastError());
+    copy_d2h(u, packed.p, sizeof(double) * total, s);  // ends with the stream synchronised
   });
 }
 

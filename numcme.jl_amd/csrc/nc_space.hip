@@ -386,7 +386,10 @@ nc_space_s* space_create(int ns, int nr, const int32_t* stoich, int64_t n_init, 
     sp->nr = nr;
     sp->stoich.assign(stoich, stoich + (size_t)ns * nr);
     sp->d_stoich.alloc((size_t)ns * nr);
-    NC_CHECK_HIP(hipMemcpy(sp->d_stoich.p, stoich, sizeof(int32_t) * ns * nr, hipMemcpyHostToDevice));
+    // on the space's stream from its own host copy (the constructor ends with a
+    // synchronisation of that stream)
+    NC_CHECK_HIP(hipMemcpyAsync(sp->d_stoich.p, sp->stoich.data(), sizeof(int32_t) * ns * nr, hipMemcpyHostToDevice,
+                                sp->stream.s));
     space_reserve(sp, std::max<int64_t>(n_init, 1));
     space_rebuild_table(sp, std::max<int64_t>(n_init, 1));
     if (n_init) {

@@ -191,6 +191,7 @@ SIGNATURES = {
     "nc_batch_info": [P, PI64, PI64],
     "nc_batch_solve": [P, C.POINTER(nc_solve_opts), PD, I32, PD, C.POINTER(nc_solve_stats)],
     "nc_batch_get_snapshot": [P, I32, I32, PD],
+    "nc_batch_get_snapshots": [P, I32, PD],
     "nc_mlp_create": [C.c_uint64, C.POINTER(P)],
     "nc_mlp_destroy": [P],
     "nc_mlp_get_params": [P, C.POINTER(C.c_float)],

@@ -91,7 +91,16 @@ class BatchFsp(L.Handle):
         self.stats = {f: getattr(st, f) for f, _ in L.nc_solve_stats._fields_}
         if not fetch:
             return None
-        return [[self.snapshot(i, k) for k in range(len(ts))] for i in range(len(self))]
+        per_k = [self.snapshots(k) for k in range(len(ts))]
+        return [[per_k[k][i] for k in range(len(ts))] for i in range(len(self))]
+
+    def snapshots(self, k):
+        """Snapshot k of every instance, downloaded at once: one array per instance (views
+        of one buffer)."""
+        counts = np.asarray(self.rowcounts, dtype=np.int64)
+        out = np.empty(int(counts.sum()))
+        L.call("nc_batch_get_snapshots", self.handle, int(k), L.ptr(out, C.c_double))
+        return np.split(out, np.cumsum(counts)[:-1])
 
     def snapshot(self, inst, k):
         out = np.empty(int(self.rowcounts[inst]))

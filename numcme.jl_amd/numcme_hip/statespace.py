@@ -31,7 +31,9 @@ class StateSpaceSparse(L.Handle):
         if X.ndim == 1 and X.size == ns and not isinstance(initstates[0], (list, tuple, np.ndarray)):
             X = X.reshape(1, ns)  # single-state constructor (:142-144)
         X = X.reshape(-1, ns)
-        if np.abs(X).max(initial=0) >= 2**31 or np.abs(S).max(initial=0) >= 2**31:
+        # min and max rather than abs: no temporary the size of the state list
+        if (X.max(initial=0) >= 2**31 or X.min(initial=0) <= -2**31
+                or S.max(initial=0) >= 2**31 or S.min(initial=0) <= -2**31):
             raise ArgumentError(L.NC_ERR_ARG, "state entries must fit in int32")
         self.stoich_matrix = S
         self.species_count, self.reaction_count = ns, nr

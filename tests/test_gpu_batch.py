@@ -67,6 +67,14 @@ def test_batch_graphs_equal_eager(gpu):
     for x, y, z in zip(a, b, c):
         for k in range(len(ts)):
             assert np.array_equal(x[k], y[k]) and np.array_equal(x[k], z[k])
+    # the one-call download of every instance equals the per-instance copies
+    for k in range(len(ts)):
+        every = B.snapshots(k)
+        assert len(every) == len(mats)
+        for i, row in enumerate(every):
+            assert np.array_equal(row, B.snapshot(i, k)) and np.array_equal(row, c[i][k])
+    with pytest.raises(nh.NumCmeHipError):
+        B.snapshots(len(ts))
 
 
 def test_batch_rejects_time_varying(gpu):
