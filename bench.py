@@ -174,7 +174,7 @@ def cpu_baseline(model_theta, krylov_tol, gpu_fp64_steps, st_fraction=0.05, log=
         "unit": "solves/s",
         "cores": cores,
         "kind": "port",
-        "label": "restatement, not reference (Julia is not available on the box)",
+        "label": "restatement, not reference (Julia is not installed on this machine)",
         "sample": f"the full FP64 C2 solve ({steps_mt} Krylov steps, m=30, tol {krylov_tol:g}) with a row-parallel "
                   f"CSR mat-vec and vector work on {cores} OpenMP threads, {secs_mt:.1f} s",
         "one_thread_csc": {
