@@ -120,6 +120,24 @@ struct RkEpilogue {
 bool matvec_rk_fusable(const nc_matrix_s* A);
 // out = A(t) v (FP64) and the epilogue's stage input
 void matvec_launch_rk(nc_matrix_s* A, double t, const void* v, double* out, hipStream_t s, const RkEpilogue& epi);
+// The DP5(4) error pass in the epilogue of the step's last mat-vec, the one that forms
+// k_7 = f(y_{n+1}) (the same generators as RkEpilogue): per row
+// r^2 = (h sum_j e_j k_j / (atol + rtol max(|u|, |y_{n+1}|)))^2, summed per row workgroup
+// into part[b] (the mat-vec's row workgroups in row order) and per sink row r into
+// part[blocks + r], blocks = matvec_err_blocks(A) - nr; with n_ex > 0 also the sink values
+// the event check reads: ex[r] = y_{n+1}, ex[nr + r] = k_1, ex[2 nr + r] = k_7 at sink row r.
+struct RkErrEpilogue {
+  const double* k[6];  // k_1 .. k_6 (k_2 is not read: its weight is zero)
+  double e[7];         // the error weights; e[6] multiplies the new k_7
+  double h, atol, rtol;
+  const double* u;
+  const double* ynew;
+  double* part;  // device-visible (the solver passes pinned host memory)
+  double* ex;
+  int n_ex;
+};
+int64_t matvec_err_parts(const nc_matrix_s* A);  // row workgroups + sink rows
+void matvec_launch_err(nc_matrix_s* A, double t, const void* v, double* out, hipStream_t s, const RkErrEpilogue& epi);
 // Arnoldi projection sums computed by the mat-vec (nc_dual.h): besides out = A v, every
 // row workgroup writes the sums of its 512 rows, with v = V_{nv-1} and q = A v, into
 // part (value-major, nblk = the mat-vec's row workgroups). Time-invariant stored

@@ -409,6 +409,15 @@ struct MatArgs {
   double* rk_ytmp;        // the last stage: y_{n+1} in FP64 too (else null)
   VT* rk_vin;             // the next mat-vec's input copy (not the buffer this launch reads)
   int32_t rk_mask;
+  // DP5(4) error pass (ERR kernels)
+  const double* err_k[6];
+  double err_e[7];
+  double err_h, err_atol, err_rtol;
+  const double* err_u;
+  const double* err_ynew;
+  double* err_part;
+  double* err_ex;
+  int32_t err_nex;
   // Arnoldi projection sums (DOT kernels): basis V_0..V_{dot_nv-1} with stride dot_ld
   const VT* dot_V;
   int64_t dot_ld;
@@ -702,6 +711,32 @@ __device__ inline void rk_epilogue(const MatArgs<VT>& a, int64_t i, int64_t nw, 
   }
 }
 
+// Squared scaled DP5(4) error of row i whose new stage value is k7 (ERR kernels): the
+// weighted stage sum in stage order, times h, over atol + rtol max(|u|, |y_{n+1}|).
+template <typename VT>
+__device__ inline double err_row(const MatArgs<VT>& a, int64_t i, double k7) {
+  double e = 0.0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    if (j != 1) e = __fma_rn(a.err_e[j], a.err_k[j][i], e);
+  e = __fma_rn(a.err_e[6], k7, e) * a.err_h;
+  const double sc = a.err_atol + a.err_rtol * fmax(fabs(a.err_u[i]), fabs(a.err_ynew[i]));
+  const double r = e / sc;
+  return r * r;
+}
+
+// Sink row q of an ERR launch (one thread): its error term and the event's sink values.
+template <typename VT>
+__device__ inline void err_sink(const MatArgs<VT>& a, int64_t blocks, int q, double k7) {
+  const int64_t o = a.n + q;
+  a.err_part[blocks + q] = err_row(a, o, k7);
+  if (a.err_nex > 0) {
+    a.err_ex[q] = a.err_ynew[o];
+    a.err_ex[a.nr + q] = a.err_k[0][o];
+    a.err_ex[2 * a.nr + q] = k7;
+  }
+}
+
 // Rows [i0, i0 + R) of the stored generator: out = A v (plus out when accumulating), the
 // FP64 row sums left in acc and v at the rows in vi (0 past the states).
 template <typename VT, typename OT, int KT, int R, int DM>
@@ -794,7 +829,9 @@ __device__ __forceinline__ void stored_rows(const MatArgs<VT>& a, const VT* __re
 // DOT: the launch also writes the Arnoldi projection sums of its 512 rows (nc_dual.h):
 // the rows' q = A v and v go through LDS to the layout of the block sums, so the
 // projection pass need not read q back.
-template <typename VT, typename OT, int KT, int R, int DM, bool EPI = false, bool DOT = false>
+// ERR: the launch also forms the DP5(4) error sums of its rows (err_row), one per row
+// workgroup and one per sink row, and the sink values of the event check.
+template <typename VT, typename OT, int KT, int R, int DM, bool EPI = false, bool DOT = false, bool ERR = false>
 __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __restrict__ v, OT* __restrict__ out,
                                                    int accumulate) {
   const int64_t row_threads = (a.n + R - 1) / R;
@@ -828,6 +865,27 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
           yy[4 * h + r] = s_y[256 * h + 4 * lane + r];
         }
       dual_block<VT>(a.dot_V, a.dot_ld, a.dot_nv, lb * kDualRows, xx, yy, a.dot_sc, a.dot_part, row_blocks, lb);
+      return;
+    }
+    if constexpr (ERR) {
+      static_assert(DM <= 2 && !EPI && !DOT, "error epilogue: stored generator");
+      double r2 = 0.0;
+      if (i0 < a.n) {
+        double vi[R], acc[R];
+        stored_rows<VT, OT, KT, R, DM>(a, v, out, i0, accumulate, acc, vi);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (i0 + r < a.n) r2 += err_row(a, i0 + r, acc[r]);
+      }
+      __shared__ double s_r2[kBlock / 64];
+      r2 = wave_sum(r2);
+      if ((threadIdx.x & 63) == 0) s_r2[threadIdx.x >> 6] = r2;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) sum += s_r2[w];
+        a.err_part[lb] = sum;
+      }
       return;
     }
     if (i0 >= a.n) return;
@@ -889,6 +947,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
         const int64_t o = a.n + r;
         out[o] = (OT)(accumulate ? (double)out[o] + s : s);
         if constexpr (EPI) rk_epilogue<VT, 1>(a, o, o + 1, &s);
+        if constexpr (ERR) err_sink(a, row_blocks, r, s);
       }
     }
     if (a.n_chunks) {
@@ -908,6 +967,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec(MatArgs<VT> a, const VT* __re
             const int64_t o = a.n + q;
             out[o] = (OT)(accumulate ? (double)out[o] + s : s);
             if constexpr (EPI) rk_epilogue<VT, 1>(a, o, o + 1, &s);
+            if constexpr (ERR) err_sink(a, row_blocks, q, s);
           }
         }
         if (threadIdx.x == 0) __hip_atomic_store(a.chunk_arrivals, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -925,13 +985,29 @@ struct LaunchCfg {
   hipEvent_t e0, e1;
 };
 
-template <typename VT, typename OT, int K, int R, int DM, bool EPI = false, bool DOT = false>
+template <typename VT, typename OT, int K, int R, int DM, bool EPI = false, bool DOT = false, bool ERR = false>
 void launch_one(const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, OT* oo, int acc) {
   if (c.e0)
-    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM, EPI, DOT>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a, vv,
-                          oo, acc);
+    hipExtLaunchKernelGGL(k_matvec<VT, OT, K, R, DM, EPI, DOT, ERR>, dim3(c.grid), dim3(kBlock), 0, c.s, c.e0, c.e1, 0, a,
+                          vv, oo, acc);
   else
-    k_matvec<VT, OT, K, R, DM, EPI, DOT><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+    k_matvec<VT, OT, K, R, DM, EPI, DOT, ERR><<<c.grid, kBlock, 0, c.s>>>(a, vv, oo, acc);
+}
+
+// The error-epilogue kernels: FP64 output, 2 rows per thread, stored generator
+template <typename VT, int DM>
+void dispatch_err_kd(int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT* vv, double* oo) {
+  switch (K) {
+    case 1: launch_one<VT, double, 1, 2, DM, false, false, true>(c, a, vv, oo, 0); break;
+    case 2: launch_one<VT, double, 2, 2, DM, false, false, true>(c, a, vv, oo, 0); break;
+    case 3: launch_one<VT, double, 3, 2, DM, false, false, true>(c, a, vv, oo, 0); break;
+    case 4: launch_one<VT, double, 4, 2, DM, false, false, true>(c, a, vv, oo, 0); break;
+    case 5: launch_one<VT, double, 5, 2, DM, false, false, true>(c, a, vv, oo, 0); break;
+    case 6: launch_one<VT, double, 6, 2, DM, false, false, true>(c, a, vv, oo, 0); break;
+    case 7: launch_one<VT, double, 7, 2, DM, false, false, true>(c, a, vv, oo, 0); break;
+    case 8: launch_one<VT, double, 8, 2, DM, false, false, true>(c, a, vv, oo, 0); break;
+    default: launch_one<VT, double, 0, 2, 2, false, false, true>(c, a, vv, oo, 0); break;
+  }
 }
 
 // The projection-epilogue kernels: FP64 output, 2 rows per thread, every slot's column
@@ -1045,7 +1121,7 @@ void dispatch_r(int R, int K, const LaunchCfg& c, const MatArgs<VT>& a, const VT
 template <typename VT>
 void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool accumulate, hipStream_t s,
                      bool out_fp64, hipEvent_t e0, hipEvent_t e1, const RkEpilogue* epi = nullptr,
-                     const DualDots* dots = nullptr) {
+                     const DualDots* dots = nullptr, const RkErrEpilogue* err = nullptr) {
   MatArgs<VT> a{};
   a.n = A->n;
   a.ld = A->ld;
@@ -1096,6 +1172,28 @@ void launch_matvec_t(nc_matrix_s* A, double t, const void* v, void* out, bool ac
     a.rk_vin = reinterpret_cast<VT*>(epi->vin);
     a.rk_mask = epi->mask;
     dispatch_epi<VT>(A->K, c, a, vv, reinterpret_cast<double*>(out));
+    return;
+  }
+  if (err) {
+    if (!matvec_rk_fusable(A) || accumulate || !out_fp64 || R != 2)
+      throw Error(NC_ERR_INTERNAL, "mat-vec error epilogue not available");
+    for (int j = 0; j < 6; ++j) a.err_k[j] = err->k[j] ? err->k[j] : err->k[0];
+    for (int j = 0; j < 7; ++j) a.err_e[j] = err->e[j];
+    a.err_h = err->h;
+    a.err_atol = err->atol;
+    a.err_rtol = err->rtol;
+    a.err_u = err->u;
+    a.err_ynew = err->ynew;
+    a.err_part = err->part;
+    a.err_ex = err->ex;
+    a.err_nex = err->n_ex;
+    const uint32_t all = A->K >= 32 ? 0xffffffffu : ((1u << A->K) - 1u);
+    if (a.dia_mask == 0u)
+      dispatch_err_kd<VT, 0>(A->K, c, a, vv, reinterpret_cast<double*>(out));
+    else if ((a.dia_mask & all) == all)
+      dispatch_err_kd<VT, 1>(A->K, c, a, vv, reinterpret_cast<double*>(out));
+    else
+      launch_one<VT, double, 0, 2, 2, false, false, true>(c, a, vv, reinterpret_cast<double*>(out), 0);
     return;
   }
   if (dots) {
@@ -1591,6 +1689,28 @@ void matvec_launch_rk(nc_matrix_s* A, double t, const void* v, double* out, hipS
     launch_matvec_t<float>(A, t, v, out, false, s, true, e0, e1, &epi);
   else
     launch_matvec_t<double>(A, t, v, out, false, s, true, e0, e1, &epi);
+  NC_CHECK_HIP(hipGetLastError());
+  if (timed) A->ev_used++;
+  A->launches++;
+}
+
+int64_t matvec_err_parts(const nc_matrix_s* A) {
+  const int64_t row_threads = (A->n + 1) / 2;  // 2 rows per thread
+  return (row_threads + kBlock - 1) / kBlock + A->nr;
+}
+
+void matvec_launch_err(nc_matrix_s* A, double t, const void* v, double* out, hipStream_t s, const RkErrEpilogue& epi) {
+  if (A->precision == NC_FP32)
+    joint_refresh<float>(A, t);
+  else
+    joint_refresh<double>(A, t);
+  if (s != A->stream.s) throw Error(NC_ERR_INTERNAL, "error-epilogue mat-vec off the matrix stream");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed = matvec_timing_slot(A, &e0, &e1);
+  if (A->precision == NC_FP32)
+    launch_matvec_t<float>(A, t, v, out, false, s, true, e0, e1, nullptr, nullptr, &epi);
+  else
+    launch_matvec_t<double>(A, t, v, out, false, s, true, e0, e1, nullptr, nullptr, &epi);
   NC_CHECK_HIP(hipGetLastError());
   if (timed) A->ev_used++;
   A->launches++;

@@ -38,6 +38,7 @@ struct nc_solver_s {
   double* hbuf = nullptr;       // pinned host copy of kbuf
   double* hcoef = nullptr;      // pinned host staging of coef
   double* hpart = nullptr;      // pinned host copy of per-block partial sums
+  int64_t hpart_cap = 0;        // partial slots of hpart before its sink-value region
   double* hpart_dev = nullptr;  // the same pinned block as a device address: the RK error
                                 // pass writes its partials there directly
   double beta = 0.0, t_new = 0.0, anorm = 0.0;

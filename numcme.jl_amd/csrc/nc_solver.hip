@@ -662,8 +662,10 @@ const double h_dp_a[7][6] = NC_DP_A;  // the same table for the host (mat-vec ep
 #undef NC_DP_A
 const double h_dp_c[7] = {0, 1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
 // b - b* (5th minus embedded 4th order weights)
-__constant__ double c_dp_e[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525,
-                                 -1.0 / 40};
+#define NC_DP_E {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525, -1.0 / 40}
+__constant__ double c_dp_e[7] = NC_DP_E;
+const double h_dp_e[7] = NC_DP_E;  // the same weights for the host (the mat-vec's error epilogue)
+#undef NC_DP_E
 
 // Stage j lives in slot j of k, except that with sw = 1 stages 0 and 6 trade slots (an
 // accepted step hands k6 = f(y_{n+1}) on as the next k0 by flipping sw, not by copying).
@@ -821,9 +823,11 @@ void solver_reduce_setup(nc_solver_s* S, int nvals) {
     NC_CHECK_HIP(hipMemsetAsync(S->fold.p, 0, sizeof(double) * (kPartStrideMax + 1), S->s));
   }
   if (!S->hpart) {
-    // partials (up to kMaxRedBlocks) and the RK step's 3 R gathered sink values, then the
-    // carried sinks of u and the step's three sink sets (4 R) behind them
-    S->hpart = static_cast<double*>(pinned_alloc(sizeof(double) * (kMaxRedBlocks + 7 * kMaxReactions)));
+    // partials (up to kMaxRedBlocks, or one per row workgroup and sink row of the mat-vec's
+    // error epilogue) and the RK step's 3 R gathered sink values, then the carried sinks of
+    // u and the step's three sink sets (4 R) behind them
+    S->hpart_cap = std::max<int64_t>(kMaxRedBlocks, matvec_err_parts(S->A));
+    S->hpart = static_cast<double*>(pinned_alloc(sizeof(double) * (S->hpart_cap + 7 * kMaxReactions)));
     NC_CHECK_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&S->hpart_dev), S->hpart, 0));
   }
 }
@@ -1265,6 +1269,10 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
   // the stage inputs alternate between two buffers, since a mat-vec gathers from its input
   // while its epilogue writes the next one.
   const bool fuse = S->rkv2.p && (S->SA ? sens_rk_fusable(S->SA) : matvec_rk_fusable(A));
+  // the error pass in the epilogue of the step's last mat-vec (plain generator)
+  const int64_t err_parts = matvec_err_parts(A);
+  const bool efuse = fuse && !S->SA && err_parts <= S->hpart_cap;
+  const int Rg = (ev && R > 0) ? R : 0;
   auto stages = [&](double h) {
     unsigned char* cur = S->rkv.p;
     unsigned char* nxt = S->rkv2.p;
@@ -1297,6 +1305,19 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
         else
           matvec_launch_rk(A, ts, cur, kp(stage), s, e);
         std::swap(cur, nxt);
+      } else if (efuse && stage == 6) {
+        RkErrEpilogue e{};
+        for (int j = 0; j < 6; ++j) e.k[j] = kp(j);
+        for (int j = 0; j < 7; ++j) e.e[j] = h_dp_e[j];
+        e.h = h;
+        e.atol = atol;
+        e.rtol = rtol;
+        e.u = S->u.p;
+        e.ynew = S->ytmp.p;
+        e.part = S->hpart_dev;
+        e.ex = S->hpart_dev + err_parts;
+        e.n_ex = Rg;
+        matvec_launch_err(A, ts, cur, kp(stage), s, e);
       } else {
         op_apply(S, ts, cur, kp(stage));
       }
@@ -1328,7 +1349,7 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
     S->h = std::max(S->h, 1e-12);
   }
   // pinned: sinks of u, y_{n+1}, k0, k6 (R each), behind the readback region
-  double* hs = S->hpart + kMaxRedBlocks + 3 * kMaxReactions;
+  double* hs = S->hpart + S->hpart_cap + 3 * kMaxReactions;
   if (ev && R > 0) {
     NC_CHECK_HIP(hipMemcpyAsync(hs, S->u.p + ns, sizeof(double) * R, hipMemcpyDeviceToHost, s));
     NC_CHECK_HIP(hipStreamSynchronize(s));
@@ -1342,19 +1363,26 @@ void rk_advance(nc_solver_s* S, double t_target, EventSpec* ev, bool one_step) {
     const bool clamped = h < S->h;
     stages(h);
     // after stage 6 the stage input is y + h*sum b_j k_j (row 7 of A): ytmp = y_{n+1}
-    const int Rg = (ev && R > 0) ? R : 0;
     // the partials and the gathered sinks go straight to the pinned host block: the step's
     // read-back needs no copy launch, only the synchronisation
-    k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->ksw, ns, Rg, S->hpart_dev);
-    NC_CHECK_HIP(hipGetLastError());
-    NC_CHECK_HIP(hipStreamSynchronize(s));
-    for (int r = 0; r < Rg; ++r) {  // sinks of y_{n+1}, k0 and k6 behind the partials
-      hs[R + r] = S->hpart[S->red_blocks + r];
-      hs[2 * R + r] = S->hpart[S->red_blocks + R + r];
-      hs[3 * R + r] = S->hpart[S->red_blocks + 2 * R + r];
+    if (!efuse) {
+      k_rk_err<<<S->red_blocks, kBlock, 0, s>>>(N, h, atol, rtol, S->u.p, S->ytmp.p, k, S->ksw, ns, Rg, S->hpart_dev);
+      NC_CHECK_HIP(hipGetLastError());
     }
-    errsum = 0.0;
-    for (unsigned b = 0; b < S->red_blocks; ++b) errsum += S->hpart[b];
+    NC_CHECK_HIP(hipStreamSynchronize(s));
+    const int64_t np = efuse ? err_parts : (int64_t)S->red_blocks;
+    for (int r = 0; r < Rg; ++r) {  // sinks of y_{n+1}, k0 and k6 behind the partials
+      hs[R + r] = S->hpart[np + r];
+      hs[2 * R + r] = S->hpart[np + R + r];
+      hs[3 * R + r] = S->hpart[np + 2 * R + r];
+    }
+    // four running sums (the epilogue's partials number one per row workgroup), then added
+    double es[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t b = 0;
+    for (; b + 4 <= np; b += 4)
+      for (int q = 0; q < 4; ++q) es[q] += S->hpart[b + q];
+    for (; b < np; ++b) es[0] += S->hpart[b];
+    errsum = (es[0] + es[1]) + (es[2] + es[3]);
     S->st.host_syncs++;
     matvec_harvest_timing(A);
     const double err = std::sqrt(errsum / (double)S->N);
